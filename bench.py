@@ -1,0 +1,215 @@
+#!/usr/bin/env python3
+"""Benchmark of the IANE convolution hot path on one MI355X.
+
+Headline workload (the one the metric is quoted on): 3-D FP32 convolution of a
+256^3 volume (U[0,1) noise + 200 Gaussian blobs, seed 2) with the separable
+15x15x15 Gaussian (sigma 2.5), zero border. One step = one
+iane_hip_convolve_separable call over the whole volume, inputs resident in HBM.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
+
+Prints ONE JSON line. ``roofline`` is for the dominant kernel
+(sep3_fused_kernel<15>): achieved = algorithmic bytes per launch
+(256^3 voxels x (4 B read + 4 B written)) / average launch time measured
+with HIP events on the stream the kernel runs on. ``cpu_baseline`` times
+the reference's own CPU convolve (compiled from its headers into
+oracle/_ref) on a 16-plane z-slab of the same workload with the dense
+15^3 outer-product kernel, which is what the reference runs for it.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import pathlib
+import sys
+import time
+
+import numpy as np
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+METRIC = "Mvoxel/s 3-D conv (%HBM roofline); RL time-to-equal-likelihood, learned vs plain"
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+class Timer:
+    """HIP events recorded on an explicit stream through the C-ABI."""
+
+    def __init__(self, stream):
+        from iane_amd import _lib
+        import ctypes
+        self._lib = _lib
+        self.stream = stream
+        self.start = ctypes.c_void_p()
+        self.stop = ctypes.c_void_p()
+        _lib.call("iane_hip_event_create", ctypes.byref(self.start))
+        _lib.call("iane_hip_event_create", ctypes.byref(self.stop))
+
+    def time(self, fn, steps: int) -> float:
+        import ctypes
+        _lib = self._lib
+        _lib.call("iane_hip_stream_sync", self.stream)
+        _lib.call("iane_hip_event_record", self.start, self.stream)
+        for _ in range(steps):
+            fn()
+        _lib.call("iane_hip_event_record", self.stop, self.stream)
+        ms = ctypes.c_float()
+        _lib.call("iane_hip_event_elapsed_ms", ctypes.byref(ms), self.start, self.stop)
+        return ms.value
+
+
+def traffic_from_profiles(kernel_tag: str):
+    """HBM bytes per launch from a committed rocprofv3 PMC summary, if present."""
+    path = ROOT / "profiles" / f"{kernel_tag}_traffic.json"
+    if not path.exists():
+        return None
+    try:
+        return float(json.loads(path.read_text())["hbm_bytes_per_launch"])
+    except Exception:
+        return None
+
+
+def cpu_baseline(volume: np.ndarray, taps: np.ndarray, planes: int = 16):
+    """Reference CPU convolve on a z-slab, threads = the box's CPU share (<= 16)."""
+    from oracle import oracle
+    cores = min(16, os.cpu_count() or 1)
+    t = taps.astype(np.float64)
+    dense = np.multiply.outer(np.multiply.outer(t, t), t).astype(np.float32)
+    z0 = volume.shape[0] // 2 - planes // 2
+    slab = np.ascontiguousarray(volume[z0:z0 + planes])
+    kind = "reference" if oracle.have_reference() else "port"
+    run = (lambda: oracle.convolve_ref(slab, dense, np.float32, oracle.ZERO, threads=cores)) if kind == "reference" \
+        else (lambda: oracle.convolve(slab, dense, np.float32, oracle.ZERO, threads=cores))
+    run()  # warm-up (run_on.hpp:43-59: warmup then samples)
+    samples = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        run()
+        samples.append(time.perf_counter() - t0)
+    median = sorted(samples)[1]
+    return {"value": slab.size / median / 1e6, "unit": "Mvoxel/s", "cores": cores, "kind": kind,
+            "sample": f"{planes}-plane z-slab ({slab.shape[2]}x{slab.shape[1]}x{planes}) of the 256^3 workload, "
+                      f"dense 15^3 outer-product kernel (the reference has no separable path), zero border, "
+                      f"cpu_policy{{threads={cores}, grain=4096}}, median of 3 after 1 warm-up",
+            "median_s": median}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--quick", action="store_true", help="headline only")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(slab decomposition with halo exchange is designed, not built)", file=sys.stderr)
+        return 2
+
+    import ctypes
+    import torch
+    import iane_amd
+    from iane_amd import _lib, synth, border_mode, convolution_options, precision
+
+    torch.cuda.init()
+    n = args.size
+    log(f"[bench] generating 256^3 workload (seed 2)")
+    data = synth.volume_blobs(n, seed=2)
+    vol_h, taps_h = data["volume"], data["taps"]
+    stream_p = ctypes.c_void_p()
+    _lib.call("iane_hip_stream_create", ctypes.byref(stream_p))
+    stream = stream_p.value
+    policy = iane_amd.hip_policy(stream)
+    timer = Timer(stream)
+
+    vol = torch.from_numpy(vol_h).cuda()
+    taps = torch.from_numpy(taps_h).cuda()
+    out = torch.empty_like(vol)
+    torch.cuda.synchronize()
+    opts = convolution_options(border_mode.zero, precision.fp32)
+
+    def step():
+        iane_amd.convolve_separable(vol, [taps, taps, taps], out, opts, policy)
+
+    log(f"[bench] separable 15^3 fp32: warmup {args.warmup}")
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    _lib.call("iane_hip_stream_sync", stream)
+    log(f"[bench] separable 15^3 fp32: timing {args.steps} steps")
+    t0 = time.perf_counter()
+    ms = timer.time(step, args.steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = ms / args.steps
+    voxels = vol_h.size
+    value = voxels / (ms_per_step / 1e3) / 1e6
+    bytes_per_launch = voxels * 8.0
+    achieved = bytes_per_launch / (ms_per_step / 1e3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles("sep3_fused_256"),
+                "kernel": "sep3_fused_kernel<15>", "bytes_per_launch": bytes_per_launch,
+                "avg_launch_us": round(ms_per_step * 1e3, 2)}
+    log(f"[bench] separable: {ms_per_step * 1e3:.1f} us/step, {value:.0f} Mvoxel/s, {achieved:.0f} GB/s")
+
+    extra = {}
+    if not args.quick:
+        # dense random 15^3 kernel, reference (FP64, bit-exact) arithmetic: compute bound
+        dense = torch.from_numpy(data["dense"]).cuda()
+        for name, prec in (("dense15_reference", precision.reference), ("dense15_fp32", precision.fp32)):
+            o = convolution_options(border_mode.zero, prec)
+            fn = lambda: iane_amd.convolve(vol, dense, out, o, policy)  # noqa: E731
+            fn()
+            _lib.call("iane_hip_stream_sync", stream)
+            k = 3
+            t = timer.time(fn, k) / k
+            flops = 2.0 * voxels * 15 ** 3
+            extra[name] = {"ms": round(t, 3), "Mvoxel_s": round(voxels / t / 1e3, 1),
+                           "TFLOP_s": round(flops / t / 1e9, 1),
+                           "frac_of_fp64_vector_peak": round(flops / t / 1e9 / FP64_PEAK_TFLOPS, 3)}
+            log(f"[bench] {name}: {t:.2f} ms")
+        # u16 volume, 3^3 binomial int kernel, int32 result (bit-exact)
+        v16 = torch.from_numpy(data["volume_u16"].view(np.int16)).cuda().view(torch.uint16)
+        binom = torch.from_numpy(data["binomial"]).cuda()
+        o32 = torch.empty(vol.shape, dtype=torch.int32, device="cuda")
+        o = convolution_options(border_mode.zero, precision.reference)
+        fn = lambda: iane_amd.convolve(v16, binom, o32, o, policy)  # noqa: E731
+        fn()
+        _lib.call("iane_hip_stream_sync", stream)
+        k = 20
+        t = timer.time(fn, k) / k
+        extra["u16_binomial3"] = {"ms": round(t, 4), "Mvoxel_s": round(voxels / t / 1e3, 1),
+                                  "GB_s": round(voxels * 6 / t / 1e6, 1),
+                                  "frac_hbm": round(voxels * 6 / t / 1e6 / HBM_PEAK_GBS, 3)}
+        log(f"[bench] u16 3^3: {t * 1e3:.1f} us")
+
+    base = None
+    if not args.no_cpu:
+        log("[bench] cpu baseline (reference CPU convolve, z-slab)")
+        base = cpu_baseline(vol_h, taps_h)
+        log(f"[bench] cpu baseline: {base['value']:.2f} Mvoxel/s on {base['cores']} cores")
+
+    line = {
+        "metric": METRIC, "value": round(value, 1), "unit": "Mvoxel/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (Philox4x32-10 seed 2: U[0,1) + 200 Gaussian blobs)",
+        "config": {"workload": f"3-D FP32 convolution {n}^3, separable 15x15x15 Gaussian sigma 2.5, zero border",
+                   "voxels": voxels, "parallelism": "single GPU"},
+        "roofline": roofline, "cpu_baseline": base, "wall_s_timed": round(wall, 3), "extra": extra,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

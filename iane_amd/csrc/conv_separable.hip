@@ -1,0 +1,404 @@
+// Separable convolution behind iane_hip_convolve_separable: the kernel is the
+// outer product of one odd-length tap vector per axis. With either border rule
+// this equals the reference convolve with the dense outer-product kernel
+// (include/iane/algorithms/convolution.hpp:23-117): the clamp rule clamps each
+// axis independently and the zero rule zeroes a sample when any axis is out of
+// range, and both factor over axes.
+//
+// Fast path (rank 3, FP32 accumulation, equal odd tap lengths K <= 15): one
+// fused kernel. A workgroup owns a 64 x 32 (x, y) output tile and streams a
+// chunk of z planes. For every input plane it stages the plane region with a
+// K-1 halo in x and y into LDS, filters along y (columns of the halo-wide
+// region), then along x, and pushes the filtered plane into a per-thread
+// register ring of K planes from which the z filter produces one output plane
+// per input plane. Input is read once per chunk (plus the z halo), output
+// written once.
+//
+// General path: one 1-D pass per axis through stream-ordered temporaries.
+#include "common.hpp"
+
+namespace iane_hip {
+
+void validate_convolution(const view4& aSrc, const view4& aKer, const view4& aDst);
+
+namespace {
+
+constexpr int kSepThreads = 512;
+constexpr int kSepTileX = 64;
+constexpr int kSepTileY = 32;
+constexpr int kSepColsPerThread = kSepTileX * kSepTileY / kSepThreads; // 4 (consecutive y)
+
+struct sep_args {
+	const char* src;
+	int src_type;
+	int nx, ny, nz;
+	long long sx, sy, sz;       // source byte strides
+	char* dst;
+	int dst_type;
+	long long dx, dy, dz;       // destination byte strides
+	int border;
+	unsigned tiles_x, tiles_y, chunks_z;
+	int chunk;                  // output planes per workgroup
+	const char* taps[3];        // device tap vectors
+	long long tap_stride[3];
+	int tap_type[3];
+};
+
+__device__ __forceinline__ bool clamp_or_zero(int& aCoord, int aExtent, int aBorder)
+{
+	if (aCoord >= 0 && aCoord < aExtent) {
+		return true;
+	}
+	if (aBorder == IANE_BORDER_ZERO) {
+		return false;
+	}
+	aCoord = aCoord < 0 ? 0 : aExtent - 1;
+	return true;
+}
+
+template<int K>
+__global__ __launch_bounds__(kSepThreads) void sep3_fused_kernel(sep_args a)
+{
+	constexpr int C = K / 2;
+	constexpr int cols = kSepTileX + K - 1;   // staged x extent
+	constexpr int rows = kSepTileY + K - 1;   // staged y extent
+	constexpr int pitch = cols + 1;           // odd pitch: column reads stay conflict-free
+	__shared__ float in_s[rows * pitch];
+	__shared__ float yr_s[kSepTileY * pitch]; // y-filtered, still x-halo wide
+
+	const unsigned block = xcd_remap(blockIdx.x, gridDim.x);
+	const unsigned tile_x = block % a.tiles_x;
+	const unsigned rest = block / a.tiles_x;
+	const unsigned tile_y = rest % a.tiles_y;
+	const unsigned chunk = rest / a.tiles_y;
+	const int x0 = static_cast<int>(tile_x) * kSepTileX;
+	const int y0 = static_cast<int>(tile_y) * kSepTileY;
+	const int z0 = static_cast<int>(chunk) * a.chunk;
+	const int zend = min(z0 + a.chunk, a.nz);
+	const int tid = threadIdx.x;
+
+	float kx[K], ky[K], kz[K];
+#pragma unroll
+	for (int i = 0; i < K; ++i) {
+		kx[i] = load_as_float(a.taps[0] + i * a.tap_stride[0], a.tap_type[0]);
+		ky[i] = load_as_float(a.taps[1] + i * a.tap_stride[1], a.tap_type[1]);
+		kz[i] = load_as_float(a.taps[2] + i * a.tap_stride[2], a.tap_type[2]);
+	}
+
+	// Thread ownership for the x filter and the z ring: column x = tid % 64,
+	// rows y = (tid / 64) * 4 + r.
+	const int ox = tid % kSepTileX;
+	const int oy = (tid / kSepTileX) * kSepColsPerThread;
+
+	float ring[K][kSepColsPerThread];
+#pragma unroll
+	for (int j = 0; j < K; ++j) {
+#pragma unroll
+		for (int r = 0; r < kSepColsPerThread; ++r) {
+			ring[j][r] = 0.0f;
+		}
+	}
+
+	const int planes = (zend - z0) + K - 1; // input planes z0 - C .. zend - 1 + C
+	for (int base = 0; base < planes; base += K) {
+#pragma unroll
+		for (int j = 0; j < K; ++j) {
+			const int p = base + j;
+			if (p >= planes) {
+				break;
+			}
+			int zi = z0 - C + p;
+			const bool plane_in = clamp_or_zero(zi, a.nz, a.border);
+			float xy[kSepColsPerThread];
+			if (plane_in) {
+				__syncthreads(); // previous plane's readers of in_s / yr_s are done
+				const char* plane = a.src + zi * a.sz;
+				for (int i = tid; i < rows * cols; i += kSepThreads) {
+					const int ly = i / cols;
+					const int lx = i - ly * cols;
+					int xi = x0 - C + lx;
+					int yi = y0 - C + ly;
+					float v = 0.0f;
+					if (clamp_or_zero(xi, a.nx, a.border) && clamp_or_zero(yi, a.ny, a.border)) {
+						v = load_as_float(plane + xi * a.sx + yi * a.sy, a.src_type);
+					}
+					in_s[ly * pitch + lx] = v;
+				}
+				__syncthreads();
+				// y filter over the halo-wide region: kSepTileY x cols outputs.
+				for (int i = tid; i < (kSepTileY / 4) * cols; i += kSepThreads) {
+					const int lx = i % cols;
+					const int gy = (i / cols) * 4;
+					float w[K + 3];
+#pragma unroll
+					for (int t = 0; t < K + 3; ++t) {
+						w[t] = in_s[(gy + t) * pitch + lx];
+					}
+#pragma unroll
+					for (int r = 0; r < 4; ++r) {
+						float s = 0.0f;
+#pragma unroll
+						for (int q = 0; q < K; ++q) {
+							s = fmaf(w[r + K - 1 - q], ky[q], s);
+						}
+						yr_s[(gy + r) * pitch + lx] = s;
+					}
+				}
+				__syncthreads();
+				// x filter: each thread its column ox for rows oy .. oy+3.
+#pragma unroll
+				for (int r = 0; r < kSepColsPerThread; ++r) {
+					const float* row = yr_s + (oy + r) * pitch + ox;
+					float s = 0.0f;
+#pragma unroll
+					for (int q = 0; q < K; ++q) {
+						s = fmaf(row[K - 1 - q], kx[q], s);
+					}
+					xy[r] = s;
+				}
+			} else {
+#pragma unroll
+				for (int r = 0; r < kSepColsPerThread; ++r) {
+					xy[r] = 0.0f;
+				}
+			}
+			// push into the ring: slot (p mod K) == j because base is a multiple of K
+#pragma unroll
+			for (int r = 0; r < kSepColsPerThread; ++r) {
+				ring[j][r] = xy[r];
+			}
+			if (p >= K - 1) {
+				const int zo = z0 + p - (K - 1);
+				float out[kSepColsPerThread];
+#pragma unroll
+				for (int r = 0; r < kSepColsPerThread; ++r) {
+					out[r] = 0.0f;
+				}
+#pragma unroll
+				for (int q = 0; q < K; ++q) {
+					const int slot = ((j - q) % K + K) % K;
+#pragma unroll
+					for (int r = 0; r < kSepColsPerThread; ++r) {
+						out[r] = fmaf(ring[slot][r], kz[q], out[r]);
+					}
+				}
+				const int px = x0 + ox;
+				if (px < a.nx) {
+#pragma unroll
+					for (int r = 0; r < kSepColsPerThread; ++r) {
+						const int py = y0 + oy + r;
+						if (py < a.ny) {
+							store_converted(a.dst + px * a.dx + py * a.dy + zo * a.dz, a.dst_type, out[r]);
+						}
+					}
+				}
+			}
+		}
+	}
+}
+
+// One 1-D pass along `axis` of a rank-4-padded view; Acc-typed temporaries in
+// between passes, final pass converts to the destination type.
+template<class Acc>
+__global__ __launch_bounds__(256) void axis_pass_kernel(view4 aIn, view4 aOut, const char* aTaps, int aTapType,
+	long long aTapStride, int aTaps_n, int aAxis, int aBorder, long long aCount)
+{
+	const long long index = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (index >= aCount) {
+		return;
+	}
+	int p[4];
+	long long rest = index;
+	for (int i = 0; i < 4; ++i) {
+		p[i] = static_cast<int>(rest % aOut.extent[i]);
+		rest /= aOut.extent[i];
+	}
+	long long base = 0;
+	long long obase = 0;
+	for (int i = 0; i < 4; ++i) {
+		if (i != aAxis) {
+			base += static_cast<long long>(p[i]) * aIn.stride[i];
+		}
+		obase += static_cast<long long>(p[i]) * aOut.stride[i];
+	}
+	const int c = aTaps_n / 2;
+	const int extent = aIn.extent[aAxis];
+	Acc sum = Acc(0);
+	for (int q = 0; q < aTaps_n; ++q) {
+		int coord = p[aAxis] + c - q;
+		Acc sample = Acc(0);
+		if (clamp_or_zero(coord, extent, aBorder)) {
+			sample = load_as<Acc>(aIn.ptr + base + coord * aIn.stride[aAxis], aIn.type);
+		}
+		sum = fma(sample, load_as<Acc>(aTaps + q * aTapStride, aTapType), sum);
+	}
+	store_converted(aOut.ptr + obase, aOut.type, sum);
+}
+
+view4 packed_temp(const view4& aLike, int aType, void* aPtr)
+{
+	view4 v = aLike;
+	v.ptr = static_cast<char*>(aPtr);
+	v.type = aType;
+	long long stride = dtype_size(aType);
+	for (int i = 0; i < 4; ++i) {
+		v.stride[i] = stride;
+		stride *= v.extent[i];
+	}
+	return v;
+}
+
+template<int K>
+void launch_fused(sep_args& a, hipStream_t aStream)
+{
+	a.tiles_x = ceil_div(a.nx, kSepTileX);
+	a.tiles_y = ceil_div(a.ny, kSepTileY);
+	// chunk so that the grid covers the chip at least twice: 256 CUs
+	const long long xy = static_cast<long long>(a.tiles_x) * a.tiles_y;
+	int chunk = a.nz;
+	while (chunk > 8 && xy * ceil_div(a.nz, chunk) < 256) {
+		chunk = (chunk + 1) / 2;
+	}
+	a.chunk = chunk;
+	a.chunks_z = ceil_div(a.nz, chunk);
+	const long long blocks = xy * a.chunks_z;
+	sep3_fused_kernel<K><<<static_cast<unsigned>(blocks), kSepThreads, 0, aStream>>>(a);
+}
+
+} // namespace
+
+void convolve_separable(const view4& aSrc, const view4* aTaps, int aN, const view4& aDst, int aBorder,
+	int aPrecision, hipStream_t aStream, bool aAllowFused)
+{
+	const int rank = aSrc.rank;
+	bool fused = aAllowFused && rank == 3 && aPrecision == IANE_PREC_FP32;
+	const int k = aTaps[0].extent[0];
+	for (int i = 0; i < aN; ++i) {
+		fused = fused && aTaps[i].extent[0] == k;
+	}
+	fused = fused && k <= 15;
+	if (fused) {
+		sep_args a{};
+		for (int axis = 0; axis < 3; ++axis) {
+			a.taps[axis] = aTaps[axis].ptr;
+			a.tap_stride[axis] = aTaps[axis].stride[0];
+			a.tap_type[axis] = aTaps[axis].type;
+		}
+		a.src = aSrc.ptr;
+		a.src_type = aSrc.type;
+		a.nx = aSrc.extent[0];
+		a.ny = aSrc.extent[1];
+		a.nz = aSrc.extent[2];
+		a.sx = aSrc.stride[0];
+		a.sy = aSrc.stride[1];
+		a.sz = aSrc.stride[2];
+		a.dst = aDst.ptr;
+		a.dst_type = aDst.type;
+		a.dx = aDst.stride[0];
+		a.dy = aDst.stride[1];
+		a.dz = aDst.stride[2];
+		a.border = aBorder;
+		switch (k) {
+		case 1: launch_fused<1>(a, aStream); break;
+		case 3: launch_fused<3>(a, aStream); break;
+		case 5: launch_fused<5>(a, aStream); break;
+		case 7: launch_fused<7>(a, aStream); break;
+		case 9: launch_fused<9>(a, aStream); break;
+		case 11: launch_fused<11>(a, aStream); break;
+		case 13: launch_fused<13>(a, aStream); break;
+		default: launch_fused<15>(a, aStream); break;
+		}
+		check_hip(hipGetLastError(), "Separable convolution launch");
+		return;
+	}
+	// general path: one pass per axis
+	const bool wide = aPrecision != IANE_PREC_FP32;
+	const int temp_type = wide ? IANE_F64 : IANE_F32;
+	const long long count = element_count(aDst);
+	void* temps[2] = {nullptr, nullptr};
+	const int passes = rank;
+	for (int t = 0; t < 2 && t < passes - 1; ++t) {
+		check_hip(hipMallocAsync(&temps[t], static_cast<size_t>(count) * dtype_size(temp_type), aStream),
+			"Allocate separable temporary");
+	}
+	view4 in = aSrc;
+	for (int axis = 0; axis < passes; ++axis) {
+		const bool last = axis == passes - 1;
+		const view4 out = last ? aDst : packed_temp(aDst, temp_type, temps[axis % 2]);
+		if (wide) {
+			axis_pass_kernel<double><<<ceil_div(count, 256), 256, 0, aStream>>>(in, out, aTaps[axis].ptr,
+				aTaps[axis].type, aTaps[axis].stride[0], aTaps[axis].extent[0], axis, aBorder, count);
+		} else {
+			axis_pass_kernel<float><<<ceil_div(count, 256), 256, 0, aStream>>>(in, out, aTaps[axis].ptr,
+				aTaps[axis].type, aTaps[axis].stride[0], aTaps[axis].extent[0], axis, aBorder, count);
+		}
+		check_hip(hipGetLastError(), "Separable pass launch");
+		in = out;
+	}
+	for (int t = 0; t < 2; ++t) {
+		if (temps[t]) {
+			check_hip(hipFreeAsync(temps[t], aStream), "Free separable temporary");
+		}
+	}
+}
+
+} // namespace iane_hip
+
+using namespace iane_hip;
+
+extern "C" int iane_hip_convolve_separable(const iane_view_desc* aSrc, const iane_view_desc* aTaps, int aN,
+	const iane_view_desc* aDst, int aBorder, int aPrecision, void* aStream)
+{
+	return guarded([&] {
+		const auto src = make_view4(aSrc, "source");
+		const auto dst = make_view4(aDst, "destination");
+		if (!aTaps || aN != src.rank) {
+			invalid("separable convolution needs one tap vector per axis");
+		}
+		view4 taps[IANE_MAX_RANK];
+		view4 shape = src;
+		for (int i = 0; i < aN; ++i) {
+			taps[i] = make_view4(&aTaps[i], "taps");
+			if (taps[i].rank != 1 || (taps[i].type != IANE_F32 && taps[i].type != IANE_F64)) {
+				invalid("separable taps must be rank-1 f32 or f64 views");
+			}
+			shape.extent[i] = taps[i].extent[0];
+		}
+		for (int i = aN; i < 4; ++i) {
+			shape.extent[i] = 1;
+		}
+		shape.type = taps[0].type;
+		validate_convolution(src, shape, dst);
+		if (aBorder != IANE_BORDER_CLAMP && aBorder != IANE_BORDER_ZERO) {
+			invalid("unknown border mode");
+		}
+		if (aPrecision != IANE_PREC_REFERENCE && aPrecision != IANE_PREC_FP32) {
+			fail(IANE_ERR_UNSUPPORTED, "separable convolution supports REFERENCE and FP32 precision");
+		}
+		if (element_count(dst) == 0) {
+			return;
+		}
+		convolve_separable(src, taps, aN, dst, aBorder, aPrecision, static_cast<hipStream_t>(aStream), true);
+	});
+}
+
+// Test hook: force the per-axis pass path.
+extern "C" int iane_hip_convolve_separable_passes(const iane_view_desc* aSrc, const iane_view_desc* aTaps, int aN,
+	const iane_view_desc* aDst, int aBorder, int aPrecision, void* aStream)
+{
+	return guarded([&] {
+		const auto src = make_view4(aSrc, "source");
+		const auto dst = make_view4(aDst, "destination");
+		if (!aTaps || aN != src.rank) {
+			invalid("separable convolution needs one tap vector per axis");
+		}
+		view4 taps[IANE_MAX_RANK];
+		for (int i = 0; i < aN; ++i) {
+			taps[i] = make_view4(&aTaps[i], "taps");
+		}
+		if (element_count(dst) == 0) {
+			return;
+		}
+		convolve_separable(src, taps, aN, dst, aBorder, aPrecision, static_cast<hipStream_t>(aStream), false);
+	});
+}

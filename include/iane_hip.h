@@ -1,0 +1,130 @@
+/*
+ * iane_hip.h — C-ABI of libiane_hip.so, the MI355X (gfx950) backend of the
+ * IANE convolution / deconvolution path.
+ *
+ * Every entry point takes plain pointers, sizes and small descriptor structs.
+ * No HIP, torch or C++ types appear in the signatures: a stream is passed as
+ * `void*` (a hipStream_t, NULL = the null stream). All work is enqueued on that
+ * stream and the call returns without waiting, exactly like the reference's
+ * device path returns a completion handle (include/iane/execution/context.hpp:51-59).
+ *
+ * Status convention: every function returns an int status, 0 on success.
+ * The message of the last failure on the calling thread is returned by
+ * iane_hip_last_error(). The C++ headers in include/iane map the codes to the
+ * exception types the reference throws (std::invalid_argument,
+ * std::length_error, std::overflow_error, std::runtime_error).
+ *
+ * Image layout follows the reference views (include/iane/view/host_image_view.hpp:11-81):
+ * a base pointer, `int` extents and `int` BYTE strides per axis, axis 0 fastest.
+ */
+#ifndef IANE_HIP_H
+#define IANE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+enum {
+	IANE_OK = 0,
+	IANE_ERR_INVALID_ARGUMENT = 1, /* std::invalid_argument in the reference */
+	IANE_ERR_UNSUPPORTED = 2,      /* dtype / rank / size not provided by this backend */
+	IANE_ERR_HIP = 3,              /* std::runtime_error("<op>: <hip error>") */
+	IANE_ERR_LENGTH = 4,           /* std::length_error (scratch reservation exceeded) */
+	IANE_ERR_OVERFLOW = 5          /* std::overflow_error (layout not representable) */
+};
+
+/* ---- element types (arithmetic elements of the reference's convolve) ---- */
+enum {
+	IANE_U8 = 0,
+	IANE_U16 = 1,
+	IANE_I32 = 2,
+	IANE_F32 = 3,
+	IANE_F64 = 4,
+	IANE_C64 = 5 /* iane::fft_complex {float real, imag}, include/iane/algorithms/fft/complex.hpp:6-9 */
+};
+
+/* ---- border rule, include/iane/locator/image_locator.hpp:14 (same order) ---- */
+enum { IANE_BORDER_CLAMP = 0, IANE_BORDER_ZERO = 1 };
+
+/* ---- arithmetic of a convolution ---------------------------------------
+ * REFERENCE: FP64 fma per tap in kernel storage order, exactly the sum the
+ *   reference forms (include/iane/algorithms/convolution.hpp:36-49), then the
+ *   reference's output conversion (:51-75). Bit-exact to the reference.
+ * FP32: FP32 fma, same tap order; float outputs within ~1e-6 relative.
+ * SPLIT_BF16 / SPLIT_FP16: MFMA with hi+lo split operands and FP32
+ *   accumulation (large kernels); float destinations only.            */
+enum {
+	IANE_PREC_REFERENCE = 0,
+	IANE_PREC_FP32 = 1,
+	IANE_PREC_SPLIT_BF16 = 2,
+	IANE_PREC_SPLIT_FP16 = 3
+};
+
+#define IANE_MAX_RANK 4
+
+/* A strided view: include/iane/view/device_image_view.hpp:11-60 as plain data. */
+typedef struct iane_view_desc {
+	void* ptr;
+	int32_t rank;                        /* 1..IANE_MAX_RANK */
+	int32_t dtype;                       /* IANE_U8 .. IANE_C64 */
+	int32_t extent[IANE_MAX_RANK];       /* axis 0 fastest */
+	int32_t stride_bytes[IANE_MAX_RANK]; /* byte strides, may be negative */
+} iane_view_desc;
+
+/* ---- runtime --------------------------------------------------------- */
+const char* iane_hip_last_error(void);
+int iane_hip_version(void);
+int iane_hip_device_count(int* count);
+int iane_hip_set_device(int device);
+int iane_hip_malloc(void** ptr, size_t bytes);           /* device_image_resource, resource_cuda.hpp:26-41 */
+int iane_hip_free(void* ptr);
+int iane_hip_host_malloc(void** ptr, size_t bytes);      /* pinned staging for run_on */
+int iane_hip_host_free(void* ptr);
+int iane_hip_stream_create(void** stream);                /* run_on.hpp:145-153 (non-blocking stream) */
+int iane_hip_stream_destroy(void* stream);
+int iane_hip_stream_sync(void* stream);                   /* cuda_execution::wait, context.hpp:57-58 */
+int iane_hip_device_sync(void);
+/* kind: 0 host->host, 1 host->device, 2 device->host, 3 device->device (hipMemcpyKind order) */
+int iane_hip_memcpy_async(void* dst, const void* src, size_t bytes, int kind, void* stream);
+/* Strided copy between two views of the same element type and extent
+ * (copy.hpp:55-108: host<->device transfer of packed or per-plane strided views). */
+int iane_hip_copy_view(const iane_view_desc* src, const iane_view_desc* dst, int kind, void* stream);
+int iane_hip_memset_async(void* dst, int value, size_t bytes, void* stream);
+/* Events for kernel timing on a given stream. */
+int iane_hip_event_create(void** event);
+int iane_hip_event_destroy(void* event);
+int iane_hip_event_record(void* event, void* stream);
+int iane_hip_event_elapsed_ms(float* ms, void* start, void* stop);
+
+/* ---- convolution ----------------------------------------------------
+ * Replaces iane::convolve(src, kernel, dst, convolution_options{border},
+ * cuda_policy{stream}) for device views, include/iane/algorithms/convolution.hpp:97-117.
+ * Kernel extents must be positive and odd (:107-113), ranks equal (:100-101),
+ * dst extents equal src extents (include/iane/algorithms/transform.hpp:51-63).
+ * Element types: src/kernel/dst each one of U8, U16, I32, F32, F64.     */
+int iane_hip_convolve(const iane_view_desc* src, const iane_view_desc* kernel,
+	const iane_view_desc* dst, int border, int precision, void* stream);
+
+/* Separable convolution: the kernel is the outer product taps[0] (axis 0)
+ * x taps[1] x ... (each a rank-1 F32/F64 view of odd extent). Equal, up to
+ * rounding, to iane_hip_convolve with the dense outer-product kernel.
+ * No reference counterpart: the reference always runs the dense kernel. */
+int iane_hip_convolve_separable(const iane_view_desc* src, const iane_view_desc* taps,
+	int n_taps, const iane_view_desc* dst, int border, int precision, void* stream);
+
+/* Test hooks: the untiled one-thread-per-output kernel and the per-axis
+ * separable passes, so tests can compare every path against the oracle. */
+int iane_hip_convolve_untiled(const iane_view_desc* src, const iane_view_desc* kernel,
+	const iane_view_desc* dst, int border, int precision, void* stream);
+int iane_hip_convolve_separable_passes(const iane_view_desc* src, const iane_view_desc* taps,
+	int n_taps, const iane_view_desc* dst, int border, int precision, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* IANE_HIP_H */

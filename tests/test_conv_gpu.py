@@ -1,0 +1,279 @@
+"""GPU parity of iane_hip_convolve / iane_hip_convolve_separable against the oracle.
+
+REFERENCE precision must be bit-identical to the reference's CPU convolve
+(oracle, golden fixtures written by the reference). FP32 precision and the
+separable FP32 path must agree with the FP64 oracle to
+    max|gpu - oracle64| <= TOL_FP32 * max|oracle64|      (TOL_FP32 = 1e-5).
+"""
+from __future__ import annotations
+
+import pathlib
+
+import numpy as np
+import pytest
+
+import iane_amd
+from iane_amd import border_mode, convolution_options, precision
+from oracle import oracle
+from tests import cases
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = pathlib.Path(__file__).resolve().parent / "golden"
+TOL_FP32 = 1e-5
+DT = [np.uint8, np.uint16, np.int32, np.float32, np.float64]
+
+
+def dev(a: np.ndarray):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def host(t) -> np.ndarray:
+    return t.cpu().numpy()
+
+
+def empty(shape, dtype):
+    """Device image pre-filled with a sentinel so unwritten outputs show."""
+    return dev(np.full(tuple(shape), 77, dtype=dtype))
+
+
+def gpu_convolve(src, ker, ddt, border, prec=precision.reference, untiled=False):
+    d = empty(src.shape, ddt)
+    fn = iane_amd.convolve_untiled if untiled else iane_amd.convolve
+    fn(dev(src), dev(ker), d, convolution_options(border_mode(border), prec)).wait()
+    return host(d)
+
+
+def same_bits(a, b):
+    np.testing.assert_array_equal(np.ascontiguousarray(a).view(np.uint8), np.ascontiguousarray(b).view(np.uint8))
+
+
+# ------------------------------------------------------ known answers -----
+@pytest.mark.parametrize("rank", [2, 3, 4])
+def test_pattern_known_answer(rank):
+    src, src2, kernel, first, second = cases.pattern_case(rank)
+    same_bits(gpu_convolve(src, kernel, np.int32, cases.ZERO), first)
+    same_bits(gpu_convolve(src2, kernel, np.int32, cases.ZERO), second)
+
+
+def test_scalar_coordinates_known_answer():
+    src, kernel, expected = cases.scalar_case()
+    same_bits(gpu_convolve(src, kernel, np.int32, cases.ZERO), expected)
+
+
+@pytest.mark.parametrize("border", [cases.CLAMP, cases.ZERO])
+def test_crop_flip_known_answer(border):
+    source, _, kernel, expected = cases.crop_cases()
+    d_source = dev(source)
+    crop = d_source[:, 1:3]  # device subview: pointer offset, parent strides
+    out = empty((2, 2), np.uint8)
+    iane_amd.convolve(crop, dev(kernel), out, convolution_options(border_mode(border))).wait()
+    same_bits(host(out), expected[border])
+
+
+@pytest.mark.parametrize("border", [cases.CLAMP, cases.ZERO])
+def test_texture_case_exact(border):
+    source, crop, kernel = cases.texture_case()
+    expected = oracle.convolve(np.ascontiguousarray(crop), kernel, np.float32, border)
+    d = dev(source)[1:4, 1:4]
+    out = empty((3, 3), np.float32)
+    iane_amd.convolve(d, dev(kernel), out, convolution_options(border_mode(border))).wait()
+    same_bits(host(out), expected)
+
+
+# ---------------------------------------------------- golden fixtures -----
+@pytest.mark.parametrize("path", sorted(GOLDEN.glob("conv_*.npz")), ids=lambda p: p.stem)
+def test_golden_bit_exact(path):
+    data = np.load(path, allow_pickle=False)
+    got = gpu_convolve(data["src"], data["kernel"], data["expected"].dtype, int(data["border"]))
+    same_bits(got, data["expected"])
+
+
+# ---------------------------------------------------- random matrix -------
+def _case(rng, shape, kshape, sdt, kdt):
+    if np.issubdtype(sdt, np.integer):
+        hi = {np.uint8: 256, np.uint16: 65536, np.int32: 3000}[sdt]
+        lo = -3000 if sdt == np.int32 else 0
+        src = rng.integers(lo, hi, shape).astype(sdt)
+    else:
+        src = (rng.random(shape) * 100 - 10).astype(sdt)
+    if np.issubdtype(kdt, np.integer):
+        ker = rng.integers(-4, 5, kshape).astype(kdt)
+    else:
+        ker = (rng.random(kshape) - 0.3).astype(kdt)
+    return src, ker
+
+
+@pytest.mark.parametrize("sdt", DT)
+@pytest.mark.parametrize("ddt", DT)
+def test_dtype_matrix_bit_exact(sdt, ddt):
+    rng = np.random.default_rng(abs(hash((str(sdt), str(ddt)))) % 2**32)
+    for kdt in (np.int32, np.float32, np.float64):
+        for shape, kshape in [((70,), (5,)), ((33, 70), (3, 7)), ((5, 40, 67), (3, 5, 9)),
+                              ((3, 4, 20, 66), (1, 3, 3, 5))]:
+            for border in (cases.CLAMP, cases.ZERO):
+                src, ker = _case(rng, shape, kshape, sdt, kdt)
+                expected = oracle.convolve(src, ker, ddt, border)
+                same_bits(gpu_convolve(src, ker, ddt, border), expected)
+
+
+@pytest.mark.parametrize("kx", [1, 3, 5, 7, 9, 11, 13, 15, 17, 21, 31])
+@pytest.mark.parametrize("border", [cases.CLAMP, cases.ZERO])
+def test_kernel_widths_tiled_and_untiled(kx, border):
+    rng = np.random.default_rng(kx)
+    src, ker = _case(rng, (3, 37, 131), (3, 5, kx), np.float32, np.float32)
+    expected = oracle.convolve(src, ker, np.float32, border)
+    same_bits(gpu_convolve(src, ker, np.float32, border), expected)
+    same_bits(gpu_convolve(src, ker, np.float32, border, untiled=True), expected)
+
+
+def test_large_kernel_untiled_fallback():
+    rng = np.random.default_rng(4)
+    src, ker = _case(rng, (40, 130), (41, 101), np.float32, np.float64)  # tile does not fit in LDS
+    expected = oracle.convolve(src, ker, np.float64, cases.ZERO)
+    same_bits(gpu_convolve(src, ker, np.float64, cases.ZERO), expected)
+
+
+def test_strided_and_reversed_views():
+    from iane_amd import _lib
+    rng = np.random.default_rng(8)
+    base = (rng.random((9, 40, 70)) * 10).astype(np.float32)
+    ker = (rng.random((3, 3, 5)) - 0.5).astype(np.float32)
+    dbase = dev(base)
+    dker = dev(ker)
+    views = [(base[1:8, 3:35, 2:69], dbase[1:8, 3:35, 2:69]), (base[::2, :, 1::3], dbase[::2, :, 1::3])]
+    for border in (cases.CLAMP, cases.ZERO):
+        for h, d in views:
+            expected = oracle.convolve(h, ker, np.float32, border)
+            out_base = empty((h.shape[0], h.shape[1], h.shape[2] * 2), np.float32)
+            out = out_base[:, :, ::2]  # strided destination
+            iane_amd.convolve(d, dker, out, convolution_options(border_mode(border))).wait()
+            same_bits(host(out), expected)
+        # negative stride along IANE axis 1 (numpy axis 1): a flipped view of the same storage
+        desc = iane_amd.view_desc(dbase)
+        desc.ptr = dbase.data_ptr() + (base.shape[1] - 1) * desc.stride_bytes[1]
+        desc.stride_bytes[1] = -desc.stride_bytes[1]
+        out = empty(base.shape, np.float32)
+        _lib.call("iane_hip_convolve", desc, iane_amd.view_desc(dker), iane_amd.view_desc(out), border, 0, 0)
+        iane_amd.hip_execution(0).wait()
+        same_bits(host(out), oracle.convolve(base[:, ::-1, :], ker, np.float32, border))
+
+
+def test_empty_image_is_a_no_op():
+    import torch
+    src = torch.zeros((0, 5), device="cuda")
+    dst = torch.zeros((0, 5), device="cuda")
+    iane_amd.convolve(src, dev(np.ones((3, 3), np.float32)), dst).wait()
+
+
+def test_errors_match_reference():
+    import torch
+    a = torch.zeros((8, 8), device="cuda")
+    with pytest.raises(iane_amd.invalid_argument, match="odd"):
+        iane_amd.convolve(a, torch.ones((3, 4), device="cuda"), a.clone())
+    with pytest.raises(iane_amd.invalid_argument, match="dimensions"):
+        iane_amd.convolve(a, torch.ones((3, 3), device="cuda"), torch.zeros((8, 9), device="cuda"))
+    with pytest.raises(iane_amd.invalid_argument):
+        iane_amd.convolve(a, torch.ones((3,), device="cuda"), a.clone())
+    with pytest.raises(iane_amd.invalid_argument, match="host"):
+        iane_amd.convolve(a.cpu(), torch.ones((3, 3)), a.cpu())
+
+
+# ---------------------------------------------------- FP32 precision ------
+def rel_err(got, ref64):
+    return float(np.max(np.abs(got.astype(np.float64) - ref64)) / max(np.max(np.abs(ref64)), 1e-300))
+
+
+@pytest.mark.parametrize("border", [cases.CLAMP, cases.ZERO])
+def test_fp32_precision_dense15(border):
+    vol = __import__("iane_amd.synth", fromlist=["x"]).volume_blobs(40, seed=2, blobs=8)
+    ref64 = oracle.convolve(vol["volume"], vol["dense"], np.float64, border, threads=8)
+    got = gpu_convolve(vol["volume"], vol["dense"], np.float32, border, prec=precision.fp32)
+    assert rel_err(got, ref64) <= TOL_FP32
+
+
+# ---------------------------------------------------- separable -----------
+@pytest.mark.parametrize("k", [3, 5, 7, 9, 11, 13, 15])
+@pytest.mark.parametrize("border", [cases.CLAMP, cases.ZERO])
+def test_separable_fused_fp32(k, border):
+    rng = np.random.default_rng(k)
+    src = rng.random((37, 45, 70)).astype(np.float32)
+    taps = [rng.random(k).astype(np.float32) for _ in range(3)]
+    t64 = [t.astype(np.float64) for t in taps]
+    dense = np.multiply.outer(np.multiply.outer(t64[2], t64[1]), t64[0])
+    ref64 = oracle.convolve(src, dense, np.float64, border, threads=8)
+    out = empty(src.shape, np.float32)
+    iane_amd.convolve_separable(dev(src), [dev(t) for t in taps], out,
+                                convolution_options(border_mode(border), precision.fp32)).wait()
+    assert rel_err(host(out), ref64) <= TOL_FP32
+
+
+@pytest.mark.parametrize("border", [cases.CLAMP, cases.ZERO])
+def test_separable_passes_reference_bit_exact(border):
+    """REFERENCE-precision passes equal the oracle's per-axis FP64 convolutions."""
+    rng = np.random.default_rng(12)
+    src = (rng.random((6, 9, 21)) * 50).astype(np.float32)
+    taps = [rng.random(k).astype(np.float64) for k in (5, 3, 7)]
+    step = src
+    for axis, t in enumerate(taps):
+        kshape = [1, 1, 1]
+        kshape[2 - axis] = t.size
+        last = axis == 2
+        step = oracle.convolve(np.ascontiguousarray(step), t.reshape(kshape), np.float32 if last else np.float64,
+                               border)
+    out = empty(src.shape, np.float32)
+    iane_amd.convolve_separable(dev(src), [dev(t) for t in taps], out,
+                                convolution_options(border_mode(border), precision.reference)).wait()
+    same_bits(host(out), step)
+
+
+# ------------------------------------- full size (256^3) properties -------
+def test_full_size_delta_shift_and_integer_linearity():
+    """At the benchmark size: a delta kernel is an exact shift, and the u16 3^3
+    binomial convolution is exactly linear."""
+    n = 256
+    rng = np.random.default_rng(1)
+    vol = rng.random((n, n, n), dtype=np.float32)
+    ker = np.zeros((15, 15, 15), np.float32)
+    ker[7 + 2, 7 - 3, 7 + 1] = 1.0  # q - c = (+1, -3, +2) in IANE (x, y, z): sample at p - (q - c)
+    out = empty(vol.shape, np.float32)
+    iane_amd.convolve(dev(vol), dev(ker), out, convolution_options(border_mode.zero)).wait()
+    expected = np.zeros_like(vol)
+    expected[2:, :n - 3, 1:] = vol[:-2, 3:, :-1]
+    same_bits(host(out), expected)
+    a = rng.integers(0, 30000, (n, n, n)).astype(np.uint16)
+    b = rng.integers(0, 30000, (n, n, n)).astype(np.uint16)
+    b3 = np.array([1, 2, 1], np.int32)
+    binom = np.multiply.outer(np.multiply.outer(b3, b3), b3).astype(np.int32)
+    outs = []
+    for x in (a, b, (a.astype(np.int64) + b).astype(np.int32)):
+        o = empty(x.shape, np.int32)
+        iane_amd.convolve(dev(x), dev(binom), o, convolution_options(border_mode.zero)).wait()
+        outs.append(host(o).astype(np.int64))
+    np.testing.assert_array_equal(outs[0] + outs[1], outs[2])
+
+
+def test_full_size_separable_vs_sampled_oracle():
+    """Separable 15^3 Gaussian at 256^3 against FP64 brute force at sampled voxels (edges included)."""
+    from iane_amd import synth
+    vol = synth.volume_blobs(256, seed=2)
+    v = vol["volume"]
+    taps = vol["taps"]
+    out = empty(v.shape, np.float32)
+    iane_amd.convolve_separable(dev(v), [dev(taps)] * 3, out,
+                                convolution_options(border_mode.zero, precision.fp32)).wait()
+    got = host(out)
+    t = taps.astype(np.float64)
+    dense = np.multiply.outer(np.multiply.outer(t, t), t)
+    rng = np.random.default_rng(0)
+    pts = rng.integers(0, 256, (300, 3))
+    pts[:20] = [[0, 0, 0]] * 20
+    pts[20:40, 0] = 255
+    vp = np.pad(v.astype(np.float64), 7)
+    worst = 0.0
+    for z, y, x in pts:
+        window = vp[z:z + 15, y:y + 15, x:x + 15]
+        ref = float(np.sum(window * dense[::-1, ::-1, ::-1]))
+        worst = max(worst, abs(got[z, y, x] - ref) / max(abs(ref), 1e-3))
+    assert worst <= TOL_FP32
