@@ -161,3 +161,10 @@ def convolve_untiled(src, kernel, dst, options: convolution_options | None = Non
     _lib.call("iane_hip_convolve_untiled", view_desc(src), view_desc(kernel), view_desc(dst),
               int(options.border), int(options.precision), stream)
     return hip_execution(stream)
+
+
+from .prepared import (execution_context, fft_forward, fft_inverse, irfft, make_execution_context,  # noqa: E402
+                       rfft, rfft_extent)
+
+__all__ += ["execution_context", "make_execution_context", "fft_forward", "fft_inverse", "rfft", "irfft",
+            "rfft_extent"]

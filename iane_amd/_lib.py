@@ -65,6 +65,8 @@ desc_p = ctypes.POINTER(iane_view_desc)
 # name -> (argtypes); every function returns int status
 SIGNATURES = {
     "iane_hip_device_count": [ctypes.POINTER(ctypes.c_int)],
+    "iane_hip_tune_set": [ctypes.c_char_p, ctypes.c_int],
+    "iane_hip_tune_get": [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)],
     "iane_hip_set_device": [ctypes.c_int],
     "iane_hip_malloc": [ctypes.POINTER(vp), sz],
     "iane_hip_free": [vp],

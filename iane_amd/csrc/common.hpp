@@ -39,6 +39,10 @@ inline void check_hip(hipError_t aStatus, const char* aOperation)
 
 void set_last_error(const std::string& aMessage);
 
+// Launch-shape knobs (tile sizes, thread counts, chunking) set by the
+// autotuner through iane_hip_tune_set; `aDefault` when unset.
+int tune_get(const char* aKey, int aDefault);
+
 // Run a C-ABI body, mapping exceptions to status codes.
 template<class F>
 int guarded(F&& aBody)
@@ -69,7 +73,7 @@ inline int dtype_size(int aType)
 	case IANE_F32: return 4;
 	case IANE_F64: return 8;
 	case IANE_C64: return 8;
-	default: invalid("unknown element type " + std::to_string(aType));
+	default: invalid("unknown element type " + std::to_string(aType)); return 0;
 	}
 }
 
@@ -95,6 +99,22 @@ __device__ __forceinline__ float load_as_float(const char* aPtr, int aType)
 	case IANE_I32: return static_cast<float>(*reinterpret_cast<const int32_t*>(aPtr));
 	case IANE_F32: return *reinterpret_cast<const float*>(aPtr);
 	default: return static_cast<float>(*reinterpret_cast<const double*>(aPtr));
+	}
+}
+
+// Call f(T{}) with the element type T of runtime code `aType`. Used to hoist
+// the type dispatch out of load loops: a switch around each load puts every
+// load in its own basic block and the compiler then waits for each one at the
+// join, serialising the memory traffic.
+template<class F>
+__device__ __forceinline__ void with_element_type(int aType, F&& aFunction)
+{
+	switch (aType) {
+	case IANE_U8: aFunction(uint8_t{}); break;
+	case IANE_U16: aFunction(uint16_t{}); break;
+	case IANE_I32: aFunction(int32_t{}); break;
+	case IANE_F32: aFunction(float{}); break;
+	default: aFunction(double{}); break;
 	}
 }
 
@@ -237,5 +257,14 @@ __device__ __forceinline__ unsigned xcd_remap(unsigned aBlock, unsigned aCount)
 }
 
 inline unsigned ceil_div(long long a, long long b) { return static_cast<unsigned>((a + b - 1) / b); }
+
+// Workgroup barrier for LDS traffic only: waits for this wave's LDS operations
+// (lgkmcnt) and then for the other waves, but not for outstanding global loads
+// (vmcnt), so register prefetches of the next tile stay in flight across it.
+// __syncthreads() carries a workgroup fence that drains every counter.
+__device__ __forceinline__ void lds_barrier()
+{
+	asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
 
 } // namespace iane_hip

@@ -23,10 +23,7 @@ void validate_convolution(const view4& aSrc, const view4& aKer, const view4& aDs
 
 namespace {
 
-constexpr int kSepThreads = 512;
 constexpr int kSepTileX = 64;
-constexpr int kSepTileY = 32;
-constexpr int kSepColsPerThread = kSepTileX * kSepTileY / kSepThreads; // 4 (consecutive y)
 
 struct sep_args {
 	const char* src;
@@ -56,13 +53,22 @@ __device__ __forceinline__ bool clamp_or_zero(int& aCoord, int aExtent, int aBor
 	return true;
 }
 
-template<int K>
+// Uniform value -> scalar register (the taps are the same for every lane).
+__device__ __forceinline__ float uniform_float(float aValue)
+{
+	return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(aValue)));
+}
+
+template<int K, int kSepThreads, int kSepTileY>
 __global__ __launch_bounds__(kSepThreads) void sep3_fused_kernel(sep_args a)
 {
+	constexpr int kSepColsPerThread = kSepTileX * kSepTileY / kSepThreads; // consecutive x columns per thread
+	constexpr int kRowThreads = kSepTileX / kSepColsPerThread;
 	constexpr int C = K / 2;
 	constexpr int cols = kSepTileX + K - 1;   // staged x extent
 	constexpr int rows = kSepTileY + K - 1;   // staged y extent
-	constexpr int pitch = cols + 1;           // odd pitch: column reads stay conflict-free
+	constexpr int pitch = cols + 1;
+	constexpr int per_thread = (rows * cols + kSepThreads - 1) / kSepThreads;
 	__shared__ float in_s[rows * pitch];
 	__shared__ float yr_s[kSepTileY * pitch]; // y-filtered, still x-halo wide
 
@@ -80,15 +86,29 @@ __global__ __launch_bounds__(kSepThreads) void sep3_fused_kernel(sep_args a)
 	float kx[K], ky[K], kz[K];
 #pragma unroll
 	for (int i = 0; i < K; ++i) {
-		kx[i] = load_as_float(a.taps[0] + i * a.tap_stride[0], a.tap_type[0]);
-		ky[i] = load_as_float(a.taps[1] + i * a.tap_stride[1], a.tap_type[1]);
-		kz[i] = load_as_float(a.taps[2] + i * a.tap_stride[2], a.tap_type[2]);
+		kx[i] = uniform_float(load_as_float(a.taps[0] + i * a.tap_stride[0], a.tap_type[0]));
+		ky[i] = uniform_float(load_as_float(a.taps[1] + i * a.tap_stride[1], a.tap_type[1]));
+		kz[i] = uniform_float(load_as_float(a.taps[2] + i * a.tap_stride[2], a.tap_type[2]));
 	}
 
-	// Thread ownership for the x filter and the z ring: column x = tid % 64,
-	// rows y = (tid / 64) * 4 + r.
-	const int ox = tid % kSepTileX;
-	const int oy = (tid / kSepTileX) * kSepColsPerThread;
+	// Staging offsets of this thread's elements (same for every plane).
+	int offset[per_thread]; // byte offset within a plane (host checks it fits in int)
+	bool valid[per_thread];
+#pragma unroll
+	for (int e = 0; e < per_thread; ++e) {
+		const int i = tid + e * kSepThreads;
+		const int ly = i / cols;
+		const int lx = i - ly * cols;
+		int xi = x0 - C + lx;
+		int yi = y0 - C + ly;
+		valid[e] = i < rows * cols && clamp_or_zero(xi, a.nx, a.border) && clamp_or_zero(yi, a.ny, a.border);
+		offset[e] = valid[e] ? static_cast<int>(xi * a.sx + yi * a.sy) : 0;
+	}
+
+	// Thread ownership for the x filter and the z ring: row y = tid / kRowThreads,
+	// columns x = (tid % kRowThreads) * kSepColsPerThread + r (a sliding window along x).
+	const int ox = (tid % kRowThreads) * kSepColsPerThread;
+	const int oy = tid / kRowThreads;
 
 	float ring[K][kSepColsPerThread];
 #pragma unroll
@@ -100,6 +120,42 @@ __global__ __launch_bounds__(kSepThreads) void sep3_fused_kernel(sep_args a)
 	}
 
 	const int planes = (zend - z0) + K - 1; // input planes z0 - C .. zend - 1 + C
+	// Register prefetch of the next staged plane: its global loads are in
+	// flight while the current plane is filtered.
+	float pending[kSepColsPerThread];
+	int pending_z = -1;
+	auto flush = [&]() {
+		const int py = y0 + oy;
+		if (pending_z >= 0 && py < a.ny) {
+#pragma unroll
+			for (int r = 0; r < kSepColsPerThread; ++r) {
+				const int px = x0 + ox + r;
+				if (px < a.nx) {
+					*reinterpret_cast<float*>(a.dst + px * a.dx + py * a.dy + pending_z * a.dz) = pending[r];
+				}
+			}
+		}
+		pending_z = -1;
+	};
+	float pre[per_thread];
+	bool pre_in = false;
+	auto fetch = [&](int aPlane) {
+		int zi = z0 - C + aPlane;
+		const bool in = clamp_or_zero(zi, a.nz, a.border);
+		// loads are unconditional (invalid lanes read plane element 0) and
+		// selected afterwards, so no load sits behind a branch
+		// (the fused path takes f32 sources only: a per-element type switch
+		// would put each load behind a branch and serialise them)
+		// The raw values stay in `pre` and are masked only when written to LDS,
+		// so nothing consumes them until the next plane is staged.
+		const char* base = a.src + (in ? zi : 0) * a.sz;
+#pragma unroll
+		for (int e = 0; e < per_thread; ++e) {
+			pre[e] = *reinterpret_cast<const float*>(base + offset[e]);
+		}
+		pre_in = in;
+	};
+	fetch(0);
 	for (int base = 0; base < planes; base += K) {
 #pragma unroll
 		for (int j = 0; j < K; ++j) {
@@ -111,21 +167,23 @@ __global__ __launch_bounds__(kSepThreads) void sep3_fused_kernel(sep_args a)
 			const bool plane_in = clamp_or_zero(zi, a.nz, a.border);
 			float xy[kSepColsPerThread];
 			if (plane_in) {
-				__syncthreads(); // previous plane's readers of in_s / yr_s are done
-				const char* plane = a.src + zi * a.sz;
-				for (int i = tid; i < rows * cols; i += kSepThreads) {
-					const int ly = i / cols;
-					const int lx = i - ly * cols;
-					int xi = x0 - C + lx;
-					int yi = y0 - C + ly;
-					float v = 0.0f;
-					if (clamp_or_zero(xi, a.nx, a.border) && clamp_or_zero(yi, a.ny, a.border)) {
-						v = load_as_float(plane + xi * a.sx + yi * a.sy, a.src_type);
+#pragma unroll
+				for (int e = 0; e < per_thread; ++e) {
+					const int i = tid + e * kSepThreads;
+					if (i < rows * cols) {
+						const int ly = i / cols;
+						in_s[ly * pitch + (i - ly * cols)] = (pre_in && valid[e]) ? pre[e] : 0.0f;
 					}
-					in_s[ly * pitch + lx] = v;
 				}
-				__syncthreads();
-				// y filter over the halo-wide region: kSepTileY x cols outputs.
+			}
+			if (p + 1 < planes) {
+				fetch(p + 1);
+			}
+			// the previous plane's outputs go out now and complete during this plane's filters
+			flush();
+			if (plane_in) {
+				lds_barrier(); // staged plane visible; previous x filter done with yr_s
+				// y filter over the halo-wide region: kSepTileY x cols outputs, 4 rows per item.
 				for (int i = tid; i < (kSepTileY / 4) * cols; i += kSepThreads) {
 					const int lx = i % cols;
 					const int gy = (i / cols) * 4;
@@ -144,17 +202,23 @@ __global__ __launch_bounds__(kSepThreads) void sep3_fused_kernel(sep_args a)
 						yr_s[(gy + r) * pitch + lx] = s;
 					}
 				}
-				__syncthreads();
-				// x filter: each thread its column ox for rows oy .. oy+3.
+				lds_barrier(); // yr_s complete; in_s free for the next plane
+				{
+					const float* row = yr_s + oy * pitch + ox;
+					float w[kSepColsPerThread + K - 1];
 #pragma unroll
-				for (int r = 0; r < kSepColsPerThread; ++r) {
-					const float* row = yr_s + (oy + r) * pitch + ox;
-					float s = 0.0f;
-#pragma unroll
-					for (int q = 0; q < K; ++q) {
-						s = fmaf(row[K - 1 - q], kx[q], s);
+					for (int t = 0; t < kSepColsPerThread + K - 1; ++t) {
+						w[t] = row[t];
 					}
-					xy[r] = s;
+#pragma unroll
+					for (int r = 0; r < kSepColsPerThread; ++r) {
+						float s = 0.0f;
+#pragma unroll
+						for (int q = 0; q < K; ++q) {
+							s = fmaf(w[r + K - 1 - q], kx[q], s);
+						}
+						xy[r] = s;
+					}
 				}
 			} else {
 #pragma unroll
@@ -182,19 +246,15 @@ __global__ __launch_bounds__(kSepThreads) void sep3_fused_kernel(sep_args a)
 						out[r] = fmaf(ring[slot][r], kz[q], out[r]);
 					}
 				}
-				const int px = x0 + ox;
-				if (px < a.nx) {
 #pragma unroll
-					for (int r = 0; r < kSepColsPerThread; ++r) {
-						const int py = y0 + oy + r;
-						if (py < a.ny) {
-							store_converted(a.dst + px * a.dx + py * a.dy + zo * a.dz, a.dst_type, out[r]);
-						}
-					}
+				for (int r = 0; r < kSepColsPerThread; ++r) {
+					pending[r] = out[r];
 				}
+				pending_z = zo;
 			}
 		}
 	}
+	flush();
 }
 
 // One 1-D pass along `axis` of a rank-4-padded view; Acc-typed temporaries in
@@ -249,20 +309,38 @@ view4 packed_temp(const view4& aLike, int aType, void* aPtr)
 }
 
 template<int K>
-void launch_fused(sep_args& a, hipStream_t aStream)
+void launch_fused(sep_args& a, hipStream_t aStream, int aThreads)
 {
+	const int tile_y = tune_get("sep3.tile_y", 16) == 32 ? 32 : 16;
 	a.tiles_x = ceil_div(a.nx, kSepTileX);
-	a.tiles_y = ceil_div(a.ny, kSepTileY);
+	a.tiles_y = ceil_div(a.ny, tile_y);
 	// chunk so that the grid covers the chip at least twice: 256 CUs
 	const long long xy = static_cast<long long>(a.tiles_x) * a.tiles_y;
-	int chunk = a.nz;
-	while (chunk > 8 && xy * ceil_div(a.nz, chunk) < 256) {
-		chunk = (chunk + 1) / 2;
+	int chunk = tune_get("sep3.chunk", 0);
+	if (chunk <= 0) {
+		const long long want = tune_get("sep3.min_blocks", 512);
+		chunk = a.nz;
+		while (chunk > 32 && xy * ceil_div(a.nz, chunk) < want) {
+			chunk = (chunk + 1) / 2;
+		}
 	}
 	a.chunk = chunk;
 	a.chunks_z = ceil_div(a.nz, chunk);
 	const long long blocks = xy * a.chunks_z;
-	sep3_fused_kernel<K><<<static_cast<unsigned>(blocks), kSepThreads, 0, aStream>>>(a);
+	const unsigned nb = static_cast<unsigned>(blocks);
+	if (aThreads == 512) {
+		if (tile_y == 32) {
+			sep3_fused_kernel<K, 512, 32><<<nb, 512, 0, aStream>>>(a);
+		} else {
+			sep3_fused_kernel<K, 512, 16><<<nb, 512, 0, aStream>>>(a);
+		}
+	} else {
+		if (tile_y == 32) {
+			sep3_fused_kernel<K, 256, 32><<<nb, 256, 0, aStream>>>(a);
+		} else {
+			sep3_fused_kernel<K, 256, 16><<<nb, 256, 0, aStream>>>(a);
+		}
+	}
 }
 
 } // namespace
@@ -271,12 +349,16 @@ void convolve_separable(const view4& aSrc, const view4* aTaps, int aN, const vie
 	int aPrecision, hipStream_t aStream, bool aAllowFused)
 {
 	const int rank = aSrc.rank;
-	bool fused = aAllowFused && rank == 3 && aPrecision == IANE_PREC_FP32;
+	bool fused = aAllowFused && rank == 3 && aPrecision == IANE_PREC_FP32 && aSrc.type == IANE_F32 &&
+		aDst.type == IANE_F32;
 	const int k = aTaps[0].extent[0];
 	for (int i = 0; i < aN; ++i) {
 		fused = fused && aTaps[i].extent[0] == k;
 	}
 	fused = fused && k <= 15;
+	// staged offsets within one plane are kept in 32-bit registers
+	fused = fused && (static_cast<long long>(aSrc.extent[0]) + 64) * std::llabs(aSrc.stride[0]) +
+		(static_cast<long long>(aSrc.extent[1]) + 64) * std::llabs(aSrc.stride[1]) < (1LL << 31);
 	if (fused) {
 		sep_args a{};
 		for (int axis = 0; axis < 3; ++axis) {
@@ -298,15 +380,16 @@ void convolve_separable(const view4& aSrc, const view4* aTaps, int aN, const vie
 		a.dy = aDst.stride[1];
 		a.dz = aDst.stride[2];
 		a.border = aBorder;
+		const int threads = tune_get("sep3.threads", 512);
 		switch (k) {
-		case 1: launch_fused<1>(a, aStream); break;
-		case 3: launch_fused<3>(a, aStream); break;
-		case 5: launch_fused<5>(a, aStream); break;
-		case 7: launch_fused<7>(a, aStream); break;
-		case 9: launch_fused<9>(a, aStream); break;
-		case 11: launch_fused<11>(a, aStream); break;
-		case 13: launch_fused<13>(a, aStream); break;
-		default: launch_fused<15>(a, aStream); break;
+		case 1: launch_fused<1>(a, aStream, threads); break;
+		case 3: launch_fused<3>(a, aStream, threads); break;
+		case 5: launch_fused<5>(a, aStream, threads); break;
+		case 7: launch_fused<7>(a, aStream, threads); break;
+		case 9: launch_fused<9>(a, aStream, threads); break;
+		case 11: launch_fused<11>(a, aStream, threads); break;
+		case 13: launch_fused<13>(a, aStream, threads); break;
+		default: launch_fused<15>(a, aStream, threads); break;
 		}
 		check_hip(hipGetLastError(), "Separable convolution launch");
 		return;

@@ -1,0 +1,435 @@
+// FFT plans behind iane_hip_fft_*: the drop-in for the reference's prepared
+// fft_forward / fft_inverse / rfft / irfft (include/iane/algorithms/fft/fft.hpp:196-231,
+// rfft.hpp:107-161), generalised from 2-D to ranks 1..3.
+//
+// Conventions (fft.hpp:228-231, docs/prepared_algorithms.md:26-29): forward uses
+// the negative exponent, the inverse is unnormalised, R2C writes the packed
+// half spectrum with extent {n0/2+1, n1, ...} along axis 0 (layout.hpp:15-21),
+// C2R leaves its input untouched (rfft.hpp:72-83: the reference copies it into
+// scratch; here the first inverse pass reads it and writes the workspace).
+//
+// A plan is a list of passes, one per axis; each pass is one launch of
+// fft_pass_kernel (lines of the axis held in LDS, all radix stages in one kernel).
+#include "fft.hpp"
+
+#include <map>
+#include <memory>
+#include <mutex>
+#include <utility>
+#include <vector>
+
+namespace iane_hip {
+
+bool factor_radices(int aM, fft_radix_plan& aPlan)
+{
+	aPlan.m = aM;
+	aPlan.stages = 0;
+	int rest = aM;
+	auto push = [&](int r) {
+		if (aPlan.stages >= kFftMaxStages) {
+			return false;
+		}
+		aPlan.radix[aPlan.stages++] = r;
+		rest /= r;
+		return true;
+	};
+	while (rest % 8 == 0) {
+		if (!push(8)) return false;
+	}
+	while (rest % 4 == 0) {
+		if (!push(4)) return false;
+	}
+	while (rest % 2 == 0) {
+		if (!push(2)) return false;
+	}
+	for (int r : {3, 5, 7, 11, 13}) {
+		while (rest % r == 0) {
+			if (!push(r)) return false;
+		}
+	}
+	for (int r = 17; rest > 1 && r <= 64; r += 2) {
+		while (rest % r == 0) {
+			if (!push(r)) return false;
+		}
+	}
+	return rest == 1;
+}
+
+namespace {
+std::mutex g_twiddle_mutex;
+std::map<std::pair<int, int>, float2*> g_twiddles;
+} // namespace
+
+const float2* twiddle_table(int aM)
+{
+	int device = 0;
+	check_hip(hipGetDevice(&device), "Query HIP device");
+	std::lock_guard lock(g_twiddle_mutex);
+	auto& slot = g_twiddles[{device, aM}];
+	if (!slot) {
+		std::vector<float2> host(aM);
+		for (int k = 0; k < aM; ++k) {
+			const double angle = -2.0 * 3.14159265358979323846 * static_cast<double>(k) / aM;
+			host[k] = make_float2(static_cast<float>(std::cos(angle)), static_cast<float>(std::sin(angle)));
+		}
+		float2* table = nullptr;
+		check_hip(hipMalloc(&table, sizeof(float2) * aM), "Allocate FFT twiddles");
+		check_hip(hipMemcpy(table, host.data(), sizeof(float2) * aM, hipMemcpyHostToDevice), "Upload FFT twiddles");
+		slot = table;
+	}
+	return slot;
+}
+
+int next_fft_length(int aN)
+{
+	for (int n = aN + (aN & 1); ; n += 2) {
+		int r = n;
+		for (int p : {2, 3, 5, 7}) {
+			while (r % p == 0) {
+				r /= p;
+			}
+		}
+		if (r == 1) {
+			return n;
+		}
+	}
+}
+
+namespace {
+
+__device__ inline void store_real_lines(float* out, const line_geom& g, int aOuter, int aLine0, int count, int n,
+	int store_n, const float2* buf, int pitch)
+{
+	const long long base = aOuter * g.outer + aLine0 * g.line;
+	const bool even = (n & 1) == 0;
+	const int total = count * store_n;
+	for (int t = threadIdx.x; t < total; t += blockDim.x) {
+		int w, i;
+		if (g.elem == 1) {
+			w = t / store_n;
+			i = t - w * store_n;
+		} else {
+			i = t / count;
+			w = t - i * count;
+		}
+		const float2 v = buf[w * pitch + (even ? i / 2 : i)];
+		out[base + w * g.line + i * g.elem] = (even && (i & 1)) ? v.y : v.x;
+	}
+}
+
+template<int MODE>
+__global__ __launch_bounds__(kFftThreads) void fft_pass_kernel(fft_pass_args a)
+{
+	__shared__ float2 lds[2 * kFftLdsElems];
+	float2* A = lds;
+	float2* B = lds + kFftLdsElems;
+	int o, l0, count;
+	decode_group(a, xcd_remap(blockIdx.x, gridDim.x), o, l0, count);
+	const int m = a.plan.m;
+	const int n = a.n;
+	if constexpr (MODE == PASS_C2C) {
+		load_complex_lines(static_cast<const float2*>(a.in), a.gin, o, l0, count, a.load_n, m, A, a.pitch);
+		__syncthreads();
+		const float2* r = run_stages(a.plan, A, B, count, a.pitch, a.tw_m, a.sign);
+		store_complex_lines(static_cast<float2*>(a.out), a.gout, o, l0, count, a.store_n, r, a.pitch);
+	} else if constexpr (MODE == PASS_C2C_MULT) {
+		load_complex_lines(static_cast<const float2*>(a.in), a.gin, o, l0, count, a.load_n, m, A, a.pitch);
+		__syncthreads();
+		float2* r = run_stages(a.plan, A, B, count, a.pitch, a.tw_m, -1.0f);
+		const long long hbase = o * a.gmult.outer + l0 * a.gmult.line;
+		for (int t = threadIdx.x; t < count * m; t += blockDim.x) {
+			int w, k;
+			if (a.gmult.elem == 1) {
+				w = t / m;
+				k = t - w * m;
+			} else {
+				k = t / count;
+				w = t - k * count;
+			}
+			float2 h = a.mult[hbase + w * a.gmult.line + k * a.gmult.elem];
+			if (a.mult_conj) {
+				h = cconj(h);
+			}
+			r[w * a.pitch + k] = cscale(cmul(r[w * a.pitch + k], h), a.mult_scale);
+		}
+		__syncthreads();
+		const float2* q = run_stages(a.plan, r, r == A ? B : A, count, a.pitch, a.tw_m, 1.0f);
+		store_complex_lines(static_cast<float2*>(a.out), a.gout, o, l0, count, a.store_n, q, a.pitch);
+	} else if constexpr (MODE == PASS_R2C) {
+		load_real_lines(static_cast<const float*>(a.in), a.gin, o, l0, count, n, a.load_n, A, a.pitch);
+		__syncthreads();
+		float2* r = run_stages(a.plan, A, B, count, a.pitch, a.tw_m, -1.0f);
+		if ((n & 1) == 0) {
+			float2* x = r == A ? B : A;
+			r2c_post(r, x, count, n, a.pitch, a.tw_n);
+			__syncthreads();
+			r = x;
+		}
+		store_complex_lines(static_cast<float2*>(a.out), a.gout, o, l0, count, a.store_n, r, a.pitch);
+	} else { // PASS_C2R
+		if ((n & 1) == 0) {
+			load_complex_lines(static_cast<const float2*>(a.in), a.gin, o, l0, count, a.load_n, m + 1, A, a.pitch);
+			__syncthreads();
+			c2r_pre(A, B, count, n, a.pitch, a.tw_n);
+			__syncthreads();
+			const float2* r = run_stages(a.plan, B, A, count, a.pitch, a.tw_m, 1.0f);
+			store_real_lines(static_cast<float*>(a.out), a.gout, o, l0, count, n, a.store_n, r, a.pitch);
+		} else {
+			load_complex_lines(static_cast<const float2*>(a.in), a.gin, o, l0, count, a.load_n, m, A, a.pitch);
+			__syncthreads();
+			c2r_extend_odd(A, count, n, a.pitch);
+			__syncthreads();
+			const float2* r = run_stages(a.plan, A, B, count, a.pitch, a.tw_m, 1.0f);
+			store_real_lines(static_cast<float*>(a.out), a.gout, o, l0, count, n, a.store_n, r, a.pitch);
+		}
+	}
+}
+
+} // namespace
+
+// Fill the radix plan, tables, LDS pitch and bundle of a pass for a transform
+// of logical length n along lines of `lines` x `outer`.
+void setup_pass(fft_pass_args& a, int aMode, int aN, int aLines, int aOuter)
+{
+	a.mode = aMode;
+	a.n = aN;
+	const bool half = (aMode == PASS_R2C || aMode == PASS_C2R) && (aN % 2 == 0);
+	const int m = half ? aN / 2 : aN;
+	if (!factor_radices(m, a.plan)) {
+		fail(IANE_ERR_UNSUPPORTED, "FFT length " + std::to_string(aN) + " has a prime factor above 64");
+	}
+	a.pitch = m + 1;
+	if (a.pitch > kFftLdsElems) {
+		fail(IANE_ERR_UNSUPPORTED, "FFT line length " + std::to_string(aN) + " exceeds the LDS line limit");
+	}
+	a.tw_m = twiddle_table(m);
+	a.tw_n = half ? twiddle_table(aN) : nullptr;
+	a.lines = aLines;
+	a.outer = aOuter;
+	a.bundle = std::max(1, std::min(aLines, kFftLdsElems / a.pitch));
+	a.groups_per_outer = ceil_div(aLines, a.bundle);
+}
+
+void launch_pass(const fft_pass_args& a, hipStream_t aStream)
+{
+	const long long blocks = static_cast<long long>(a.groups_per_outer) * a.outer;
+	if (blocks == 0) {
+		return;
+	}
+	if (blocks > 0x7fffffffLL) {
+		fail(IANE_ERR_UNSUPPORTED, "FFT grid too large");
+	}
+	const unsigned nb = static_cast<unsigned>(blocks);
+	switch (a.mode) {
+	case PASS_C2C: fft_pass_kernel<PASS_C2C><<<nb, kFftThreads, 0, aStream>>>(a); break;
+	case PASS_R2C: fft_pass_kernel<PASS_R2C><<<nb, kFftThreads, 0, aStream>>>(a); break;
+	case PASS_C2R: fft_pass_kernel<PASS_C2R><<<nb, kFftThreads, 0, aStream>>>(a); break;
+	default: fft_pass_kernel<PASS_C2C_MULT><<<nb, kFftThreads, 0, aStream>>>(a); break;
+	}
+	check_hip(hipGetLastError(), "FFT pass launch");
+}
+
+// Geometry of the lines along `axis` of a packed array with extents e[0..rank-1].
+void axis_lines(const int* e, int rank, int axis, line_geom& g, int& lines, int& outer)
+{
+	long long inner = 1;
+	for (int i = 0; i < axis; ++i) {
+		inner *= e[i];
+	}
+	long long above = 1;
+	for (int i = axis + 1; i < rank; ++i) {
+		above *= e[i];
+	}
+	if (axis == 0) {
+		g = {1, e[0], 0};
+		lines = static_cast<int>(above);
+		outer = 1;
+	} else {
+		g = {inner, 1, inner * e[axis]};
+		lines = static_cast<int>(inner);
+		outer = static_cast<int>(above);
+	}
+}
+
+} // namespace iane_hip
+
+using namespace iane_hip;
+
+enum { BUF_IN = 0, BUF_OUT = 1, BUF_WS = 2 };
+
+struct iane_fft_plan {
+	int rank = 0;
+	int kind = 0;
+	int extent[3] = {1, 1, 1};      // logical (real for R2C/C2R) extents
+	size_t in_bytes = 0;
+	size_t out_bytes = 0;
+	size_t workspace = 0;
+	std::vector<fft_pass_args> passes;
+	std::vector<std::pair<int, int>> buffers; // (in, out) per pass
+};
+
+extern "C" {
+
+int iane_hip_fft_plan_create(iane_fft_plan** aPlan, int aRank, const int32_t* aExtent, int aKind)
+{
+	return guarded([&] {
+		if (!aPlan || !aExtent) {
+			invalid("FFT plan output or extent is null");
+		}
+		*aPlan = nullptr;
+		if (aRank < 1 || aRank > 3) {
+			fail(IANE_ERR_UNSUPPORTED, "FFT rank must be 1..3");
+		}
+		if (aKind < IANE_FFT_C2C_FORWARD || aKind > IANE_FFT_C2R) {
+			invalid("unknown FFT kind");
+		}
+		auto plan = std::make_unique<iane_fft_plan>();
+		plan->rank = aRank;
+		plan->kind = aKind;
+		int spec[3] = {1, 1, 1};
+		long long count = 1;
+		for (int i = 0; i < aRank; ++i) {
+			if (aExtent[i] <= 0) {
+				invalid(aKind >= IANE_FFT_R2C ? "Real FFT requires positive dimensions" : "FFT requires positive dimensions");
+			}
+			plan->extent[i] = aExtent[i];
+			spec[i] = aExtent[i];
+			count *= aExtent[i];
+		}
+		const bool real = aKind == IANE_FFT_R2C || aKind == IANE_FFT_C2R;
+		if (real) {
+			spec[0] = aExtent[0] / 2 + 1;
+		}
+		long long spec_count = 1;
+		for (int i = 0; i < aRank; ++i) {
+			spec_count *= spec[i];
+		}
+		const size_t real_bytes = static_cast<size_t>(count) * sizeof(float);
+		const size_t spec_bytes = static_cast<size_t>(spec_count) * sizeof(float2);
+		if (aKind == IANE_FFT_C2C_FORWARD || aKind == IANE_FFT_C2C_INVERSE) {
+			const float sign = aKind == IANE_FFT_C2C_FORWARD ? -1.0f : 1.0f;
+			for (int axis = 0; axis < aRank; ++axis) {
+				fft_pass_args a{};
+				line_geom g;
+				int lines, outer;
+				axis_lines(spec, aRank, axis, g, lines, outer);
+				setup_pass(a, PASS_C2C, spec[axis], lines, outer);
+				a.sign = sign;
+				a.load_n = a.store_n = spec[axis];
+				a.gin = a.gout = g;
+				plan->passes.push_back(a);
+				plan->buffers.push_back({axis == 0 ? BUF_IN : BUF_OUT, BUF_OUT});
+			}
+			plan->in_bytes = plan->out_bytes = spec_bytes;
+		} else if (aKind == IANE_FFT_R2C) {
+			fft_pass_args a{};
+			line_geom g;
+			int lines, outer;
+			axis_lines(plan->extent, aRank, 0, g, lines, outer);
+			setup_pass(a, PASS_R2C, plan->extent[0], lines, outer);
+			a.load_n = plan->extent[0];
+			a.store_n = spec[0];
+			a.gin = g;
+			a.gout = {1, spec[0], 0};
+			plan->passes.push_back(a);
+			plan->buffers.push_back({BUF_IN, BUF_OUT});
+			for (int axis = 1; axis < aRank; ++axis) {
+				fft_pass_args c{};
+				axis_lines(spec, aRank, axis, g, lines, outer);
+				setup_pass(c, PASS_C2C, spec[axis], lines, outer);
+				c.sign = -1.0f;
+				c.load_n = c.store_n = spec[axis];
+				c.gin = c.gout = g;
+				plan->passes.push_back(c);
+				plan->buffers.push_back({BUF_OUT, BUF_OUT});
+			}
+			plan->in_bytes = real_bytes;
+			plan->out_bytes = spec_bytes;
+		} else { // C2R
+			int src = BUF_IN;
+			for (int axis = aRank - 1; axis >= 1; --axis) {
+				fft_pass_args c{};
+				line_geom g;
+				int lines, outer;
+				axis_lines(spec, aRank, axis, g, lines, outer);
+				setup_pass(c, PASS_C2C, spec[axis], lines, outer);
+				c.sign = 1.0f;
+				c.load_n = c.store_n = spec[axis];
+				c.gin = c.gout = g;
+				plan->passes.push_back(c);
+				plan->buffers.push_back({src, BUF_WS});
+				src = BUF_WS;
+			}
+			fft_pass_args a{};
+			line_geom g;
+			int lines, outer;
+			axis_lines(plan->extent, aRank, 0, g, lines, outer);
+			setup_pass(a, PASS_C2R, plan->extent[0], lines, outer);
+			a.load_n = spec[0];
+			a.store_n = plan->extent[0];
+			a.gin = {1, spec[0], 0};
+			a.gout = g;
+			plan->passes.push_back(a);
+			plan->buffers.push_back({src, BUF_OUT});
+			plan->workspace = aRank > 1 ? spec_bytes : 0;
+			plan->in_bytes = spec_bytes;
+			plan->out_bytes = real_bytes;
+		}
+		*aPlan = plan.release();
+	});
+}
+
+int iane_hip_fft_plan_workspace_bytes(const iane_fft_plan* aPlan, size_t* aBytes)
+{
+	return guarded([&] {
+		if (!aPlan || !aBytes) {
+			invalid("FFT plan or output is null");
+		}
+		*aBytes = aPlan->workspace;
+	});
+}
+
+// Validation of fft.hpp:20-66 / rfft.hpp:29-47: non-null, complex-aligned,
+// disjoint buffers; the workspace must not overlap either.
+int iane_hip_fft_exec(iane_fft_plan* aPlan, const void* aIn, void* aOut, void* aWorkspace, void* aStream)
+{
+	return guarded([&] {
+		if (!aPlan) {
+			invalid("FFT plan is null");
+		}
+		auto addr = [](const void* p) { return reinterpret_cast<uintptr_t>(p); };
+		if (!aIn || !aOut || addr(aIn) % alignof(float2) || addr(aOut) % alignof(float2)) {
+			invalid("FFT requires aligned input and output buffers");
+		}
+		auto overlap = [&](const void* a, size_t na, const void* b, size_t nb) {
+			return addr(a) <= addr(b) ? addr(b) - addr(a) < na : addr(a) - addr(b) < nb;
+		};
+		if (overlap(aIn, aPlan->in_bytes, aOut, aPlan->out_bytes)) {
+			invalid("FFT requires disjoint input and output buffers");
+		}
+		if (aPlan->workspace) {
+			if (!aWorkspace || addr(aWorkspace) % alignof(float2)) {
+				fail(IANE_ERR_LENGTH, "Prepared operation exceeded its scratch reservation");
+			}
+			if (overlap(aWorkspace, aPlan->workspace, aIn, aPlan->in_bytes) ||
+				overlap(aWorkspace, aPlan->workspace, aOut, aPlan->out_bytes)) {
+				invalid("FFT workspace overlaps an input or output buffer");
+			}
+		}
+		auto stream = static_cast<hipStream_t>(aStream);
+		void* buffers[3] = {const_cast<void*>(aIn), aOut, aWorkspace};
+		for (size_t i = 0; i < aPlan->passes.size(); ++i) {
+			fft_pass_args a = aPlan->passes[i];
+			a.in = buffers[aPlan->buffers[i].first];
+			a.out = buffers[aPlan->buffers[i].second];
+			launch_pass(a, stream);
+		}
+	});
+}
+
+int iane_hip_fft_plan_destroy(iane_fft_plan* aPlan)
+{
+	return guarded([&] { delete aPlan; });
+}
+
+} // extern "C"

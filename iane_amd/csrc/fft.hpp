@@ -1,0 +1,402 @@
+// Stockham FFT building blocks shared by the FFT plans (fft.hip) and the
+// FFT-based Richardson–Lucy pipeline (rl.hip).
+//
+// Every kernel works on "lines": complete 1-D transforms of length m held in
+// LDS by one workgroup. A workgroup processes `bundle` lines at once; lines
+// along axis 0 are contiguous rows, lines along axes 1/2 are bundled with
+// their axis-0 neighbours so global loads and stores stay coalesced. Radix
+// stages (2, 3, 4, 5, 7, 8, 11, 13, generic odd) ping-pong between two LDS
+// buffers in the Stockham autosort order, so results land in natural order.
+//
+// Real transforms along axis 0 use the half-length complex trick for even n
+// (z[j] = x[2j] + i x[2j+1], one m = n/2 FFT and an O(n) post/pre-process)
+// and a full-length complex FFT for odd n.
+#pragma once
+
+#include "common.hpp"
+
+namespace iane_hip {
+
+constexpr int kFftThreads = 256;
+constexpr int kFftLdsElems = 4096;   // complex elements per LDS buffer (32 KiB; two buffers)
+constexpr int kFftMaxStages = 20;
+
+struct fft_radix_plan {
+	int m;                       // complex line length
+	int stages;
+	int radix[kFftMaxStages];
+};
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b)
+{
+	return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// i * a
+__device__ __forceinline__ float2 cmuli(float2 a) { return make_float2(-a.y, a.x); }
+
+// R-point DFT in registers; sign -1 forward, +1 inverse; tw = W_m table, m = line length.
+template<int R>
+__device__ __forceinline__ void dft(float2 (&v)[R], float sign, const float2* tw, int m)
+{
+	if constexpr (R == 2) {
+		const float2 a = v[0], b = v[1];
+		v[0] = cadd(a, b);
+		v[1] = csub(a, b);
+	} else if constexpr (R == 3) {
+		const float c = -0.5f;
+		const float s = sign * 0.86602540378443864676f;
+		const float2 t = cadd(v[1], v[2]);
+		const float2 u = csub(v[1], v[2]);
+		const float2 base = make_float2(v[0].x + c * t.x, v[0].y + c * t.y);
+		const float2 rot = cmuli(cscale(u, s));
+		v[0] = cadd(v[0], t);
+		v[1] = cadd(base, rot);
+		v[2] = csub(base, rot);
+	} else if constexpr (R == 4) {
+		const float2 a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+		const float2 b0 = cadd(v[1], v[3]);
+		float2 b1 = csub(v[1], v[3]);
+		b1 = sign < 0 ? make_float2(b1.y, -b1.x) : make_float2(-b1.y, b1.x); // (sign i) * b1
+		v[0] = cadd(a0, b0);
+		v[2] = csub(a0, b0);
+		v[1] = cadd(a1, b1);
+		v[3] = csub(a1, b1);
+	} else if constexpr (R == 5) {
+		const float c1 = 0.30901699437494742410f, c2 = -0.80901699437494742410f;
+		const float s1 = sign * 0.95105651629515357212f, s2 = sign * 0.58778525229247312917f;
+		const float2 b1 = cadd(v[1], v[4]), b2 = cadd(v[2], v[3]);
+		const float2 d1 = csub(v[1], v[4]), d2 = csub(v[2], v[3]);
+		const float2 r1 = make_float2(v[0].x + c1 * b1.x + c2 * b2.x, v[0].y + c1 * b1.y + c2 * b2.y);
+		const float2 r2 = make_float2(v[0].x + c2 * b1.x + c1 * b2.x, v[0].y + c2 * b1.y + c1 * b2.y);
+		const float2 i1 = cmuli(make_float2(s1 * d1.x + s2 * d2.x, s1 * d1.y + s2 * d2.y));
+		const float2 i2 = cmuli(make_float2(s2 * d1.x - s1 * d2.x, s2 * d1.y - s1 * d2.y));
+		v[0] = cadd(v[0], cadd(b1, b2));
+		v[1] = cadd(r1, i1);
+		v[4] = csub(r1, i1);
+		v[2] = cadd(r2, i2);
+		v[3] = csub(r2, i2);
+	} else if constexpr (R == 8) {
+		float2 e[4] = {v[0], v[2], v[4], v[6]};
+		float2 o[4] = {v[1], v[3], v[5], v[7]};
+		dft<4>(e, sign, tw, m);
+		dft<4>(o, sign, tw, m);
+		const float r = 0.70710678118654752440f;
+		// w8^k = exp(sign * 2 pi i k / 8)
+		const float2 w1 = make_float2(r, sign * r);
+		const float2 w3 = make_float2(-r, sign * r);
+		o[1] = cmul(o[1], w1);
+		o[2] = sign < 0 ? make_float2(o[2].y, -o[2].x) : make_float2(-o[2].y, o[2].x);
+		o[3] = cmul(o[3], w3);
+#pragma unroll
+		for (int k = 0; k < 4; ++k) {
+			v[k] = cadd(e[k], o[k]);
+			v[k + 4] = csub(e[k], o[k]);
+		}
+	} else {
+		// odd prime radix: direct DFT, twiddles from the W_m table
+		float2 out[R];
+		const int step = m / R;
+#pragma unroll
+		for (int k = 0; k < R; ++k) {
+			float2 acc = v[0];
+#pragma unroll
+			for (int r = 1; r < R; ++r) {
+				float2 w = tw[((r * k) % R) * step];
+				if (sign > 0) {
+					w = cconj(w);
+				}
+				acc = cadd(acc, cmul(v[r], w));
+			}
+			out[k] = acc;
+		}
+#pragma unroll
+		for (int k = 0; k < R; ++k) {
+			v[k] = out[k];
+		}
+	}
+}
+
+// One Stockham stage of radix R over `bundle` lines (LDS line pitch `pitch`).
+template<int R>
+__device__ __forceinline__ void stockham_stage(const float2* src, float2* dst, int m, int Ns, int bundle, int pitch,
+	const float2* tw, float sign)
+{
+	const int nb = m / R;
+	const int total = bundle * nb;
+	const int step = m / (Ns * R);
+	for (int t = threadIdx.x; t < total; t += blockDim.x) {
+		const int w = t / nb;
+		const int j = t - w * nb;
+		const float2* s = src + w * pitch;
+		float2 v[R];
+#pragma unroll
+		for (int r = 0; r < R; ++r) {
+			v[r] = s[j + r * nb];
+		}
+		const int k = j % Ns;
+		if (Ns > 1) {
+#pragma unroll
+			for (int r = 1; r < R; ++r) {
+				float2 wv = tw[k * r * step];
+				if (sign > 0) {
+					wv = cconj(wv);
+				}
+				v[r] = cmul(v[r], wv);
+			}
+		}
+		dft<R>(v, sign, tw, m);
+		const int out = (j - k) * R + k;
+		float2* d = dst + w * pitch;
+#pragma unroll
+		for (int r = 0; r < R; ++r) {
+			d[out + r * Ns] = v[r];
+		}
+	}
+}
+
+// Generic radix (any odd factor without a specialisation), runtime R <= 64.
+__device__ inline void stockham_stage_generic(const float2* src, float2* dst, int m, int R, int Ns, int bundle,
+	int pitch, const float2* tw, float sign)
+{
+	const int nb = m / R;
+	const int total = bundle * nb;
+	const int step = m / (Ns * R);
+	const int dstep = m / R;
+	for (int t = threadIdx.x; t < total; t += blockDim.x) {
+		const int w = t / nb;
+		const int j = t - w * nb;
+		const float2* s = src + w * pitch;
+		const int k = j % Ns;
+		const int out = (j - k) * R + k;
+		float2* d = dst + w * pitch;
+		for (int q = 0; q < R; ++q) {
+			float2 acc = make_float2(0.0f, 0.0f);
+			for (int r = 0; r < R; ++r) {
+				float2 v = s[j + r * nb];
+				if (Ns > 1 && r > 0) {
+					float2 wv = tw[k * r * step];
+					if (sign > 0) {
+						wv = cconj(wv);
+					}
+					v = cmul(v, wv);
+				}
+				float2 wd = tw[((r * q) % R) * dstep];
+				if (sign > 0) {
+					wd = cconj(wd);
+				}
+				acc = cadd(acc, cmul(v, wd));
+			}
+			d[out + q * Ns] = acc;
+		}
+	}
+}
+
+// Run every stage of `plan` on the lines in buffer `a` (scratch `b`).
+// Returns the buffer holding the result. Ends with a barrier.
+__device__ inline float2* run_stages(const fft_radix_plan& plan, float2* a, float2* b, int bundle, int pitch,
+	const float2* tw, float sign)
+{
+	int Ns = 1;
+	for (int s = 0; s < plan.stages; ++s) {
+		const int R = plan.radix[s];
+		switch (R) {
+		case 2: stockham_stage<2>(a, b, plan.m, Ns, bundle, pitch, tw, sign); break;
+		case 3: stockham_stage<3>(a, b, plan.m, Ns, bundle, pitch, tw, sign); break;
+		case 4: stockham_stage<4>(a, b, plan.m, Ns, bundle, pitch, tw, sign); break;
+		case 5: stockham_stage<5>(a, b, plan.m, Ns, bundle, pitch, tw, sign); break;
+		case 7: stockham_stage<7>(a, b, plan.m, Ns, bundle, pitch, tw, sign); break;
+		case 8: stockham_stage<8>(a, b, plan.m, Ns, bundle, pitch, tw, sign); break;
+		case 11: stockham_stage<11>(a, b, plan.m, Ns, bundle, pitch, tw, sign); break;
+		case 13: stockham_stage<13>(a, b, plan.m, Ns, bundle, pitch, tw, sign); break;
+		default: stockham_stage_generic(a, b, plan.m, R, Ns, bundle, pitch, tw, sign); break;
+		}
+		__syncthreads();
+		float2* t = a;
+		a = b;
+		b = t;
+		Ns *= R;
+	}
+	return a;
+}
+
+// Host: factor m into supported radices (largest first; any leftover factor
+// becomes one generic stage, limited to 64).
+bool factor_radices(int aM, fft_radix_plan& aPlan);
+
+// Host: device table W_m^k = exp(-2 pi i k / m), k < m, computed in double;
+// cached per (device, m) for the life of the process.
+const float2* twiddle_table(int aM);
+
+// Smallest even length >= n whose prime factors are all in {2, 3, 5, 7}.
+int next_fft_length(int aN);
+
+// ---------------------------------------------------------------------------
+// Line geometry: point k of line (o, l) sits at o*outer + l*line + k*elem
+// (element units of the buffer's type).
+struct line_geom {
+	long long elem;
+	long long line;
+	long long outer;
+};
+
+enum { PASS_C2C = 0, PASS_R2C = 1, PASS_C2R = 2, PASS_C2C_MULT = 3 };
+
+struct fft_pass_args {
+	const void* in;
+	void* out;
+	int mode;
+	int n;            // logical length: complex length (C2C) or real length (R2C/C2R)
+	float sign;       // C2C direction (-1 forward, +1 inverse)
+	int load_n;       // points loaded per line (reals for R2C, bins for C2R), the rest are zero
+	int store_n;      // points stored per line (complex for C2C/R2C, reals for C2R)
+	line_geom gin;
+	line_geom gout;
+	int lines;        // lines per outer group
+	int outer;        // outer groups
+	int bundle;       // lines per workgroup
+	int pitch;        // LDS pitch per line (complex elements)
+	unsigned groups_per_outer;
+	fft_radix_plan plan; // the complex transform run in LDS (length m)
+	const float2* tw_m;
+	const float2* tw_n;  // W_n table for the real pre/post processing (even n)
+	const float2* mult;  // PASS_C2C_MULT: spectrum multiplier, read with gmult
+	line_geom gmult;
+	int mult_conj;
+	float mult_scale;
+};
+
+__device__ __forceinline__ void decode_group(const fft_pass_args& a, unsigned aBlock, int& aOuter, int& aLine0,
+	int& aCount)
+{
+	aOuter = static_cast<int>(aBlock / a.groups_per_outer);
+	aLine0 = static_cast<int>(aBlock % a.groups_per_outer) * a.bundle;
+	aCount = min(a.bundle, a.lines - aLine0);
+}
+
+// Load complex points k < load_n (zero up to `width`) of `count` lines into buf.
+__device__ inline void load_complex_lines(const float2* in, const line_geom& g, int aOuter, int aLine0, int count,
+	int load_n, int width, float2* buf, int pitch)
+{
+	const long long base = aOuter * g.outer + aLine0 * g.line;
+	const int total = count * width;
+	if (g.elem == 1) {
+		for (int t = threadIdx.x; t < total; t += blockDim.x) {
+			const int w = t / width;
+			const int k = t - w * width;
+			buf[w * pitch + k] = k < load_n ? in[base + w * g.line + k] : make_float2(0.0f, 0.0f);
+		}
+	} else {
+		for (int t = threadIdx.x; t < total; t += blockDim.x) {
+			const int k = t / count;
+			const int w = t - k * count;
+			buf[w * pitch + k] = k < load_n ? in[base + w * g.line + k * g.elem] : make_float2(0.0f, 0.0f);
+		}
+	}
+}
+
+__device__ inline void store_complex_lines(float2* out, const line_geom& g, int aOuter, int aLine0, int count,
+	int store_n, const float2* buf, int pitch)
+{
+	const long long base = aOuter * g.outer + aLine0 * g.line;
+	const int total = count * store_n;
+	if (g.elem == 1) {
+		for (int t = threadIdx.x; t < total; t += blockDim.x) {
+			const int w = t / store_n;
+			const int k = t - w * store_n;
+			out[base + w * g.line + k] = buf[w * pitch + k];
+		}
+	} else {
+		for (int t = threadIdx.x; t < total; t += blockDim.x) {
+			const int k = t / count;
+			const int w = t - k * count;
+			out[base + w * g.line + k * g.elem] = buf[w * pitch + k];
+		}
+	}
+}
+
+// Real line of length n (points >= load_n are zero) packed for the complex FFT:
+// even n: z[j] = x[2j] + i x[2j+1] (j < n/2); odd n: z[j] = x[j].
+__device__ inline void load_real_lines(const float* in, const line_geom& g, int aOuter, int aLine0, int count,
+	int n, int load_n, float2* buf, int pitch)
+{
+	const long long base = aOuter * g.outer + aLine0 * g.line;
+	const bool even = (n & 1) == 0;
+	const int width = even ? n / 2 : n;
+	const int total = count * width;
+	for (int t = threadIdx.x; t < total; t += blockDim.x) {
+		int w, j;
+		if (g.elem == 1) {
+			w = t / width;
+			j = t - w * width;
+		} else {
+			j = t / count;
+			w = t - j * count;
+		}
+		const float* line = in + base + w * g.line;
+		float2 v;
+		if (even) {
+			const int i0 = 2 * j, i1 = 2 * j + 1;
+			v.x = i0 < load_n ? line[i0 * g.elem] : 0.0f;
+			v.y = i1 < load_n ? line[i1 * g.elem] : 0.0f;
+		} else {
+			v.x = j < load_n ? line[j * g.elem] : 0.0f;
+			v.y = 0.0f;
+		}
+		buf[w * pitch + j] = v;
+	}
+}
+
+// even n, after the m = n/2 forward FFT of the packed line in `z`:
+// X[k] = E[k] + W_n^k O[k], k = 0..m, written to `x` (pitch >= m + 1).
+__device__ inline void r2c_post(const float2* z, float2* x, int count, int n, int pitch, const float2* tw_n)
+{
+	const int m = n / 2;
+	const int total = count * (m + 1);
+	for (int t = threadIdx.x; t < total; t += blockDim.x) {
+		const int w = t / (m + 1);
+		const int k = t - w * (m + 1);
+		const float2* zl = z + w * pitch;
+		const float2 zk = zl[k == m ? 0 : k];
+		const float2 zc = cconj(zl[k == 0 ? 0 : m - k]);
+		const float2 e = cscale(cadd(zk, zc), 0.5f);
+		const float2 d = csub(zk, zc);
+		const float2 o = make_float2(0.5f * d.y, -0.5f * d.x); // d / (2i)
+		x[w * pitch + k] = cadd(e, cmul(tw_n[k], o));
+	}
+}
+
+// even n, before the m = n/2 inverse FFT: from bins X[0..m] in `x` build
+// Z'[k] = (X[k] + conj X[m-k]) + i (X[k] - conj X[m-k]) W_n^{-k}, k < m, into `z`.
+__device__ inline void c2r_pre(const float2* x, float2* z, int count, int n, int pitch, const float2* tw_n)
+{
+	const int m = n / 2;
+	const int total = count * m;
+	for (int t = threadIdx.x; t < total; t += blockDim.x) {
+		const int w = t / m;
+		const int k = t - w * m;
+		const float2* xl = x + w * pitch;
+		const float2 xk = xl[k];
+		const float2 xc = cconj(xl[m - k]);
+		const float2 s = cadd(xk, xc);
+		const float2 d = cmul(csub(xk, xc), cconj(tw_n[k]));
+		z[w * pitch + k] = cadd(s, cmuli(d));
+	}
+}
+
+// odd n, before the inverse FFT: Hermitian extension of bins X[0..n/2] in place.
+__device__ inline void c2r_extend_odd(float2* x, int count, int n, int pitch)
+{
+	const int h = n / 2;
+	const int total = count * (n - h - 1);
+	for (int t = threadIdx.x; t < total; t += blockDim.x) {
+		const int w = t / (n - h - 1);
+		const int k = h + 1 + (t - w * (n - h - 1));
+		x[w * pitch + k] = cconj(x[w * pitch + n - k]);
+	}
+}
+
+} // namespace iane_hip

@@ -4,6 +4,9 @@
 // include/iane/execution/context.hpp:36-59, include/iane/algorithms/copy.hpp:55-108).
 #include "common.hpp"
 
+#include <map>
+#include <mutex>
+
 namespace iane_hip {
 
 namespace {
@@ -11,6 +14,22 @@ thread_local std::string g_last_error;
 }
 
 void set_last_error(const std::string& aMessage) { g_last_error = aMessage; }
+
+namespace {
+std::mutex g_tune_mutex;
+std::map<std::string, int>& tune_table()
+{
+	static std::map<std::string, int> table;
+	return table;
+}
+} // namespace
+
+int tune_get(const char* aKey, int aDefault)
+{
+	std::lock_guard lock(g_tune_mutex);
+	const auto it = tune_table().find(aKey);
+	return it == tune_table().end() ? aDefault : it->second;
+}
 
 namespace {
 
@@ -52,6 +71,31 @@ extern "C" {
 const char* iane_hip_last_error(void) { return g_last_error.c_str(); }
 
 int iane_hip_version(void) { return 1; }
+
+int iane_hip_tune_set(const char* aKey, int aValue)
+{
+	return guarded([&] {
+		if (!aKey) {
+			invalid("tuning key is null");
+		}
+		std::lock_guard lock(g_tune_mutex);
+		if (aValue < 0) {
+			tune_table().erase(aKey);
+		} else {
+			tune_table()[aKey] = aValue;
+		}
+	});
+}
+
+int iane_hip_tune_get(const char* aKey, int* aValue)
+{
+	return guarded([&] {
+		if (!aKey || !aValue) {
+			invalid("tuning key or output is null");
+		}
+		*aValue = tune_get(aKey, -1);
+	});
+}
 
 int iane_hip_device_count(int* aCount)
 {

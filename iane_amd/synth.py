@@ -186,7 +186,8 @@ def volume_blobs(n: int = 256, seed: int = 2, blobs: int = 200):
 
 def _fftconvolve_same(x: np.ndarray, k: np.ndarray) -> np.ndarray:
     shape = [n + m - 1 for n, m in zip(x.shape, k.shape)]
-    full = np.fft.irfftn(np.fft.rfftn(x, shape) * np.fft.rfftn(k, shape), shape)
+    axes = list(range(x.ndim))
+    full = np.fft.irfftn(np.fft.rfftn(x, shape, axes) * np.fft.rfftn(k, shape, axes), shape, axes)
     return full[tuple(slice(m // 2, m // 2 + n) for n, m in zip(x.shape, k.shape))]
 
 

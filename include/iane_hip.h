@@ -94,6 +94,11 @@ int iane_hip_memcpy_async(void* dst, const void* src, size_t bytes, int kind, vo
  * (copy.hpp:55-108: host<->device transfer of packed or per-plane strided views). */
 int iane_hip_copy_view(const iane_view_desc* src, const iane_view_desc* dst, int kind, void* stream);
 int iane_hip_memset_async(void* dst, int value, size_t bytes, void* stream);
+/* Launch-shape knobs (tile sizes, thread counts, z chunking) chosen by the
+ * autotuner (iane_amd/autotune.py); a negative value restores the default.
+ * Known keys: "sep3.threads" (256|512), "sep3.chunk", "sep3.min_blocks". */
+int iane_hip_tune_set(const char* key, int value);
+int iane_hip_tune_get(const char* key, int* value);
 /* Events for kernel timing on a given stream. */
 int iane_hip_event_create(void** event);
 int iane_hip_event_destroy(void* event);
@@ -115,6 +120,23 @@ int iane_hip_convolve(const iane_view_desc* src, const iane_view_desc* kernel,
  * No reference counterpart: the reference always runs the dense kernel. */
 int iane_hip_convolve_separable(const iane_view_desc* src, const iane_view_desc* taps,
 	int n_taps, const iane_view_desc* dst, int border, int precision, void* stream);
+
+/* ---- FFT: replaces the prepared fft_forward / fft_inverse / rfft / irfft
+ * (include/iane/algorithms/fft/fft.hpp:196-231, rfft.hpp:107-161) ---------
+ * Stockham FFT, FP32, packed buffers, rank 1..3 (the reference is 2-D only,
+ * fft.hpp:22). Forward uses the negative exponent, the inverse is
+ * unnormalised. extent = the complex extents (C2C) or the REAL extents
+ * (R2C/C2R); the spectrum of R2C/C2R is {n0/2+1, n1, n2} (layout.hpp:15-21).
+ * C2R leaves its input intact (rfft.hpp:72-83). Lengths: any whose prime
+ * factors are <= 64, each line <= 4095 points (the line is held in LDS).
+ * Buffers must be complex-aligned and disjoint (fft.hpp:20-66).           */
+enum { IANE_FFT_C2C_FORWARD = 0, IANE_FFT_C2C_INVERSE = 1, IANE_FFT_R2C = 2, IANE_FFT_C2R = 3 };
+typedef struct iane_fft_plan iane_fft_plan;
+int iane_hip_fft_plan_create(iane_fft_plan** plan, int rank, const int32_t* extent, int kind);
+/* scratch the caller supplies to exec (execution_context scratch, prepared.hpp:120-146) */
+int iane_hip_fft_plan_workspace_bytes(const iane_fft_plan* plan, size_t* bytes);
+int iane_hip_fft_exec(iane_fft_plan* plan, const void* in, void* out, void* workspace, void* stream);
+int iane_hip_fft_plan_destroy(iane_fft_plan* plan);
 
 /* Test hooks: the untiled one-thread-per-output kernel and the per-axis
  * separable passes, so tests can compare every path against the oracle. */

@@ -156,7 +156,7 @@ def rfft(x: np.ndarray) -> np.ndarray:
 def irfft(spectrum: np.ndarray, real_shape) -> np.ndarray:
     """Unnormalised C2R inverse of ``rfft``."""
     n = int(np.prod(real_shape))
-    return np.fft.irfftn(np.asarray(spectrum, dtype=np.complex128), s=real_shape) * n
+    return np.fft.irfftn(np.asarray(spectrum, dtype=np.complex128), s=real_shape, axes=list(range(len(real_shape)))) * n
 
 
 def dft_bruteforce(x: np.ndarray) -> np.ndarray:
@@ -176,7 +176,8 @@ def conv_zero_fft(x: np.ndarray, psf: np.ndarray) -> np.ndarray:
     """Zero-border convolution with an odd centred kernel, as a float64 FFT
     product on a padded domain (equal to ``convolve(..., border=ZERO)`` in exact arithmetic)."""
     shape = [n + k - 1 for n, k in zip(x.shape, psf.shape)]
-    full = np.fft.irfftn(np.fft.rfftn(x, shape) * np.fft.rfftn(psf, shape), shape)
+    axes = list(range(x.ndim))
+    full = np.fft.irfftn(np.fft.rfftn(x, shape, axes) * np.fft.rfftn(psf, shape, axes), shape, axes)
     sl = tuple(slice(k // 2, k // 2 + n) for n, k in zip(x.shape, psf.shape))
     return full[sl]
 

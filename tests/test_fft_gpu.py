@@ -1,0 +1,197 @@
+"""GPU parity of the Stockham FFT drop-ins (fft_forward / fft_inverse / rfft / irfft).
+
+Known answers restated from test/unit/prepared/prepared_cases.hpp:73-123,
+test_prepared.cpp:39-59,160-183 and real_fft_cases.hpp:12-77; larger sizes
+against the FP64 oracle (numpy.fft, itself pinned by a brute-force DFT in
+tests/test_oracle_fft.py). Tolerance for FP32 transforms:
+    max|gpu - oracle64| <= 1e-5 * max|oracle64|   (TOL_FFT)
+plus the reference's own absolute tolerances on its small cases.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import iane_amd
+from iane_amd import execution_context, fft_forward, fft_inverse, irfft, rfft, rfft_extent
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+TOL_FFT = 1e-5
+
+
+def dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def host(t):
+    return t.cpu().numpy()
+
+
+def cempty(shape):
+    import torch
+    return torch.full(tuple(shape), complex(7, 7), dtype=torch.complex64, device="cuda")
+
+
+def rempty(shape):
+    import torch
+    return torch.full(tuple(shape), 7.0, dtype=torch.float32, device="cuda")
+
+
+def rel(got, ref):
+    return float(np.max(np.abs(got - ref)) / max(np.max(np.abs(ref)), 1e-30))
+
+
+# ------------------------------------------------------ known answers -----
+def test_c2c_delta_known_answer_and_round_trip():
+    """prepared_cases.hpp:73-123: delta at index 1 of a 4x2 image."""
+    x = np.zeros((2, 4), np.complex64)
+    x[0, 1] = 1.0
+    src, out, restored = dev(x), cempty((2, 4)), cempty((2, 4))
+    ctx = execution_context()
+    fwd = ctx.prepare(fft_forward, src, out)
+    inv = ctx.prepare(fft_inverse, out, restored)
+    ctx.run(fwd, src, out)
+    ctx.run(inv, out, restored)
+    ctx.run(fwd, src, out).wait()
+    got = host(out)
+    real = np.array([1, 0, -1, 0], np.float32)
+    imag = np.array([0, -1, 0, 1], np.float32)
+    for row in range(2):
+        np.testing.assert_allclose(got[row].real, real, atol=1e-5)
+        np.testing.assert_allclose(got[row].imag, imag, atol=1e-5)
+    rt = host(restored)
+    np.testing.assert_allclose(rt.real, 8 * x.real, atol=1e-5)
+    np.testing.assert_allclose(rt.imag, 0, atol=1e-5)
+    assert ctx.scratch_capacity() == 0
+
+
+def test_c2c_constant_dc():
+    """test_prepared.cpp:39-59: constant 2 on 4x2 -> DC 16, zeros elsewhere."""
+    x = np.full((2, 4), 2.0, np.complex64)
+    src, out = dev(x), cempty((2, 4))
+    ctx = execution_context()
+    op = ctx.prepare(fft_forward, src, out)
+    ctx.run(op, src, out).wait()
+    got = host(out)
+    expected = np.zeros((2, 4), np.complex64)
+    expected[0, 0] = 16
+    np.testing.assert_allclose(got, expected, atol=1e-5)
+
+
+def test_c2c_aliasing_and_shape_errors():
+    x = dev(np.zeros((2, 4), np.complex64))
+    ctx = execution_context()
+    with pytest.raises(iane_amd.invalid_argument):
+        op = ctx.prepare(fft_forward, x, x)
+        ctx.run(op, x, x)
+    out = cempty((2, 4))
+    op = ctx.prepare(fft_forward, x, out)
+    with pytest.raises(iane_amd.invalid_argument, match="prepared shape"):
+        ctx.run(op, dev(np.zeros((4, 2), np.complex64)), cempty((4, 2)))
+    other = execution_context()
+    with pytest.raises(iane_amd.invalid_argument, match="different execution context"):
+        other.run(op, x, out)
+    with pytest.raises(iane_amd.invalid_argument, match="packed"):
+        ctx.prepare(fft_forward, dev(np.zeros((2, 8), np.complex64))[:, ::2], out)
+
+
+@pytest.mark.parametrize("shape", [(3, 4), (3, 5), (4, 6)])
+def test_real_fft_known_answer(shape):
+    """real_fft_cases.hpp:12-77: sizes 4x3, 5x3, 6x4 (IANE {w, h})."""
+    count = shape[0] * shape[1]
+    x = ((np.arange(count) * 7) % 11 - 4).astype(np.float32).reshape(shape)
+    packed_shape = rfft_extent(shape)
+    src, packed, out = dev(x), cempty(packed_shape), rempty(shape)
+    ctx = execution_context()
+    fwd = ctx.prepare(rfft, src, packed)
+    inv = ctx.prepare(irfft, packed, out)
+    ctx.run(fwd, src, packed)
+    before = host(packed)
+    ctx.run(inv, packed, out).wait()
+    after = host(packed)
+    np.testing.assert_allclose(host(out), count * x, atol=2e-4)
+    np.testing.assert_array_equal(host(src), x)
+    brute = oracle.dft_bruteforce(x.astype(np.float64))
+    np.testing.assert_allclose(before.real, brute[:, :packed_shape[1]].real, atol=3e-5)
+    np.testing.assert_allclose(before.imag, brute[:, :packed_shape[1]].imag, atol=3e-5)
+    np.testing.assert_array_equal(after.view(np.uint32), before.view(np.uint32))  # C2R preserved input
+
+
+def test_real_fft_layout_errors():
+    ctx = execution_context()
+    x = dev(np.zeros((3, 4), np.float32))
+    with pytest.raises(iane_amd.invalid_argument, match="matching real and packed"):
+        ctx.prepare(rfft, x, cempty((3, 4)))
+    op = ctx.prepare(rfft, x, cempty((3, 3)))
+    with pytest.raises(iane_amd.invalid_argument):
+        ctx.run(op, dev(np.zeros((3, 6), np.float32)), cempty((3, 4)))
+
+
+# ------------------------------------------------------ oracle sweeps -----
+LENGTHS = [1, 2, 3, 5, 7, 8, 12, 16, 17, 30, 31, 64, 90, 128, 143, 160, 224, 280, 288, 512, 1000, 2048]
+
+
+@pytest.mark.parametrize("n", LENGTHS)
+def test_c2c_1d_lengths(n):
+    rng = np.random.default_rng(n)
+    x = (rng.standard_normal((5, n)) + 1j * rng.standard_normal((5, n))).astype(np.complex64)
+    ctx = execution_context()
+    for alg, ref in ((fft_forward, np.fft.fft(x.astype(np.complex128), axis=-1)),
+                     (fft_inverse, np.fft.ifft(x.astype(np.complex128), axis=-1) * n)):
+        # a batch of 5 independent 1-D lines = rank-1 plans on each row
+        for row in range(5):
+            src, out = dev(x[row]), cempty((n,))
+            op = ctx.prepare(alg, src, out)
+            ctx.run(op, src, out).wait()
+            assert rel(host(out), ref[row]) <= TOL_FFT * max(1.0, np.log2(max(n, 2)) / 4)
+
+
+@pytest.mark.parametrize("shape", [(6, 10), (9, 16, 20), (16, 30, 28), (7, 5, 3), (64, 64, 64)])
+def test_c2c_nd(shape):
+    rng = np.random.default_rng(len(shape))
+    x = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+    ctx = execution_context()
+    src, out = dev(x), cempty(shape)
+    op = ctx.prepare(fft_forward, src, out)
+    ctx.run(op, src, out).wait()
+    assert rel(host(out), oracle.fft_forward(x)) <= TOL_FFT
+    back = cempty(shape)
+    op2 = ctx.prepare(fft_inverse, out, back)
+    ctx.run(op2, out, back).wait()
+    assert rel(host(back), x * np.prod(shape)) <= TOL_FFT
+
+
+@pytest.mark.parametrize("shape", [(7,), (8,), (31,), (3, 9), (10, 12), (5, 6, 7), (9, 20, 30), (40, 56, 70),
+                                   (160, 280, 280)])
+def test_r2c_c2r_nd(shape):
+    rng = np.random.default_rng(sum(shape))
+    x = rng.standard_normal(shape).astype(np.float32)
+    spec_shape = rfft_extent(shape)
+    ctx = execution_context()
+    src, spec, back = dev(x), cempty(spec_shape), rempty(shape)
+    f = ctx.prepare(rfft, src, spec)
+    b = ctx.prepare(irfft, spec, back)
+    ctx.run(f, src, spec)
+    saved = host(spec)
+    ctx.run(b, spec, back).wait()
+    assert rel(saved, oracle.rfft(x)) <= TOL_FFT
+    assert rel(host(back), x.astype(np.float64) * np.prod(shape)) <= TOL_FFT
+    np.testing.assert_array_equal(host(spec).view(np.uint32), saved.view(np.uint32))
+    # Parseval on the full spectrum (size independent)
+    if len(shape) == 3 and shape[0] >= 100:
+        full = oracle.rfft(x)
+        energy = np.sum(np.abs(saved[..., 0]) ** 2) + 2 * np.sum(np.abs(saved[..., 1:]) ** 2)
+        if shape[-1] % 2 == 0:
+            energy -= np.sum(np.abs(saved[..., -1]) ** 2)
+        assert abs(energy / np.prod(shape) - np.sum(x.astype(np.float64) ** 2)) <= 1e-4 * np.sum(x ** 2)
+        del full
+
+
+def test_fft_workspace_reported_for_c2r():
+    ctx = execution_context()
+    spec, back = cempty((6, 9)), rempty((6, 16))
+    op = ctx.prepare(irfft, spec, back)
+    assert ctx.scratch_capacity() == 6 * 9 * 8
+    assert op.state.scratch_bytes() == 6 * 9 * 8
