@@ -95,6 +95,8 @@ SIGNATURES = {
     "iane_hip_rl_ratio": [vp, vp, vp, i64, f32, vp, vp],
     "iane_hip_rl_update": [vp, vp, vp, i64, f32, vp],
     "iane_hip_wiener_apply": [vp, vp, i64, f32, vp],
+    "iane_hip_real_to_complex": [vp, vp, i64, vp],
+    "iane_hip_complex_real_scale": [vp, vp, i64, f32, vp],
     "iane_hip_rl_plan_create": [ctypes.POINTER(vp), ctypes.c_int, ctypes.POINTER(i32), desc_p, ctypes.c_int,
                                 ctypes.c_int, f32, vp],
     "iane_hip_rl_plan_workspace_bytes": [vp, ctypes.POINTER(sz)],

@@ -1,0 +1,787 @@
+// Richardson–Lucy and Wiener deconvolution behind iane_hip_rl_* / iane_hip_wiener_apply.
+//
+// The loop is the reference's (examples/rl_deconvolution/main.cu:51-81):
+//   x0 = max(y, eps);  per iteration:
+//   p = H x (zero border);  r = y / max(p, eps);  c = H^T r (flipped PSF);
+//   x <- x * (s > 0 ? c / max(s, eps) : 0),  s = H^T 1.
+// Pointwise steps use the same float operations as the reference functors
+// (main.cu:11-49), so the DIRECT method in REFERENCE precision is bit-identical
+// to the reference loop.
+//
+// The FFT method works on a zero-padded domain n_a >= N_a + K_a/2 (smooth
+// lengths), which makes the circular product equal to the zero-border linear
+// convolution on the image. One convolution is five line passes:
+//   R2C along x (rows of the image) -> C2C along y -> [C2C along z, multiply by
+//   the PSF spectrum, inverse along z] in one kernel -> inverse along y -> C2R
+//   along x. Loads read only the valid (unpadded) extent and stores write only
+//   what a later pass reads. The C2R of one convolution and the R2C of the next
+//   happen in the same row kernel with the RL pointwise step between them
+//   (ratio + log-likelihood, or the multiplicative update), so neither the
+//   prediction nor the ratio image ever touches HBM.
+#include "fft.hpp"
+
+#include <memory>
+#include <vector>
+
+namespace iane_hip {
+
+void convolve_direct(const view4& aSrc, const view4& aKer, const view4& aDst, int aBorder, int aPrecision,
+	hipStream_t aStream, bool aForceSimple);
+int convolve_split(const view4& aSrc, const view4& aKer, const view4& aDst, int aBorder, int aPrecision,
+	hipStream_t aStream);
+void setup_pass(fft_pass_args& a, int aMode, int aN, int aLines, int aOuter);
+void launch_pass(const fft_pass_args& a, hipStream_t aStream);
+
+namespace {
+
+constexpr int kPointThreads = 256;
+constexpr int kPartialBlocks = 1024;
+
+// ---------------------------------------------------------------- pointwise --
+// main.cu:12-19
+__global__ void rl_initial_kernel(const float* y, float* x, long long n, float eps)
+{
+	for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
+		 i += static_cast<long long>(gridDim.x) * blockDim.x) {
+		const float v = y[i];
+		x[i] = v > eps ? v : eps;
+	}
+}
+
+__device__ __forceinline__ double block_sum(double v, double* shared)
+{
+	for (int off = 32; off > 0; off >>= 1) {
+		v += __shfl_down(v, off, 64);
+	}
+	const int lane = threadIdx.x & 63;
+	const int wave = threadIdx.x >> 6;
+	if (lane == 0) {
+		shared[wave] = v;
+	}
+	__syncthreads();
+	double total = 0.0;
+	if (threadIdx.x == 0) {
+		for (int w = 0; w < (blockDim.x + 63) / 64; ++w) {
+			total += shared[w];
+		}
+	}
+	return total;
+}
+
+// main.cu:22-29 plus the Poisson log-likelihood partial sum(y log max(p, eps) - p)
+__global__ void rl_ratio_kernel(const float* y, const float* p, float* r, long long n, float eps, double* partials)
+{
+	__shared__ double shared[kPointThreads / 64];
+	double ll = 0.0;
+	for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
+		 i += static_cast<long long>(gridDim.x) * blockDim.x) {
+		const float obs = y[i];
+		const float pred = p[i];
+		const float floor = pred > eps ? pred : eps;
+		r[i] = obs / floor;
+		ll += static_cast<double>(obs) * log(static_cast<double>(floor)) - static_cast<double>(pred);
+	}
+	if (partials) {
+		const double total = block_sum(ll, shared);
+		if (threadIdx.x == 0) {
+			partials[blockIdx.x] = total;
+		}
+	}
+}
+
+// main.cu:32-49, :74-77
+__device__ __forceinline__ float edge_factor(float c, float s, float eps)
+{
+	return s > 0.0f ? c / (s > eps ? s : eps) : 0.0f;
+}
+
+__global__ void rl_update_kernel(float* x, const float* c, const float* s, long long n, float eps)
+{
+	for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
+		 i += static_cast<long long>(gridDim.x) * blockDim.x) {
+		x[i] = x[i] * edge_factor(c[i], s[i], eps);
+	}
+}
+
+// Deterministic final sum of the per-block partials (fixed order).
+__global__ void ll_finalize_kernel(const double* partials, int n, double* out)
+{
+	__shared__ double shared[kPointThreads / 64];
+	double v = 0.0;
+	for (int i = threadIdx.x; i < n; i += blockDim.x) {
+		v += partials[i];
+	}
+	const double total = block_sum(v, shared);
+	if (threadIdx.x == 0) {
+		*out = total;
+	}
+}
+
+// wiener/main.cu:21-32: X * conj(H) / (|H|^2 + noise), factor in double
+__global__ void wiener_kernel(float2* spectrum, const float2* response, long long n, float noise)
+{
+	for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
+		 i += static_cast<long long>(gridDim.x) * blockDim.x) {
+		const float2 o = spectrum[i];
+		const double real = response[i].x;
+		const double imag = response[i].y;
+		const double denominator = real * real + imag * imag + noise;
+		spectrum[i] = make_float2(static_cast<float>((o.x * real + o.y * imag) / denominator),
+			static_cast<float>((o.y * real - o.x * imag) / denominator));
+	}
+}
+
+// wiener/main.cu:13-19 (real_sample) and :35-41 (encode_sample's real part x scale)
+__global__ void real_to_complex_kernel(const float* in, float2* out, long long n)
+{
+	for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
+		 i += static_cast<long long>(gridDim.x) * blockDim.x) {
+		out[i] = make_float2(in[i], 0.0f);
+	}
+}
+
+__global__ void complex_real_scale_kernel(const float2* in, float* out, long long n, float scale)
+{
+	for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
+		 i += static_cast<long long>(gridDim.x) * blockDim.x) {
+		out[i] = in[i].x * scale;
+	}
+}
+
+unsigned point_blocks(long long n)
+{
+	return static_cast<unsigned>(std::min<long long>(kPartialBlocks, std::max<long long>(1, (n + 255) / 256)));
+}
+
+// ------------------------------------------------------------- FFT row step --
+enum { EPI_RATIO = 0, EPI_UPDATE = 1 };
+
+struct rl_row_args {
+	fft_pass_args f;        // spectrum rows (in and out), m = n0/2, lines = ny rows, outer = nz planes
+	float* x;               // estimate (EPI_UPDATE)
+	const float* y;         // observed (EPI_RATIO)
+	const float* s;         // sensitivity (EPI_UPDATE)
+	long long real_line;    // real image row stride (= nx)
+	long long real_outer;   // real image plane stride (= nx*ny)
+	int nx;
+	float eps;
+	int forward_next;       // run the R2C of the new row (0 on the last update)
+	double* partials;       // EPI_RATIO: log-likelihood partials per block (may be null)
+};
+
+// C2R of `count` spectrum rows, the RL pointwise step on the real row, R2C of
+// the result back into the same spectrum rows.
+template<int EPI>
+__global__ __launch_bounds__(kFftThreads) void rl_row_kernel(rl_row_args a)
+{
+	__shared__ float2 lds[2 * kFftLdsElems];
+	__shared__ double shared[kFftThreads / 64];
+	float2* A = lds;
+	float2* B = lds + kFftLdsElems;
+	const fft_pass_args& f = a.f;
+	int o, l0, count;
+	decode_group(f, xcd_remap(blockIdx.x, gridDim.x), o, l0, count);
+	const int n = f.n;
+	const int m = n / 2;
+	load_complex_lines(static_cast<const float2*>(f.in), f.gin, o, l0, count, f.load_n, m + 1, A, f.pitch);
+	__syncthreads();
+	c2r_pre(A, B, count, n, f.pitch, f.tw_n);
+	__syncthreads();
+	float2* r = run_stages(f.plan, B, A, count, f.pitch, f.tw_m, 1.0f);
+	// pointwise step on real samples 2j, 2j+1 of each row
+	double ll = 0.0;
+	const long long base = o * a.real_outer + l0 * a.real_line;
+	for (int t = threadIdx.x; t < count * m; t += blockDim.x) {
+		const int w = t / m;
+		const int j = t - w * m;
+		const float2 v = r[w * f.pitch + j];
+		float out[2] = {0.0f, 0.0f};
+		const float in[2] = {v.x, v.y};
+#pragma unroll
+		for (int h = 0; h < 2; ++h) {
+			const int i = 2 * j + h;
+			if (i < a.nx) {
+				const long long idx = base + w * a.real_line + i;
+				if constexpr (EPI == EPI_RATIO) {
+					const float obs = a.y[idx];
+					const float pred = in[h];
+					const float floor = pred > a.eps ? pred : a.eps;
+					out[h] = obs / floor;
+					ll += static_cast<double>(obs) * log(static_cast<double>(floor)) - static_cast<double>(pred);
+				} else {
+					const float xn = a.x[idx] * edge_factor(in[h], a.s[idx], a.eps);
+					a.x[idx] = xn;
+					out[h] = xn;
+				}
+			}
+		}
+		r[w * f.pitch + j] = make_float2(out[0], out[1]);
+	}
+	if constexpr (EPI == EPI_RATIO) {
+		if (a.partials) {
+			const double total = block_sum(ll, shared);
+			if (threadIdx.x == 0) {
+				a.partials[blockIdx.x] = total;
+			}
+		}
+	}
+	if (!a.forward_next) {
+		return;
+	}
+	__syncthreads();
+	float2* z = run_stages(f.plan, r, r == A ? B : A, count, f.pitch, f.tw_m, -1.0f);
+	float2* xb = z == A ? B : A;
+	r2c_post(z, xb, count, n, f.pitch, f.tw_n);
+	__syncthreads();
+	store_complex_lines(static_cast<float2*>(f.out), f.gout, o, l0, count, f.store_n, xb, f.pitch);
+}
+
+// place the PSF with its centre at the origin of the padded real domain (wrap-around)
+__global__ void psf_wrap_kernel(view4 psf, float* out, int n0, int n1, int n2, float scale)
+{
+	const long long count = static_cast<long long>(psf.extent[0]) * psf.extent[1] * psf.extent[2];
+	for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < count;
+		 i += static_cast<long long>(gridDim.x) * blockDim.x) {
+		const int q0 = static_cast<int>(i % psf.extent[0]);
+		const int q1 = static_cast<int>((i / psf.extent[0]) % psf.extent[1]);
+		const int q2 = static_cast<int>(i / (static_cast<long long>(psf.extent[0]) * psf.extent[1]));
+		const float v = load_as_float(psf.ptr + q0 * psf.stride[0] + q1 * psf.stride[1] + q2 * psf.stride[2], psf.type);
+		const int d0 = ((q0 - psf.extent[0] / 2) % n0 + n0) % n0;
+		const int d1 = ((q1 - psf.extent[1] / 2) % n1 + n1) % n1;
+		const int d2 = ((q2 - psf.extent[2] / 2) % n2 + n2) % n2;
+		out[(static_cast<long long>(d2) * n1 + d1) * n0 + d0] = v * scale;
+	}
+}
+
+// reverse a packed buffer (flips every axis, main.cu:106-107) and convert to float
+__global__ void flip_kernel(view4 psf, float* out)
+{
+	const long long count = static_cast<long long>(psf.extent[0]) * psf.extent[1] * psf.extent[2] * psf.extent[3];
+	for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < count;
+		 i += static_cast<long long>(gridDim.x) * blockDim.x) {
+		long long rest = i;
+		long long off = 0;
+		for (int a = 0; a < 4; ++a) {
+			off += (rest % psf.extent[a]) * psf.stride[a];
+			rest /= psf.extent[a];
+		}
+		out[count - 1 - i] = load_as_float(psf.ptr + off, psf.type);
+	}
+}
+
+__global__ void fill_kernel(float* out, long long n, float v)
+{
+	for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
+		 i += static_cast<long long>(gridDim.x) * blockDim.x) {
+		out[i] = v;
+	}
+}
+
+} // namespace
+
+// ----------------------------------------------------------------- plan ------
+struct rl_plan_impl {
+	int rank = 0;
+	int method = 0;
+	int precision = 0;
+	float eps = 1e-6f;
+	int n[3] = {1, 1, 1};       // image extents (IANE order)
+	int k[3] = {1, 1, 1};       // PSF extents
+	long long count = 0;
+	// persistent device data
+	float* sensitivity = nullptr;
+	float* psf = nullptr;       // packed float PSF (direct)
+	float* flipped = nullptr;   // packed flipped PSF (direct)
+	float2* spectrum_h = nullptr; // PSF spectrum scaled by 1/(n0 n1 n2) (fft)
+	// fft geometry
+	int pad[3] = {1, 1, 1};
+	int h0 = 1;
+	size_t spectrum_bytes = 0;
+	std::vector<fft_pass_args> forward_mid;  // middle passes with H
+	std::vector<fft_pass_args> adjoint_mid;  // middle passes with conj(H)
+	fft_pass_args r2c{};                     // R2C of image rows into the spectrum
+	fft_pass_args c2r{};                     // C2R of spectrum rows into an image
+	rl_row_args row_tpl{};                    // row-kernel template
+	int partial_capacity = kPartialBlocks;   // log-likelihood partial sums (one per block)
+	size_t workspace = 0;
+
+	~rl_plan_impl()
+	{
+		hipFree(sensitivity);
+		hipFree(psf);
+		hipFree(flipped);
+		hipFree(spectrum_h);
+	}
+
+	view4 image_view(float* p) const
+	{
+		view4 v{};
+		v.ptr = reinterpret_cast<char*>(p);
+		v.type = IANE_F32;
+		v.rank = rank;
+		long long stride = 4;
+		for (int i = 0; i < 4; ++i) {
+			v.extent[i] = i < rank ? n[i] : 1;
+			v.stride[i] = i < rank ? stride : 0;
+			stride *= v.extent[i];
+		}
+		return v;
+	}
+
+	view4 kernel_view(float* p) const
+	{
+		view4 v{};
+		v.ptr = reinterpret_cast<char*>(p);
+		v.type = IANE_F32;
+		v.rank = rank;
+		long long stride = 4;
+		for (int i = 0; i < 4; ++i) {
+			v.extent[i] = i < rank ? k[i] : 1;
+			v.stride[i] = i < rank ? stride : 0;
+			stride *= v.extent[i];
+		}
+		return v;
+	}
+
+	void convolve(const float* in, const float* kernel, float* out, hipStream_t stream) const
+	{
+		const view4 src = image_view(const_cast<float*>(in));
+		const view4 ker = kernel_view(const_cast<float*>(kernel));
+		const view4 dst = image_view(out);
+		if ((precision == IANE_PREC_SPLIT_BF16 || precision == IANE_PREC_SPLIT_FP16) &&
+			convolve_split(src, ker, dst, IANE_BORDER_ZERO, precision, stream)) {
+			return;
+		}
+		convolve_direct(src, ker, dst, IANE_BORDER_ZERO,
+			precision == IANE_PREC_REFERENCE ? IANE_PREC_REFERENCE : IANE_PREC_FP32, stream, false);
+	}
+
+	// Spectrum geometry helpers (packed h0 x n1 x n2 complex array).
+	void build_fft(hipStream_t stream);
+	void run_mid(const std::vector<fft_pass_args>& passes, float2* spec, hipStream_t stream) const
+	{
+		for (fft_pass_args a : passes) {
+			a.in = spec;
+			a.out = spec;
+			launch_pass(a, stream);
+		}
+	}
+};
+
+void rl_plan_impl::build_fft(hipStream_t stream)
+{
+	for (int i = 0; i < 3; ++i) {
+		pad[i] = i < rank ? next_fft_length(n[i] + k[i] / 2) : 1;
+	}
+	h0 = pad[0] / 2 + 1;
+	const long long spec_count = static_cast<long long>(h0) * pad[1] * pad[2];
+	spectrum_bytes = static_cast<size_t>(spec_count) * sizeof(float2);
+	const long long row = h0;               // spectrum row stride
+	const long long plane = row * pad[1];   // spectrum plane stride
+	const int nx = n[0], ny = n[1], nz = rank == 3 ? n[2] : 1;
+	// R2C of image rows (y < ny, z < nz), zero beyond nx
+	setup_pass(r2c, PASS_R2C, pad[0], ny, nz);
+	r2c.load_n = nx;
+	r2c.store_n = h0;
+	r2c.gin = {1, nx, static_cast<long long>(nx) * ny};
+	r2c.gout = {1, row, plane};
+	// C2R of spectrum rows, only x < nx stored
+	setup_pass(c2r, PASS_C2R, pad[0], ny, nz);
+	c2r.load_n = h0;
+	c2r.store_n = nx;
+	c2r.gin = {1, row, plane};
+	c2r.gout = {1, nx, static_cast<long long>(nx) * ny};
+	// middle passes
+	const float scale = 1.0f;
+	auto mid = [&](int conj) {
+		std::vector<fft_pass_args> passes;
+		if (rank == 3) {
+			fft_pass_args yf{};
+			setup_pass(yf, PASS_C2C, pad[1], h0, nz);
+			yf.sign = -1.0f;
+			yf.load_n = ny;
+			yf.store_n = pad[1];
+			yf.gin = yf.gout = {row, 1, plane};
+			fft_pass_args zm{};
+			setup_pass(zm, PASS_C2C_MULT, pad[2], static_cast<int>(row * pad[1]), 1);
+			zm.load_n = nz;
+			zm.store_n = nz;
+			zm.gin = zm.gout = zm.gmult = {plane, 1, 0};
+			zm.mult = spectrum_h;
+			zm.mult_conj = conj;
+			zm.mult_scale = scale;
+			fft_pass_args yi{};
+			setup_pass(yi, PASS_C2C, pad[1], h0, nz);
+			yi.sign = 1.0f;
+			yi.load_n = pad[1];
+			yi.store_n = ny;
+			yi.gin = yi.gout = {row, 1, plane};
+			passes = {yf, zm, yi};
+		} else {
+			fft_pass_args ym{};
+			setup_pass(ym, PASS_C2C_MULT, pad[1], h0, 1);
+			ym.load_n = ny;
+			ym.store_n = ny;
+			ym.gin = ym.gout = ym.gmult = {row, 1, 0};
+			ym.mult = spectrum_h;
+			ym.mult_conj = conj;
+			ym.mult_scale = scale;
+			passes = {ym};
+		}
+		return passes;
+	};
+	// PSF spectrum: wrap the PSF into the padded domain, full R2C + forward passes
+	const long long real_count = static_cast<long long>(pad[0]) * pad[1] * pad[2];
+	float* wrapped = nullptr;
+	check_hip(hipMallocAsync(reinterpret_cast<void**>(&wrapped), real_count * sizeof(float), stream), "Allocate PSF buffer");
+	check_hip(hipMemsetAsync(wrapped, 0, real_count * sizeof(float), stream), "Clear PSF buffer");
+	check_hip(hipMalloc(&spectrum_h, spectrum_bytes), "Allocate PSF spectrum");
+	{
+		view4 kp = kernel_view(psf);
+		psf_wrap_kernel<<<point_blocks(static_cast<long long>(k[0]) * k[1] * k[2]), kPointThreads, 0, stream>>>(
+			kp, wrapped, pad[0], pad[1], pad[2], static_cast<float>(1.0 / static_cast<double>(real_count)));
+		check_hip(hipGetLastError(), "PSF wrap launch");
+		fft_pass_args full{};
+		setup_pass(full, PASS_R2C, pad[0], pad[1], pad[2]);
+		full.load_n = pad[0];
+		full.store_n = h0;
+		full.gin = {1, pad[0], static_cast<long long>(pad[0]) * pad[1]};
+		full.gout = {1, row, plane};
+		full.in = wrapped;
+		full.out = spectrum_h;
+		launch_pass(full, stream);
+		for (int axis = 1; axis < rank; ++axis) {
+			fft_pass_args c{};
+			if (axis == 1) {
+				setup_pass(c, PASS_C2C, pad[1], h0, pad[2]);
+				c.gin = c.gout = {row, 1, plane};
+			} else {
+				setup_pass(c, PASS_C2C, pad[2], static_cast<int>(row * pad[1]), 1);
+				c.gin = c.gout = {plane, 1, 0};
+			}
+			c.sign = -1.0f;
+			c.load_n = c.store_n = pad[axis];
+			c.in = c.out = spectrum_h;
+			launch_pass(c, stream);
+		}
+	}
+	check_hip(hipFreeAsync(wrapped, stream), "Free PSF buffer");
+	forward_mid = mid(0);
+	adjoint_mid = mid(1);
+	// row-kernel template: the spectrum rows of the image region
+	row_tpl.f = c2r;
+	row_tpl.f.gout = {1, row_tpl.f.gin.line, row_tpl.f.gin.outer};
+	row_tpl.f.store_n = h0;
+	row_tpl.real_line = nx;
+	row_tpl.real_outer = static_cast<long long>(nx) * ny;
+	row_tpl.nx = nx;
+	row_tpl.eps = eps;
+	partial_capacity = static_cast<int>(std::max<long long>(kPartialBlocks,
+		static_cast<long long>(row_tpl.f.groups_per_outer) * row_tpl.f.outer));
+	workspace = ((spectrum_bytes + 255) / 256) * 256 + static_cast<size_t>(partial_capacity) * sizeof(double);
+}
+
+} // namespace iane_hip
+
+using namespace iane_hip;
+
+struct iane_rl_plan {
+	std::unique_ptr<rl_plan_impl> impl;
+};
+
+namespace {
+
+unsigned row_blocks(const fft_pass_args& a)
+{
+	return static_cast<unsigned>(static_cast<long long>(a.groups_per_outer) * a.outer);
+}
+
+template<int EPI>
+void launch_row(const rl_row_args& a, hipStream_t s)
+{
+	rl_row_kernel<EPI><<<row_blocks(a.f), kFftThreads, 0, s>>>(a);
+	check_hip(hipGetLastError(), "RL row launch");
+}
+
+} // namespace
+
+extern "C" {
+
+int iane_hip_rl_initial(const float* aObserved, float* aEstimate, int64_t aCount, float aEps, void* aStream)
+{
+	return guarded([&] {
+		if (aCount < 0 || (aCount && (!aObserved || !aEstimate))) {
+			invalid("RL buffers are null");
+		}
+		if (aCount) {
+			rl_initial_kernel<<<point_blocks(aCount), kPointThreads, 0, static_cast<hipStream_t>(aStream)>>>(
+				aObserved, aEstimate, aCount, aEps);
+			check_hip(hipGetLastError(), "RL initial launch");
+		}
+	});
+}
+
+int iane_hip_rl_ratio(const float* aObserved, const float* aPredicted, float* aRatio, int64_t aCount, float aEps,
+	double* aLlAccum, void* aStream)
+{
+	return guarded([&] {
+		if (aCount < 0 || (aCount && (!aObserved || !aPredicted || !aRatio))) {
+			invalid("RL buffers are null");
+		}
+		if (aCount == 0) {
+			return;
+		}
+		auto stream = static_cast<hipStream_t>(aStream);
+		double* partials = nullptr;
+		const unsigned blocks = point_blocks(aCount);
+		if (aLlAccum) {
+			check_hip(hipMallocAsync(reinterpret_cast<void**>(&partials), blocks * sizeof(double), stream),
+				"Allocate partials");
+		}
+		rl_ratio_kernel<<<blocks, kPointThreads, 0, stream>>>(aObserved, aPredicted, aRatio, aCount, aEps, partials);
+		check_hip(hipGetLastError(), "RL ratio launch");
+		if (aLlAccum) {
+			ll_finalize_kernel<<<1, kPointThreads, 0, stream>>>(partials, blocks, aLlAccum);
+			check_hip(hipFreeAsync(partials, stream), "Free partials");
+		}
+	});
+}
+
+int iane_hip_rl_update(float* aEstimate, const float* aCorrection, const float* aSensitivity, int64_t aCount,
+	float aEps, void* aStream)
+{
+	return guarded([&] {
+		if (aCount < 0 || (aCount && (!aEstimate || !aCorrection || !aSensitivity))) {
+			invalid("RL buffers are null");
+		}
+		if (aCount) {
+			rl_update_kernel<<<point_blocks(aCount), kPointThreads, 0, static_cast<hipStream_t>(aStream)>>>(
+				aEstimate, aCorrection, aSensitivity, aCount, aEps);
+			check_hip(hipGetLastError(), "RL update launch");
+		}
+	});
+}
+
+int iane_hip_wiener_apply(void* aSpectrum, const void* aResponse, int64_t aCount, float aNoise, void* aStream)
+{
+	return guarded([&] {
+		if (aCount < 0 || (aCount && (!aSpectrum || !aResponse))) {
+			invalid("Wiener buffers are null");
+		}
+		if (!(aNoise > 0.0f) || !std::isfinite(aNoise)) {
+			invalid("Noise must be finite and positive");
+		}
+		if (aCount) {
+			wiener_kernel<<<point_blocks(aCount), kPointThreads, 0, static_cast<hipStream_t>(aStream)>>>(
+				static_cast<float2*>(aSpectrum), static_cast<const float2*>(aResponse), aCount, aNoise);
+			check_hip(hipGetLastError(), "Wiener launch");
+		}
+	});
+}
+
+int iane_hip_real_to_complex(const float* aIn, void* aOut, int64_t aCount, void* aStream)
+{
+	return guarded([&] {
+		if (aCount < 0 || (aCount && (!aIn || !aOut))) {
+			invalid("buffers are null");
+		}
+		if (aCount) {
+			real_to_complex_kernel<<<point_blocks(aCount), kPointThreads, 0, static_cast<hipStream_t>(aStream)>>>(
+				aIn, static_cast<float2*>(aOut), aCount);
+			check_hip(hipGetLastError(), "real to complex launch");
+		}
+	});
+}
+
+int iane_hip_complex_real_scale(const void* aIn, float* aOut, int64_t aCount, float aScale, void* aStream)
+{
+	return guarded([&] {
+		if (aCount < 0 || (aCount && (!aIn || !aOut))) {
+			invalid("buffers are null");
+		}
+		if (aCount) {
+			complex_real_scale_kernel<<<point_blocks(aCount), kPointThreads, 0, static_cast<hipStream_t>(aStream)>>>(
+				static_cast<const float2*>(aIn), aOut, aCount, aScale);
+			check_hip(hipGetLastError(), "complex real scale launch");
+		}
+	});
+}
+
+int iane_hip_rl_plan_create(iane_rl_plan** aPlan, int aRank, const int32_t* aExtent, const iane_view_desc* aPsf,
+	int aMethod, int aPrecision, float aEpsilon, void* aStream)
+{
+	return guarded([&] {
+		if (!aPlan || !aExtent) {
+			invalid("RL plan output or extent is null");
+		}
+		*aPlan = nullptr;
+		if (aRank != 2 && aRank != 3) {
+			fail(IANE_ERR_UNSUPPORTED, "RL plans support rank 2 and 3 images");
+		}
+		if (aMethod != IANE_RL_DIRECT && aMethod != IANE_RL_FFT) {
+			invalid("unknown RL method");
+		}
+		if (!(aEpsilon > 0.0f) || !std::isfinite(aEpsilon)) {
+			invalid("Iterations and finite epsilon must be positive");
+		}
+		const view4 psf = make_view4(aPsf, "psf");
+		if (psf.rank != aRank) {
+			invalid("Convolution requires a kernel and images of matching rank");
+		}
+		auto impl = std::make_unique<rl_plan_impl>();
+		impl->rank = aRank;
+		impl->method = aMethod;
+		impl->precision = aPrecision;
+		impl->eps = aEpsilon;
+		impl->count = 1;
+		for (int i = 0; i < aRank; ++i) {
+			if (aExtent[i] <= 0) {
+				invalid("RL requires positive image dimensions");
+			}
+			if (psf.extent[i] <= 0 || psf.extent[i] % 2 == 0) {
+				invalid("Convolution requires positive odd kernel dimensions");
+			}
+			impl->n[i] = aExtent[i];
+			impl->k[i] = psf.extent[i];
+			impl->count *= aExtent[i];
+		}
+		auto stream = static_cast<hipStream_t>(aStream);
+		const long long kcount = static_cast<long long>(impl->k[0]) * impl->k[1] * impl->k[2];
+		check_hip(hipMalloc(&impl->psf, kcount * sizeof(float)), "Allocate PSF");
+		check_hip(hipMalloc(&impl->flipped, kcount * sizeof(float)), "Allocate flipped PSF");
+		check_hip(hipMalloc(&impl->sensitivity, impl->count * sizeof(float)), "Allocate sensitivity");
+		{
+			// packed float copy of the PSF, then its flip (reverse of the packed buffer)
+			view4 packed = impl->kernel_view(impl->flipped);
+			flip_kernel<<<point_blocks(kcount), kPointThreads, 0, stream>>>(psf, impl->flipped);
+			flip_kernel<<<point_blocks(kcount), kPointThreads, 0, stream>>>(packed, impl->psf);
+			check_hip(hipGetLastError(), "PSF copy launch");
+		}
+		if (aMethod == IANE_RL_DIRECT) {
+			// s = H^T 1 (main.cu:62-63): the flipped PSF over an image of ones, zero border
+			float* ones = nullptr;
+			check_hip(hipMallocAsync(reinterpret_cast<void**>(&ones), impl->count * sizeof(float), stream), "Allocate ones");
+			fill_kernel<<<point_blocks(impl->count), kPointThreads, 0, stream>>>(ones, impl->count, 1.0f);
+			impl->convolve(ones, impl->flipped, impl->sensitivity, stream);
+			check_hip(hipFreeAsync(ones, stream), "Free ones");
+			impl->workspace = 2 * impl->count * sizeof(float) + static_cast<size_t>(kPartialBlocks + 64) * sizeof(double);
+		} else {
+			impl->build_fft(stream);
+			// s = H^T 1 through the same pruned passes
+			float* ones = nullptr;
+			float2* spec = nullptr;
+			check_hip(hipMallocAsync(reinterpret_cast<void**>(&ones), impl->count * sizeof(float), stream), "Allocate ones");
+			check_hip(hipMallocAsync(reinterpret_cast<void**>(&spec), impl->spectrum_bytes, stream), "Allocate spectrum");
+			fill_kernel<<<point_blocks(impl->count), kPointThreads, 0, stream>>>(ones, impl->count, 1.0f);
+			fft_pass_args a = impl->r2c;
+			a.in = ones;
+			a.out = spec;
+			launch_pass(a, stream);
+			impl->run_mid(impl->adjoint_mid, spec, stream);
+			fft_pass_args c = impl->c2r;
+			c.in = spec;
+			c.out = impl->sensitivity;
+			launch_pass(c, stream);
+			check_hip(hipFreeAsync(ones, stream), "Free ones");
+			check_hip(hipFreeAsync(spec, stream), "Free spectrum");
+		}
+		auto plan = std::make_unique<iane_rl_plan>();
+		plan->impl = std::move(impl);
+		*aPlan = plan.release();
+	});
+}
+
+int iane_hip_rl_plan_workspace_bytes(const iane_rl_plan* aPlan, size_t* aBytes)
+{
+	return guarded([&] {
+		if (!aPlan || !aBytes) {
+			invalid("RL plan or output is null");
+		}
+		*aBytes = aPlan->impl->workspace;
+	});
+}
+
+int iane_hip_rl_run(iane_rl_plan* aPlan, const float* aObserved, float* aEstimate, int aIterations, int aInit,
+	void* aWorkspace, double* aLlHistory, void* aStream)
+{
+	return guarded([&] {
+		if (!aPlan) {
+			invalid("RL plan is null");
+		}
+		if (!aObserved || !aEstimate) {
+			invalid("RL buffers are null");
+		}
+		if (aIterations < 0) {
+			invalid("Iterations and finite epsilon must be positive");
+		}
+		const rl_plan_impl& p = *aPlan->impl;
+		if (!aWorkspace) {
+			fail(IANE_ERR_LENGTH, "Prepared operation exceeded its scratch reservation");
+		}
+		auto stream = static_cast<hipStream_t>(aStream);
+		const long long n = p.count;
+		if (aInit) {
+			rl_initial_kernel<<<point_blocks(n), kPointThreads, 0, stream>>>(aObserved, aEstimate, n, p.eps);
+			check_hip(hipGetLastError(), "RL initial launch");
+		}
+		if (aIterations == 0) {
+			return;
+		}
+		char* ws = static_cast<char*>(aWorkspace);
+		if (p.method == IANE_RL_DIRECT) {
+			float* predicted = reinterpret_cast<float*>(ws);
+			float* correction = predicted + n;
+			double* partials = reinterpret_cast<double*>(ws + ((2 * n * sizeof(float) + 255) / 256) * 256);
+			const unsigned blocks = point_blocks(n);
+			for (int it = 0; it < aIterations; ++it) {
+				p.convolve(aEstimate, p.psf, predicted, stream);
+				rl_ratio_kernel<<<blocks, kPointThreads, 0, stream>>>(aObserved, predicted, predicted, n, p.eps,
+					aLlHistory ? partials : nullptr);
+				if (aLlHistory) {
+					ll_finalize_kernel<<<1, kPointThreads, 0, stream>>>(partials, blocks, aLlHistory + it);
+				}
+				p.convolve(predicted, p.flipped, correction, stream);
+				rl_update_kernel<<<blocks, kPointThreads, 0, stream>>>(aEstimate, correction, p.sensitivity, n, p.eps);
+			}
+			check_hip(hipGetLastError(), "RL iteration launch");
+			return;
+		}
+		float2* spec = reinterpret_cast<float2*>(ws);
+		double* partials = reinterpret_cast<double*>(ws + ((p.spectrum_bytes + 255) / 256) * 256);
+		// spectrum of the current estimate
+		fft_pass_args a = p.r2c;
+		a.in = aEstimate;
+		a.out = spec;
+		launch_pass(a, stream);
+		p.run_mid(p.forward_mid, spec, stream);
+		rl_row_args ratio = p.row_tpl;
+		ratio.f.in = spec;
+		ratio.f.out = spec;
+		ratio.y = aObserved;
+		ratio.forward_next = 1;
+		rl_row_args update = ratio;
+		update.x = aEstimate;
+		update.s = p.sensitivity;
+		const unsigned rb = row_blocks(ratio.f);
+		for (int it = 0; it < aIterations; ++it) {
+			ratio.partials = aLlHistory ? partials : nullptr;
+			launch_row<EPI_RATIO>(ratio, stream);
+			if (aLlHistory) {
+				ll_finalize_kernel<<<1, kPointThreads, 0, stream>>>(partials, rb, aLlHistory + it);
+			}
+			p.run_mid(p.adjoint_mid, spec, stream);
+			update.forward_next = it + 1 < aIterations;
+			launch_row<EPI_UPDATE>(update, stream);
+			if (it + 1 < aIterations) {
+				p.run_mid(p.forward_mid, spec, stream);
+			}
+		}
+	});
+}
+
+int iane_hip_rl_plan_destroy(iane_rl_plan* aPlan)
+{
+	return guarded([&] { delete aPlan; });
+}
+
+} // extern "C"

@@ -22,9 +22,12 @@ _W0, _W1 = np.uint32(0x9E3779B9), np.uint32(0xBB67AE85)
 _MASK = np.uint64(0xFFFFFFFF)
 
 
-def philox4x32(seed: int, stream: int, n_blocks: int, offset: int = 0) -> np.ndarray:
-    """``n_blocks`` Philox4x32-10 outputs (shape (n_blocks, 4), uint32)."""
-    idx = np.arange(offset, offset + n_blocks, dtype=np.uint64)
+def philox4x32(seed: int, stream: int, n_blocks: int, offset: int = 0, counters=None) -> np.ndarray:
+    """``n_blocks`` Philox4x32-10 outputs (shape (n_blocks, 4), uint32) for the
+    counters offset .. offset + n_blocks - 1, or for the given ``counters``."""
+    idx = np.arange(offset, offset + n_blocks, dtype=np.uint64) if counters is None else \
+        np.asarray(counters, dtype=np.uint64)
+    n_blocks = idx.size
     c0 = (idx & _MASK).astype(np.uint32)
     c1 = (idx >> np.uint64(32)).astype(np.uint32)
     c2 = np.zeros(n_blocks, dtype=np.uint32)
@@ -99,24 +102,25 @@ def poisson(lam: np.ndarray, seed: int, stream: int) -> np.ndarray:
         inv_alpha = 1.1239 + 1.1328 / (b - 3.4)
         v_r = 0.9277 - 3.6224 / (b - 2)
         result = np.full(m, -1.0)
-        pending = np.ones(m, dtype=bool)
         attempt = 0
-        from math import lgamma
-        lg = np.vectorize(lgamma)
-        while pending.any():
-            blocks = philox4x32(seed, stream + 1, m, offset=attempt * m)
+        from scipy.special import gammaln as lg
+        idx = np.arange(m)
+        while idx.size:
+            # sample i, attempt t uses counter block t*m + i (only pending samples are drawn)
+            blocks = philox4x32(seed, stream + 1, 0, counters=attempt * m + idx)
             u = (blocks[:, 0] >> np.uint32(8)).astype(np.float64) / 16777216.0 - 0.5
             v = (blocks[:, 1] >> np.uint32(8)).astype(np.float64) / 16777216.0
             us = 0.5 - np.abs(u)
+            A, B, IA, VR, L, LL = a[idx], b[idx], inv_alpha[idx], v_r[idx], lb[idx], loglam[idx]
             with np.errstate(divide="ignore", invalid="ignore"):
-                k = np.floor((2 * a / us + b) * u + lb + 0.43)
-                quick = (us >= 0.07) & (v <= v_r)
+                k = np.floor((2 * A / us + B) * u + L + 0.43)
+                quick = (us >= 0.07) & (v <= VR)
                 ok_range = ~((k < 0) | ((us < 0.013) & (v > us)))
-                lhs = np.log(v * inv_alpha / (a / (us * us) + b))
-                rhs = -lb + k * loglam - lg(np.maximum(k, 0) + 1)
-                accept = pending & (quick | (ok_range & (lhs <= rhs)))
-            result[accept] = k[accept]
-            pending &= ~accept
+                lhs = np.log(v * IA / (A / (us * us) + B))
+                rhs = -L + k * LL - lg(np.maximum(k, 0) + 1)
+                accept = quick | (ok_range & (lhs <= rhs))
+            result[idx[accept]] = k[accept]
+            idx = idx[~accept]
             attempt += 1
             if attempt > 200:
                 raise RuntimeError("PTRS did not converge")

@@ -138,6 +138,48 @@ int iane_hip_fft_plan_workspace_bytes(const iane_fft_plan* plan, size_t* bytes);
 int iane_hip_fft_exec(iane_fft_plan* plan, const void* in, void* out, void* workspace, void* stream);
 int iane_hip_fft_plan_destroy(iane_fft_plan* plan);
 
+/* ---- Richardson–Lucy / Wiener pointwise steps: the functors of
+ * examples/rl_deconvolution/main.cu:11-49 and examples/wiener/main.cu:21-32
+ * with the same float arithmetic. count = elements; packed buffers.       */
+/* estimate = observed > eps ? observed : eps                  (main.cu:12-19) */
+int iane_hip_rl_initial(const float* observed, float* estimate, int64_t count, float epsilon, void* stream);
+/* ratio = observed / max(predicted, eps) (main.cu:22-29); ratio may alias predicted.
+ * ll_out != NULL: *ll_out (device double) = sum(y log max(p, eps) - p), the
+ * Poisson log-likelihood without log(y!) (no reference counterpart).        */
+int iane_hip_rl_ratio(const float* observed, const float* predicted, float* ratio, int64_t count,
+	float epsilon, double* ll_out, void* stream);
+/* estimate *= (sens > 0 ? corr / max(sens, eps) : 0)           (main.cu:32-49,74-77) */
+int iane_hip_rl_update(float* estimate, const float* correction, const float* sensitivity,
+	int64_t count, float epsilon, void* stream);
+/* spectrum = spectrum * conj(H) / (|H|^2 + noise), factor in double (wiener/main.cu:21-32);
+ * spectrum/response: IANE_C64 buffers of count elements. */
+int iane_hip_wiener_apply(void* spectrum, const void* response, int64_t count, float noise, void* stream);
+/* out = (in, 0) as IANE_C64 (wiener/main.cu:13-19, real_sample) */
+int iane_hip_real_to_complex(const float* in, void* out, int64_t count, void* stream);
+/* out = real(in) * scale (wiener/main.cu:35-41, the normalisation of encode_sample) */
+int iane_hip_complex_real_scale(const void* in, float* out, int64_t count, float scale, void* stream);
+
+/* ---- Richardson–Lucy prepared algorithm: prepare_rl + its closure,
+ * examples/rl_deconvolution/main.cu:51-81, rank 2 or 3, FP32 images, zero border.
+ * IANE_RL_DIRECT: direct convolutions with the PSF and its flip (main.cu:106-107)
+ *   at `precision`; REFERENCE precision is bit-identical to the reference loop.
+ * IANE_RL_FFT: convolutions on a zero-padded domain (n >= N + K/2 per axis,
+ *   equal to the zero-border convolution up to FP32 rounding).
+ * The plan owns the PSF, its flip / spectrum and the sensitivity H^T 1
+ * (main.cu:62-63); the per-run temporaries live in the caller's workspace.  */
+enum { IANE_RL_DIRECT = 0, IANE_RL_FFT = 1 };
+typedef struct iane_rl_plan iane_rl_plan;
+/* extent: image extents (axis 0 fastest); psf: rank-matching view, odd extents */
+int iane_hip_rl_plan_create(iane_rl_plan** plan, int rank, const int32_t* extent,
+	const iane_view_desc* psf, int method, int precision, float epsilon, void* stream);
+int iane_hip_rl_plan_workspace_bytes(const iane_rl_plan* plan, size_t* bytes);
+/* init != 0: estimate = max(observed, eps) first (main.cu:68); otherwise the
+ * estimate buffer is the warm start. ll_history (device, `iterations` doubles,
+ * may be NULL): log-likelihood of each iteration's prediction H x_i.        */
+int iane_hip_rl_run(iane_rl_plan* plan, const float* observed, float* estimate, int iterations,
+	int init, void* workspace, double* ll_history, void* stream);
+int iane_hip_rl_plan_destroy(iane_rl_plan* plan);
+
 /* Test hooks: the untiled one-thread-per-output kernel and the per-axis
  * separable passes, so tests can compare every path against the oracle. */
 int iane_hip_convolve_untiled(const iane_view_desc* src, const iane_view_desc* kernel,

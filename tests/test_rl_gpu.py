@@ -1,0 +1,166 @@
+"""GPU parity of the Richardson–Lucy and Wiener loops.
+
+* DIRECT method, REFERENCE precision: bit-identical to the reference RL loop
+  (golden fixtures written by the reference's own convolve/zip/copy).
+* FFT method: against the FP64 oracle (numpy FFT restatement of the loop,
+  itself pinned in tests/test_oracle_fft.py); tolerance on the estimate
+      max|gpu - oracle64| <= TOL_RL * max|oracle64|   (TOL_RL = 1e-4 after <= 10 iterations)
+  and on the log-likelihood |ll_gpu - ll64| <= 1e-6 |ll64|.
+* Pointwise steps: bit-identical to the same float32 operations on the host.
+* Full size (256x256x128, PSF 31x31x63): FFT RL vs the FP64 oracle for 3 iterations,
+  and the EM property (non-decreasing log-likelihood) over 20 iterations.
+"""
+from __future__ import annotations
+
+import pathlib
+
+import numpy as np
+import pytest
+
+from iane_amd import deconvolution, precision, synth
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+GOLDEN = pathlib.Path(__file__).resolve().parent / "golden"
+TOL_RL = 1e-4
+
+
+def dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def host(t):
+    return t.cpu().numpy()
+
+
+def rel(got, ref):
+    return float(np.max(np.abs(got.astype(np.float64) - ref)) / max(np.max(np.abs(ref)), 1e-30))
+
+
+@pytest.mark.parametrize("path", sorted(GOLDEN.glob("rl_*.npz")), ids=lambda p: p.stem)
+def test_direct_reference_precision_bit_exact(path):
+    data = np.load(path, allow_pickle=False)
+    est = deconvolution.richardson_lucy(dev(data["observed"]), dev(data["psf"]), int(data["iterations"]),
+                                        float(data["epsilon"]), method=deconvolution.DIRECT,
+                                        precision=precision.reference)
+    np.testing.assert_array_equal(host(est).view(np.uint32), data["expected"].view(np.uint32))
+
+
+def _case(shape, psf_sigma, psf_shape, peak=500.0, seed=3):
+    truth = np.full(shape, 0.05)
+    synth.add_blobs(truth, 6, (1.0, 3.0), (0.3, 1.0), seed, 1)
+    psf = synth.gaussian_kernel(psf_sigma, psf_shape)
+    blurred = oracle.conv_zero_fft(truth * peak, psf)
+    y = synth.poisson(np.maximum(blurred, 0), seed, 2).reshape(shape).astype(np.float32)
+    return y, psf.astype(np.float32)
+
+
+@pytest.mark.parametrize("shape,psf_shape", [((40, 50), (7, 9)), ((33, 21), (5, 15)), ((20, 24, 28), (5, 7, 9)),
+                                             ((17, 30, 26), (11, 5, 3))])
+def test_fft_method_vs_fp64_oracle(shape, psf_shape):
+    y, psf = _case(shape, [1.2] * len(shape), psf_shape)
+    iters = 10
+    est, ll = deconvolution.richardson_lucy(dev(y), dev(psf), iters, method=deconvolution.FFT, return_ll=True)
+    ref, ll_ref = oracle.rl(y, psf.astype(np.float64), iters)
+    assert rel(host(est), ref) <= TOL_RL
+    np.testing.assert_allclose(host(ll), ll_ref, rtol=1e-6)
+
+
+@pytest.mark.parametrize("shape,psf_shape", [((30, 34), (7, 5)), ((12, 16, 20), (3, 5, 7))])
+def test_direct_fp32_and_fft_agree(shape, psf_shape):
+    y, psf = _case(shape, [1.0] * len(shape), psf_shape)
+    a = deconvolution.richardson_lucy(dev(y), dev(psf), 8, method=deconvolution.DIRECT, precision=precision.fp32)
+    b = deconvolution.richardson_lucy(dev(y), dev(psf), 8, method=deconvolution.FFT)
+    ref, _ = oracle.rl(y, psf.astype(np.float64), 8)
+    assert rel(host(a), ref) <= TOL_RL
+    assert rel(host(b), ref) <= TOL_RL
+
+
+def test_warm_start_continues_the_iteration():
+    import torch
+    y, psf = _case((24, 28), (1.3, 1.1), (7, 7))
+    plan = deconvolution.rl_plan(y.shape, dev(psf), deconvolution.FFT)
+    full = torch.empty_like(dev(y))
+    plan.run(dev(y), full, 6).wait()
+    part = torch.empty_like(full)
+    plan.run(dev(y), part, 4).wait()
+    plan.run(dev(y), part, 2, init=False).wait()
+    np.testing.assert_array_equal(host(full), host(part))
+
+
+def test_pointwise_steps_match_float32_semantics():
+    import torch
+    rng = np.random.default_rng(0)
+    n = 10007
+    y = rng.poisson(5.0, n).astype(np.float32)
+    p = (rng.random(n) * 10).astype(np.float32)
+    p[:50] = 0.0
+    s = rng.random(n).astype(np.float32)
+    s[:30] = 0.0
+    s[30:60] = 1e-9
+    c = rng.random(n).astype(np.float32)
+    x = (rng.random(n) + 0.5).astype(np.float32)
+    eps = np.float32(1e-6)
+    dy, dp, ds, dc, dx = map(dev, (y, p, s, c, x))
+    ratio = torch.empty_like(dp)
+    ll = torch.zeros(1, dtype=torch.float64, device="cuda")
+    deconvolution.rl_ratio(dy, dp, ratio, 1e-6, ll).wait()
+    floor = np.where(p > eps, p, eps)
+    np.testing.assert_array_equal(host(ratio), y / floor)
+    ll_ref = np.sum(y.astype(np.float64) * np.log(floor.astype(np.float64)) - p.astype(np.float64))
+    assert abs(host(ll)[0] - ll_ref) <= 1e-9 * abs(ll_ref)
+    deconvolution.rl_update(dx, dc, ds, 1e-6).wait()
+    factor = np.where(s > 0, c / np.where(s > eps, s, eps), np.float32(0)).astype(np.float32)
+    np.testing.assert_array_equal(host(dx), x * factor)
+    init = torch.empty_like(dy)
+    y2 = y.copy()
+    y2[:10] = 0
+    deconvolution.rl_initial(dev(y2), init, 1e-6).wait()
+    np.testing.assert_array_equal(host(init), np.where(y2 > eps, y2, eps))
+
+
+def test_errors():
+    import iane_amd
+    y, psf = _case((16, 16), (1.0, 1.0), (5, 5))
+    with pytest.raises(iane_amd.invalid_argument, match="odd"):
+        deconvolution.rl_plan(y.shape, dev(np.ones((4, 5), np.float32)))
+    with pytest.raises(iane_amd.invalid_argument, match="epsilon"):
+        deconvolution.rl_plan(y.shape, dev(psf), epsilon=0.0)
+    plan = deconvolution.rl_plan(y.shape, dev(psf))
+    with pytest.raises(iane_amd.invalid_argument, match="shape"):
+        plan.run(dev(np.zeros((16, 17), np.float32)), dev(np.zeros((16, 17), np.float32)), 1)
+
+
+# ------------------------------------------------------------ full size ----
+def test_full_size_fft_rl_vs_oracle_and_em_property():
+    """256x256x128 volume, PSF 31x31x63 (sigma 1.5, 1.5, 3.0), peak 1000 photons."""
+    import torch
+    truth, psf, y = synth.rl_volume_data()
+    dy, dpsf = dev(y), dev(psf)
+    plan = deconvolution.rl_plan(y.shape, dpsf, deconvolution.FFT)
+    est = torch.empty_like(dy)
+    ll = torch.zeros(20, dtype=torch.float64, device="cuda")
+    plan.run(dy, est, 2, ll_history=ll).wait()
+    ref, ll_ref = oracle.rl(y, psf.astype(np.float64), 2)
+    assert rel(host(est), ref) <= TOL_RL
+    np.testing.assert_allclose(host(ll)[:2], ll_ref, rtol=1e-6)
+    plan.run(dy, est, 20, ll_history=ll).wait()
+    history = host(ll)
+    assert np.all(np.diff(history) >= -1e-7 * np.abs(history[:-1]))
+
+
+# -------------------------------------------------------------- Wiener -----
+def test_wiener_matches_oracle_and_delta_identity():
+    rng = np.random.default_rng(2)
+    img = (rng.random((48, 40)) * 200).astype(np.float32)
+    psf = synth.gaussian_kernel((1.5, 1.2), (9, 7))
+    padded = oracle.pad_psf_centered(psf, img.shape).astype(np.complex64)
+    got = host(deconvolution.wiener(dev(img), dev(padded), 0.002))
+    ref = oracle.wiener(img, padded.astype(np.complex128), 0.002)
+    assert rel(got, ref) <= 1e-5
+    delta = np.zeros((3, 3))
+    delta[1, 1] = 1
+    ident = host(deconvolution.wiener(dev(img), dev(oracle.pad_psf_centered(delta, img.shape).astype(np.complex64)),
+                                      1e-9))
+    np.testing.assert_allclose(ident, img, rtol=1e-5, atol=1e-3)
