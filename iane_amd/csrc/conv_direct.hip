@@ -264,7 +264,7 @@ void validate_convolution(const view4& aSrc, const view4& aKer, const view4& aDs
 	}
 	for (int i = 0; i < 4; ++i) {
 		if (aSrc.extent[i] != aDst.extent[i]) {
-			invalid("Image dimensions must match");
+			invalid("Source and destination require matching nonnegative dimensions");
 		}
 	}
 	if (aSrc.type == IANE_C64 || aKer.type == IANE_C64 || aDst.type == IANE_C64) {
