@@ -125,6 +125,34 @@ def bench_rl(timer, stream, iterations: int = 200):
             "padded_domain": "280x280x160 (x, y, z) R2C", "workspace_bytes": plan.workspace_bytes}
 
 
+def bench_rl_batch(timer, stream, iterations: int = 100):
+    """64 RL problems (edges U{32..192}, own PSFs, seed 4), 100 iterations:
+    one hipGraph launch of the batched engine vs the same volumes run one plan at a time."""
+    import torch
+    from iane_amd import deconvolution, hip_policy, synth
+    log("[bench] RL batch: generating 64 volumes (seed 4)")
+    data = synth.rl_batch_data(count=64, seed=4)
+    policy = hip_policy(stream)
+    obs = [torch.from_numpy(o).cuda() for o, _ in data]
+    psfs = [torch.from_numpy(p).cuda() for _, p in data]
+    est = [torch.empty_like(o) for o in obs]
+    t0 = time.perf_counter()
+    batch = deconvolution.rl_batch([o.shape for o in obs], psfs, policy=policy)
+    prepare_s = time.perf_counter() - t0
+    batch.run(obs, est, iterations, use_graph=True).wait()  # capture + first replay
+    graph_ms = timer.time(lambda: batch.run(obs, est, iterations, use_graph=True), 1)
+    eager_ms = timer.time(lambda: batch.run(obs, est, iterations, use_graph=False), 1)
+    plans = [deconvolution.rl_plan(o.shape, p, deconvolution.FFT, policy=policy) for o, p in zip(obs, psfs)]
+    plans[0].run(obs[0], est[0], 1).wait()
+    single_ms = timer.time(lambda: [pl.run(o, e, iterations) for pl, o, e in zip(plans, obs, est)], 1)
+    voxels = sum(o.numel() for o in obs)
+    log(f"[bench] RL batch: graph {graph_ms:.1f} ms, batched eager {eager_ms:.1f} ms, one plan at a time {single_ms:.1f} ms")
+    return {"volumes": len(obs), "voxels": voxels, "iterations": iterations, "graph_ms": round(graph_ms, 2),
+            "batched_eager_ms": round(eager_ms, 2), "per_volume_plans_ms": round(single_ms, 2),
+            "speedup_vs_per_volume": round(single_ms / graph_ms, 2),
+            "Mvoxel_iter_s": round(voxels * iterations / graph_ms / 1e3, 1), "prepare_s": round(prepare_s, 2)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -218,6 +246,7 @@ def main() -> int:
 
     if not args.quick:
         extra["rl_fft_256x256x128_psf31x31x63"] = bench_rl(timer, stream)
+        extra["rl_batch_64_volumes"] = bench_rl_batch(timer, stream)
 
     base = None
     if not args.no_cpu:

@@ -102,6 +102,11 @@ SIGNATURES = {
     "iane_hip_rl_plan_workspace_bytes": [vp, ctypes.POINTER(sz)],
     "iane_hip_rl_run": [vp, vp, vp, ctypes.c_int, ctypes.c_int, vp, vp, vp],
     "iane_hip_rl_plan_destroy": [vp],
+    "iane_hip_rl_batch_create": [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int, ctypes.POINTER(i32), desc_p, f32, vp],
+    "iane_hip_rl_batch_workspace_bytes": [vp, ctypes.POINTER(sz)],
+    "iane_hip_rl_batch_run": [vp, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int, vp, vp,
+                              ctypes.c_int, vp],
+    "iane_hip_rl_batch_destroy": [vp],
 }
 
 _lib = None

@@ -399,4 +399,93 @@ __device__ inline void c2r_extend_odd(float2* x, int count, int n, int pitch)
 	}
 }
 
+__device__ inline void store_real_lines(float* out, const line_geom& g, int aOuter, int aLine0, int count, int n,
+	int store_n, const float2* buf, int pitch)
+{
+	const long long base = aOuter * g.outer + aLine0 * g.line;
+	const bool even = (n & 1) == 0;
+	const int total = count * store_n;
+	for (int t = threadIdx.x; t < total; t += blockDim.x) {
+		int w, i;
+		if (g.elem == 1) {
+			w = t / store_n;
+			i = t - w * store_n;
+		} else {
+			i = t / count;
+			w = t - i * count;
+		}
+		const float2 v = buf[w * pitch + (even ? i / 2 : i)];
+		out[base + w * g.line + i * g.elem] = (even && (i & 1)) ? v.y : v.x;
+	}
+}
+
+// One workgroup's share of a line pass: the lines of group `aBlock` of `a`.
+// `lds` holds two buffers of kFftLdsElems complex elements.
+template<int MODE>
+__device__ inline void fft_pass_body(const fft_pass_args& a, unsigned aBlock, float2* lds)
+{
+	float2* A = lds;
+	float2* B = lds + kFftLdsElems;
+	int o, l0, count;
+	decode_group(a, aBlock, o, l0, count);
+	const int m = a.plan.m;
+	const int n = a.n;
+	if constexpr (MODE == PASS_C2C) {
+		load_complex_lines(static_cast<const float2*>(a.in), a.gin, o, l0, count, a.load_n, m, A, a.pitch);
+		__syncthreads();
+		const float2* r = run_stages(a.plan, A, B, count, a.pitch, a.tw_m, a.sign);
+		store_complex_lines(static_cast<float2*>(a.out), a.gout, o, l0, count, a.store_n, r, a.pitch);
+	} else if constexpr (MODE == PASS_C2C_MULT) {
+		load_complex_lines(static_cast<const float2*>(a.in), a.gin, o, l0, count, a.load_n, m, A, a.pitch);
+		__syncthreads();
+		float2* r = run_stages(a.plan, A, B, count, a.pitch, a.tw_m, -1.0f);
+		const long long hbase = o * a.gmult.outer + l0 * a.gmult.line;
+		for (int t = threadIdx.x; t < count * m; t += blockDim.x) {
+			int w, k;
+			if (a.gmult.elem == 1) {
+				w = t / m;
+				k = t - w * m;
+			} else {
+				k = t / count;
+				w = t - k * count;
+			}
+			float2 h = a.mult[hbase + w * a.gmult.line + k * a.gmult.elem];
+			if (a.mult_conj) {
+				h = cconj(h);
+			}
+			r[w * a.pitch + k] = cscale(cmul(r[w * a.pitch + k], h), a.mult_scale);
+		}
+		__syncthreads();
+		const float2* q = run_stages(a.plan, r, r == A ? B : A, count, a.pitch, a.tw_m, 1.0f);
+		store_complex_lines(static_cast<float2*>(a.out), a.gout, o, l0, count, a.store_n, q, a.pitch);
+	} else if constexpr (MODE == PASS_R2C) {
+		load_real_lines(static_cast<const float*>(a.in), a.gin, o, l0, count, n, a.load_n, A, a.pitch);
+		__syncthreads();
+		float2* r = run_stages(a.plan, A, B, count, a.pitch, a.tw_m, -1.0f);
+		if ((n & 1) == 0) {
+			float2* x = r == A ? B : A;
+			r2c_post(r, x, count, n, a.pitch, a.tw_n);
+			__syncthreads();
+			r = x;
+		}
+		store_complex_lines(static_cast<float2*>(a.out), a.gout, o, l0, count, a.store_n, r, a.pitch);
+	} else { // PASS_C2R
+		if ((n & 1) == 0) {
+			load_complex_lines(static_cast<const float2*>(a.in), a.gin, o, l0, count, a.load_n, m + 1, A, a.pitch);
+			__syncthreads();
+			c2r_pre(A, B, count, n, a.pitch, a.tw_n);
+			__syncthreads();
+			const float2* r = run_stages(a.plan, B, A, count, a.pitch, a.tw_m, 1.0f);
+			store_real_lines(static_cast<float*>(a.out), a.gout, o, l0, count, n, a.store_n, r, a.pitch);
+		} else {
+			load_complex_lines(static_cast<const float2*>(a.in), a.gin, o, l0, count, a.load_n, m, A, a.pitch);
+			__syncthreads();
+			c2r_extend_odd(A, count, n, a.pitch);
+			__syncthreads();
+			const float2* r = run_stages(a.plan, A, B, count, a.pitch, a.tw_m, 1.0f);
+			store_real_lines(static_cast<float*>(a.out), a.gout, o, l0, count, n, a.store_n, r, a.pitch);
+		}
+	}
+}
+
 } // namespace iane_hip

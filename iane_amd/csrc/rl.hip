@@ -171,16 +171,16 @@ struct rl_row_args {
 
 // C2R of `count` spectrum rows, the RL pointwise step on the real row, R2C of
 // the result back into the same spectrum rows.
+// One workgroup's rows (group `aBlock` of `a`); the log-likelihood partial of
+// the group goes to a.partials[aSlot].
 template<int EPI>
-__global__ __launch_bounds__(kFftThreads) void rl_row_kernel(rl_row_args a)
+__device__ inline void rl_row_body(const rl_row_args& a, unsigned aBlock, unsigned aSlot, float2* lds, double* shared)
 {
-	__shared__ float2 lds[2 * kFftLdsElems];
-	__shared__ double shared[kFftThreads / 64];
 	float2* A = lds;
 	float2* B = lds + kFftLdsElems;
 	const fft_pass_args& f = a.f;
 	int o, l0, count;
-	decode_group(f, xcd_remap(blockIdx.x, gridDim.x), o, l0, count);
+	decode_group(f, aBlock, o, l0, count);
 	const int n = f.n;
 	const int m = n / 2;
 	load_complex_lines(static_cast<const float2*>(f.in), f.gin, o, l0, count, f.load_n, m + 1, A, f.pitch);
@@ -221,7 +221,7 @@ __global__ __launch_bounds__(kFftThreads) void rl_row_kernel(rl_row_args a)
 		if (a.partials) {
 			const double total = block_sum(ll, shared);
 			if (threadIdx.x == 0) {
-				a.partials[blockIdx.x] = total;
+				a.partials[aSlot] = total;
 			}
 		}
 	}
@@ -234,6 +234,80 @@ __global__ __launch_bounds__(kFftThreads) void rl_row_kernel(rl_row_args a)
 	r2c_post(z, xb, count, n, f.pitch, f.tw_n);
 	__syncthreads();
 	store_complex_lines(static_cast<float2*>(f.out), f.gout, o, l0, count, f.store_n, xb, f.pitch);
+}
+
+template<int EPI>
+__global__ __launch_bounds__(kFftThreads) void rl_row_kernel(rl_row_args a)
+{
+	__shared__ float2 lds[2 * kFftLdsElems];
+	__shared__ double shared[kFftThreads / 64];
+	rl_row_body<EPI>(a, xcd_remap(blockIdx.x, gridDim.x), blockIdx.x, lds, shared);
+}
+
+// ---------------------------------------------------- batched (many volumes) --
+// One launch covers the line groups of every volume: block b belongs to the
+// volume v with offsets[v] <= b < offsets[v + 1] (offsets are prefix sums).
+__device__ __forceinline__ int volume_of(const unsigned* offsets, int count, unsigned b)
+{
+	int lo = 0, hi = count;
+	while (hi - lo > 1) {
+		const int mid = (lo + hi) >> 1;
+		if (offsets[mid] <= b) {
+			lo = mid;
+		} else {
+			hi = mid;
+		}
+	}
+	return lo;
+}
+
+template<int MODE>
+__global__ __launch_bounds__(kFftThreads) void fft_pass_batched_kernel(const fft_pass_args* args, const unsigned* offsets,
+	int count)
+{
+	__shared__ float2 lds[2 * kFftLdsElems];
+	const unsigned b = xcd_remap(blockIdx.x, gridDim.x);
+	const int v = volume_of(offsets, count, b);
+	fft_pass_body<MODE>(args[v], b - offsets[v], lds);
+}
+
+template<int EPI>
+__global__ __launch_bounds__(kFftThreads) void rl_row_batched_kernel(const rl_row_args* args, const unsigned* offsets,
+	int count)
+{
+	__shared__ float2 lds[2 * kFftLdsElems];
+	__shared__ double shared[kFftThreads / 64];
+	const unsigned b = xcd_remap(blockIdx.x, gridDim.x);
+	const int v = volume_of(offsets, count, b);
+	rl_row_body<EPI>(args[v], b - offsets[v], b, lds, shared);
+}
+
+// log-likelihood of volume v (partials offsets[v] .. offsets[v+1]-1) into out[v * stride]
+__global__ void ll_finalize_batched_kernel(const double* partials, const unsigned* offsets, double* out, int stride)
+{
+	__shared__ double shared[kPointThreads / 64];
+	const int v = blockIdx.x;
+	double s = 0.0;
+	for (unsigned i = offsets[v] + threadIdx.x; i < offsets[v + 1]; i += blockDim.x) {
+		s += partials[i];
+	}
+	const double total = block_sum(s, shared);
+	if (threadIdx.x == 0) {
+		out[static_cast<long long>(v) * stride] = total;
+	}
+}
+
+__global__ void rl_initial_batched_kernel(const float* const* y, float* const* x, const long long* counts, float eps)
+{
+	const int v = blockIdx.y;
+	const long long n = counts[v];
+	const float* yv = y[v];
+	float* xv = x[v];
+	for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
+		 i += static_cast<long long>(gridDim.x) * blockDim.x) {
+		const float o = yv[i];
+		xv[i] = o > eps ? o : eps;
+	}
 }
 
 // place the PSF with its centre at the origin of the padded real domain (wrap-around)
@@ -489,6 +563,81 @@ struct iane_rl_plan {
 	std::unique_ptr<rl_plan_impl> impl;
 };
 
+// Validate and prepare one RL problem (shared by single plans and batches).
+std::unique_ptr<rl_plan_impl> make_rl_plan(int aRank, const int32_t* aExtent, const view4& psf, int aMethod,
+	int aPrecision, float aEpsilon, hipStream_t stream)
+{
+	if (aRank != 2 && aRank != 3) {
+		fail(IANE_ERR_UNSUPPORTED, "RL plans support rank 2 and 3 images");
+	}
+	if (aMethod != IANE_RL_DIRECT && aMethod != IANE_RL_FFT) {
+		invalid("unknown RL method");
+	}
+	if (!(aEpsilon > 0.0f) || !std::isfinite(aEpsilon)) {
+		invalid("Iterations and finite epsilon must be positive");
+	}
+	if (psf.rank != aRank) {
+		invalid("Convolution requires a kernel and images of matching rank");
+	}
+	auto impl = std::make_unique<rl_plan_impl>();
+	impl->rank = aRank;
+	impl->method = aMethod;
+	impl->precision = aPrecision;
+	impl->eps = aEpsilon;
+	impl->count = 1;
+	for (int i = 0; i < aRank; ++i) {
+		if (aExtent[i] <= 0) {
+			invalid("RL requires positive image dimensions");
+		}
+		if (psf.extent[i] <= 0 || psf.extent[i] % 2 == 0) {
+			invalid("Convolution requires positive odd kernel dimensions");
+		}
+		impl->n[i] = aExtent[i];
+		impl->k[i] = psf.extent[i];
+		impl->count *= aExtent[i];
+	}
+	const long long kcount = static_cast<long long>(impl->k[0]) * impl->k[1] * impl->k[2];
+	check_hip(hipMalloc(&impl->psf, kcount * sizeof(float)), "Allocate PSF");
+	check_hip(hipMalloc(&impl->flipped, kcount * sizeof(float)), "Allocate flipped PSF");
+	check_hip(hipMalloc(&impl->sensitivity, impl->count * sizeof(float)), "Allocate sensitivity");
+	{
+		// packed float copy of the PSF, then its flip (reverse of the packed buffer)
+		view4 packed = impl->kernel_view(impl->flipped);
+		flip_kernel<<<point_blocks(kcount), kPointThreads, 0, stream>>>(psf, impl->flipped);
+		flip_kernel<<<point_blocks(kcount), kPointThreads, 0, stream>>>(packed, impl->psf);
+		check_hip(hipGetLastError(), "PSF copy launch");
+	}
+	if (aMethod == IANE_RL_DIRECT) {
+		// s = H^T 1 (main.cu:62-63): the flipped PSF over an image of ones, zero border
+		float* ones = nullptr;
+		check_hip(hipMallocAsync(reinterpret_cast<void**>(&ones), impl->count * sizeof(float), stream), "Allocate ones");
+		fill_kernel<<<point_blocks(impl->count), kPointThreads, 0, stream>>>(ones, impl->count, 1.0f);
+		impl->convolve(ones, impl->flipped, impl->sensitivity, stream);
+		check_hip(hipFreeAsync(ones, stream), "Free ones");
+		impl->workspace = 2 * impl->count * sizeof(float) + static_cast<size_t>(kPartialBlocks + 64) * sizeof(double);
+	} else {
+		impl->build_fft(stream);
+		// s = H^T 1 through the same pruned passes
+		float* ones = nullptr;
+		float2* spec = nullptr;
+		check_hip(hipMallocAsync(reinterpret_cast<void**>(&ones), impl->count * sizeof(float), stream), "Allocate ones");
+		check_hip(hipMallocAsync(reinterpret_cast<void**>(&spec), impl->spectrum_bytes, stream), "Allocate spectrum");
+		fill_kernel<<<point_blocks(impl->count), kPointThreads, 0, stream>>>(ones, impl->count, 1.0f);
+		fft_pass_args a = impl->r2c;
+		a.in = ones;
+		a.out = spec;
+		launch_pass(a, stream);
+		impl->run_mid(impl->adjoint_mid, spec, stream);
+		fft_pass_args c = impl->c2r;
+		c.in = spec;
+		c.out = impl->sensitivity;
+		launch_pass(c, stream);
+		check_hip(hipFreeAsync(ones, stream), "Free ones");
+		check_hip(hipFreeAsync(spec, stream), "Free spectrum");
+	}
+	return impl;
+}
+
 namespace {
 
 unsigned row_blocks(const fft_pass_args& a)
@@ -615,76 +764,8 @@ int iane_hip_rl_plan_create(iane_rl_plan** aPlan, int aRank, const int32_t* aExt
 			invalid("RL plan output or extent is null");
 		}
 		*aPlan = nullptr;
-		if (aRank != 2 && aRank != 3) {
-			fail(IANE_ERR_UNSUPPORTED, "RL plans support rank 2 and 3 images");
-		}
-		if (aMethod != IANE_RL_DIRECT && aMethod != IANE_RL_FFT) {
-			invalid("unknown RL method");
-		}
-		if (!(aEpsilon > 0.0f) || !std::isfinite(aEpsilon)) {
-			invalid("Iterations and finite epsilon must be positive");
-		}
-		const view4 psf = make_view4(aPsf, "psf");
-		if (psf.rank != aRank) {
-			invalid("Convolution requires a kernel and images of matching rank");
-		}
-		auto impl = std::make_unique<rl_plan_impl>();
-		impl->rank = aRank;
-		impl->method = aMethod;
-		impl->precision = aPrecision;
-		impl->eps = aEpsilon;
-		impl->count = 1;
-		for (int i = 0; i < aRank; ++i) {
-			if (aExtent[i] <= 0) {
-				invalid("RL requires positive image dimensions");
-			}
-			if (psf.extent[i] <= 0 || psf.extent[i] % 2 == 0) {
-				invalid("Convolution requires positive odd kernel dimensions");
-			}
-			impl->n[i] = aExtent[i];
-			impl->k[i] = psf.extent[i];
-			impl->count *= aExtent[i];
-		}
-		auto stream = static_cast<hipStream_t>(aStream);
-		const long long kcount = static_cast<long long>(impl->k[0]) * impl->k[1] * impl->k[2];
-		check_hip(hipMalloc(&impl->psf, kcount * sizeof(float)), "Allocate PSF");
-		check_hip(hipMalloc(&impl->flipped, kcount * sizeof(float)), "Allocate flipped PSF");
-		check_hip(hipMalloc(&impl->sensitivity, impl->count * sizeof(float)), "Allocate sensitivity");
-		{
-			// packed float copy of the PSF, then its flip (reverse of the packed buffer)
-			view4 packed = impl->kernel_view(impl->flipped);
-			flip_kernel<<<point_blocks(kcount), kPointThreads, 0, stream>>>(psf, impl->flipped);
-			flip_kernel<<<point_blocks(kcount), kPointThreads, 0, stream>>>(packed, impl->psf);
-			check_hip(hipGetLastError(), "PSF copy launch");
-		}
-		if (aMethod == IANE_RL_DIRECT) {
-			// s = H^T 1 (main.cu:62-63): the flipped PSF over an image of ones, zero border
-			float* ones = nullptr;
-			check_hip(hipMallocAsync(reinterpret_cast<void**>(&ones), impl->count * sizeof(float), stream), "Allocate ones");
-			fill_kernel<<<point_blocks(impl->count), kPointThreads, 0, stream>>>(ones, impl->count, 1.0f);
-			impl->convolve(ones, impl->flipped, impl->sensitivity, stream);
-			check_hip(hipFreeAsync(ones, stream), "Free ones");
-			impl->workspace = 2 * impl->count * sizeof(float) + static_cast<size_t>(kPartialBlocks + 64) * sizeof(double);
-		} else {
-			impl->build_fft(stream);
-			// s = H^T 1 through the same pruned passes
-			float* ones = nullptr;
-			float2* spec = nullptr;
-			check_hip(hipMallocAsync(reinterpret_cast<void**>(&ones), impl->count * sizeof(float), stream), "Allocate ones");
-			check_hip(hipMallocAsync(reinterpret_cast<void**>(&spec), impl->spectrum_bytes, stream), "Allocate spectrum");
-			fill_kernel<<<point_blocks(impl->count), kPointThreads, 0, stream>>>(ones, impl->count, 1.0f);
-			fft_pass_args a = impl->r2c;
-			a.in = ones;
-			a.out = spec;
-			launch_pass(a, stream);
-			impl->run_mid(impl->adjoint_mid, spec, stream);
-			fft_pass_args c = impl->c2r;
-			c.in = spec;
-			c.out = impl->sensitivity;
-			launch_pass(c, stream);
-			check_hip(hipFreeAsync(ones, stream), "Free ones");
-			check_hip(hipFreeAsync(spec, stream), "Free spectrum");
-		}
+		auto impl = make_rl_plan(aRank, aExtent, make_view4(aPsf, "psf"), aMethod, aPrecision, aEpsilon,
+			static_cast<hipStream_t>(aStream));
 		auto plan = std::make_unique<iane_rl_plan>();
 		plan->impl = std::move(impl);
 		*aPlan = plan.release();
@@ -782,6 +863,327 @@ int iane_hip_rl_run(iane_rl_plan* aPlan, const float* aObserved, float* aEstimat
 int iane_hip_rl_plan_destroy(iane_rl_plan* aPlan)
 {
 	return guarded([&] { delete aPlan; });
+}
+
+} // extern "C"
+
+// ------------------------------------------------------------------ batches --
+// Many RL problems of uneven sizes, each with its own PSF, advanced together:
+// every pass of the iteration is ONE launch that covers the line groups of all
+// volumes (per-volume pass descriptors in device memory, prefix-summed block
+// offsets), and the whole `iterations`-long sequence is captured once into a
+// hipGraph and replayed with a single graph launch. Results per volume are
+// identical to the single-volume FFT plan.
+namespace {
+
+template<class Args>
+struct batched_launch {
+	std::vector<Args> host;
+	std::vector<unsigned> offsets;
+	Args* d_args = nullptr;
+	unsigned* d_offsets = nullptr;
+	unsigned blocks = 0;
+
+	batched_launch() = default;
+	batched_launch(const batched_launch&) = delete;
+	batched_launch& operator=(const batched_launch&) = delete;
+	~batched_launch()
+	{
+		hipFree(d_args);
+		hipFree(d_offsets);
+	}
+
+	// offsets from the per-volume group counts; device copies of both tables
+	void finalize(const fft_pass_args& (*aPass)(const Args&))
+	{
+		offsets.assign(host.size() + 1, 0);
+		for (size_t v = 0; v < host.size(); ++v) {
+			const fft_pass_args& f = aPass(host[v]);
+			offsets[v + 1] = offsets[v] + static_cast<unsigned>(static_cast<long long>(f.groups_per_outer) * f.outer);
+		}
+		blocks = offsets.back();
+		check_hip(hipMalloc(&d_args, sizeof(Args) * host.size()), "Allocate batch descriptors");
+		check_hip(hipMalloc(&d_offsets, sizeof(unsigned) * offsets.size()), "Allocate batch offsets");
+		check_hip(hipMemcpy(d_offsets, offsets.data(), sizeof(unsigned) * offsets.size(), hipMemcpyHostToDevice),
+			"Upload batch offsets");
+	}
+
+	void upload(hipStream_t aStream) const
+	{
+		check_hip(hipMemcpyAsync(d_args, host.data(), sizeof(Args) * host.size(), hipMemcpyHostToDevice, aStream),
+			"Upload batch descriptors");
+	}
+};
+
+const fft_pass_args& pass_of(const fft_pass_args& a) { return a; }
+const fft_pass_args& pass_of_row(const rl_row_args& a) { return a.f; }
+
+void launch_batched(const batched_launch<fft_pass_args>& b, int aMode, hipStream_t s)
+{
+	const int n = static_cast<int>(b.host.size());
+	switch (aMode) {
+	case PASS_C2C: fft_pass_batched_kernel<PASS_C2C><<<b.blocks, kFftThreads, 0, s>>>(b.d_args, b.d_offsets, n); break;
+	case PASS_R2C: fft_pass_batched_kernel<PASS_R2C><<<b.blocks, kFftThreads, 0, s>>>(b.d_args, b.d_offsets, n); break;
+	case PASS_C2R: fft_pass_batched_kernel<PASS_C2R><<<b.blocks, kFftThreads, 0, s>>>(b.d_args, b.d_offsets, n); break;
+	default: fft_pass_batched_kernel<PASS_C2C_MULT><<<b.blocks, kFftThreads, 0, s>>>(b.d_args, b.d_offsets, n); break;
+	}
+}
+
+} // namespace
+
+struct iane_rl_batch {
+	int count = 0;
+	int rank = 0;
+	float eps = 1e-6f;
+	std::vector<std::unique_ptr<rl_plan_impl>> plans;
+	batched_launch<fft_pass_args> r2c;
+	std::vector<std::unique_ptr<batched_launch<fft_pass_args>>> fwd;
+	std::vector<std::unique_ptr<batched_launch<fft_pass_args>>> adj;
+	std::vector<int> mid_modes;
+	batched_launch<rl_row_args> ratio;
+	batched_launch<rl_row_args> update_next;
+	batched_launch<rl_row_args> update_last;
+	std::vector<size_t> spectrum_offset;
+	size_t partial_offset = 0;
+	size_t workspace = 0;
+	const float** d_observed = nullptr;
+	float** d_estimate = nullptr;
+	long long* d_counts = nullptr;
+	long long max_count = 0;
+	// replayable graph of one run configuration
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t exec = nullptr;
+	int g_iterations = -1;
+	int g_init = -1;
+	double* g_ll = nullptr;
+	void* g_stream = nullptr;
+	void* g_workspace = nullptr;
+
+	~iane_rl_batch()
+	{
+		if (exec) {
+			hipGraphExecDestroy(exec);
+		}
+		if (graph) {
+			hipGraphDestroy(graph);
+		}
+		hipFree(d_observed);
+		hipFree(d_estimate);
+		hipFree(d_counts);
+	}
+
+	// the launch sequence of `iterations` RL steps (captured into the graph)
+	void enqueue(int aIterations, int aInit, double* aLl, hipStream_t s) const
+	{
+		if (aInit) {
+			const unsigned gx = static_cast<unsigned>(std::min<long long>(256, (max_count + 255) / 256));
+			rl_initial_batched_kernel<<<dim3(gx, count), kPointThreads, 0, s>>>(d_observed, d_estimate, d_counts, eps);
+		}
+		launch_batched(r2c, PASS_R2C, s);
+		for (size_t p = 0; p < fwd.size(); ++p) {
+			launch_batched(*fwd[p], mid_modes[p], s);
+		}
+		for (int it = 0; it < aIterations; ++it) {
+			rl_row_batched_kernel<EPI_RATIO><<<ratio.blocks, kFftThreads, 0, s>>>(ratio.d_args, ratio.d_offsets, count);
+			if (aLl) {
+				ll_finalize_batched_kernel<<<count, kPointThreads, 0, s>>>(ratio.host[0].partials, ratio.d_offsets,
+					aLl + it, aIterations);
+			}
+			for (size_t p = 0; p < adj.size(); ++p) {
+				launch_batched(*adj[p], mid_modes[p], s);
+			}
+			const auto& up = it + 1 < aIterations ? update_next : update_last;
+			rl_row_batched_kernel<EPI_UPDATE><<<up.blocks, kFftThreads, 0, s>>>(up.d_args, up.d_offsets, count);
+			if (it + 1 < aIterations) {
+				for (size_t p = 0; p < fwd.size(); ++p) {
+					launch_batched(*fwd[p], mid_modes[p], s);
+				}
+			}
+		}
+		check_hip(hipGetLastError(), "RL batch launch");
+	}
+};
+
+extern "C" {
+
+int iane_hip_rl_batch_create(iane_rl_batch** aBatch, int aCount, int aRank, const int32_t* aExtents,
+	const iane_view_desc* aPsfs, float aEpsilon, void* aStream)
+{
+	return guarded([&] {
+		if (!aBatch || !aExtents || !aPsfs) {
+			invalid("RL batch output, extents or PSFs are null");
+		}
+		*aBatch = nullptr;
+		if (aCount <= 0) {
+			invalid("RL batch needs at least one volume");
+		}
+		auto stream = static_cast<hipStream_t>(aStream);
+		auto batch = std::make_unique<iane_rl_batch>();
+		batch->count = aCount;
+		batch->rank = aRank;
+		batch->eps = aEpsilon;
+		size_t offset = 0;
+		auto align = [](size_t x) { return (x + 255) / 256 * 256; };
+		for (int v = 0; v < aCount; ++v) {
+			batch->plans.push_back(make_rl_plan(aRank, aExtents + v * aRank, make_view4(&aPsfs[v], "psf"), IANE_RL_FFT,
+				IANE_PREC_FP32, aEpsilon, stream));
+			batch->spectrum_offset.push_back(offset);
+			offset += align(batch->plans.back()->spectrum_bytes);
+			batch->max_count = std::max(batch->max_count, batch->plans.back()->count);
+		}
+		const size_t mids = batch->plans[0]->forward_mid.size();
+		for (size_t p = 0; p < mids; ++p) {
+			batch->fwd.push_back(std::make_unique<batched_launch<fft_pass_args>>());
+			batch->adj.push_back(std::make_unique<batched_launch<fft_pass_args>>());
+			batch->mid_modes.push_back(batch->plans[0]->forward_mid[p].mode);
+		}
+		for (const auto& plan : batch->plans) {
+			batch->r2c.host.push_back(plan->r2c);
+			for (size_t p = 0; p < mids; ++p) {
+				batch->fwd[p]->host.push_back(plan->forward_mid[p]);
+				batch->adj[p]->host.push_back(plan->adjoint_mid[p]);
+			}
+			rl_row_args row = plan->row_tpl;
+			row.forward_next = 1;
+			batch->ratio.host.push_back(row);
+			batch->update_next.host.push_back(row);
+			row.forward_next = 0;
+			batch->update_last.host.push_back(row);
+		}
+		batch->r2c.finalize(pass_of);
+		for (size_t p = 0; p < mids; ++p) {
+			batch->fwd[p]->finalize(pass_of);
+			batch->adj[p]->finalize(pass_of);
+		}
+		batch->ratio.finalize(pass_of_row);
+		batch->update_next.finalize(pass_of_row);
+		batch->update_last.finalize(pass_of_row);
+		batch->partial_offset = offset;
+		batch->workspace = offset + static_cast<size_t>(batch->ratio.blocks) * sizeof(double);
+		check_hip(hipMalloc(reinterpret_cast<void**>(&batch->d_observed), sizeof(float*) * aCount), "Allocate tables");
+		check_hip(hipMalloc(reinterpret_cast<void**>(&batch->d_estimate), sizeof(float*) * aCount), "Allocate tables");
+		check_hip(hipMalloc(reinterpret_cast<void**>(&batch->d_counts), sizeof(long long) * aCount), "Allocate tables");
+		std::vector<long long> counts;
+		for (const auto& plan : batch->plans) {
+			counts.push_back(plan->count);
+		}
+		check_hip(hipMemcpy(batch->d_counts, counts.data(), sizeof(long long) * aCount, hipMemcpyHostToDevice),
+			"Upload counts");
+		check_hip(hipStreamSynchronize(stream), "RL batch prepare");
+		*aBatch = batch.release();
+	});
+}
+
+int iane_hip_rl_batch_workspace_bytes(const iane_rl_batch* aBatch, size_t* aBytes)
+{
+	return guarded([&] {
+		if (!aBatch || !aBytes) {
+			invalid("RL batch or output is null");
+		}
+		*aBytes = aBatch->workspace;
+	});
+}
+
+int iane_hip_rl_batch_run(iane_rl_batch* aBatch, const float* const* aObserved, float* const* aEstimates,
+	int aIterations, int aInit, void* aWorkspace, double* aLlHistory, int aUseGraph, void* aStream)
+{
+	return guarded([&] {
+		if (!aBatch || !aObserved || !aEstimates) {
+			invalid("RL batch or buffer tables are null");
+		}
+		if (!aWorkspace) {
+			fail(IANE_ERR_LENGTH, "Prepared operation exceeded its scratch reservation");
+		}
+		if (aIterations < 0) {
+			invalid("Iterations and finite epsilon must be positive");
+		}
+		iane_rl_batch& b = *aBatch;
+		auto stream = static_cast<hipStream_t>(aStream);
+		if (aUseGraph && !stream) {
+			invalid("graph capture needs a created (non-null) stream");
+		}
+		char* ws = static_cast<char*>(aWorkspace);
+		double* partials = reinterpret_cast<double*>(ws + b.partial_offset);
+		for (int v = 0; v < b.count; ++v) {
+			if (!aObserved[v] || !aEstimates[v]) {
+				invalid("RL batch buffers are null");
+			}
+			float2* spec = reinterpret_cast<float2*>(ws + b.spectrum_offset[v]);
+			auto bind = [&](fft_pass_args& f, const void* in, void* out) {
+				f.in = in;
+				f.out = out;
+			};
+			bind(b.r2c.host[v], aEstimates[v], spec);
+			for (size_t p = 0; p < b.fwd.size(); ++p) {
+				bind(b.fwd[p]->host[v], spec, spec);
+				bind(b.adj[p]->host[v], spec, spec);
+			}
+			for (auto* rows : {&b.ratio, &b.update_next, &b.update_last}) {
+				rl_row_args& r = rows->host[v];
+				bind(r.f, spec, spec);
+				r.y = aObserved[v];
+				r.x = aEstimates[v];
+				r.s = b.plans[v]->sensitivity;
+				r.partials = partials;
+			}
+		}
+		// descriptor tables are read by the kernels (and by the captured graph) from device memory
+		b.r2c.upload(stream);
+		for (size_t p = 0; p < b.fwd.size(); ++p) {
+			b.fwd[p]->upload(stream);
+			b.adj[p]->upload(stream);
+		}
+		b.ratio.upload(stream);
+		b.update_next.upload(stream);
+		b.update_last.upload(stream);
+		check_hip(hipMemcpyAsync(b.d_observed, aObserved, sizeof(float*) * b.count, hipMemcpyHostToDevice, stream),
+			"Upload tables");
+		check_hip(hipMemcpyAsync(b.d_estimate, aEstimates, sizeof(float*) * b.count, hipMemcpyHostToDevice, stream),
+			"Upload tables");
+		if (aIterations == 0 && !aInit) {
+			return;
+		}
+		if (!aUseGraph) {
+			b.enqueue(aIterations, aInit, aLlHistory, stream);
+			return;
+		}
+		const bool reuse = b.exec && b.g_iterations == aIterations && b.g_init == aInit && b.g_ll == aLlHistory &&
+			b.g_stream == aStream && b.g_workspace == aWorkspace;
+		if (!reuse) {
+			if (b.exec) {
+				check_hip(hipGraphExecDestroy(b.exec), "Destroy RL graph");
+				b.exec = nullptr;
+			}
+			if (b.graph) {
+				check_hip(hipGraphDestroy(b.graph), "Destroy RL graph");
+				b.graph = nullptr;
+			}
+			check_hip(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "Begin RL graph capture");
+			try {
+				b.enqueue(aIterations, aInit, aLlHistory, stream);
+			} catch (...) {
+				hipGraph_t discard = nullptr;
+				hipStreamEndCapture(stream, &discard);
+				if (discard) {
+					hipGraphDestroy(discard);
+				}
+				throw;
+			}
+			check_hip(hipStreamEndCapture(stream, &b.graph), "End RL graph capture");
+			check_hip(hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0), "Instantiate RL graph");
+			b.g_iterations = aIterations;
+			b.g_init = aInit;
+			b.g_ll = aLlHistory;
+			b.g_stream = aStream;
+			b.g_workspace = aWorkspace;
+		}
+		check_hip(hipGraphLaunch(b.exec, stream), "Launch RL graph");
+	});
+}
+
+int iane_hip_rl_batch_destroy(iane_rl_batch* aBatch)
+{
+	return guarded([&] { delete aBatch; });
 }
 
 } // extern "C"

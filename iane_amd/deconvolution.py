@@ -83,6 +83,67 @@ class rl_plan:
             self._handle = None
 
 
+class rl_batch:
+    """Many RL problems of uneven shapes with their own PSFs, advanced together:
+    one launch per pass over every volume, the run captured once into a hipGraph
+    and replayed. Per volume identical to ``rl_plan(method=FFT)``."""
+
+    def __init__(self, shapes, psfs, epsilon: float = 1e-6, policy=None):
+        import torch
+        shapes = [tuple(int(s) for s in sh) for sh in shapes]
+        if not shapes or len(shapes) != len(psfs):
+            raise invalid_argument("one PSF per volume is required")
+        rank = len(shapes[0])
+        if any(len(s) != rank for s in shapes):
+            raise invalid_argument("all volumes of a batch have the same rank")
+        self.shapes = shapes
+        self.stream = _stream(policy)
+        extents = (ctypes.c_int32 * (rank * len(shapes)))(*[e for s in shapes for e in reversed(s)])
+        descs = (_lib.iane_view_desc * len(psfs))(*[view_desc(p) for p in psfs])
+        handle = ctypes.c_void_p()
+        _lib.call("iane_hip_rl_batch_create", ctypes.byref(handle), len(shapes), rank, extents, descs,
+                  ctypes.c_float(epsilon), self.stream)
+        self._handle = handle
+        nbytes = ctypes.c_size_t()
+        _lib.call("iane_hip_rl_batch_workspace_bytes", handle, ctypes.byref(nbytes))
+        self.workspace_bytes = nbytes.value
+        self._workspace = torch.empty(max(nbytes.value, 1), dtype=torch.uint8, device="cuda")
+
+    def run(self, observed, estimates, iterations: int, *, init: bool = True, ll_history=None,
+            use_graph: bool = True) -> hip_execution:
+        """``ll_history``: float64 device tensor of shape (count, iterations) or None."""
+        import torch
+        n = len(self.shapes)
+        if len(observed) != n or len(estimates) != n:
+            raise invalid_argument("one observed and one estimate image per volume")
+        for o, e, s in zip(observed, estimates, self.shapes):
+            for t in (o, e):
+                if tuple(t.shape) != s or t.dtype != torch.float32 or not t.is_contiguous() or t.device.type != "cuda":
+                    raise invalid_argument("volumes must be packed float32 device images of the prepared shapes")
+        obs = (ctypes.c_void_p * n)(*[o.data_ptr() for o in observed])
+        est = (ctypes.c_void_p * n)(*[e.data_ptr() for e in estimates])
+        ll_ptr = 0
+        if ll_history is not None:
+            if ll_history.dtype != torch.float64 or tuple(ll_history.shape) != (n, iterations) \
+                    or not ll_history.is_contiguous():
+                raise invalid_argument("ll_history must be a contiguous float64 (count, iterations) tensor")
+            ll_ptr = ll_history.data_ptr()
+        self._keep = (obs, est)
+        _lib.call("iane_hip_rl_batch_run", self._handle, obs, est, int(iterations), 1 if init else 0,
+                  self._workspace.data_ptr(), ll_ptr, 1 if use_graph else 0, self.stream)
+        return hip_execution(self.stream)
+
+    def __del__(self):
+        handle = getattr(self, "_handle", None)
+        if handle:
+            try:
+                _lib.load().iane_hip_stream_sync(self.stream)
+                _lib.load().iane_hip_rl_batch_destroy(handle)
+            except Exception:
+                pass
+            self._handle = None
+
+
 def richardson_lucy(observed, psf, iterations: int = 20, epsilon: float = 1e-6, *, method: int = FFT,
                     precision: int = precision_t.fp32, warm_start=None, return_ll: bool = False, policy=None):
     """One-shot RL: returns the estimate (and the log-likelihood history)."""

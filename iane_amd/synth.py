@@ -188,6 +188,30 @@ def volume_blobs(n: int = 256, seed: int = 2, blobs: int = 200):
             "volume_u16": vol_u16, "binomial": binomial}
 
 
+def rl_batch_data(count: int = 64, seed: int = 4, edge_range=(32, 192), sigma_range=(1.0, 4.0),
+                  peak_range=(200.0, 2000.0), blobs_per_megavoxel: float = 150.0):
+    """``count`` RL problems of uneven shapes: each axis length U{32..192}, its own
+    anisotropic Gaussian PSF (sigma U[1,4] per axis, support 2*ceil(3 sigma)+1),
+    Poisson noise at a peak U[200, 2000] photons. Returns a list of
+    (observed float32, psf float32) in array order (z, y, x)."""
+    u = uniform(seed, 0, count * 7).reshape(count, 7)
+    out = []
+    for v in range(count):
+        shape = tuple(int(edge_range[0] + np.floor(u[v, a] * (edge_range[1] - edge_range[0] + 1))) for a in range(3))
+        sigma = tuple(sigma_range[0] + u[v, 3 + a] * (sigma_range[1] - sigma_range[0]) for a in range(3))
+        size = tuple(2 * int(np.ceil(3 * s)) + 1 for s in sigma)
+        peak = peak_range[0] + u[v, 6] * (peak_range[1] - peak_range[0])
+        truth = np.full(shape, 0.01)
+        add_blobs(truth, max(4, int(blobs_per_megavoxel * np.prod(shape) / 1e6)), (1.0, 4.0), (0.1, 1.0),
+                  seed * 1000 + v, 1)
+        psf = gaussian_kernel(sigma, size)
+        blurred = _fftconvolve_same(truth, psf)
+        blurred *= peak / blurred.max()
+        observed = poisson(np.maximum(blurred, 0.0), seed * 1000 + v, 2).reshape(shape)
+        out.append((observed.astype(np.float32), psf.astype(np.float32)))
+    return out
+
+
 def _fftconvolve_same(x: np.ndarray, k: np.ndarray) -> np.ndarray:
     shape = [n + m - 1 for n, m in zip(x.shape, k.shape)]
     axes = list(range(x.ndim))

@@ -180,6 +180,23 @@ int iane_hip_rl_run(iane_rl_plan* plan, const float* observed, float* estimate, 
 	int init, void* workspace, double* ll_history, void* stream);
 int iane_hip_rl_plan_destroy(iane_rl_plan* plan);
 
+/* ---- Batched Richardson–Lucy (no reference counterpart) ----------------
+ * `count` volumes of one rank with their own extents (extents[v*rank + a])
+ * and PSFs, FFT method, FP32. Each pass of an iteration is one launch over
+ * the lines of every volume; with use_graph != 0 the whole run is captured
+ * into a hipGraph on first use and replayed (same iterations/init/ll/
+ * workspace/stream) with one graph launch; the stream must not be NULL then.
+ * Per volume the result equals iane_hip_rl_run with IANE_RL_FFT bit for bit.
+ * observed/estimates: host arrays of `count` device pointers (packed f32).
+ * ll_history: device doubles, count x iterations (row v = volume v), or NULL. */
+typedef struct iane_rl_batch iane_rl_batch;
+int iane_hip_rl_batch_create(iane_rl_batch** batch, int count, int rank, const int32_t* extents,
+	const iane_view_desc* psfs, float epsilon, void* stream);
+int iane_hip_rl_batch_workspace_bytes(const iane_rl_batch* batch, size_t* bytes);
+int iane_hip_rl_batch_run(iane_rl_batch* batch, const float* const* observed, float* const* estimates,
+	int iterations, int init, void* workspace, double* ll_history, int use_graph, void* stream);
+int iane_hip_rl_batch_destroy(iane_rl_batch* batch);
+
 /* Test hooks: the untiled one-thread-per-output kernel and the per-axis
  * separable passes, so tests can compare every path against the oracle. */
 int iane_hip_convolve_untiled(const iane_view_desc* src, const iane_view_desc* kernel,
