@@ -20,9 +20,12 @@ INCLUDE = ROOT.parent / "include"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 # Packed FP32 instructions are disabled for all device code (some of their
-# forms have produced wrong results on the target boxes).
+# forms have produced wrong results on the target boxes). Contraction into
+# fma is decided per source expression (-ffp-contract=on), so a device
+# function inlined into different kernels (single-volume and batched passes,
+# tuned launch shapes) performs exactly the same arithmetic.
 FLAGS = [
-    "-std=c++20", "-O3", "-fPIC", f"--offload-arch={ARCH}",
+    "-std=c++20", "-O3", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=on",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-result", f"-I{INCLUDE}",
 ]
