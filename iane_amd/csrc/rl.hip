@@ -241,7 +241,9 @@ __global__ __launch_bounds__(kFftThreads) void rl_row_kernel(rl_row_args a)
 {
 	__shared__ float2 lds[2 * kFftLdsElems];
 	__shared__ double shared[kFftThreads / 64];
-	rl_row_body<EPI>(a, xcd_remap(blockIdx.x, gridDim.x), blockIdx.x, lds, shared);
+	// the partial of line group g goes to slot g (the batched kernel does the same)
+	const unsigned group = xcd_remap(blockIdx.x, gridDim.x);
+	rl_row_body<EPI>(a, group, group, lds, shared);
 }
 
 // ---------------------------------------------------- batched (many volumes) --
