@@ -107,6 +107,24 @@ SIGNATURES = {
     "iane_hip_rl_batch_run": [vp, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int, vp, vp,
                               ctypes.c_int, vp],
     "iane_hip_rl_batch_destroy": [vp],
+    "iane_hip_rl_plan_blur": [vp, vp, vp, vp, vp],
+    "iane_hip_cnn_conv3d": [vp, vp, vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                            ctypes.c_int, ctypes.c_int, vp],
+    "iane_hip_cnn_transpose_weights": [vp, vp, ctypes.c_int, ctypes.c_int, vp],
+    "iane_hip_cnn_wgrad": [vp, vp, vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp],
+    "iane_hip_cnn_relu_backward": [vp, vp, i64, vp],
+    "iane_hip_cnn_pool2": [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp],
+    "iane_hip_cnn_pool2_backward": [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp],
+    "iane_hip_cnn_up2_add": [vp, vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp],
+    "iane_hip_cnn_up2_backward": [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp],
+    "iane_hip_cnn_to_bf16": [vp, vp, i64, f32, vp],
+    "iane_hip_cnn_adam": [vp, vp, vp, vp, vp, i64, f32, f32, f32, f32, ctypes.c_int, vp],
+    "iane_hip_cnn_residual_mse": [vp, vp, vp, vp, vp, i64, vp, ctypes.POINTER(ctypes.c_int), vp],
+    "iane_hip_cnn_warm_start": [vp, vp, vp, i64, f32, f32, vp],
+    "iane_hip_synth_blobs": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, f32, vp, ctypes.c_int, vp],
+    "iane_hip_synth_poisson": [vp, vp, i64, ctypes.c_uint, vp],
+    "iane_hip_max": [vp, i64, vp, vp, vp],
+    "iane_hip_scale": [vp, i64, f32, vp],
 }
 
 _lib = None

@@ -784,6 +784,31 @@ int iane_hip_rl_plan_workspace_bytes(const iane_rl_plan* aPlan, size_t* aBytes)
 	});
 }
 
+int iane_hip_rl_plan_blur(iane_rl_plan* aPlan, const float* aIn, float* aOut, void* aWorkspace, void* aStream)
+{
+	return guarded([&] {
+		if (!aPlan || !aIn || !aOut || !aWorkspace) {
+			invalid("RL blur buffers are null");
+		}
+		const rl_plan_impl& p = *aPlan->impl;
+		auto stream = static_cast<hipStream_t>(aStream);
+		if (p.method == IANE_RL_DIRECT) {
+			p.convolve(aIn, p.psf, aOut, stream);
+			return;
+		}
+		float2* spec = static_cast<float2*>(aWorkspace);
+		fft_pass_args a = p.r2c;
+		a.in = aIn;
+		a.out = spec;
+		launch_pass(a, stream);
+		p.run_mid(p.forward_mid, spec, stream);
+		fft_pass_args c = p.c2r;
+		c.in = spec;
+		c.out = aOut;
+		launch_pass(c, stream);
+	});
+}
+
 int iane_hip_rl_run(iane_rl_plan* aPlan, const float* aObserved, float* aEstimate, int aIterations, int aInit,
 	void* aWorkspace, double* aLlHistory, void* aStream)
 {

@@ -179,6 +179,8 @@ int iane_hip_rl_plan_workspace_bytes(const iane_rl_plan* plan, size_t* bytes);
 int iane_hip_rl_run(iane_rl_plan* plan, const float* observed, float* estimate, int iterations,
 	int init, void* workspace, double* ll_history, void* stream);
 int iane_hip_rl_plan_destroy(iane_rl_plan* plan);
+/* out = H in (the plan's zero-border convolution with the PSF); workspace as for run */
+int iane_hip_rl_plan_blur(iane_rl_plan* plan, const float* in, float* out, void* workspace, void* stream);
 
 /* ---- Batched Richardson–Lucy (no reference counterpart) ----------------
  * `count` volumes of one rank with their own extents (extents[v*rank + a])
@@ -196,6 +198,37 @@ int iane_hip_rl_batch_workspace_bytes(const iane_rl_batch* batch, size_t* bytes)
 int iane_hip_rl_batch_run(iane_rl_batch* batch, const float* const* observed, float* const* estimates,
 	int iterations, int init, void* workspace, double* ll_history, int use_graph, void* stream);
 int iane_hip_rl_batch_destroy(iane_rl_batch* batch);
+
+/* ---- Learned RL warm start: kernels of a small 3-D CNN (no reference counterpart) ----
+ * Activations channels-last [z][y][x][c] (BF16 unless stated), weights BF16
+ * [cout][27][cin], tap = (dz*3 + dy)*3 + dx, zero padding. FP32 accumulation. */
+/* out = [relu](bias + 3x3x3 cross-correlation); out f32 or bf16; cout <= 64 */
+int iane_hip_cnn_conv3d(const void* in, const void* weights, const float* bias, void* out, int out_bf16, int relu,
+	int nx, int ny, int nz, int cin, int cout, void* stream);
+/* wt[ci][t][co] = w[co][26 - t][ci]: the input gradient is conv3d with wt */
+int iane_hip_cnn_transpose_weights(const void* w, void* wt, int cin, int cout, void* stream);
+/* dw[co][t][ci] += sum_p dy[p][co] in[p + d(t) - 1][ci], dbias[co] += sum_p dy[p][co] (FP32, accumulating) */
+int iane_hip_cnn_wgrad(const void* in, const float* dy, float* dw, float* dbias, int nx, int ny, int nz, int cin,
+	int cout, void* stream);
+int iane_hip_cnn_relu_backward(const void* y, float* dy, int64_t n, void* stream);
+int iane_hip_cnn_pool2(const void* in, void* out, int nx, int ny, int nz, int c, void* stream);
+int iane_hip_cnn_pool2_backward(const float* dy, float* dx, int nx, int ny, int nz, int c, int accumulate, void* stream);
+int iane_hip_cnn_up2_add(const void* coarse, const void* skip, void* out, int nx, int ny, int nz, int c, void* stream);
+int iane_hip_cnn_up2_backward(const float* dfine, float* dcoarse, int nx, int ny, int nz, int c, void* stream);
+int iane_hip_cnn_to_bf16(const float* in, void* out, int64_t n, float scale, void* stream);
+int iane_hip_cnn_adam(float* w, const float* g, float* m, float* v, void* w_bf16, int64_t n, float lr, float beta1,
+	float beta2, float eps, int step, void* stream);
+int iane_hip_cnn_residual_mse(const float* u, const float* r, const float* target, float* pred, float* grad, int64_t n,
+	double* partials, int* blocks_used, void* stream);
+/* x = max(scale * (u + r), eps) */
+int iane_hip_cnn_warm_start(const float* u, const float* r, float* x, int64_t n, float scale, float eps, void* stream);
+/* training-data synthesis on the device: blobs (cz, cy, cx, sigma, amp) x count, Poisson
+ * sampling (inversion / PTRS over Philox4x32-10), max reduction, scaling */
+int iane_hip_synth_blobs(float* out, int nx, int ny, int nz, float background, const float* blobs, int count,
+	void* stream);
+int iane_hip_synth_poisson(const float* lambda, float* out, int64_t n, unsigned seed, void* stream);
+int iane_hip_max(const float* in, int64_t n, float* partial, float* out, void* stream);
+int iane_hip_scale(float* x, int64_t n, float scale, void* stream);
 
 /* Test hooks: the untiled one-thread-per-output kernel and the per-axis
  * separable passes, so tests can compare every path against the oracle. */
