@@ -32,6 +32,7 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+MFMA16_PEAK_TFLOPS = 2500.0
 METRIC = "Mvoxel/s 3-D conv (%HBM roofline); RL time-to-equal-likelihood, learned vs plain"
 
 
@@ -112,7 +113,7 @@ def bench_rl(timer, stream, iterations: int = 200):
     t0 = time.perf_counter()
     plan = deconvolution.rl_plan(y.shape, dpsf, deconvolution.FFT, policy=hip_policy(stream))
     est = torch.empty_like(dy)
-    ll = torch.zeros(iterations, dtype=torch.float64, device="cuda")
+    ll = torch.empty(iterations, dtype=torch.float64, device="cuda")  # every entry is written by the run
     plan.run(dy, est, 1).wait()
     prepare_s = time.perf_counter() - t0
     ms = timer.time(lambda: plan.run(dy, est, iterations, ll_history=ll), 1)
@@ -151,6 +152,32 @@ def bench_rl_batch(timer, stream, iterations: int = 100):
             "batched_eager_ms": round(eager_ms, 2), "per_volume_plans_ms": round(single_ms, 2),
             "speedup_vs_per_volume": round(single_ms / graph_ms, 2),
             "Mvoxel_iter_s": round(voxels * iterations / graph_ms / 1e3, 1), "prepare_s": round(prepare_s, 2)}
+
+
+def bench_learned(stream, volumes: int = 4):
+    """Learned warm start + batched RL vs plain batched RL on 256x256x128 volumes
+    (blobs, PSF sigma (3, 1.5, 1.5) on 63x31x31, peak 1000, Poisson; made on the
+    GPU from seed 3): GPU time until every volume's Poisson log-likelihood reaches
+    that of plain 200-iteration RL."""
+    import torch
+    from iane_amd import hip_policy, learned
+    if not learned.WEIGHTS.exists():
+        return {"skipped": f"no trained weights at {learned.WEIGHTS}"}
+    policy = hip_policy(stream)
+    net = learned.warm_start_net(policy=policy).load()
+    psf = torch.from_numpy(learned.psf_kernel()).cuda()
+    log(f"[bench] learned: {volumes} observed volumes 256x256x128")
+    ys = list(learned.observed_volumes(volumes, (128, 256, 256), seed=3, policy=policy))
+    out = learned.batch_time_to_likelihood(net, ys, [psf] * volumes, iterations=200, max_iterations=120,
+                                           policy=policy)
+    log(f"[bench] learned: plain {out['plain_ms']:.1f} ms, learned {out['learned_ms']:.1f} ms "
+        f"(inference {out['inference_ms']:.1f} ms, k={out['k_max']}), speedup {out['speedup']:.2f}x")
+    out["plain_ms"] = round(out["plain_ms"], 2)
+    out["learned_ms"] = round(out["learned_ms"], 2)
+    out["inference_ms"] = round(out["inference_ms"], 2)
+    out["speedup"] = round(out["speedup"], 2)
+    out["network_parameters"] = net.parameter_count()
+    return out
 
 
 def main() -> int:
@@ -217,7 +244,8 @@ def main() -> int:
     if not args.quick:
         # dense random 15^3 kernel, reference (FP64, bit-exact) arithmetic: compute bound
         dense = torch.from_numpy(data["dense"]).cuda()
-        for name, prec in (("dense15_reference", precision.reference), ("dense15_fp32", precision.fp32)):
+        for name, prec in (("dense15_reference", precision.reference), ("dense15_fp32", precision.fp32),
+                           ("dense15_split_fp16", precision.split_fp16), ("dense15_split_bf16", precision.split_bf16)):
             o = convolution_options(border_mode.zero, prec)
             fn = lambda: iane_amd.convolve(vol, dense, out, o, policy)  # noqa: E731
             fn()
@@ -225,9 +253,14 @@ def main() -> int:
             k = 3
             t = timer.time(fn, k) / k
             flops = 2.0 * voxels * 15 ** 3
-            extra[name] = {"ms": round(t, 3), "Mvoxel_s": round(voxels / t / 1e3, 1),
-                           "TFLOP_s": round(flops / t / 1e9, 1),
-                           "frac_of_fp64_vector_peak": round(flops / t / 1e9 / FP64_PEAK_TFLOPS, 3)}
+            tflops = flops / t / 1e9
+            entry = {"ms": round(t, 3), "Mvoxel_s": round(voxels / t / 1e3, 1), "TFLOP_s": round(tflops, 1)}
+            if prec in (precision.split_fp16, precision.split_bf16):
+                # algorithmic 2*N*K flops against the dense FP16/BF16 MFMA peak
+                entry["frac_of_mfma_peak"] = round(tflops / MFMA16_PEAK_TFLOPS, 4)
+            else:
+                entry["frac_of_fp64_vector_peak"] = round(tflops / FP64_PEAK_TFLOPS, 3)
+            extra[name] = entry
             log(f"[bench] {name}: {t:.2f} ms")
         # u16 volume, 3^3 binomial int kernel, int32 result (bit-exact)
         v16 = torch.from_numpy(data["volume_u16"].view(np.int16)).cuda().view(torch.uint16)
@@ -247,6 +280,7 @@ def main() -> int:
     if not args.quick:
         extra["rl_fft_256x256x128_psf31x31x63"] = bench_rl(timer, stream)
         extra["rl_batch_64_volumes"] = bench_rl_batch(timer, stream)
+        extra["rl_learned_vs_plain_256x256x128"] = bench_learned(stream)
 
     base = None
     if not args.no_cpu:

@@ -1,11 +1,391 @@
-// Split-precision MFMA direct convolution (placeholder until the MFMA tiles land).
+// Split-precision MFMA direct convolution (IANE_PREC_SPLIT_FP16 / _BF16).
+//
+// The reference sums taps in FP64 (include/iane/algorithms/convolution.hpp:36-49);
+// this path computes the same sum on the matrix cores with FP32 accumulation.
+// Every FP32 operand v is split as v ~ hi + lo * 2^-L (hi, lo 16-bit), and a
+// product is formed from three MFMAs: hi*hi into one accumulator and
+// hi*lo + lo*hi into a second one that is scaled by 2^-L at the end (the lo*lo
+// term is below FP32 resolution for FP16). FP16 keeps 11+11 significant bits
+// per operand (power-of-two scales keep the values inside FP16 range); BF16
+// keeps 8+8 bits with the FP32 exponent range.
+//
+// GEMM form. Along x the convolution is a Toeplitz product: for a window of
+// 16 outputs starting at x0, out[x0 + m] = sum_k A[m][k] * in[x0 - hx + k] with
+// A[m][k] = kernel[qz][qy][m + 2 hx - k] (zero outside the kernel). One
+// v_mfma_f32_16x16x32 takes that A (16 x 32) and, as its 16 columns, the
+// windows of 16 consecutive x-tiles of ONE input row: a wave covers a
+// 256-wide output strip of a row with one MFMA per (input row, qz, qy). A wave
+// owns the accumulators of 16 output rows (y .. y+15 at one z) and streams the
+// input rows past them; the A matrices of a chunk of qy values stay in
+// registers, so each row load feeds up to 16 x 3 MFMAs.
+//
+// Border handling is done once, up front: the source is written into a padded
+// buffer (hx, hy, hz margins filled by the border rule, values already split
+// into hi and lo planes), so the hot loop has no bounds checks and every load
+// is 16-byte aligned.
 #include "common.hpp"
 
-namespace iane_hip {
+#include <type_traits>
 
-int convolve_split(const view4&, const view4&, const view4&, int, int, hipStream_t)
+namespace iane_hip {
+namespace {
+
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 b16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kRows = 16;         // output rows per wave
+constexpr int kStrip = 256;       // output x per wave (16 tiles of 16)
+constexpr int kWaves = 4;         // waves per workgroup
+constexpr int kLoShiftFp16 = 12;  // lo = (v - hi) * 2^12 for FP16
+
+struct split_geom {
+	int nx, ny, nz;       // image extents
+	int hx, hy, hz;       // kernel half extents
+	int kz;               // kernel extent along z
+	int qy_padded;        // ky rounded up to the qy chunk
+	int px, py, pz;       // padded buffer extents (halves per row, rows, planes)
+	int strips, yblocks;  // wave tasks per plane
+	long long plane;      // halves per padded plane
+	long long part;       // halves per hi (or lo) buffer
+};
+
+template<class T16>
+struct split16;
+
+template<>
+struct split16<_Float16> {
+	using vec = f16x8;
+	static constexpr int lo_shift = kLoShiftFp16;
+	__device__ static _Float16 to(float v) { return static_cast<_Float16>(v); }
+	__device__ static float from(_Float16 v) { return static_cast<float>(v); }
+	__device__ static f32x4 mfma(vec a, vec b, f32x4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
+};
+
+template<>
+struct split16<__bf16> {
+	using vec = b16x8;
+	static constexpr int lo_shift = 0;
+	__device__ static __bf16 to(float v) { return static_cast<__bf16>(v); }
+	__device__ static float from(__bf16 v) { return static_cast<float>(v); }
+	__device__ static f32x4 mfma(vec a, vec b, f32x4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
+};
+
+// ------------------------------------------------------------------ scales --
+// Per-block maximum of |element| over a strided rank<=3 view.
+__global__ void absmax_partial_kernel(view4 v, float* partial)
 {
-	return 0;
+	__shared__ float shared[4];
+	const long long count = static_cast<long long>(v.extent[0]) * v.extent[1] * v.extent[2];
+	float m = 0.0f;
+	for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < count;
+		 i += static_cast<long long>(gridDim.x) * blockDim.x) {
+		const long long x = i % v.extent[0];
+		const long long y = (i / v.extent[0]) % v.extent[1];
+		const long long z = i / (static_cast<long long>(v.extent[0]) * v.extent[1]);
+		const char* p = v.ptr + x * v.stride[0] + y * v.stride[1] + z * v.stride[2];
+		m = fmaxf(m, fabsf(static_cast<float>(load_as_double(p, v.type))));
+	}
+	for (int off = 32; off > 0; off >>= 1) {
+		m = fmaxf(m, __shfl_down(m, off, 64));
+	}
+	if ((threadIdx.x & 63) == 0) {
+		shared[threadIdx.x >> 6] = m;
+	}
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		float r = shared[0];
+		for (int w = 1; w < static_cast<int>(blockDim.x >> 6); ++w) {
+			r = fmaxf(r, shared[w]);
+		}
+		partial[blockIdx.x] = r;
+	}
+}
+
+// scale[slot] = 2^(14 - e) with max < 2^e (1 for a zero maximum, or when `aUnit`).
+__global__ void scale_from_partials_kernel(const float* partial, int count, float* scale, int slot, int unit)
+{
+	__shared__ float shared[4];
+	float m = 0.0f;
+	for (int i = threadIdx.x; i < count; i += blockDim.x) {
+		m = fmaxf(m, partial[i]);
+	}
+	for (int off = 32; off > 0; off >>= 1) {
+		m = fmaxf(m, __shfl_down(m, off, 64));
+	}
+	if ((threadIdx.x & 63) == 0) {
+		shared[threadIdx.x >> 6] = m;
+	}
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		float r = shared[0];
+		for (int w = 1; w < static_cast<int>(blockDim.x >> 6); ++w) {
+			r = fmaxf(r, shared[w]);
+		}
+		int e = 0;
+		frexpf(r, &e);
+		scale[slot] = (unit || r == 0.0f) ? 1.0f : ldexpf(1.0f, 14 - e);
+	}
+}
+
+// ------------------------------------------------------------ padded split --
+// hi/lo planes of the padded source: padded (xp, yp, zp) holds the sample at
+// (xp - hx, yp - hy, zp - hz) under the border rule.
+template<class T16>
+__global__ void split_pad_kernel(view4 src, int border, split_geom g, const float* scale, T16* out)
+{
+	using S = split16<T16>;
+	const float sd = scale[0];
+	const float lo_scale = static_cast<float>(1 << S::lo_shift);
+	const long long total = g.part;
+	for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < total;
+		 i += static_cast<long long>(gridDim.x) * blockDim.x) {
+		const int xp = static_cast<int>(i % g.px);
+		const int yp = static_cast<int>((i / g.px) % g.py);
+		const int zp = static_cast<int>(i / (static_cast<long long>(g.px) * g.py));
+		int x = xp - g.hx, y = yp - g.hy, z = zp - g.hz;
+		float v = 0.0f;
+		const bool inside = x >= 0 && x < g.nx && y >= 0 && y < g.ny && z >= 0 && z < g.nz;
+		if (inside || border == IANE_BORDER_CLAMP) {
+			x = min(max(x, 0), g.nx - 1);
+			y = min(max(y, 0), g.ny - 1);
+			z = min(max(z, 0), g.nz - 1);
+			v = load_as_float(src.ptr + x * src.stride[0] + y * src.stride[1] + z * src.stride[2], src.type) * sd;
+		}
+		const T16 hi = S::to(v);
+		out[i] = hi;
+		out[i + total] = S::to((v - S::from(hi)) * lo_scale);
+	}
+}
+
+// ------------------------------------------------------------------ A table --
+// table[((qz * qy_padded + qy) * steps + s) * 2 + part][lane][j]: the A operand
+// (hi or lo part) of the Toeplitz matrix of kernel row (qz, qy), K-step s, in
+// the MFMA lane layout (lane l holds A[l % 16][32 s + 8 (l / 16) + j]).
+template<class T16>
+__global__ void toeplitz_table_kernel(view4 ker, int hx, int qy_padded, int steps, const float* scale, T16* table)
+{
+	using S = split16<T16>;
+	const float sk = scale[1];
+	const float lo_scale = static_cast<float>(1 << S::lo_shift);
+	const int kx = ker.extent[0], ky = ker.extent[1], kz = ker.extent[2];
+	const long long entries = static_cast<long long>(kz) * qy_padded * steps * 64 * 8;
+	for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < entries;
+		 i += static_cast<long long>(gridDim.x) * blockDim.x) {
+		const int j = static_cast<int>(i % 8);
+		const int lane = static_cast<int>((i / 8) % 64);
+		const long long row = i / (8 * 64);  // (qz * qy_padded + qy) * steps + s
+		const int s = static_cast<int>(row % steps);
+		const int qy = static_cast<int>((row / steps) % qy_padded);
+		const int qz = static_cast<int>(row / (static_cast<long long>(steps) * qy_padded));
+		const int m = lane % 16;
+		const int k = 32 * s + 8 * (lane / 16) + j;
+		const int qx = m + 2 * hx - k;
+		float v = 0.0f;
+		if (qy < ky && qx >= 0 && qx < kx) {
+			v = static_cast<float>(load_as_double(ker.ptr + qx * ker.stride[0] + qy * ker.stride[1] + qz * ker.stride[2],
+				ker.type)) * sk;
+		}
+		const T16 hi = S::to(v);
+		const long long base = (row * 2) * 64 * 8 + lane * 8 + j;
+		table[base] = hi;
+		table[base + 64 * 8] = S::to((v - S::from(hi)) * lo_scale);
+	}
+}
+
+// ------------------------------------------------------------- main kernel --
+template<class T16, int QC, int STEPS>
+__global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(1, 1)))
+void conv_split_kernel(const T16* __restrict__ padded, const T16* __restrict__ table, const float* __restrict__ scale,
+	view4 dst, split_geom g, unsigned tasks)
+{
+	using S = split16<T16>;
+	using vec = typename S::vec;
+	const unsigned block = xcd_remap(blockIdx.x, gridDim.x);
+	// The 4 waves of a workgroup take 4 consecutive z planes of one (strip,
+	// y block), and consecutive workgroups continue along z: workgroups that
+	// run together on one XCD then read overlapping input planes from its L2.
+	const unsigned wave = threadIdx.x >> 6;
+	const unsigned zgroups = (static_cast<unsigned>(g.nz) + kWaves - 1) / kWaves;
+	const unsigned zgroup = block % zgroups;
+	const unsigned column = block / zgroups;
+	const int z = static_cast<int>(zgroup * kWaves + wave);
+	if (block >= tasks || z >= g.nz) {
+		return;
+	}
+	const int strip = static_cast<int>(column % g.strips);
+	const int y0 = static_cast<int>(column / g.strips) * kRows;
+	const int x0 = strip * kStrip;
+	const int lane = threadIdx.x & 63;
+	const int n = lane & 15;
+	const int grp = lane >> 4;
+
+	f32x4 acc_hi[kRows];
+	f32x4 acc_lo[kRows];
+#pragma unroll
+	for (int r = 0; r < kRows; ++r) {
+		acc_hi[r] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+		acc_lo[r] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+	}
+	const long long lane_x = x0 + 16 * n + 8 * grp;  // padded x of this lane's first B element
+	const vec* tab = reinterpret_cast<const vec*>(table);
+	const vec* hi_rows = reinterpret_cast<const vec*>(padded);
+	const vec* lo_rows = reinterpret_cast<const vec*>(padded + g.part);
+	const int chunks = g.qy_padded / QC;
+
+	for (int qz = 0; qz < g.kz; ++qz) {
+		const long long plane = static_cast<long long>(z + 2 * g.hz - qz) * g.plane;
+		for (int c = 0; c < chunks; ++c) {
+			const int q0 = c * QC;
+			vec a_hi[QC][STEPS];
+			vec a_lo[QC][STEPS];
+#pragma unroll
+			for (int j = 0; j < QC; ++j) {
+#pragma unroll
+				for (int s = 0; s < STEPS; ++s) {
+					const long long row = (static_cast<long long>(qz) * g.qy_padded + q0 + j) * STEPS + s;
+					a_hi[j][s] = tab[(row * 2) * 64 + lane];
+					a_lo[j][s] = tab[(row * 2 + 1) * 64 + lane];
+				}
+			}
+			// input row u pairs with output row r = u - (QC - 1) + j through qy = q0 + j
+			const int row0 = y0 + 2 * g.hy - q0 - (QC - 1);
+#pragma unroll
+			for (int u = 0; u < kRows + QC - 1; ++u) {
+				const int yp = min(max(row0 + u, 0), g.py - 1);  // rows < 0 only meet zero A (padded qy)
+				const long long base = (plane + static_cast<long long>(yp) * g.px + lane_x) / 8;
+				vec b_hi[STEPS];
+				vec b_lo[STEPS];
+#pragma unroll
+				for (int s = 0; s < STEPS; ++s) {
+					b_hi[s] = hi_rows[base + 4 * s];
+					b_lo[s] = lo_rows[base + 4 * s];
+				}
+#pragma unroll
+				for (int j = 0; j < QC; ++j) {
+					const int r = u - (QC - 1) + j;
+					if (r >= 0 && r < kRows) {
+#pragma unroll
+						for (int s = 0; s < STEPS; ++s) {
+							acc_hi[r] = S::mfma(a_hi[j][s], b_hi[s], acc_hi[r]);
+							acc_lo[r] = S::mfma(a_hi[j][s], b_lo[s], acc_lo[r]);
+							acc_lo[r] = S::mfma(a_lo[j][s], b_hi[s], acc_lo[r]);
+						}
+					}
+				}
+			}
+		}
+	}
+
+	const float unscale = 1.0f / (scale[0] * scale[1]);
+	const float lo_unscale = 1.0f / static_cast<float>(1 << S::lo_shift);
+	const int xs = x0 + 16 * n + 4 * grp;
+#pragma unroll
+	for (int r = 0; r < kRows; ++r) {
+		const int y = y0 + r;
+		if (y >= g.ny) {
+			break;
+		}
+		char* row = dst.ptr + static_cast<long long>(y) * dst.stride[1] + static_cast<long long>(z) * dst.stride[2];
+#pragma unroll
+		for (int i = 0; i < 4; ++i) {
+			const int x = xs + i;
+			if (x < g.nx) {
+				const float v = (acc_hi[r][i] + acc_lo[r][i] * lo_unscale) * unscale;
+				char* p = row + static_cast<long long>(x) * dst.stride[0];
+				if (dst.type == IANE_F32) {
+					*reinterpret_cast<float*>(p) = v;
+				} else {
+					*reinterpret_cast<double*>(p) = static_cast<double>(v);
+				}
+			}
+		}
+	}
+}
+
+template<class T16, int QC, int STEPS>
+void launch_split(const view4& aSrc, const view4& aKer, const view4& aDst, int aBorder, hipStream_t aStream)
+{
+	split_geom g{};
+	g.nx = aSrc.extent[0];
+	g.ny = aSrc.extent[1];
+	g.nz = aSrc.extent[2];
+	g.hx = aKer.extent[0] / 2;
+	g.hy = aKer.extent[1] / 2;
+	g.hz = aKer.extent[2] / 2;
+	g.kz = aKer.extent[2];
+	g.qy_padded = ((aKer.extent[1] + QC - 1) / QC) * QC;
+	g.strips = (g.nx + kStrip - 1) / kStrip;
+	g.yblocks = (g.ny + kRows - 1) / kRows;
+	g.px = g.strips * kStrip + 32 * STEPS;
+	g.py = g.yblocks * kRows + 2 * g.hy;
+	g.pz = g.nz + 2 * g.hz;
+	g.plane = static_cast<long long>(g.px) * g.py;
+	g.part = g.plane * g.pz;
+
+	const long long table_halves = static_cast<long long>(g.kz) * g.qy_padded * STEPS * 2 * 64 * 8;
+	constexpr int partial_blocks = 1024;
+	const size_t bytes_scale = 256;
+	const size_t bytes_partial = sizeof(float) * partial_blocks;
+	const size_t bytes_table = sizeof(T16) * static_cast<size_t>(table_halves);
+	const size_t bytes_padded = sizeof(T16) * static_cast<size_t>(2 * g.part);
+	char* work = nullptr;
+	check_hip(hipMallocAsync(reinterpret_cast<void**>(&work), bytes_scale + bytes_partial + bytes_table + bytes_padded,
+		aStream), "split convolution workspace");
+	auto* scale = reinterpret_cast<float*>(work);
+	auto* partial = reinterpret_cast<float*>(work + bytes_scale);
+	auto* table = reinterpret_cast<T16*>(work + bytes_scale + bytes_partial);
+	auto* padded = reinterpret_cast<T16*>(work + bytes_scale + bytes_partial + bytes_table);
+	const int unit = std::is_same_v<T16, __bf16> ? 1 : 0;
+
+	absmax_partial_kernel<<<partial_blocks, 256, 0, aStream>>>(aSrc, partial);
+	scale_from_partials_kernel<<<1, 256, 0, aStream>>>(partial, partial_blocks, scale, 0, unit);
+	absmax_partial_kernel<<<64, 256, 0, aStream>>>(aKer, partial);
+	scale_from_partials_kernel<<<1, 256, 0, aStream>>>(partial, 64, scale, 1, unit);
+	split_pad_kernel<T16><<<ceil_div(g.part, 256 * 8), 256, 0, aStream>>>(aSrc, aBorder, g, scale, padded);
+	toeplitz_table_kernel<T16><<<ceil_div(table_halves / 2, 256), 256, 0, aStream>>>(aKer, g.hx, g.qy_padded, STEPS,
+		scale, table);
+	const unsigned tasks = static_cast<unsigned>(g.strips * g.yblocks) * ceil_div(g.nz, kWaves);
+	conv_split_kernel<T16, QC, STEPS><<<tasks, 64 * kWaves, 0, aStream>>>(padded, table, scale, aDst, g, tasks);
+	check_hip(hipGetLastError(), "split convolution launch");
+	check_hip(hipFreeAsync(work, aStream), "split convolution workspace");
+}
+
+template<class T16>
+bool dispatch_split(const view4& aSrc, const view4& aKer, const view4& aDst, int aBorder, hipStream_t aStream)
+{
+	const int kx = aKer.extent[0], ky = aKer.extent[1];
+	const int steps = (15 + kx + 31) / 32;
+	if (steps == 1 && ky <= 8) {
+		launch_split<T16, 8, 1>(aSrc, aKer, aDst, aBorder, aStream);
+	} else if (steps == 1) {
+		launch_split<T16, 15, 1>(aSrc, aKer, aDst, aBorder, aStream);
+	} else if (steps == 2) {
+		launch_split<T16, 8, 2>(aSrc, aKer, aDst, aBorder, aStream);
+	} else {
+		return false;
+	}
+	return true;
+}
+
+} // namespace
+
+// Returns false for shapes this path does not tile (the caller then runs the
+// FP32 direct kernel): rank-4 images, kernels wider than 49 along x, or
+// non-float destinations.
+int convolve_split(const view4& aSrc, const view4& aKer, const view4& aDst, int aBorder, int aPrecision,
+	hipStream_t aStream)
+{
+	if (aSrc.extent[3] != 1 || aKer.extent[3] != 1 || (aDst.type != IANE_F32 && aDst.type != IANE_F64)) {
+		return 0;
+	}
+	if (aSrc.extent[0] >= (1 << 24) || aSrc.extent[1] >= (1 << 24)) {
+		return 0;
+	}
+	if (aPrecision == IANE_PREC_SPLIT_FP16) {
+		return dispatch_split<_Float16>(aSrc, aKer, aDst, aBorder, aStream) ? 1 : 0;
+	}
+	return dispatch_split<__bf16>(aSrc, aKer, aDst, aBorder, aStream) ? 1 : 0;
 }
 
 } // namespace iane_hip

@@ -120,8 +120,11 @@ __global__ __launch_bounds__(kSepThreads) void sep3_fused_kernel(sep_args a)
 	}
 
 	const int planes = (zend - z0) + K - 1; // input planes z0 - C .. zend - 1 + C
-	// Register prefetch of the next staged plane: its global loads are in
-	// flight while the current plane is filtered.
+	// Register prefetch P planes deep: the loads of planes p+1 .. p+P are in
+	// flight while plane p is filtered (one plane of filtering is shorter than
+	// the memory latency under load). Slot of plane p = p mod P, a compile-time
+	// index because the plane loop is unrolled by K and P divides K.
+	constexpr int P = (K % 3 == 0) ? 3 : 1;
 	float pending[kSepColsPerThread];
 	int pending_z = -1;
 	auto flush = [&]() {
@@ -137,9 +140,9 @@ __global__ __launch_bounds__(kSepThreads) void sep3_fused_kernel(sep_args a)
 		}
 		pending_z = -1;
 	};
-	float pre[per_thread];
-	bool pre_in = false;
-	auto fetch = [&](int aPlane) {
+	float pre[P][per_thread];
+	bool pre_in[P];
+	auto fetch = [&](int aPlane, int aSlot) {
 		int zi = z0 - C + aPlane;
 		const bool in = clamp_or_zero(zi, a.nz, a.border);
 		// loads are unconditional (invalid lanes read plane element 0) and
@@ -147,15 +150,20 @@ __global__ __launch_bounds__(kSepThreads) void sep3_fused_kernel(sep_args a)
 		// (the fused path takes f32 sources only: a per-element type switch
 		// would put each load behind a branch and serialise them)
 		// The raw values stay in `pre` and are masked only when written to LDS,
-		// so nothing consumes them until the next plane is staged.
+		// so nothing consumes them until their plane is staged.
 		const char* base = a.src + (in ? zi : 0) * a.sz;
 #pragma unroll
 		for (int e = 0; e < per_thread; ++e) {
-			pre[e] = *reinterpret_cast<const float*>(base + offset[e]);
+			pre[aSlot][e] = *reinterpret_cast<const float*>(base + offset[e]);
 		}
-		pre_in = in;
+		pre_in[aSlot] = in;
 	};
-	fetch(0);
+#pragma unroll
+	for (int q = 0; q < P; ++q) {
+		if (q < planes) {
+			fetch(q, q);
+		}
+	}
 	for (int base = 0; base < planes; base += K) {
 #pragma unroll
 		for (int j = 0; j < K; ++j) {
@@ -163,6 +171,7 @@ __global__ __launch_bounds__(kSepThreads) void sep3_fused_kernel(sep_args a)
 			if (p >= planes) {
 				break;
 			}
+			const int slot = j % P;
 			int zi = z0 - C + p;
 			const bool plane_in = clamp_or_zero(zi, a.nz, a.border);
 			float xy[kSepColsPerThread];
@@ -172,12 +181,12 @@ __global__ __launch_bounds__(kSepThreads) void sep3_fused_kernel(sep_args a)
 					const int i = tid + e * kSepThreads;
 					if (i < rows * cols) {
 						const int ly = i / cols;
-						in_s[ly * pitch + (i - ly * cols)] = (pre_in && valid[e]) ? pre[e] : 0.0f;
+						in_s[ly * pitch + (i - ly * cols)] = (pre_in[slot] && valid[e]) ? pre[slot][e] : 0.0f;
 					}
 				}
 			}
-			if (p + 1 < planes) {
-				fetch(p + 1);
+			if (p + P < planes) {
+				fetch(p + P, slot);
 			}
 			// the previous plane's outputs go out now and complete during this plane's filters
 			flush();

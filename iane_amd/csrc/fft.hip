@@ -100,7 +100,7 @@ namespace {
 template<int MODE>
 __global__ __launch_bounds__(kFftThreads) void fft_pass_kernel(fft_pass_args a)
 {
-	__shared__ float2 lds[2 * kFftLdsElems];
+	extern __shared__ float2 lds[];
 	fft_pass_body<MODE>(a, xcd_remap(blockIdx.x, gridDim.x), lds);
 }
 
@@ -125,7 +125,13 @@ void setup_pass(fft_pass_args& a, int aMode, int aN, int aLines, int aOuter)
 	a.tw_n = half ? twiddle_table(aN) : nullptr;
 	a.lines = aLines;
 	a.outer = aOuter;
-	a.bundle = std::max(1, std::min(aLines, kFftLdsElems / a.pitch));
+	// Lines per workgroup: as many as fit an LDS budget of "fft.lds_elems" complex
+	// elements per buffer (autotuned; default kFftLdsElems). A smaller budget means
+	// less LDS per workgroup and more resident workgroups per CU. Every line is
+	// transformed the same way whatever the bundle, so results do not depend on it.
+	const int budget = std::clamp(tune_get("fft.lds_elems", kFftLdsElems), a.pitch, kFftLdsElems);
+	a.bundle = std::max(1, std::min(aLines, budget / a.pitch));
+	a.lds_elems = a.bundle * a.pitch;
 	a.groups_per_outer = ceil_div(aLines, a.bundle);
 }
 
@@ -139,11 +145,12 @@ void launch_pass(const fft_pass_args& a, hipStream_t aStream)
 		fail(IANE_ERR_UNSUPPORTED, "FFT grid too large");
 	}
 	const unsigned nb = static_cast<unsigned>(blocks);
+	const size_t lds = pass_lds_bytes(a);
 	switch (a.mode) {
-	case PASS_C2C: fft_pass_kernel<PASS_C2C><<<nb, kFftThreads, 0, aStream>>>(a); break;
-	case PASS_R2C: fft_pass_kernel<PASS_R2C><<<nb, kFftThreads, 0, aStream>>>(a); break;
-	case PASS_C2R: fft_pass_kernel<PASS_C2R><<<nb, kFftThreads, 0, aStream>>>(a); break;
-	default: fft_pass_kernel<PASS_C2C_MULT><<<nb, kFftThreads, 0, aStream>>>(a); break;
+	case PASS_C2C: fft_pass_kernel<PASS_C2C><<<nb, kFftThreads, lds, aStream>>>(a); break;
+	case PASS_R2C: fft_pass_kernel<PASS_R2C><<<nb, kFftThreads, lds, aStream>>>(a); break;
+	case PASS_C2R: fft_pass_kernel<PASS_C2R><<<nb, kFftThreads, lds, aStream>>>(a); break;
+	default: fft_pass_kernel<PASS_C2C_MULT><<<nb, kFftThreads, lds, aStream>>>(a); break;
 	}
 	check_hip(hipGetLastError(), "FFT pass launch");
 }

@@ -259,6 +259,7 @@ struct fft_pass_args {
 	int outer;        // outer groups
 	int bundle;       // lines per workgroup
 	int pitch;        // LDS pitch per line (complex elements)
+	int lds_elems;    // complex elements per LDS buffer (bundle * pitch); two buffers
 	unsigned groups_per_outer;
 	fft_radix_plan plan; // the complex transform run in LDS (length m)
 	const float2* tw_m;
@@ -419,13 +420,16 @@ __device__ inline void store_real_lines(float* out, const line_geom& g, int aOut
 	}
 }
 
+// Dynamic LDS of a launch running pass `a`: two buffers of a.lds_elems.
+inline size_t pass_lds_bytes(const fft_pass_args& a) { return 2 * sizeof(float2) * static_cast<size_t>(a.lds_elems); }
+
 // One workgroup's share of a line pass: the lines of group `aBlock` of `a`.
-// `lds` holds two buffers of kFftLdsElems complex elements.
+// `lds` holds two buffers of a.lds_elems complex elements.
 template<int MODE>
 __device__ inline void fft_pass_body(const fft_pass_args& a, unsigned aBlock, float2* lds)
 {
 	float2* A = lds;
-	float2* B = lds + kFftLdsElems;
+	float2* B = lds + a.lds_elems;
 	int o, l0, count;
 	decode_group(a, aBlock, o, l0, count);
 	const int m = a.plan.m;

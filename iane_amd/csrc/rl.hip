@@ -176,9 +176,9 @@ struct rl_row_args {
 template<int EPI>
 __device__ inline void rl_row_body(const rl_row_args& a, unsigned aBlock, unsigned aSlot, float2* lds, double* shared)
 {
-	float2* A = lds;
-	float2* B = lds + kFftLdsElems;
 	const fft_pass_args& f = a.f;
+	float2* A = lds;
+	float2* B = lds + f.lds_elems;
 	int o, l0, count;
 	decode_group(f, aBlock, o, l0, count);
 	const int n = f.n;
@@ -239,7 +239,7 @@ __device__ inline void rl_row_body(const rl_row_args& a, unsigned aBlock, unsign
 template<int EPI>
 __global__ __launch_bounds__(kFftThreads) void rl_row_kernel(rl_row_args a)
 {
-	__shared__ float2 lds[2 * kFftLdsElems];
+	extern __shared__ float2 lds[];
 	__shared__ double shared[kFftThreads / 64];
 	// the partial of line group g goes to slot g (the batched kernel does the same)
 	const unsigned group = xcd_remap(blockIdx.x, gridDim.x);
@@ -267,7 +267,7 @@ template<int MODE>
 __global__ __launch_bounds__(kFftThreads) void fft_pass_batched_kernel(const fft_pass_args* args, const unsigned* offsets,
 	int count)
 {
-	__shared__ float2 lds[2 * kFftLdsElems];
+	extern __shared__ float2 lds[];
 	const unsigned b = xcd_remap(blockIdx.x, gridDim.x);
 	const int v = volume_of(offsets, count, b);
 	fft_pass_body<MODE>(args[v], b - offsets[v], lds);
@@ -277,7 +277,7 @@ template<int EPI>
 __global__ __launch_bounds__(kFftThreads) void rl_row_batched_kernel(const rl_row_args* args, const unsigned* offsets,
 	int count)
 {
-	__shared__ float2 lds[2 * kFftLdsElems];
+	extern __shared__ float2 lds[];
 	__shared__ double shared[kFftThreads / 64];
 	const unsigned b = xcd_remap(blockIdx.x, gridDim.x);
 	const int v = volume_of(offsets, count, b);
@@ -650,7 +650,7 @@ unsigned row_blocks(const fft_pass_args& a)
 template<int EPI>
 void launch_row(const rl_row_args& a, hipStream_t s)
 {
-	rl_row_kernel<EPI><<<row_blocks(a.f), kFftThreads, 0, s>>>(a);
+	rl_row_kernel<EPI><<<row_blocks(a.f), kFftThreads, pass_lds_bytes(a.f), s>>>(a);
 	check_hip(hipGetLastError(), "RL row launch");
 }
 
@@ -910,6 +910,7 @@ struct batched_launch {
 	Args* d_args = nullptr;
 	unsigned* d_offsets = nullptr;
 	unsigned blocks = 0;
+	size_t lds_bytes = 0;  // dynamic LDS of the launch: the largest of the volumes' passes
 
 	batched_launch() = default;
 	batched_launch(const batched_launch&) = delete;
@@ -927,6 +928,7 @@ struct batched_launch {
 		for (size_t v = 0; v < host.size(); ++v) {
 			const fft_pass_args& f = aPass(host[v]);
 			offsets[v + 1] = offsets[v] + static_cast<unsigned>(static_cast<long long>(f.groups_per_outer) * f.outer);
+			lds_bytes = std::max(lds_bytes, pass_lds_bytes(f));
 		}
 		blocks = offsets.back();
 		check_hip(hipMalloc(&d_args, sizeof(Args) * host.size()), "Allocate batch descriptors");
@@ -949,10 +951,10 @@ void launch_batched(const batched_launch<fft_pass_args>& b, int aMode, hipStream
 {
 	const int n = static_cast<int>(b.host.size());
 	switch (aMode) {
-	case PASS_C2C: fft_pass_batched_kernel<PASS_C2C><<<b.blocks, kFftThreads, 0, s>>>(b.d_args, b.d_offsets, n); break;
-	case PASS_R2C: fft_pass_batched_kernel<PASS_R2C><<<b.blocks, kFftThreads, 0, s>>>(b.d_args, b.d_offsets, n); break;
-	case PASS_C2R: fft_pass_batched_kernel<PASS_C2R><<<b.blocks, kFftThreads, 0, s>>>(b.d_args, b.d_offsets, n); break;
-	default: fft_pass_batched_kernel<PASS_C2C_MULT><<<b.blocks, kFftThreads, 0, s>>>(b.d_args, b.d_offsets, n); break;
+	case PASS_C2C: fft_pass_batched_kernel<PASS_C2C><<<b.blocks, kFftThreads, b.lds_bytes, s>>>(b.d_args, b.d_offsets, n); break;
+	case PASS_R2C: fft_pass_batched_kernel<PASS_R2C><<<b.blocks, kFftThreads, b.lds_bytes, s>>>(b.d_args, b.d_offsets, n); break;
+	case PASS_C2R: fft_pass_batched_kernel<PASS_C2R><<<b.blocks, kFftThreads, b.lds_bytes, s>>>(b.d_args, b.d_offsets, n); break;
+	default: fft_pass_batched_kernel<PASS_C2C_MULT><<<b.blocks, kFftThreads, b.lds_bytes, s>>>(b.d_args, b.d_offsets, n); break;
 	}
 }
 
@@ -1011,7 +1013,7 @@ struct iane_rl_batch {
 			launch_batched(*fwd[p], mid_modes[p], s);
 		}
 		for (int it = 0; it < aIterations; ++it) {
-			rl_row_batched_kernel<EPI_RATIO><<<ratio.blocks, kFftThreads, 0, s>>>(ratio.d_args, ratio.d_offsets, count);
+			rl_row_batched_kernel<EPI_RATIO><<<ratio.blocks, kFftThreads, ratio.lds_bytes, s>>>(ratio.d_args, ratio.d_offsets, count);
 			if (aLl) {
 				ll_finalize_batched_kernel<<<count, kPointThreads, 0, s>>>(ratio.host[0].partials, ratio.d_offsets,
 					aLl + it, aIterations);
@@ -1020,7 +1022,7 @@ struct iane_rl_batch {
 				launch_batched(*adj[p], mid_modes[p], s);
 			}
 			const auto& up = it + 1 < aIterations ? update_next : update_last;
-			rl_row_batched_kernel<EPI_UPDATE><<<up.blocks, kFftThreads, 0, s>>>(up.d_args, up.d_offsets, count);
+			rl_row_batched_kernel<EPI_UPDATE><<<up.blocks, kFftThreads, up.lds_bytes, s>>>(up.d_args, up.d_offsets, count);
 			if (it + 1 < aIterations) {
 				for (size_t p = 0; p < fwd.size(); ++p) {
 					launch_batched(*fwd[p], mid_modes[p], s);

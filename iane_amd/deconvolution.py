@@ -110,8 +110,12 @@ class rl_batch:
         self._workspace = torch.empty(max(nbytes.value, 1), dtype=torch.uint8, device="cuda")
 
     def run(self, observed, estimates, iterations: int, *, init: bool = True, ll_history=None,
-            use_graph: bool = True) -> hip_execution:
-        """``ll_history``: float64 device tensor of shape (count, iterations) or None."""
+            use_graph: bool | None = None) -> hip_execution:
+        """``ll_history``: float64 device tensor of shape (count, iterations) or None.
+        ``use_graph`` (default: whenever the batch runs on a created stream) replays
+        the iteration sequence as one hipGraph; graph capture needs a created stream."""
+        if use_graph is None:
+            use_graph = self.stream != 0
         import torch
         n = len(self.shapes)
         if len(observed) != n or len(estimates) != n:
@@ -150,7 +154,8 @@ def richardson_lucy(observed, psf, iterations: int = 20, epsilon: float = 1e-6, 
     import torch
     plan = rl_plan(observed.shape, psf, method, precision, epsilon, policy)
     estimate = warm_start.clone() if warm_start is not None else torch.empty_like(observed)
-    ll = torch.zeros(max(iterations, 1), dtype=torch.float64, device="cuda") if return_ll else None
+    # entries 0..iterations-1 are all written by the run (on the plan's stream)
+    ll = torch.empty(max(iterations, 1), dtype=torch.float64, device="cuda") if return_ll else None
     plan.run(observed, estimate, iterations, init=warm_start is None, ll_history=ll).wait()
     return (estimate, ll[:iterations]) if return_ll else estimate
 

@@ -46,6 +46,11 @@ def _ptr(t) -> int:
     return t.data_ptr()
 
 
+def _zero(t, stream) -> None:
+    """Clear a device tensor on `stream` (torch's own fill would run on torch's stream)."""
+    _lib.call("iane_hip_memset_async", _ptr(t), 0, t.numel() * t.element_size(), stream)
+
+
 @dataclasses.dataclass
 class _acts:
     u: object = None
@@ -85,6 +90,8 @@ class warm_start_net:
                 d[name] = torch.zeros_like(self.w32[name])
             for d in (self.gb, self.mb, self.vb):
                 d[name] = torch.zeros_like(self.b32[name])
+        # the fills and copies above ran on torch's stream; the network's kernels use self.stream
+        torch.cuda.current_stream().synchronize()
         self._refresh_bf16()
 
     # ------------------------------------------------------------ weights --
@@ -109,25 +116,31 @@ class warm_start_net:
     def load(self, path=WEIGHTS):
         import torch
         data = np.load(path, allow_pickle=False)
+        _lib.call("iane_hip_stream_sync", self.stream)
         for name, *_ in LAYERS:
             self.w32[name].copy_(torch.from_numpy(data[f"{name}_w"]))
             self.b32[name].copy_(torch.from_numpy(data[f"{name}_b"]))
+        torch.cuda.current_stream().synchronize()
         self.step_count = int(data["step"])
         self._refresh_bf16()
         return self
 
     # ------------------------------------------------------------ forward --
-    def _conv(self, name, x, shape, out, out_bf16=True, relu=True):
+    def _conv(self, name, x, shape, out, out_bf16=True, relu=True, stream=None):
         cin, cout = self.w32[name].shape[2], self.w32[name].shape[0]
         nz, ny, nx = shape
         _call("iane_hip_cnn_conv3d", _ptr(x), _ptr(self.wbf[name]), _ptr(self.b32[name]), _ptr(out),
-              1 if out_bf16 else 0, 1 if relu else 0, nx, ny, nz, cin, cout, self.stream)
+              1 if out_bf16 else 0, 1 if relu else 0, nx, ny, nz, cin, cout,
+              self.stream if stream is None else stream)
         return out
 
-    def forward(self, u, keep: bool = False):
+    def forward(self, u, keep: bool = False, stream=None):
         """u: float32 (nz, ny, nx) normalised volume, extents divisible by 4.
-        Returns r (float32 (nz, ny, nx)); prediction = u + r."""
+        Returns r (float32 (nz, ny, nx)); prediction = u + r. ``stream``
+        overrides the network's stream for this pass (inference on a caller's
+        stream; the weights must be ready on it)."""
         import torch
+        s = self.stream if stream is None else stream
         nz, ny, nx = u.shape
         if nz % 4 or ny % 4 or nx % 4:
             raise _lib.invalid_argument("warm-start network needs extents divisible by 4")
@@ -135,23 +148,23 @@ class warm_start_net:
         bf = torch.bfloat16
         a = _acts(u=u)
         a.ubf = torch.empty(s0 + (1,), dtype=bf, device="cuda")
-        _call("iane_hip_cnn_to_bf16", _ptr(u), _ptr(a.ubf), u.numel(), ctypes.c_float(1.0), self.stream)
-        a.a1a = self._conv("c1a", a.ubf, s0, torch.empty(s0 + (16,), dtype=bf, device="cuda"))
-        a.a1b = self._conv("c1b", a.a1a, s0, torch.empty(s0 + (16,), dtype=bf, device="cuda"))
+        _call("iane_hip_cnn_to_bf16", _ptr(u), _ptr(a.ubf), u.numel(), ctypes.c_float(1.0), s)
+        a.a1a = self._conv("c1a", a.ubf, s0, torch.empty(s0 + (16,), dtype=bf, device="cuda"), stream=s)
+        a.a1b = self._conv("c1b", a.a1a, s0, torch.empty(s0 + (16,), dtype=bf, device="cuda"), stream=s)
         a.p1 = torch.empty(s1 + (16,), dtype=bf, device="cuda")
-        _call("iane_hip_cnn_pool2", _ptr(a.a1b), _ptr(a.p1), nx, ny, nz, 16, self.stream)
-        a.a2a = self._conv("c2a", a.p1, s1, torch.empty(s1 + (32,), dtype=bf, device="cuda"))
+        _call("iane_hip_cnn_pool2", _ptr(a.a1b), _ptr(a.p1), nx, ny, nz, 16, s)
+        a.a2a = self._conv("c2a", a.p1, s1, torch.empty(s1 + (32,), dtype=bf, device="cuda"), stream=s)
         a.p2 = torch.empty(s2 + (32,), dtype=bf, device="cuda")
-        _call("iane_hip_cnn_pool2", _ptr(a.a2a), _ptr(a.p2), nx // 2, ny // 2, nz // 2, 32, self.stream)
-        a.a3a = self._conv("c3a", a.p2, s2, torch.empty(s2 + (64,), dtype=bf, device="cuda"))
-        a.a3b = self._conv("c3b", a.a3a, s2, torch.empty(s2 + (32,), dtype=bf, device="cuda"))
+        _call("iane_hip_cnn_pool2", _ptr(a.a2a), _ptr(a.p2), nx // 2, ny // 2, nz // 2, 32, s)
+        a.a3a = self._conv("c3a", a.p2, s2, torch.empty(s2 + (64,), dtype=bf, device="cuda"), stream=s)
+        a.a3b = self._conv("c3b", a.a3a, s2, torch.empty(s2 + (32,), dtype=bf, device="cuda"), stream=s)
         a.s2 = torch.empty(s1 + (32,), dtype=bf, device="cuda")
-        _call("iane_hip_cnn_up2_add", _ptr(a.a3b), _ptr(a.a2a), _ptr(a.s2), nx // 2, ny // 2, nz // 2, 32, self.stream)
-        a.a2b = self._conv("c2b", a.s2, s1, torch.empty(s1 + (16,), dtype=bf, device="cuda"))
+        _call("iane_hip_cnn_up2_add", _ptr(a.a3b), _ptr(a.a2a), _ptr(a.s2), nx // 2, ny // 2, nz // 2, 32, s)
+        a.a2b = self._conv("c2b", a.s2, s1, torch.empty(s1 + (16,), dtype=bf, device="cuda"), stream=s)
         a.s1 = torch.empty(s0 + (16,), dtype=bf, device="cuda")
-        _call("iane_hip_cnn_up2_add", _ptr(a.a2b), _ptr(a.a1b), _ptr(a.s1), nx, ny, nz, 16, self.stream)
+        _call("iane_hip_cnn_up2_add", _ptr(a.a2b), _ptr(a.a1b), _ptr(a.s1), nx, ny, nz, 16, s)
         a.r = self._conv("c1c", a.s1, s0, torch.empty(s0, dtype=torch.float32, device="cuda"), out_bf16=False,
-                         relu=False)
+                         relu=False, stream=s)
         if keep:
             self._saved = a
         return a.r
@@ -187,8 +200,8 @@ class warm_start_net:
         nz, ny, nx = a.u.shape
         s0, s1, s2 = (nz, ny, nx), (nz // 2, ny // 2, nx // 2), (nz // 4, ny // 4, nx // 4)
         for name, *_ in LAYERS:
-            self.gw[name].zero_()
-            self.gb[name].zero_()
+            _zero(self.gw[name], self.stream)
+            _zero(self.gb[name], self.stream)
         dr4 = dr.reshape(s0 + (1,))
         self._wgrad("c1c", a.s1, dr4, s0)
         ds1 = self._dgrad("c1c", dr4, s0)                                   # (s0, 16)
@@ -233,7 +246,7 @@ class warm_start_net:
         import torch
         r = self.forward(u, keep=True)
         grad = torch.empty_like(r)
-        partials = torch.zeros(1024, dtype=torch.float64, device="cuda")
+        partials = torch.empty(1024, dtype=torch.float64, device="cuda")  # the first `used` entries are written
         used = ctypes.c_int()
         _call("iane_hip_cnn_residual_mse", _ptr(u), _ptr(r), _ptr(target), 0, _ptr(grad), u.numel(), _ptr(partials),
               ctypes.byref(used), self.stream)
@@ -255,37 +268,73 @@ def normalisation(y, policy=None):
     return u, scale
 
 
+def _padded_extents(shape):
+    return tuple(-(-int(n) // 4) * 4 for n in shape)
+
+
 def warm_start(net: warm_start_net, y, epsilon: float = 1e-6, policy=None):
-    """x0 = max((u + net(u)) / scale, eps) for an observed float32 volume y."""
+    """x0 = max((u + net(u)) / scale, eps) for an observed float32 volume y.
+
+    Extents that are not multiples of 4 are zero-padded at the high end for the
+    network (two pooling levels) and the prediction is cropped back. Runs on the
+    policy's stream (default: the network's stream)."""
     import torch
-    stream = _stream(policy)
-    u, scale = normalisation(y, policy)
-    r = net.forward(u)
-    x0 = torch.empty_like(y)
-    _call("iane_hip_cnn_warm_start", _ptr(u), _ptr(r), _ptr(x0), y.numel(), ctypes.c_float(1.0 / scale),
+    from . import view_desc
+    stream = net.stream if policy is None else _stream(policy)
+    policy = hip_policy(stream)
+    shape = tuple(y.shape)
+    padded = _padded_extents(shape)
+    if padded != shape:
+        yp = torch.empty(padded, dtype=torch.float32, device=y.device)
+        _zero(yp, stream)
+        crop = yp[:shape[0], :shape[1], :shape[2]]
+        _call("iane_hip_copy_view", ctypes.byref(view_desc(y)), ctypes.byref(view_desc(crop)), 3, stream)
+    else:
+        yp = y
+    u, scale = normalisation(yp, policy)
+    r = net.forward(u, stream=stream)
+    xp = torch.empty_like(yp)
+    _call("iane_hip_cnn_warm_start", _ptr(u), _ptr(r), _ptr(xp), yp.numel(), ctypes.c_float(1.0 / scale),
           ctypes.c_float(epsilon), stream)
+    if padded == shape:
+        return xp
+    x0 = torch.empty_like(y)
+    _call("iane_hip_copy_view", ctypes.byref(view_desc(xp[:shape[0], :shape[1], :shape[2]])),
+          ctypes.byref(view_desc(x0)), 3, stream)
     return x0
 
 
 # ------------------------------------------------------------ training data --
-def make_pairs(count: int, shape=(128, 128, 128), iterations: int = 200, seed: int = 5, psf_sigma=(3.0, 1.5, 1.5),
-               psf_size=(63, 31, 31), peak: float = 1000.0, blobs_per_voxel: float = 300 / (256 * 256 * 128),
-               policy=None):
-    """``count`` (observed, RL-`iterations` result) pairs made on the GPU: blobs
+PSF_SIGMA = (3.0, 1.5, 1.5)
+PSF_SIZE = (63, 31, 31)
+PEAK = 1000.0
+BLOBS_PER_MEGAVOXEL = 300 / (256 * 256 * 128) * 1e6
+
+
+def psf_kernel(sigma=PSF_SIGMA, size=PSF_SIZE):
+    """The instrument PSF the network is trained for: anisotropic Gaussian, sigma
+    (3, 1.5, 1.5) voxels in (z, y, x) on a 63x31x31 support, unit sum (the PSF
+    of the 256x256x128 RL workload)."""
+    from . import synth
+    return synth.gaussian_kernel(sigma, size).astype(np.float32)
+
+
+def observed_volumes(count: int, shape=(128, 256, 256), seed: int = 5, psf=None, peak_range=(PEAK, PEAK),
+                     blobs_per_megavoxel: float = BLOBS_PER_MEGAVOXEL, policy=None):
+    """Yields ``count`` observed float32 device volumes made on the GPU: blobs
     (sigma U[1,4], amplitude U[0.1,1]) on 0.01 background -> blur by the PSF
-    (FFT, zero border) -> scale to `peak` -> Poisson -> plain RL."""
+    (FFT, zero border) -> scale to a peak U[peak_range] -> Poisson."""
     import torch
     from . import deconvolution, synth
     stream = _stream(policy)
-    pol = hip_policy(stream)
-    psf = torch.from_numpy(synth.gaussian_kernel(psf_sigma, psf_size).astype(np.float32)).cuda()
-    plan = deconvolution.rl_plan(shape, psf, deconvolution.FFT, policy=pol)
+    psf = torch.from_numpy(psf_kernel() if psf is None else np.asarray(psf, np.float32)).cuda()
+    plan = deconvolution.rl_plan(shape, psf, deconvolution.FFT, policy=hip_policy(stream))
     n = int(np.prod(shape))
-    nblobs = max(4, int(round(blobs_per_voxel * n)))
+    nblobs = max(4, int(round(blobs_per_megavoxel * n / 1e6)))
     truth = torch.empty(shape, dtype=torch.float32, device="cuda")
     blurred = torch.empty_like(truth)
     scratch = torch.empty(4097, dtype=torch.float32, device="cuda")
-    ys, xs = [], []
+    peaks = synth.uniform(seed, 8, count)
     for i in range(count):
         u = synth.uniform(seed * 100003 + i, 7, nblobs * 5).reshape(nblobs, 5)
         params = np.empty((nblobs, 5), np.float32)
@@ -299,11 +348,26 @@ def make_pairs(count: int, shape=(128, 128, 128), iterations: int = 200, seed: i
               nblobs, stream)
         _call("iane_hip_rl_plan_blur", plan._handle, _ptr(truth), _ptr(blurred), _ptr(plan._workspace), stream)
         _call("iane_hip_max", _ptr(blurred), n, _ptr(scratch), _ptr(scratch) + 4096 * 4, stream)
-        top = float(scratch[4096].item())
+        top = float(scratch[4096].item())  # synchronises: dparams is no longer in use after this
+        peak = peak_range[0] + peaks[i] * (peak_range[1] - peak_range[0])
         _call("iane_hip_scale", _ptr(blurred), n, ctypes.c_float(peak / max(top, 1e-12)), stream)
         y = torch.empty_like(truth)
         _call("iane_hip_synth_poisson", _ptr(blurred), _ptr(y), n, ctypes.c_uint(seed * 7919 + i), stream)
-        x = torch.empty_like(truth)
+        yield y
+
+
+def make_pairs(count: int, shape=(128, 128, 128), iterations: int = 200, seed: int = 5, psf=None,
+               peak_range=(PEAK, PEAK), blobs_per_megavoxel: float = BLOBS_PER_MEGAVOXEL, policy=None):
+    """``count`` (observed, RL-``iterations`` result) pairs: observed_volumes()
+    followed by plain RL from max(y, eps)."""
+    import torch
+    from . import deconvolution
+    stream = _stream(policy)
+    psf_t = torch.from_numpy(psf_kernel() if psf is None else np.asarray(psf, np.float32)).cuda()
+    plan = deconvolution.rl_plan(shape, psf_t, deconvolution.FFT, policy=hip_policy(stream))
+    ys, xs = [], []
+    for y in observed_volumes(count, shape, seed, psf, peak_range, blobs_per_megavoxel, policy):
+        x = torch.empty_like(y)
         plan.run(y, x, iterations)
         ys.append(y)
         xs.append(x)
@@ -311,8 +375,10 @@ def make_pairs(count: int, shape=(128, 128, 128), iterations: int = 200, seed: i
     return ys, xs
 
 
-def train(net: warm_start_net, ys, xs, steps: int, lr: float = 1e-3, log_every: int = 50, log=print, policy=None):
-    """Adam on the MSE between u + r and the normalised RL result; one pair per step."""
+def train(net: warm_start_net, ys, xs, steps: int, lr: float = 1e-3, log_every: int = 50, log=print, policy=None,
+          lr_final: float = 1e-5):
+    """Adam on the MSE between u + r and the normalised RL result; one pair per
+    step; the learning rate decays from ``lr`` to ``lr_final`` on a cosine."""
     import torch
     stream = _stream(policy)
     history = []
@@ -325,7 +391,8 @@ def train(net: warm_start_net, ys, xs, steps: int, lr: float = 1e-3, log_every: 
         _call("iane_hip_scale", _ptr(target), target.numel(), ctypes.c_float(scale), stream)
         partials, used, grad = net.loss_and_grad(u, target)
         net.backward(grad)
-        net.adam_step(lr)
+        frac = step / max(1, steps - 1)
+        net.adam_step(lr_final + 0.5 * (lr - lr_final) * (1.0 + np.cos(np.pi * frac)))
         if step % log_every == 0 or step == steps - 1:
             loss = float(partials[:used].cpu().numpy().sum()) / u.numel()
             history.append((step, loss))
@@ -336,3 +403,143 @@ def train(net: warm_start_net, ys, xs, steps: int, lr: float = 1e-3, log_every: 
 def synth_uniform_order(count: int, steps: int):
     from .synth import uniform
     return uniform(97, 3, max(count, steps))
+
+
+# --------------------------------------------------------------- evaluation --
+def _first_reaching(ll: np.ndarray, target: float) -> int:
+    """Index of the first log-likelihood >= target (ll[j] is that of x_j), or -1."""
+    hit = np.nonzero(ll >= target)[0]
+    return int(hit[0]) if hit.size else -1
+
+
+def iterations_to_likelihood(net: warm_start_net, observed, psf, iterations: int = 200, max_iterations: int = 400,
+                             epsilon: float = 1e-6, policy=None):
+    """For each observed volume: the log-likelihood of plain RL after
+    ``iterations`` steps (the target) and how many RL steps from the network's
+    warm start reach it. Returns a list of dicts (k = -1 if never reached)."""
+    import torch
+    from . import deconvolution
+    pol = policy or hip_policy()
+    out = []
+    for y in observed:
+        plan = deconvolution.rl_plan(tuple(y.shape), psf, deconvolution.FFT, epsilon=epsilon, policy=pol)
+        x = torch.empty_like(y)
+        ll = torch.empty(iterations + 1, dtype=torch.float64, device="cuda")  # every entry is written
+        plan.run(y, x, iterations + 1, ll_history=ll)
+        target = float(ll[iterations].item())
+        x0 = warm_start(net, y, epsilon, pol)
+        llw = torch.empty(max_iterations + 1, dtype=torch.float64, device="cuda")
+        plan.run(y, x0, max_iterations + 1, init=False, ll_history=llw)
+        lw = llw.cpu().numpy()
+        out.append({"shape": tuple(y.shape), "target": target, "ll_warm0": float(lw[0]),
+                    "ll_plain0": float(ll[0].item()), "k": _first_reaching(lw, target)})
+    return out
+
+
+def batch_time_to_likelihood(net: warm_start_net, observed, psfs, iterations: int = 200, max_iterations: int = 200,
+                             repeats: int = 3, epsilon: float = 1e-6, policy=None):
+    """GPU time for a batch of volumes to reach the log-likelihood of plain
+    ``iterations``-step RL: plain = one batched RL launch sequence of
+    ``iterations`` steps; learned = network inference for every volume + the
+    batched RL from the warm starts for k = max over volumes of the steps each
+    needs. Times are HIP-event milliseconds on the policy's stream (median of
+    ``repeats``). The batch is replayed as a hipGraph, which needs a created
+    stream: with the null stream a stream is made for the measurement."""
+    pol = policy or hip_policy()
+    if pol.handle() == 0:
+        handle = ctypes.c_void_p()
+        _lib.call("iane_hip_stream_create", ctypes.byref(handle))
+        _lib.call("iane_hip_device_sync")  # the inputs were made on the null stream
+        try:
+            return batch_time_to_likelihood(net, observed, psfs, iterations, max_iterations, repeats, epsilon,
+                                            hip_policy(handle.value))
+        finally:
+            _lib.call("iane_hip_stream_sync", handle.value)
+            _lib.call("iane_hip_stream_destroy", handle.value)
+    import torch
+    from . import deconvolution
+    stream = pol.handle()
+    batch = deconvolution.rl_batch([tuple(y.shape) for y in observed], psfs, epsilon, policy=pol)
+    n = len(observed)
+    xs = [torch.empty_like(y) for y in observed]
+    ll = torch.empty((n, iterations + 1), dtype=torch.float64, device="cuda")  # every entry is written
+    batch.run(observed, xs, iterations + 1, ll_history=ll)
+    targets = ll[:, iterations].cpu().numpy()
+    x0s = [warm_start(net, y, epsilon, pol) for y in observed]
+    llw = torch.empty((n, max_iterations + 1), dtype=torch.float64, device="cuda")
+    batch.run(observed, x0s, max_iterations + 1, init=False, ll_history=llw)
+    lw = llw.cpu().numpy()
+    ks = [_first_reaching(lw[v], targets[v]) for v in range(n)]
+    k = max(ks) if min(ks) >= 0 else -1
+
+    def timed(fn):
+        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        results = []
+        fn()  # untimed: graph capture and allocator warm-up
+        for _ in range(repeats):
+            _lib.call("iane_hip_stream_sync", stream)
+            start.record(torch.cuda.ExternalStream(stream))
+            fn()
+            stop.record(torch.cuda.ExternalStream(stream))
+            stop.synchronize()
+            results.append(start.elapsed_time(stop))
+        return float(np.median(results))
+
+    plain_ms = timed(lambda: batch.run(observed, xs, iterations))
+    learned_ms = float("nan")
+    infer_ms = timed(lambda: [warm_start(net, y, epsilon, pol) for y in observed])
+    if k >= 0:
+        def learned_path():
+            starts_now = [warm_start(net, y, epsilon, pol) for y in observed]
+            batch.run(observed, starts_now, k, init=False)
+        learned_ms = timed(learned_path)
+    return {"volumes": n, "plain_ms": plain_ms, "learned_ms": learned_ms, "inference_ms": infer_ms,
+            "k_max": k, "k_per_volume": ks, "speedup": plain_ms / learned_ms if k >= 0 else 0.0}
+
+
+# --------------------------------------------------------------- training CLI --
+def main(argv=None):
+    """python -m iane_amd.learned --pairs 2000 --steps 20000 --out weights.npz
+
+    The committed weights came from three runs, each starting from the last
+    (pre-training on a broader mix of PSFs and photon counts, then the
+    256x256x128 workload's distribution):
+      --pairs 384 --steps 6000 --lr 1e-3 --seed 5 --psf-size 19 11 11 --peak 200 2000 --blobs-per-mvox 150
+      --pairs 768 --steps 22000 --lr 6e-4 --seed 6 --psf-size 19 11 11 --peak 200 2000 --blobs-per-mvox 150
+      --pairs 2000 --steps 24000 --lr 5e-4 --seed 5            (defaults: 63x31x31 PSF, peak 1000)"""
+    import argparse
+    import time
+    parser = argparse.ArgumentParser(prog="python -m iane_amd.learned")
+    parser.add_argument("--pairs", type=int, default=2000)
+    parser.add_argument("--edge", type=int, default=128)
+    parser.add_argument("--steps", type=int, default=20000)
+    parser.add_argument("--lr", type=float, default=1e-3)
+    parser.add_argument("--seed", type=int, default=5)
+    parser.add_argument("--iterations", type=int, default=200)
+    parser.add_argument("--out", default=str(WEIGHTS))
+    parser.add_argument("--init", default=None, help="start from these weights")
+    parser.add_argument("--psf-size", type=int, nargs=3, default=list(PSF_SIZE), metavar=("Z", "Y", "X"),
+                        help="support of the Gaussian PSF, sigma (3, 1.5, 1.5)")
+    parser.add_argument("--peak", type=float, nargs=2, default=[PEAK, PEAK], metavar=("MIN", "MAX"),
+                        help="photon peak drawn uniformly from [MIN, MAX] per pair")
+    parser.add_argument("--blobs-per-mvox", type=float, default=BLOBS_PER_MEGAVOXEL)
+    args = parser.parse_args(argv)
+    net = warm_start_net(seed=args.seed)
+    if args.init:
+        net.load(args.init)
+        net.step_count = 0
+    t0 = time.perf_counter()
+    shape = (args.edge,) * 3
+    ys, xs = make_pairs(args.pairs, shape, args.iterations, args.seed, psf=psf_kernel(PSF_SIGMA, tuple(args.psf_size)),
+                        peak_range=tuple(args.peak), blobs_per_megavoxel=args.blobs_per_mvox)
+    print(f"[train] {args.pairs} pairs of {shape} in {time.perf_counter() - t0:.1f} s", flush=True)
+    t0 = time.perf_counter()
+    train(net, ys, xs, args.steps, lr=args.lr, log_every=max(1, args.steps // 40),
+          log=lambda m: print(m, flush=True))
+    print(f"[train] {args.steps} steps in {time.perf_counter() - t0:.1f} s", flush=True)
+    net.save(args.out)
+    print(f"[train] saved {args.out}", flush=True)
+
+
+if __name__ == "__main__":
+    main()

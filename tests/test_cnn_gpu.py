@@ -45,14 +45,17 @@ def test_conv3d_layer(cin, cout):
     x = cnn.bf16(rng.standard_normal(shape + (cin,)))
     w = cnn.bf16(rng.standard_normal((cout, 27, cin)) * 0.2)
     b = rng.standard_normal(cout).astype(np.float32)
+    # device copies stay referenced until the stream is synchronised: a dropped
+    # temporary would hand its memory to the next allocation before the launch runs
+    xd, wd, bd = bf(x), bf(w), dev(b)
     out = torch.empty(shape + (cout,), dtype=torch.float32, device="cuda")
-    _lib.call("iane_hip_cnn_conv3d", bf(x).data_ptr(), bf(w).data_ptr(), dev(b).data_ptr(), out.data_ptr(), 0, 0,
+    _lib.call("iane_hip_cnn_conv3d", xd.data_ptr(), wd.data_ptr(), bd.data_ptr(), out.data_ptr(), 0, 0,
               shape[2], shape[1], shape[0], cin, cout, 0)
     ref = cnn.conv3d(x, w, b.astype(np.float64))
     assert rel(out.cpu().numpy(), ref) <= TOL_LAYER
     # ReLU + BF16 output
     outb = torch.empty(shape + (cout,), dtype=torch.bfloat16, device="cuda")
-    _lib.call("iane_hip_cnn_conv3d", bf(x).data_ptr(), bf(w).data_ptr(), dev(b).data_ptr(), outb.data_ptr(), 1, 1,
+    _lib.call("iane_hip_cnn_conv3d", xd.data_ptr(), wd.data_ptr(), bd.data_ptr(), outb.data_ptr(), 1, 1,
               shape[2], shape[1], shape[0], cin, cout, 0)
     got = outb.float().cpu().numpy()
     np.testing.assert_allclose(got, cnn.bf16(np.maximum(ref, 0)), rtol=2 ** -7, atol=1e-6)
@@ -67,18 +70,19 @@ def test_input_and_weight_gradients(cin, cout):
     w = cnn.bf16(rng.standard_normal((cout, 27, cin)) * 0.2)
     dy = rng.standard_normal(shape + (cout,)).astype(np.float32)
     # weight gradient (accumulating into zeroed buffers)
+    xd, wd, dyd, dyb = bf(x), bf(w), dev(dy), bf(cnn.bf16(dy))
     dw = torch.zeros((cout, 27, cin), dtype=torch.float32, device="cuda")
     db = torch.zeros(cout, dtype=torch.float32, device="cuda")
-    _lib.call("iane_hip_cnn_wgrad", bf(x).data_ptr(), dev(dy).data_ptr(), dw.data_ptr(), db.data_ptr(),
+    _lib.call("iane_hip_cnn_wgrad", xd.data_ptr(), dyd.data_ptr(), dw.data_ptr(), db.data_ptr(),
               shape[2], shape[1], shape[0], cin, cout, 0)
     ref_dw, ref_db = cnn.conv3d_weight_grad(x, dy.astype(np.float64))
     assert rel(dw.cpu().numpy(), ref_dw) <= TOL_LAYER
     assert rel(db.cpu().numpy(), ref_db) <= TOL_LAYER
     # input gradient = conv3d with the transposed flipped weights
     wt = torch.empty((cin, 27, cout), dtype=torch.bfloat16, device="cuda")
-    _lib.call("iane_hip_cnn_transpose_weights", bf(w).data_ptr(), wt.data_ptr(), cin, cout, 0)
+    _lib.call("iane_hip_cnn_transpose_weights", wd.data_ptr(), wt.data_ptr(), cin, cout, 0)
     dx = torch.empty(shape + (cin,), dtype=torch.float32, device="cuda")
-    _lib.call("iane_hip_cnn_conv3d", bf(cnn.bf16(dy)).data_ptr(), wt.data_ptr(), 0, dx.data_ptr(), 0, 0,
+    _lib.call("iane_hip_cnn_conv3d", dyb.data_ptr(), wt.data_ptr(), 0, dx.data_ptr(), 0, 0,
               shape[2], shape[1], shape[0], cout, cin, 0)
     assert rel(dx.cpu().numpy(), cnn.conv3d_input_grad(cnn.bf16(dy), w)) <= TOL_LAYER
 
