@@ -102,6 +102,12 @@ def cpu_baseline(volume: np.ndarray, taps: np.ndarray, planes: int = 16):
             "median_s": median}
 
 
+def _lib_sync(stream) -> None:
+    """Wait for our stream before torch (on its own stream) reads results back."""
+    from iane_amd import _lib
+    _lib.call("iane_hip_stream_sync", stream)
+
+
 def bench_rl(timer, stream, iterations: int = 200):
     """Plain FFT Richardson–Lucy on the 256x256x128 volume with the 31x31x63 PSF:
     GPU time of `iterations` iterations (inputs resident) and the final log-likelihood."""
@@ -110,6 +116,8 @@ def bench_rl(timer, stream, iterations: int = 200):
     log("[bench] RL: generating 256x256x128 volume (seed 3)")
     _, psf, y = synth.rl_volume_data()
     dy, dpsf = torch.from_numpy(y).cuda(), torch.from_numpy(psf).cuda()
+    from iane_amd import autotune
+    tuned = autotune.tune_rl_plan(y.shape, dpsf, policy=hip_policy(stream))
     t0 = time.perf_counter()
     plan = deconvolution.rl_plan(y.shape, dpsf, deconvolution.FFT, policy=hip_policy(stream))
     est = torch.empty_like(dy)
@@ -117,13 +125,15 @@ def bench_rl(timer, stream, iterations: int = 200):
     plan.run(dy, est, 1).wait()
     prepare_s = time.perf_counter() - t0
     ms = timer.time(lambda: plan.run(dy, est, iterations, ll_history=ll), 1)
+    _lib_sync(stream)
     hist = ll.cpu().numpy()
     voxels = y.size
     log(f"[bench] RL: {iterations} iterations in {ms:.2f} ms ({ms / iterations * 1e3:.1f} us/iteration)")
     return {"iterations": iterations, "ms_total": round(ms, 3), "us_per_iteration": round(ms / iterations * 1e3, 2),
             "Mvoxel_iter_s": round(voxels * iterations / ms / 1e3, 1), "ll_final": float(hist[-1]),
             "ll_first": float(hist[0]), "prepare_s": round(prepare_s, 3),
-            "padded_domain": "280x280x160 (x, y, z) R2C", "workspace_bytes": plan.workspace_bytes}
+            "padded_domain": "280x280x160 (x, y, z) R2C", "workspace_bytes": plan.workspace_bytes,
+            "launch_shape": tuned}
 
 
 def bench_rl_batch(timer, stream, iterations: int = 100):
@@ -137,6 +147,9 @@ def bench_rl_batch(timer, stream, iterations: int = 100):
     obs = [torch.from_numpy(o).cuda() for o, _ in data]
     psfs = [torch.from_numpy(p).cuda() for _, p in data]
     est = [torch.empty_like(o) for o in obs]
+    from iane_amd import autotune
+    tuned = autotune.tune_rl_batch(obs, psfs, policy=policy)
+    log(f"[bench] RL batch: autotuned {tuned}")
     t0 = time.perf_counter()
     batch = deconvolution.rl_batch([o.shape for o in obs], psfs, policy=policy)
     prepare_s = time.perf_counter() - t0
@@ -151,7 +164,8 @@ def bench_rl_batch(timer, stream, iterations: int = 100):
     return {"volumes": len(obs), "voxels": voxels, "iterations": iterations, "graph_ms": round(graph_ms, 2),
             "batched_eager_ms": round(eager_ms, 2), "per_volume_plans_ms": round(single_ms, 2),
             "speedup_vs_per_volume": round(single_ms / graph_ms, 2),
-            "Mvoxel_iter_s": round(voxels * iterations / graph_ms / 1e3, 1), "prepare_s": round(prepare_s, 2)}
+            "Mvoxel_iter_s": round(voxels * iterations / graph_ms / 1e3, 1), "prepare_s": round(prepare_s, 2),
+            "launch_shape": tuned}
 
 
 def bench_learned(stream, volumes: int = 4):
@@ -188,6 +202,10 @@ def main() -> int:
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--quick", action="store_true", help="headline only")
+    ap.add_argument("--no-tune", action="store_true", help="default launch shapes (no autotuner launches)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also copy the 256^3 volume once (device to device, 64 MiB read + 64 MiB written): "
+                         "the reference dispatch for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -219,6 +237,10 @@ def main() -> int:
     def step():
         iane_amd.convolve_separable(vol, [taps, taps, taps], out, opts, policy)
 
+    from iane_amd import autotune
+    tuned = {} if args.no_tune else autotune.tune_separable(vol, [taps, taps, taps], out, opts, policy)
+    log(f"[bench] autotuned launch shape: {tuned}")
+
     log(f"[bench] separable 15^3 fp32: warmup {args.warmup}")
     for _ in range(args.warmup):
         step()
@@ -236,11 +258,15 @@ def main() -> int:
     achieved = bytes_per_launch / (ms_per_step / 1e3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles("sep3_fused_256"),
-                "kernel": "sep3_fused_kernel<15>", "bytes_per_launch": bytes_per_launch,
+                "kernel": "sep3_fused_kernel<15>", "launch_shape": tuned, "bytes_per_launch": bytes_per_launch,
                 "avg_launch_us": round(ms_per_step * 1e3, 2)}
     log(f"[bench] separable: {ms_per_step * 1e3:.1f} us/step, {value:.0f} Mvoxel/s, {achieved:.0f} GB/s")
 
     extra = {}
+    if args.copy_calibration:
+        _lib.call("iane_hip_memcpy_async", out.data_ptr(), vol.data_ptr(), voxels * 4, 3, stream)
+        _lib.call("iane_hip_stream_sync", stream)
+        extra["copy_calibration_bytes"] = {"read": voxels * 4, "written": voxels * 4}
     if not args.quick:
         # dense random 15^3 kernel, reference (FP64, bit-exact) arithmetic: compute bound
         dense = torch.from_numpy(data["dense"]).cuda()

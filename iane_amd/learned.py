@@ -46,6 +46,13 @@ def _ptr(t) -> int:
     return t.data_ptr()
 
 
+def _host(t, stream) -> np.ndarray:
+    """Copy a device tensor to the host after the work queued on `stream` is done
+    (torch's copy runs on torch's own stream, which does not wait for ours)."""
+    _lib.call("iane_hip_stream_sync", stream)
+    return t.cpu().numpy()
+
+
 def _zero(t, stream) -> None:
     """Clear a device tensor on `stream` (torch's own fill would run on torch's stream)."""
     _lib.call("iane_hip_memset_async", _ptr(t), 0, t.numel() * t.element_size(), stream)
@@ -108,8 +115,8 @@ class warm_start_net:
         path.parent.mkdir(parents=True, exist_ok=True)
         arrays = {}
         for name, *_ in LAYERS:
-            arrays[f"{name}_w"] = self.w32[name].cpu().numpy()
-            arrays[f"{name}_b"] = self.b32[name].cpu().numpy()
+            arrays[f"{name}_w"] = _host(self.w32[name], self.stream)
+            arrays[f"{name}_b"] = _host(self.b32[name], self.stream)
         arrays["step"] = np.int64(self.step_count)
         np.savez(path, **arrays)
 
@@ -260,7 +267,7 @@ def normalisation(y, policy=None):
     stream = _stream(policy)
     scratch = torch.empty(4097, dtype=torch.float32, device="cuda")
     _call("iane_hip_max", _ptr(y), y.numel(), _ptr(scratch), _ptr(scratch) + 4096 * 4, stream)
-    peak = float(scratch[4096].item())
+    peak = float(_host(scratch[4096:], stream)[0])
     scale = 4.0 / max(peak, 1e-6)
     u = torch.empty_like(y)
     _lib.call("iane_hip_memcpy_async", _ptr(u), _ptr(y), y.numel() * 4, 3, stream)
@@ -348,7 +355,7 @@ def observed_volumes(count: int, shape=(128, 256, 256), seed: int = 5, psf=None,
               nblobs, stream)
         _call("iane_hip_rl_plan_blur", plan._handle, _ptr(truth), _ptr(blurred), _ptr(plan._workspace), stream)
         _call("iane_hip_max", _ptr(blurred), n, _ptr(scratch), _ptr(scratch) + 4096 * 4, stream)
-        top = float(scratch[4096].item())  # synchronises: dparams is no longer in use after this
+        top = float(_host(scratch[4096:], stream)[0])  # synchronises: dparams is no longer in use after this
         peak = peak_range[0] + peaks[i] * (peak_range[1] - peak_range[0])
         _call("iane_hip_scale", _ptr(blurred), n, ctypes.c_float(peak / max(top, 1e-12)), stream)
         y = torch.empty_like(truth)
@@ -375,12 +382,14 @@ def make_pairs(count: int, shape=(128, 128, 128), iterations: int = 200, seed: i
     return ys, xs
 
 
-def train(net: warm_start_net, ys, xs, steps: int, lr: float = 1e-3, log_every: int = 50, log=print, policy=None,
+def train(net: warm_start_net, ys, xs, steps: int, lr: float = 1e-3, log_every: int = 50, log=print,
           lr_final: float = 1e-5):
     """Adam on the MSE between u + r and the normalised RL result; one pair per
-    step; the learning rate decays from ``lr`` to ``lr_final`` on a cosine."""
+    step; the learning rate decays from ``lr`` to ``lr_final`` on a cosine.
+    Runs on the network's stream (the pairs must be ready on it)."""
     import torch
-    stream = _stream(policy)
+    stream = net.stream
+    policy = hip_policy(stream)
     history = []
     order = np.argsort(synth_uniform_order(len(ys), steps))
     for step in range(steps):
@@ -394,7 +403,7 @@ def train(net: warm_start_net, ys, xs, steps: int, lr: float = 1e-3, log_every: 
         frac = step / max(1, steps - 1)
         net.adam_step(lr_final + 0.5 * (lr - lr_final) * (1.0 + np.cos(np.pi * frac)))
         if step % log_every == 0 or step == steps - 1:
-            loss = float(partials[:used].cpu().numpy().sum()) / u.numel()
+            loss = float(_host(partials[:used], stream).sum()) / u.numel()
             history.append((step, loss))
             log(f"[train] step {step} mse {loss:.5f}")
     return history
@@ -426,13 +435,14 @@ def iterations_to_likelihood(net: warm_start_net, observed, psf, iterations: int
         x = torch.empty_like(y)
         ll = torch.empty(iterations + 1, dtype=torch.float64, device="cuda")  # every entry is written
         plan.run(y, x, iterations + 1, ll_history=ll)
-        target = float(ll[iterations].item())
+        lp = _host(ll, pol.handle())
+        target = float(lp[iterations])
         x0 = warm_start(net, y, epsilon, pol)
         llw = torch.empty(max_iterations + 1, dtype=torch.float64, device="cuda")
         plan.run(y, x0, max_iterations + 1, init=False, ll_history=llw)
-        lw = llw.cpu().numpy()
+        lw = _host(llw, pol.handle())
         out.append({"shape": tuple(y.shape), "target": target, "ll_warm0": float(lw[0]),
-                    "ll_plain0": float(ll[0].item()), "k": _first_reaching(lw, target)})
+                    "ll_plain0": float(lp[0]), "k": _first_reaching(lw, target)})
     return out
 
 
@@ -464,11 +474,11 @@ def batch_time_to_likelihood(net: warm_start_net, observed, psfs, iterations: in
     xs = [torch.empty_like(y) for y in observed]
     ll = torch.empty((n, iterations + 1), dtype=torch.float64, device="cuda")  # every entry is written
     batch.run(observed, xs, iterations + 1, ll_history=ll)
-    targets = ll[:, iterations].cpu().numpy()
+    targets = _host(ll, stream)[:, iterations]
     x0s = [warm_start(net, y, epsilon, pol) for y in observed]
     llw = torch.empty((n, max_iterations + 1), dtype=torch.float64, device="cuda")
     batch.run(observed, x0s, max_iterations + 1, init=False, ll_history=llw)
-    lw = llw.cpu().numpy()
+    lw = _host(llw, stream)
     ks = [_first_reaching(lw[v], targets[v]) for v in range(n)]
     k = max(ks) if min(ks) >= 0 else -1
 
