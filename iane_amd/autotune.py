@@ -6,9 +6,13 @@ changes a result: every candidate computes each output sample with the same
 operations in the same order (tests check bit equality across candidates),
 so tuning only trades occupancy, LDS use and tile overlap.
 
-    sep3.threads    256 | 512              threads per workgroup, fused separable kernel
-    sep3.tile_y     16 | 32                output rows per workgroup tile
-    sep3.chunk      0 (auto) | 16 | 32 | 64   z planes streamed per workgroup
+    sep3.waves      2 | 3                  waves per SIMD the matrix engine's registers are fitted to
+    sep3.chunk      0 (auto) | 16 .. 64    z planes streamed per workgroup (both engines)
+    sep3.threads    256 | 512              threads per workgroup (vector engine)
+    sep3.tile_y     16 | 32                output rows per workgroup tile (vector engine)
+
+``sep3.engine`` (0: x/y filters on FP32 MFMA, 1: vector ALUs) is not tuned: the
+engines sum the same FP32 products in different orders, so it changes bits.
     fft.lds_elems   1024 | 2048 | 4096     LDS budget per buffer (complex elements) of the
                                            FFT line passes and RL row steps: lines per workgroup
 
@@ -26,7 +30,8 @@ import pathlib
 
 from . import _lib
 
-SEPARABLE_SPACE = {"sep3.threads": [512, 256], "sep3.tile_y": [16, 32], "sep3.chunk": [0, 16, 32, 64]}
+SEPARABLE_SPACE = {"sep3.waves": [2, 3], "sep3.chunk": [0, 16, 21, 24, 32, 43, 64]}  # matrix-core engine (default)
+VECTOR_SEPARABLE_SPACE = {"sep3.threads": [512, 256], "sep3.tile_y": [16, 32], "sep3.chunk": [0, 16, 32, 64]}
 FFT_SPACE = {"fft.lds_elems": [4096, 2048, 1024]}
 
 
@@ -152,10 +157,12 @@ def tune_separable(src, taps, dst, options=None, policy=None, repeats: int = 10,
     import iane_amd
     policy = policy or iane_amd.hip_policy()
     stream = policy.handle()
-    key = f"{device_name()}|sep3|{tuple(src.shape)}|{tuple(int(t.numel()) for t in taps)}"
+    engine = max(get("sep3.engine"), 0)
+    space = VECTOR_SEPARABLE_SPACE if engine == 1 else SEPARABLE_SPACE
+    key = f"{device_name()}|sep3.engine{engine}|{tuple(src.shape)}|{tuple(int(t.numel()) for t in taps)}"
     time, close = _event_timer(stream)
     try:
-        return search(key, SEPARABLE_SPACE,
+        return search(key, space,
                       lambda _: time(lambda: iane_amd.convolve_separable(src, taps, dst, options, policy), repeats),
                       use_cache=use_cache)
     finally:

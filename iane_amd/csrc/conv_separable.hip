@@ -6,13 +6,14 @@
 // range, and both factor over axes.
 //
 // Fast path (rank 3, FP32 accumulation, equal odd tap lengths K <= 15): one
-// fused kernel. A workgroup owns a 64 x 32 (x, y) output tile and streams a
+// fused kernel. A workgroup owns a 64 x 16 (x, y) output tile and streams a
 // chunk of z planes. For every input plane it stages the plane region with a
-// K-1 halo in x and y into LDS, filters along y (columns of the halo-wide
-// region), then along x, and pushes the filtered plane into a per-thread
-// register ring of K planes from which the z filter produces one output plane
-// per input plane. Input is read once per chunk (plus the z halo), output
-// written once.
+// K-1 halo in x and y into LDS, filters it along x and y, and pushes the
+// filtered plane into a per-thread register ring of K planes from which the z
+// filter produces one output plane per input plane. Input is read once per
+// chunk (plus the z halo), output written once. Two engines share this plan:
+// sep3_mfma_kernel runs the x and y filters as FP32 Toeplitz products on the
+// matrix cores (default), sep3_fused_kernel on the vector ALUs.
 //
 // General path: one 1-D pass per axis through stream-ordered temporaries.
 #include "common.hpp"
@@ -39,7 +40,16 @@ struct sep_args {
 	const char* taps[3];        // device tap vectors
 	long long tap_stride[3];
 	int tap_type[3];
+	int src_span, dst_span;     // bytes of one source / destination plane that a launch addresses (matrix engine)
 };
+
+// Raw buffer resource of one plane (GFX9 word 3: 32-bit data format). Loads
+// and stores through it take a scalar base and a VGPR byte offset, and
+// offsets at or past `aBytes` read 0 / are dropped.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_resource(const char* aBase, int aBytes)
+{
+	return __builtin_amdgcn_make_buffer_rsrc(const_cast<char*>(aBase), static_cast<short>(0), aBytes, 0x00020000);
+}
 
 __device__ __forceinline__ bool clamp_or_zero(int& aCoord, int aExtent, int aBorder)
 {
@@ -266,6 +276,237 @@ __global__ __launch_bounds__(kSepThreads) void sep3_fused_kernel(sep_args a)
 	flush();
 }
 
+// ---------------------------------------------------------------------------
+// Matrix-core variant of the fused kernel (FP32 MFMA, full FP32 products).
+//
+// The x and y filters of a 64 x 16 plane tile are Toeplitz products on
+// v_mfma_f32_16x16x4_f32. Wave w owns output columns 16w .. 16w+15:
+//   x pass  T (32 staged rows x 16 columns) = S (rows x 32-wide window) * X,
+//           X[k][n] = kx[n + 2C - k]: two row tiles, 8 K-steps each;
+//   y pass  U (16 rows x 16 columns) = Y * T, K = the 32 staged rows taken in
+//           the order in which the x pass leaves them in the accumulators
+//           (K-step s, lane group g <-> staged row 16 (s / 4) + 4 g + s % 4),
+//           so T feeds the y pass from registers without an LDS round trip;
+//           Y[m][row] = ky[m + 2C - row] in that permuted order.
+// The z filter stays a register ring on the vector ALUs (4 outputs per lane).
+// The staged plane is double-buffered in LDS: one barrier per plane.
+// MFMA lane layout (16x16x4 f32): lane l holds A[l % 16][l / 16],
+// B[l / 16][l % 16] and D[4 (l / 16) + i][l % 16], i = 0..3.
+constexpr int kMxTileX = 64;
+constexpr int kMxTileY = 16;
+constexpr int kMxRows = 32;   // staged rows: 16 + K - 1 <= 30, padded to two 16-row MFMA tiles
+constexpr int kMxCols = 80;   // staged columns: the last tile's 32-wide window ends at column 79
+constexpr int kMxPitch = 84;  // 16-byte aligned rows; 84 = 20 mod 32 and mod 64: a wave's 128-bit reads of 16 rows hit disjoint banks
+constexpr int kMxPerThread = kMxRows * kMxCols / 256;
+
+// LDS position of staged column c: columns are permuted inside blocks of 16 so
+// that the 4 A operands a lane needs from one block (columns 16b + 4a + g,
+// a = 0..3) are adjacent and load with one 128-bit read.
+__device__ __forceinline__ int mx_lds_column(int aCol)
+{
+	return (aCol & ~15) + 4 * (aCol & 3) + ((aCol >> 2) & 3);
+}
+
+typedef float mx_f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ mx_f32x4 mfma_f32_16x16x4(float a, float b, mx_f32x4 c)
+{
+	return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// W: waves per SIMD the register allocation is fitted to (2 or 3).
+template<int K, int W>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))) void sep3_mfma_kernel(sep_args a)
+{
+	constexpr int C = K / 2;
+	constexpr int P = (K % 3 == 0) ? 3 : 1; // prefetch depth in planes (P divides K: compile-time slots)
+	__shared__ float stage[2][kMxRows * kMxPitch];
+
+	const unsigned block = xcd_remap(blockIdx.x, gridDim.x);
+	const unsigned tile_x = block % a.tiles_x;
+	const unsigned rest = block / a.tiles_x;
+	const unsigned tile_y = rest % a.tiles_y;
+	const unsigned chunk = rest / a.tiles_y;
+	const int x0 = static_cast<int>(tile_x) * kMxTileX;
+	const int y0 = static_cast<int>(tile_y) * kMxTileY;
+	const int z0 = static_cast<int>(chunk) * a.chunk;
+	const int zend = min(z0 + a.chunk, a.nz);
+	const int tid = threadIdx.x;
+	const int lane = tid & 63;
+	const int wave = tid >> 6;
+	const int g = lane >> 4;
+	const int n = lane & 15;
+
+	// MFMA operands built from the taps (per lane, constant over the run)
+	float bx[8], ay[8], kz[K];
+	const auto tap = [&](int aAxis, int aIndex) {
+		return (aIndex >= 0 && aIndex < K) ? load_as_float(a.taps[aAxis] + aIndex * a.tap_stride[aAxis], a.tap_type[aAxis])
+		                                   : 0.0f;
+	};
+#pragma unroll
+	for (int s = 0; s < 8; ++s) {
+		bx[s] = tap(0, n + 2 * C - (4 * s + g));
+		ay[s] = tap(1, n + 2 * C - (16 * (s / 4) + 4 * g + s % 4));
+	}
+#pragma unroll
+	for (int i = 0; i < K; ++i) {
+		kz[i] = uniform_float(load_as_float(a.taps[2] + i * a.tap_stride[2], a.tap_type[2]));
+	}
+
+	// The plane loop below has no branch but its exit: every condition is an
+	// address instead. Loads and stores go through a per-plane buffer resource
+	// (scalar registers) and fixed 32-bit lane offsets; an element outside the
+	// image (zero border) or a padding element gets an offset past the plane
+	// and reads 0, a plane outside the volume (zero border) or past the chunk
+	// gets an empty resource, and a store that must not happen goes to an
+	// empty resource. (A branchy loop makes the compiler wait for all memory
+	// operations at every join, which serialises the prefetch.)
+	unsigned offset[kMxPerThread];
+#pragma unroll
+	for (int e = 0; e < kMxPerThread; ++e) {
+		const int i = tid + e * 256;
+		const int row = i / kMxCols;
+		const int col = i - row * kMxCols;
+		int xi = x0 - C + col;
+		int yi = y0 - C + row;
+		const bool valid = row < kMxTileY + K - 1 && col < kMxTileX + K - 1 && clamp_or_zero(xi, a.nx, a.border) &&
+			clamp_or_zero(yi, a.ny, a.border);
+		offset[e] = valid ? static_cast<unsigned>(xi * a.sx + yi * a.sy) : static_cast<unsigned>(a.src_span);
+	}
+	int lds_index[kMxPerThread];
+#pragma unroll
+	for (int e = 0; e < kMxPerThread; ++e) {
+		const int i = tid + e * 256;
+		const int row = i / kMxCols;
+		lds_index[e] = row * kMxPitch + mx_lds_column(i - row * kMxCols);
+	}
+	const int ox = x0 + 16 * wave + n;  // output column of this lane
+	const int oy = y0 + 4 * g;          // first of its 4 output rows
+	unsigned store_offset[4];
+#pragma unroll
+	for (int r = 0; r < 4; ++r) {
+		store_offset[r] = (ox < a.nx && oy + r < a.ny) ? static_cast<unsigned>(ox * a.dx + (oy + r) * a.dy)
+		                                               : static_cast<unsigned>(a.dst_span);
+	}
+
+	const int planes = (zend - z0) + K - 1; // input planes z0 - C .. zend - 1 + C
+	const auto source_plane = [&](int aPlane) {
+		int zi = z0 - C + aPlane;
+		const bool in = aPlane < planes && clamp_or_zero(zi, a.nz, a.border);
+		return plane_resource(a.src + (in ? zi : 0) * a.sz, in ? a.src_span : 0);
+	};
+	float ring[K][4];
+#pragma unroll
+	for (int j = 0; j < K; ++j) {
+#pragma unroll
+		for (int r = 0; r < 4; ++r) {
+			ring[j][r] = 0.0f;
+		}
+	}
+	float pre[P][kMxPerThread];
+	auto fetch = [&](int aPlane, int aSlot) {
+		const auto plane = source_plane(aPlane);
+#pragma unroll
+		for (int e = 0; e < kMxPerThread; ++e) {
+			pre[aSlot][e] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(plane, static_cast<int>(offset[e]), 0, 0));
+		}
+	};
+	auto stage_plane = [&](int aPlane, int aSlot) {
+		float* S = stage[aPlane & 1];
+#pragma unroll
+		for (int e = 0; e < kMxPerThread; ++e) {
+			S[lds_index[e]] = pre[aSlot][e];
+		}
+	};
+	// store the z-filtered output plane z0 + aPlane - (K - 1) (dropped unless in the chunk)
+	auto z_store = [&](int aPlane, const float* aOut) {
+		const int z = z0 + aPlane - (K - 1);
+		const bool ok = z >= z0 && z < zend;
+		// integer selects (not a branch: the z filter above must stay in this block)
+		const auto plane = plane_resource(a.dst + (ok ? z : 0) * a.dz, ok ? a.dst_span : 0);
+#pragma unroll
+		for (int r = 0; r < 4; ++r) {
+			// the b32 builtins move raw 32-bit words: bit casts, not conversions
+			__builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, aOut[r]), plane,
+				static_cast<int>(store_offset[r]), 0, 0);
+		}
+	};
+#pragma unroll
+	for (int q = 0; q < P; ++q) {
+		fetch(q, q);
+	}
+	stage_plane(0, 0);
+	fetch(P, 0);
+	// Software pipeline: iteration p runs the MFMA filters of plane p while the
+	// vector ALUs finish plane p - 1 (z filter over ring slots, store) and
+	// stage plane p + 1. Ring slot of plane q is q mod K.
+	// Iteration `planes` filters an empty plane and completes the last output.
+	float prev[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+	for (int base = 0; base <= planes; base += K) {
+#pragma unroll
+		for (int j = 0; j < K; ++j) {
+			const int p = base + j;
+			if (p > planes) {
+				break;
+			}
+			// buffer p & 1 is staged, and every wave is past plane p - 1, whose
+			// buffer (p + 1) & 1 is restaged below
+			lds_barrier();
+			const float* S = stage[p & 1];
+			// A operands of both row tiles first (four 128-bit reads), then the MFMA chains
+			mx_f32x4 av[2][2];
+#pragma unroll
+			for (int mt = 0; mt < 2; ++mt) {
+				const float* rowp = S + (16 * mt + n) * kMxPitch + 16 * wave + 4 * g;
+				av[mt][0] = *reinterpret_cast<const mx_f32x4*>(rowp);       // K-steps 0..3 (block w)
+				av[mt][1] = *reinterpret_cast<const mx_f32x4*>(rowp + 16);  // K-steps 4..7 (block w + 1)
+			}
+			// A wave issues in order, so the z filter of plane p - 1 (vector ALU)
+			// is written between the MFMAs of plane p: two taps per x-pass step,
+			// while the matrix pipe works on the previous MFMA.
+			const int zr = (j + K - 1) % K;  // ring slot of plane p - 1
+#pragma unroll
+			for (int r = 0; r < 4; ++r) {
+				ring[zr][r] = prev[r];
+			}
+			float zout[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+			mx_f32x4 t[2];
+#pragma unroll
+			for (int mt = 0; mt < 2; ++mt) {
+				t[mt] = mx_f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+			}
+#pragma unroll
+			for (int s = 0; s < 8; ++s) {
+#pragma unroll
+				for (int mt = 0; mt < 2; ++mt) {
+					t[mt] = mfma_f32_16x16x4(av[mt][s / 4][s % 4], bx[s], t[mt]);
+				}
+#pragma unroll
+				for (int q = 2 * s; q < 2 * s + 2 && q < K; ++q) {
+					const int ring_slot = ((zr - q) % K + K) % K;
+#pragma unroll
+					for (int r = 0; r < 4; ++r) {
+						zout[r] = fmaf(ring[ring_slot][r], kz[q], zout[r]);
+					}
+				}
+			}
+			mx_f32x4 u = mx_f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+			for (int s = 0; s < 8; ++s) {
+				u = mfma_f32_16x16x4(ay[s], t[s / 4][s % 4], u);
+			}
+			z_store(p - 1, zout);
+			const int next_slot = (j + 1) % P;
+			stage_plane(p + 1, next_slot);
+			fetch(p + 1 + P, next_slot);
+#pragma unroll
+			for (int r = 0; r < 4; ++r) {
+				prev[r] = u[r];
+			}
+		}
+	}
+}
+
 // One 1-D pass along `axis` of a rank-4-padded view; Acc-typed temporaries in
 // between passes, final pass converts to the destination type.
 template<class Acc>
@@ -317,12 +558,11 @@ view4 packed_temp(const view4& aLike, int aType, void* aPtr)
 	return v;
 }
 
-template<int K>
-void launch_fused(sep_args& a, hipStream_t aStream, int aThreads)
+// Tiles and z chunks of a fused launch; returns the workgroup count.
+unsigned plan_tiles(sep_args& a, int aTileX, int aTileY)
 {
-	const int tile_y = tune_get("sep3.tile_y", 16) == 32 ? 32 : 16;
-	a.tiles_x = ceil_div(a.nx, kSepTileX);
-	a.tiles_y = ceil_div(a.ny, tile_y);
+	a.tiles_x = ceil_div(a.nx, aTileX);
+	a.tiles_y = ceil_div(a.ny, aTileY);
 	// chunk so that the grid covers the chip at least twice: 256 CUs
 	const long long xy = static_cast<long long>(a.tiles_x) * a.tiles_y;
 	int chunk = tune_get("sep3.chunk", 0);
@@ -335,8 +575,25 @@ void launch_fused(sep_args& a, hipStream_t aStream, int aThreads)
 	}
 	a.chunk = chunk;
 	a.chunks_z = ceil_div(a.nz, chunk);
-	const long long blocks = xy * a.chunks_z;
-	const unsigned nb = static_cast<unsigned>(blocks);
+	return static_cast<unsigned>(xy * a.chunks_z);
+}
+
+template<int K>
+void launch_mfma(sep_args& a, hipStream_t aStream)
+{
+	const unsigned nb = plan_tiles(a, kMxTileX, kMxTileY);
+	if (tune_get("sep3.waves", 2) == 3) {
+		sep3_mfma_kernel<K, 3><<<nb, 256, 0, aStream>>>(a);
+	} else {
+		sep3_mfma_kernel<K, 2><<<nb, 256, 0, aStream>>>(a);
+	}
+}
+
+template<int K>
+void launch_fused(sep_args& a, hipStream_t aStream, int aThreads)
+{
+	const int tile_y = tune_get("sep3.tile_y", 16) == 32 ? 32 : 16;
+	const unsigned nb = plan_tiles(a, kSepTileX, tile_y);
 	if (aThreads == 512) {
 		if (tile_y == 32) {
 			sep3_fused_kernel<K, 512, 32><<<nb, 512, 0, aStream>>>(a);
@@ -389,6 +646,29 @@ void convolve_separable(const view4& aSrc, const view4* aTaps, int aN, const vie
 		a.dy = aDst.stride[1];
 		a.dz = aDst.stride[2];
 		a.border = aBorder;
+		// engine 0: x and y filters on the matrix cores (FP32 MFMA); engine 1: vector ALUs.
+		// Both compute FP32 products with FP32 sums, in different orders. The
+		// matrix engine addresses a plane with unsigned 32-bit lane offsets.
+		const long long src_span = (static_cast<long long>(a.nx) - 1) * a.sx + (static_cast<long long>(a.ny) - 1) * a.sy + 4;
+		const long long dst_span = (static_cast<long long>(a.nx) - 1) * a.dx + (static_cast<long long>(a.ny) - 1) * a.dy + 4;
+		const bool matrix = tune_get("sep3.engine", 0) != 1 && a.sx >= 0 && a.sy >= 0 && a.dx >= 0 && a.dy >= 0 &&
+			src_span < (1LL << 31) && dst_span < (1LL << 31);
+		if (matrix) {
+			a.src_span = static_cast<int>(src_span);
+			a.dst_span = static_cast<int>(dst_span);
+			switch (k) {
+			case 1: launch_mfma<1>(a, aStream); break;
+			case 3: launch_mfma<3>(a, aStream); break;
+			case 5: launch_mfma<5>(a, aStream); break;
+			case 7: launch_mfma<7>(a, aStream); break;
+			case 9: launch_mfma<9>(a, aStream); break;
+			case 11: launch_mfma<11>(a, aStream); break;
+			case 13: launch_mfma<13>(a, aStream); break;
+			default: launch_mfma<15>(a, aStream); break;
+			}
+			check_hip(hipGetLastError(), "Separable convolution launch");
+			return;
+		}
 		const int threads = tune_get("sep3.threads", 512);
 		switch (k) {
 		case 1: launch_fused<1>(a, aStream, threads); break;

@@ -13,14 +13,15 @@ def cache(tmp_path, monkeypatch):
     path = tmp_path / "tune.json"
     monkeypatch.setenv("IANE_AMD_TUNE_CACHE", str(path))
     yield path
-    for key in list(autotune.SEPARABLE_SPACE) + list(autotune.FFT_SPACE):
+    for key in list(autotune.VECTOR_SEPARABLE_SPACE) + list(autotune.FFT_SPACE):
         autotune.set(key, -1)
 
 
 def test_candidates_cover_the_space_default_first():
-    cands = autotune.candidates(autotune.SEPARABLE_SPACE)
+    cands = autotune.candidates(autotune.VECTOR_SEPARABLE_SPACE)
     assert len(cands) == 2 * 2 * 4
     assert cands[0] == {"sep3.threads": 512, "sep3.tile_y": 16, "sep3.chunk": 0}
+    assert autotune.candidates(autotune.SEPARABLE_SPACE)[0] == {"sep3.waves": 2, "sep3.chunk": 0}
     assert len({tuple(sorted(c.items())) for c in cands}) == len(cands)
 
 

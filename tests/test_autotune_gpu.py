@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 def reset_knobs(tmp_path, monkeypatch):
     monkeypatch.setenv("IANE_AMD_TUNE_CACHE", str(tmp_path / "tune.json"))
     yield
-    for key in list(autotune.SEPARABLE_SPACE) + list(autotune.FFT_SPACE):
+    for key in list(autotune.VECTOR_SEPARABLE_SPACE) + list(autotune.SEPARABLE_SPACE) + list(autotune.FFT_SPACE) + ["sep3.engine"]:
         autotune.set(key, -1)
 
 
@@ -38,6 +38,15 @@ def test_separable_candidates_bit_identical(border):
         with autotune.applied(config):
             iane_amd.convolve_separable(vol, taps, out, opts).wait()
         np.testing.assert_array_equal(out.cpu().numpy().view(np.uint32), ref.view(np.uint32), err_msg=str(config))
+    with autotune.applied({"sep3.engine": 1}):
+        ref = torch.empty_like(vol)
+        iane_amd.convolve_separable(vol, taps, ref, opts).wait()
+        ref = ref.cpu().numpy()
+        for config in autotune.candidates(autotune.VECTOR_SEPARABLE_SPACE):
+            out = torch.empty_like(vol)
+            with autotune.applied(config):
+                iane_amd.convolve_separable(vol, taps, out, opts).wait()
+            np.testing.assert_array_equal(out.cpu().numpy().view(np.uint32), ref.view(np.uint32), err_msg=str(config))
 
 
 def _rl_case():

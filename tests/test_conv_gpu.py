@@ -194,9 +194,12 @@ def test_fp32_precision_dense15(border):
 
 
 # ---------------------------------------------------- separable -----------
-@pytest.mark.parametrize("k", [3, 5, 7, 9, 11, 13, 15])
+@pytest.mark.parametrize("k", [1, 3, 5, 7, 9, 11, 13, 15])
 @pytest.mark.parametrize("border", [cases.CLAMP, cases.ZERO])
-def test_separable_fused_fp32(k, border):
+@pytest.mark.parametrize("engine", [0, 1], ids=["matrix", "vector"])
+def test_separable_fused_fp32(k, border, engine):
+    """Both fused engines (x/y filters on FP32 MFMA, or on the vector ALUs)."""
+    from iane_amd import autotune
     rng = np.random.default_rng(k)
     src = rng.random((37, 45, 70)).astype(np.float32)
     taps = [rng.random(k).astype(np.float32) for _ in range(3)]
@@ -204,9 +207,26 @@ def test_separable_fused_fp32(k, border):
     dense = np.multiply.outer(np.multiply.outer(t64[2], t64[1]), t64[0])
     ref64 = oracle.convolve(src, dense, np.float64, border, threads=8)
     out = empty(src.shape, np.float32)
-    iane_amd.convolve_separable(dev(src), [dev(t) for t in taps], out,
-                                convolution_options(border_mode(border), precision.fp32)).wait()
+    with autotune.applied({"sep3.engine": engine}):
+        iane_amd.convolve_separable(dev(src), [dev(t) for t in taps], out,
+                                    convolution_options(border_mode(border), precision.fp32)).wait()
     assert rel_err(host(out), ref64) <= TOL_FP32
+
+
+@pytest.mark.parametrize("shape", [(5, 16, 64), (40, 17, 65), (3, 100, 130), (70, 33, 257)])
+def test_separable_matrix_engine_ragged_tiles(shape):
+    """Tile edges in x, y and z (partial 64 x 16 tiles, short and long z chunks)."""
+    rng = np.random.default_rng(sum(shape))
+    src = (rng.random(shape) - 0.3).astype(np.float32)
+    taps = [rng.standard_normal(7).astype(np.float32) for _ in range(3)]
+    t64 = [t.astype(np.float64) for t in taps]
+    dense = np.multiply.outer(np.multiply.outer(t64[2], t64[1]), t64[0])
+    for border in (cases.CLAMP, cases.ZERO):
+        ref64 = oracle.convolve(src, dense, np.float64, border, threads=8)
+        out = empty(src.shape, np.float32)
+        iane_amd.convolve_separable(dev(src), [dev(t) for t in taps], out,
+                                    convolution_options(border_mode(border), precision.fp32)).wait()
+        assert rel_err(host(out), ref64) <= TOL_FP32
 
 
 @pytest.mark.parametrize("border", [cases.CLAMP, cases.ZERO])
