@@ -182,6 +182,9 @@ def bench_learned(stream, volumes: int = 4):
     psf = torch.from_numpy(learned.psf_kernel()).cuda()
     log(f"[bench] learned: {volumes} observed volumes 256x256x128")
     ys = list(learned.observed_volumes(volumes, (128, 256, 256), seed=3, policy=policy))
+    from iane_amd import autotune
+    tuned = autotune.tune_rl_batch(ys, [psf] * volumes, policy=policy)
+    log(f"[bench] learned: autotuned batch {tuned}")
     out = learned.batch_time_to_likelihood(net, ys, [psf] * volumes, iterations=200, max_iterations=120,
                                            policy=policy)
     log(f"[bench] learned: plain {out['plain_ms']:.1f} ms, learned {out['learned_ms']:.1f} ms "
@@ -191,6 +194,7 @@ def bench_learned(stream, volumes: int = 4):
     out["inference_ms"] = round(out["inference_ms"], 2)
     out["speedup"] = round(out["speedup"], 2)
     out["network_parameters"] = net.parameter_count()
+    out["launch_shape"] = tuned
     return out
 
 

@@ -33,6 +33,7 @@ from . import _lib
 SEPARABLE_SPACE = {"sep3.waves": [2, 3], "sep3.chunk": [0, 16, 21, 24, 32, 43, 64]}  # matrix-core engine (default)
 VECTOR_SEPARABLE_SPACE = {"sep3.threads": [512, 256], "sep3.tile_y": [16, 32], "sep3.chunk": [0, 16, 32, 64]}
 FFT_SPACE = {"fft.lds_elems": [4096, 2048, 1024]}
+RL_BATCH_SPACE = FFT_SPACE
 
 
 def get(key: str) -> int:
@@ -190,7 +191,7 @@ def tune_rl_batch(observed, psfs, iterations: int = 4, epsilon: float = 1e-6, po
         return ms
 
     try:
-        return search(key, FFT_SPACE, measure, use_cache=use_cache)
+        return search(key, RL_BATCH_SPACE, measure, use_cache=use_cache)
     finally:
         close()
 

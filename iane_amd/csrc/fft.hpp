@@ -197,7 +197,9 @@ __device__ inline void stockham_stage_generic(const float2* src, float2* dst, in
 
 // Run every stage of `plan` on the lines in buffer `a` (scratch `b`).
 // Returns the buffer holding the result. Ends with a barrier.
-__device__ inline float2* run_stages(const fft_radix_plan& plan, float2* a, float2* b, int bundle, int pitch,
+// Always inlined: as a called function it needs a scratch stack frame and
+// its register count caps the occupancy of every kernel that calls it.
+__device__ __forceinline__ float2* run_stages(const fft_radix_plan& plan, float2* a, float2* b, int bundle, int pitch,
 	const float2* tw, float sign)
 {
 	int Ns = 1;
@@ -426,7 +428,7 @@ inline size_t pass_lds_bytes(const fft_pass_args& a) { return 2 * sizeof(float2)
 // One workgroup's share of a line pass: the lines of group `aBlock` of `a`.
 // `lds` holds two buffers of a.lds_elems complex elements.
 template<int MODE>
-__device__ inline void fft_pass_body(const fft_pass_args& a, unsigned aBlock, float2* lds)
+__device__ __forceinline__ void fft_pass_body(const fft_pass_args& a, unsigned aBlock, float2* lds)
 {
 	float2* A = lds;
 	float2* B = lds + a.lds_elems;

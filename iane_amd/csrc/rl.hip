@@ -174,7 +174,8 @@ struct rl_row_args {
 // One workgroup's rows (group `aBlock` of `a`); the log-likelihood partial of
 // the group goes to a.partials[aSlot].
 template<int EPI>
-__device__ inline void rl_row_body(const rl_row_args& a, unsigned aBlock, unsigned aSlot, float2* lds, double* shared)
+__device__ __forceinline__ void rl_row_body(const rl_row_args& a, unsigned aBlock, unsigned aSlot, float2* lds,
+	double* shared)
 {
 	const fft_pass_args& f = a.f;
 	float2* A = lds;
@@ -263,25 +264,41 @@ __device__ __forceinline__ int volume_of(const unsigned* offsets, int count, uns
 	return lo;
 }
 
+// The workgroup copies its volume's descriptor into LDS once: the pass body
+// reads descriptor fields in its inner loops, and LDS reads of a uniform
+// address are far cheaper there than repeated global loads.
+template<class T>
+__device__ __forceinline__ void copy_descriptor(T& aDst, const T& aSrc)
+{
+	static_assert(sizeof(T) % 4 == 0 && sizeof(T) <= 4 * kFftThreads, "descriptor copied one word per thread");
+	if (threadIdx.x < sizeof(T) / 4) {
+		reinterpret_cast<int*>(&aDst)[threadIdx.x] = reinterpret_cast<const int*>(&aSrc)[threadIdx.x];
+	}
+	__syncthreads();
+}
 template<int MODE>
-__global__ __launch_bounds__(kFftThreads) void fft_pass_batched_kernel(const fft_pass_args* args, const unsigned* offsets,
-	int count)
+__global__ __launch_bounds__(kFftThreads) void fft_pass_batched_kernel(
+	const fft_pass_args* __restrict__ args, const unsigned* __restrict__ offsets, int count)
 {
 	extern __shared__ float2 lds[];
+	__shared__ fft_pass_args desc;
 	const unsigned b = xcd_remap(blockIdx.x, gridDim.x);
-	const int v = volume_of(offsets, count, b);
-	fft_pass_body<MODE>(args[v], b - offsets[v], lds);
+	const int v = __builtin_amdgcn_readfirstlane(volume_of(offsets, count, b));
+	copy_descriptor(desc, args[v]);
+	fft_pass_body<MODE>(desc, b - offsets[v], lds);
 }
 
 template<int EPI>
-__global__ __launch_bounds__(kFftThreads) void rl_row_batched_kernel(const rl_row_args* args, const unsigned* offsets,
-	int count)
+__global__ __launch_bounds__(kFftThreads) void rl_row_batched_kernel(
+	const rl_row_args* __restrict__ args, const unsigned* __restrict__ offsets, int count)
 {
 	extern __shared__ float2 lds[];
 	__shared__ double shared[kFftThreads / 64];
+	__shared__ rl_row_args desc;
 	const unsigned b = xcd_remap(blockIdx.x, gridDim.x);
-	const int v = volume_of(offsets, count, b);
-	rl_row_body<EPI>(args[v], b - offsets[v], b, lds, shared);
+	const int v = __builtin_amdgcn_readfirstlane(volume_of(offsets, count, b));
+	copy_descriptor(desc, args[v]);
+	rl_row_body<EPI>(desc, b - offsets[v], b, lds, shared);
 }
 
 // log-likelihood of volume v (partials offsets[v] .. offsets[v+1]-1) into out[v * stride]
@@ -958,6 +975,12 @@ void launch_batched(const batched_launch<fft_pass_args>& b, int aMode, hipStream
 	}
 }
 
+template<int EPI>
+void launch_row_batched(const batched_launch<rl_row_args>& b, int aCount, hipStream_t s)
+{
+	rl_row_batched_kernel<EPI><<<b.blocks, kFftThreads, b.lds_bytes, s>>>(b.d_args, b.d_offsets, aCount);
+}
+
 } // namespace
 
 struct iane_rl_batch {
@@ -1013,7 +1036,7 @@ struct iane_rl_batch {
 			launch_batched(*fwd[p], mid_modes[p], s);
 		}
 		for (int it = 0; it < aIterations; ++it) {
-			rl_row_batched_kernel<EPI_RATIO><<<ratio.blocks, kFftThreads, ratio.lds_bytes, s>>>(ratio.d_args, ratio.d_offsets, count);
+			launch_row_batched<EPI_RATIO>(ratio, count, s);
 			if (aLl) {
 				ll_finalize_batched_kernel<<<count, kPointThreads, 0, s>>>(ratio.host[0].partials, ratio.d_offsets,
 					aLl + it, aIterations);
@@ -1022,7 +1045,7 @@ struct iane_rl_batch {
 				launch_batched(*adj[p], mid_modes[p], s);
 			}
 			const auto& up = it + 1 < aIterations ? update_next : update_last;
-			rl_row_batched_kernel<EPI_UPDATE><<<up.blocks, kFftThreads, up.lds_bytes, s>>>(up.d_args, up.d_offsets, count);
+			launch_row_batched<EPI_UPDATE>(up, count, s);
 			if (it + 1 < aIterations) {
 				for (size_t p = 0; p < fwd.size(); ++p) {
 					launch_batched(*fwd[p], mid_modes[p], s);

@@ -14,7 +14,8 @@ pytestmark = pytest.mark.gpu
 def reset_knobs(tmp_path, monkeypatch):
     monkeypatch.setenv("IANE_AMD_TUNE_CACHE", str(tmp_path / "tune.json"))
     yield
-    for key in list(autotune.VECTOR_SEPARABLE_SPACE) + list(autotune.SEPARABLE_SPACE) + list(autotune.FFT_SPACE) + ["sep3.engine"]:
+    for key in (list(autotune.VECTOR_SEPARABLE_SPACE) + list(autotune.SEPARABLE_SPACE) + list(autotune.RL_BATCH_SPACE)
+                + ["sep3.engine"]):
         autotune.set(key, -1)
 
 
@@ -58,7 +59,7 @@ def test_fft_candidates_bit_identical_single_and_batched():
     import torch
     obs, psfs = _rl_case()
     results = {}
-    for config in autotune.candidates(autotune.FFT_SPACE):
+    for config in autotune.candidates(autotune.RL_BATCH_SPACE):
         with autotune.applied(config):
             single = []
             for o, p in zip(obs, psfs):
@@ -83,7 +84,7 @@ def test_tuning_runs_and_applies_a_candidate():
     import torch
     obs, psfs = _rl_case()
     best = autotune.tune_rl_batch(obs, psfs, iterations=2)
-    assert best["fft.lds_elems"] in autotune.FFT_SPACE["fft.lds_elems"]
+    assert best in autotune.candidates(autotune.RL_BATCH_SPACE)
     assert autotune.get("fft.lds_elems") == best["fft.lds_elems"]
     vol = dev(np.random.default_rng(1).random((40, 40, 64)).astype(np.float32))
     taps = [dev(synth.gaussian_1d(2.5, 15).astype(np.float32))] * 3
