@@ -9,7 +9,9 @@ iane_hip_convolve_separable call over the whole volume, inputs resident in HBM.
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
 
 Prints ONE JSON line. ``roofline`` is for the dominant kernel
-(sep3_fused_kernel<15>): achieved = algorithmic bytes per launch
+(sep3_mfma_kernel<15, W>: x/y filters on FP32 matrix cores, z filter on the
+vector ALUs; ``sep3.engine=1`` selects the all-vector sep3_fused_kernel):
+achieved = algorithmic bytes per launch
 (256^3 voxels x (4 B read + 4 B written)) / average launch time measured
 with HIP events on the stream the kernel runs on. ``cpu_baseline`` times
 the reference's own CPU convolve (compiled from its headers into
@@ -260,9 +262,13 @@ def main() -> int:
     value = voxels / (ms_per_step / 1e3) / 1e6
     bytes_per_launch = voxels * 8.0
     achieved = bytes_per_launch / (ms_per_step / 1e3) / 1e9
+    if autotune.get("sep3.engine") == 1:
+        kernel, tag = "sep3_fused_kernel<15>", "sep3_fused_256"
+    else:
+        kernel, tag = f"sep3_mfma_kernel<15, {3 if autotune.get('sep3.waves') == 3 else 2}>", "sep3_mfma_256"
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles("sep3_fused_256"),
-                "kernel": "sep3_fused_kernel<15>", "launch_shape": tuned, "bytes_per_launch": bytes_per_launch,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(tag),
+                "kernel": kernel, "launch_shape": tuned, "bytes_per_launch": bytes_per_launch,
                 "avg_launch_us": round(ms_per_step * 1e3, 2)}
     log(f"[bench] separable: {ms_per_step * 1e3:.1f} us/step, {value:.0f} Mvoxel/s, {achieved:.0f} GB/s")
 
