@@ -128,7 +128,8 @@ def _stream(policy) -> int:
 
 
 def convolve(src, kernel, dst, options: convolution_options | None = None, policy=None) -> hip_execution:
-    """iane::convolve for device images (convolution.hpp:97-117).
+    """iane::convolve for device images (convolution.hpp:97-117); ``src`` may be a
+    texture view (docs/cuda_texture_images.md:5-17).
 
     ``dst[p] = sum_q kernel[q] * src[p + c - q]`` with ``c = extent // 2`` per axis.
     Raises ``invalid_argument`` for even/non-positive kernel extents, rank or
@@ -136,6 +137,8 @@ def convolve(src, kernel, dst, options: convolution_options | None = None, polic
     """
     options = options or convolution_options()
     stream = _stream(policy)
+    if isinstance(src, texture_image_view):
+        src = src.texels()  # texture source: its texels, border rule at the view boundary
     s, k, d = view_desc(src), view_desc(kernel), view_desc(dst)
     _lib.call("iane_hip_convolve", s, k, d, int(options.border), int(options.precision), stream)
     return hip_execution(stream)
@@ -162,6 +165,10 @@ def convolve_untiled(src, kernel, dst, options: convolution_options | None = Non
               int(options.border), int(options.precision), stream)
     return hip_execution(stream)
 
+
+from .texture import copy, texture_image, texture_image_view  # noqa: E402
+
+__all__ += ["texture_image", "texture_image_view", "copy"]
 
 from .prepared import (execution_context, fft_forward, fft_inverse, irfft, make_execution_context,  # noqa: E402
                        rfft, rfft_extent)

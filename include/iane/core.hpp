@@ -500,4 +500,56 @@ iane_view_desc describe(const V& aView, int aCode = element_code<typename V::val
 
 } // namespace detail
 
+// ------------------------------------------------------- texture views ----
+/// Texel types of texture images (include/iane/view/texture_image_view.hpp:13-18).
+template<class T>
+concept texture_element = std::same_as<T, float> || std::same_as<T, signed char> || std::same_as<T, unsigned char> ||
+	std::same_as<T, short> || std::same_as<T, unsigned short> || std::same_as<T, int> || std::same_as<T, unsigned int>;
+
+/**
+ * Read-only texture view (texture_image_view.hpp:20-98). gfx950 offers no
+ * image/texture sampling to HIP, so the texels live in device memory and the
+ * view reads them as a const device view with the reference's lookup rules:
+ * an integer lookup returns the texel without interpolation or normalisation,
+ * coordinates are local to the view, and a subview keeps the storage and moves
+ * its origin (a crop is an independent image for the border rule). Like the
+ * reference it is a readable device view, but neither a memory view nor a
+ * writable one.
+ */
+template<texture_element T, int N = 2>
+requires(N >= 1 && N <= 3)
+class texture_image_view {
+public:
+	using value_type = T;
+	using coord_t = typename math::dim_traits<N>::coord_t;
+	using extents_t = typename math::dim_traits<N>::extents_t;
+	struct is_device_view_tag {};
+	static constexpr int dimension = N;
+
+	explicit texture_image_view(const_device_image_view<T, N> aTexels) : mTexels(aTexels) {}
+	/// Logical extent in texels.
+	extents_t dimensions() const { return mTexels.dimensions(); }
+	/// In-bounds region whose coordinates start at zero (texture_image_view.hpp:74-80).
+	texture_image_view subview(const coord_t& aCorner, const extents_t& aSize) const
+	{
+		return texture_image_view(mTexels.subview(aCorner, aSize));
+	}
+	/// The texels of this view as device memory (how this backend stores a texture).
+	const_device_image_view<T, N> texels() const { return mTexels; }
+
+private:
+	const_device_image_view<T, N> mTexels;
+};
+
+namespace detail {
+
+// A texture view goes to the C-ABI as the device memory view of its texels.
+template<class T, int N>
+iane_view_desc describe(const texture_image_view<T, N>& aView, int aCode = element_code<T>())
+{
+	return describe(aView.texels(), aCode);
+}
+
+} // namespace detail
+
 } // namespace iane

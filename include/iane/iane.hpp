@@ -6,3 +6,4 @@
 #include <iane/prepared.hpp>
 #include <iane/deconvolution.hpp>
 #include <iane/run_on.hpp>
+#include <iane/texture.hpp>

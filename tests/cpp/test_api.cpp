@@ -10,7 +10,8 @@
 // (scalar coordinates; flip + crop, both borders), test_convolution_cuda.cu:53-86
 // (device equals host exactly), test_for_each.cpp:43-56 (policy validation),
 // prepared_cases.hpp:73-123 and real_fft_cases.hpp:12-77 (FFT), and the RL loop of
-// examples/rl_deconvolution/main.cu:51-81 (device DIRECT/reference == host loop, bitwise).
+// examples/rl_deconvolution/main.cu:51-81 (device DIRECT/reference == host loop, bitwise),
+// and the texture-image cases of test_convolution_cuda.cu:10-163.
 #include <iane/iane.hpp>
 
 #include <cmath>
@@ -366,6 +367,166 @@ void rl_device_equals_host_loop()
 	CHECK(std::memcmp(result.pointer(), estimate.pointer(), sizeof(float) * size[0] * size[1]) == 0);
 }
 
+// Texture images (test_convolution_cuda.cu:10-163): uploads from strided host and
+// device views, convolution of a cropped texture view equal to the host
+// convolve for both borders, empty textures and moves, nested 1-D crops, and
+// 3-D uploads whose slice stride is not a whole number of rows.
+static_assert(iane::concepts::DeviceView<iane::texture_image_view<float>>);
+static_assert(!iane::concepts::MemoryView<iane::texture_image_view<float>>);
+static_assert(!iane::concepts::HostView<iane::texture_image_view<float>>);
+
+void texture_upload_from_strided_views()
+{
+	using coord_t = iane::math::vec2i;
+	const coord_t size{2, 2};
+	iane::host_image<int, 2> host(coord_t{4, 3});
+	for (int i = 0; i < 12; ++i) {
+		host.pointer()[i] = i + 101;
+	}
+	iane::host_image<int, 2> output(size);
+	iane::texture_image<int> texture(size);
+	iane::device_image<int, 2> linear(host.dimensions());
+	iane::device_image<int, 2> materialized(size);
+	void* stream = nullptr;
+	iane::detail::check_status(iane_hip_stream_create(&stream));
+	const iane::hip_policy policy{stream};
+	check_throws<std::invalid_argument>([&] { iane::copy(host.const_view(), texture, policy); }, "upload extent mismatch");
+	const auto verify = [&](int aOffset) {
+		iane::copy(texture.const_view(), materialized.view(), policy);
+		iane::copy(materialized.const_view(), output.view(), policy).wait();
+		const int expected[]{106, 107, 110, 111};
+		for (int i = 0; i < 4; ++i) {
+			CHECK(output.pointer()[i] == expected[i] + aOffset);
+		}
+	};
+	const coord_t origin{1, 1};
+	iane::copy(host.const_view().subview(origin, size), texture, policy);
+	verify(0);
+	for (int i = 0; i < 12; ++i) {
+		host.pointer()[i] += 100;
+	}
+	iane::copy(host.const_view(), linear.view(), policy);
+	iane::copy(linear.const_view().subview(origin, size), texture, policy);
+	verify(100);
+	iane::detail::check_status(iane_hip_stream_destroy(stream));
+}
+
+void texture_convolution_matches_cpu()
+{
+	using coord_t = iane::math::vec2i;
+	const coord_t size{5, 4};
+	const coord_t crop_size{3, 3};
+	const coord_t origin{1, 1};
+	iane::host_image<float, 2> source(size), kernel(coord_t{3, 3}), reference(crop_size), output(crop_size);
+	for (int i = 0; i < 20; ++i) {
+		source.pointer()[i] = static_cast<float>(i - 6);
+	}
+	for (int i = 0; i < 9; ++i) {
+		kernel.pointer()[i] = static_cast<float>(i - 3) / 8.0f;
+	}
+	iane::texture_image<float> texture(size);
+	iane::device_image<float, 2> device_kernel(kernel.dimensions()), device_output(crop_size);
+	iane::copy(source.const_view(), texture).wait();
+	iane::copy(kernel.const_view(), device_kernel.view()).wait();
+	for (const auto border : {iane::border_mode::clamp, iane::border_mode::zero}) {
+		iane::convolve(source.const_view().subview(origin, crop_size), kernel.const_view(), reference.view(),
+			{.border = border});
+		iane::convolve(texture.const_view().subview(origin, crop_size), device_kernel.const_view(), device_output.view(),
+			{.border = border});
+		iane::copy(device_output.const_view(), output.view()).wait();
+		for (int i = 0; i < 9; ++i) {
+			CHECK(output.pointer()[i] == reference.pointer()[i]);
+		}
+	}
+}
+
+void empty_texture_and_move()
+{
+	using coord_t = iane::math::vec2i;
+	const coord_t size{0, 3};
+	iane::texture_image<float> empty(size);
+	auto moved = std::move(empty);
+	iane::host_image<float, 2> source(size);
+	iane::copy(source.const_view(), moved).wait();
+	CHECK(moved.const_view().dimensions()[0] == size[0]);
+	CHECK(moved.const_view().dimensions()[1] == size[1]);
+	CHECK(moved.array() == nullptr);
+	check_throws<std::invalid_argument>([] { iane::texture_image<float> bad(coord_t{-1, 2}); }, "negative extent");
+}
+
+void texture_1d_nested_crop()
+{
+	iane::host_image<int, 1> source(6);
+	for (int i = 0; i < 6; ++i) {
+		source.view()[i] = 100 + i;
+	}
+	iane::texture_image<int, 1> texture(4);
+	iane::device_image<int, 1> materialized(2);
+	iane::host_image<int, 1> output(2);
+	iane::copy(source.const_view().subview(1, 4), texture);
+	iane::copy(texture.const_view().subview(1, 3).subview(1, 2), materialized.view());
+	iane::copy(materialized.const_view(), output.view()).wait();
+	CHECK(output.view()[0] == 103);
+	CHECK(output.view()[1] == 104);
+}
+
+void texture_3d_upload_from_strided_views()
+{
+	using coord_t = iane::math::vec3i;
+	const coord_t size{3, 2, 3};
+	const coord_t crop_size{2, 2, 2};
+	const coord_t origin{1, 0, 1};
+	iane::texture_image<int, 3> texture(size);
+	iane::device_image<int, 3> materialized(crop_size);
+	iane::host_image<int, 3> output(crop_size);
+	const auto verify = [&](int aOffset) {
+		iane::copy(texture.const_view().subview(origin, crop_size), materialized.view());
+		iane::copy(materialized.const_view(), output.view()).wait();
+		for (int z = 0; z < 2; ++z) {
+			for (int y = 0; y < 2; ++y) {
+				for (int x = 0; x < 2; ++x) {
+					CHECK((output.view()[coord_t{x, y, z}] == 100 * (z + 1) + 10 * y + x + 1 + aOffset));
+				}
+			}
+		}
+	};
+	// a slice stride (44 bytes) that is not a whole number of row pitches (16 bytes)
+	int padded[33]{};
+	const iane::host_image_view<int, 3> host(padded, size, coord_t{4, 16, 44});
+	for (int z = 0; z < 3; ++z) {
+		for (int y = 0; y < 2; ++y) {
+			for (int x = 0; x < 3; ++x) {
+				host[coord_t{x, y, z}] = 100 * z + 10 * y + x;
+			}
+		}
+	}
+	iane::copy(host, texture);
+	verify(0);
+	// a cropped device view as the source
+	const coord_t storage_size{5, 4, 4};
+	const coord_t storage_origin{1, 1, 1};
+	iane::host_image<int, 3> packed(storage_size);
+	for (int i = 0; i < 5 * 4 * 4; ++i) {
+		packed.pointer()[i] = -1;
+	}
+	auto crop = packed.view().subview(storage_origin, size);
+	for (int z = 0; z < 3; ++z) {
+		for (int y = 0; y < 2; ++y) {
+			for (int x = 0; x < 3; ++x) {
+				crop[coord_t{x, y, z}] = host[coord_t{x, y, z}] + 1000;
+			}
+		}
+	}
+	iane::device_image<int, 3> linear(storage_size);
+	iane::copy(packed.const_view(), linear.view());
+	iane::copy(linear.const_view().subview(storage_origin, size), texture);
+	verify(1000);
+	// rows must be contiguous
+	const iane::host_image_view<int, 3> strided_rows(padded, coord_t{2, 2, 2}, coord_t{8, 16, 44});
+	iane::texture_image<int, 3> small(coord_t{2, 2, 2});
+	check_throws<std::invalid_argument>([&] { iane::copy(strided_rows, small); }, "non-contiguous rows");
+}
+
 } // namespace
 
 int main(int argc, char** argv)
@@ -394,6 +555,11 @@ int main(int argc, char** argv)
 		real_fft_known_answers({5, 3});
 		real_fft_known_answers({6, 4});
 		rl_device_equals_host_loop();
+		texture_upload_from_strided_views();
+		texture_convolution_matches_cpu();
+		empty_texture_and_move();
+		texture_1d_nested_crop();
+		texture_3d_upload_from_strided_views();
 	}
 	std::printf("%s: %d checks, %d failures\n", device ? "device" : "host", g_checks, g_failures);
 	return g_failures ? 1 : 0;
