@@ -170,6 +170,36 @@ def bench_rl_batch(timer, stream, iterations: int = 100):
             "launch_shape": tuned}
 
 
+def bench_sequence(timer, stream, vol):
+    """Scan / reduce on the 256^3 f32 volume: a whole f64-accumulated sum (the
+    log-likelihood reduction pattern; 4 B read per voxel), an f32 inclusive scan
+    in storage order (4 B read + 4 B written) and per-row sums along x."""
+    import torch
+    import iane_amd
+    policy = iane_amd.hip_policy(stream)
+    ctx = iane_amd.make_execution_context(policy)
+    voxels = vol.numel()
+    total = torch.empty(1, dtype=torch.float64, device="cuda")
+    scan = torch.empty_like(vol)
+    rows = torch.empty(vol.shape[:-1], dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    legs = (("reduce_sum_f32_to_f64", iane_amd.reduce("sum", 0.0), total, 4.0),
+            ("inclusive_scan_sum_f32", iane_amd.inclusive_scan("sum", 0.0), scan, 8.0),
+            ("reduce_along_x_sum_f32", iane_amd.reduce_along(0, "sum", 0.0), rows, 4.0 + 4.0 / vol.shape[-1]))
+    out = {}
+    for name, algorithm, dst, bytes_per_voxel in legs:
+        op = ctx.prepare(algorithm, vol, dst)
+        fn = lambda: ctx.run(op, vol, dst)  # noqa: E731
+        fn()
+        _lib_sync(stream)
+        k = 20
+        t = timer.time(fn, k) / k
+        gbs = voxels * bytes_per_voxel / t / 1e6
+        out[name] = {"us": round(t * 1e3, 2), "GB_s": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 3)}
+        log(f"[bench] {name}: {t * 1e3:.1f} us, {gbs:.0f} GB/s")
+    return out
+
+
 def bench_learned(stream, volumes: int = 4):
     """Learned warm start + batched RL vs plain batched RL on 256x256x128 volumes
     (blobs, PSF sigma (3, 1.5, 1.5) on 63x31x31, peak 1000, Poisson; made on the
@@ -314,6 +344,7 @@ def main() -> int:
         log(f"[bench] u16 3^3: {t * 1e3:.1f} us")
 
     if not args.quick:
+        extra["sequence_256"] = bench_sequence(timer, stream, vol)
         extra["rl_fft_256x256x128_psf31x31x63"] = bench_rl(timer, stream)
         extra["rl_batch_64_volumes"] = bench_rl_batch(timer, stream)
         extra["rl_learned_vs_plain_256x256x128"] = bench_learned(stream)

@@ -167,8 +167,11 @@ def convolve_untiled(src, kernel, dst, options: convolution_options | None = Non
 
 
 from .texture import copy, texture_image, texture_image_view  # noqa: E402
+from .sequence import (exclusive_scan, exclusive_scan_along, inclusive_scan, inclusive_scan_along,  # noqa: E402
+                       reduce, reduce_along)
 
-__all__ += ["texture_image", "texture_image_view", "copy"]
+__all__ += ["texture_image", "texture_image_view", "copy", "reduce", "reduce_along", "inclusive_scan",
+            "exclusive_scan", "inclusive_scan_along", "exclusive_scan_along"]
 
 from .prepared import (execution_context, fft_forward, fft_inverse, irfft, make_execution_context,  # noqa: E402
                        rfft, rfft_extent)

@@ -124,6 +124,11 @@ SIGNATURES = {
     "iane_hip_synth_blobs": [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, f32, vp, ctypes.c_int, vp],
     "iane_hip_synth_poisson": [vp, vp, i64, ctypes.c_uint, vp],
     "iane_hip_max": [vp, i64, vp, vp, vp],
+    "iane_hip_sequence_plan_create": [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp,
+                                      desc_p, desc_p],
+    "iane_hip_sequence_plan_workspace_bytes": [vp, ctypes.POINTER(ctypes.c_size_t)],
+    "iane_hip_sequence_run": [vp, desc_p, desc_p, vp, vp],
+    "iane_hip_sequence_plan_destroy": [vp],
     "iane_hip_scale": [vp, i64, f32, vp],
 }
 

@@ -1,0 +1,784 @@
+// Scan and reduce behind iane_hip_sequence_*: the prepared reduce /
+// reduce_along / inclusive_scan / exclusive_scan (and *_along) algorithms of
+// the reference (include/iane/algorithms/{reduce,scan}.hpp over
+// algorithms/detail/sequence.hpp) for device views, with a fixed operator set
+// (sum, product, min, max) instead of device functors, which a C ABI cannot
+// carry.
+//
+// The work is the same three stages as sequence.hpp:213-235 (chunk totals,
+// line carries, output), laid out for the GPU:
+//   row mode (the whole view, or lines along axis 0): a workgroup owns a tile
+//     of up to 2048 consecutive elements of one line, staged through LDS with
+//     coalesced loads; totals are block reductions, the carries of a line are
+//     a block scan of its tile totals, and the output pass rescans each tile
+//     from its carry (8 consecutive elements per thread, wavefront shuffles,
+//     one LDS exchange across the 4 waves) and stores through LDS again;
+//   column mode (lines along axis >= 1): one thread per line, consecutive
+//     threads on consecutive lines (adjacent addresses when axis 0 is
+//     packed), each folding its line in order; lines are cut into chunks when
+//     there are too few of them to fill the GPU.
+// Everything is HBM-bound: one read of the source (plus the tile/chunk totals)
+// and one write of the destination.
+#include "common.hpp"
+
+#include <type_traits>
+
+struct iane_sequence_plan {
+	int kind;
+	int axis;
+	int op;
+	int acc_type;
+	alignas(8) unsigned char init[8];
+	int src_type;
+	int src_rank;
+	int src_extent[4];
+	int dst_rank;
+	int dst_extent[4];
+	bool row;               // row mode (whole or axis 0), else column mode
+	long long lines;
+	long long length;
+	int per;                // row mode: elements per thread of a tile (1..16, power of two); tile = 256 * per
+	bool wave_reduce;       // row mode reduction of short lines: one wave per line
+	long long tiles;        // row mode: tiles per line
+	long long chunk;        // column mode: elements per chunk
+	long long chunks;       // column mode: chunks per line
+	size_t workspace;
+};
+
+namespace iane_hip {
+namespace {
+
+constexpr int kSeqThreads = 256;
+
+// Element addressing of a sequence problem (source and destination byte offsets).
+struct seq_map {
+	const char* src;
+	char* dst;
+	int src_type;
+	int whole;                   // 1: one line = the whole view in storage order
+	int src_packed, dst_packed;  // whole mode: offset = index * element size
+	int src_esize, dst_esize;
+	int extent[4];               // whole mode: view extents
+	long long src_stride[4], dst_stride[4];
+	int other_rank;              // line mode: axes other than the line axis, in order
+	int other_extent[3];
+	long long src_other_stride[3];
+	long long dst_other_stride[3];
+	long long src_step, dst_step; // line mode: byte stride along the line
+	long long lines, length;
+};
+
+__device__ __forceinline__ long long line_base(const long long* aStrides, const seq_map& m, long long aLine)
+{
+	long long offset = 0;
+	for (int k = 0; k < m.other_rank; ++k) {
+		const long long c = aLine % m.other_extent[k];
+		aLine /= m.other_extent[k];
+		offset += c * aStrides[k];
+	}
+	return offset;
+}
+
+__device__ __forceinline__ long long whole_offset(const long long* aStrides, const seq_map& m, long long aIndex)
+{
+	long long offset = 0;
+	for (int k = 0; k < 4; ++k) {
+		const long long c = aIndex % m.extent[k];
+		aIndex /= m.extent[k];
+		offset += c * aStrides[k];
+	}
+	return offset;
+}
+
+__device__ __forceinline__ long long src_offset(const seq_map& m, long long aBase, long long aIndex)
+{
+	if (m.whole) {
+		return m.src_packed ? aIndex * m.src_esize : whole_offset(m.src_stride, m, aIndex);
+	}
+	return aBase + aIndex * m.src_step;
+}
+
+__device__ __forceinline__ long long dst_offset(const seq_map& m, long long aBase, long long aIndex)
+{
+	if (m.whole) {
+		return m.dst_packed ? aIndex * m.dst_esize : whole_offset(m.dst_stride, m, aIndex);
+	}
+	return aBase + aIndex * m.dst_step;
+}
+
+template<class Acc>
+__device__ __forceinline__ Acc load_value(const char* aPtr, int aType)
+{
+	switch (aType) {
+	case IANE_U8: return static_cast<Acc>(*reinterpret_cast<const uint8_t*>(aPtr));
+	case IANE_U16: return static_cast<Acc>(*reinterpret_cast<const uint16_t*>(aPtr));
+	case IANE_I32: return static_cast<Acc>(*reinterpret_cast<const int32_t*>(aPtr));
+	case IANE_F32: return static_cast<Acc>(*reinterpret_cast<const float*>(aPtr));
+	default: return static_cast<Acc>(*reinterpret_cast<const double*>(aPtr));
+	}
+}
+
+// op(accumulated, next) in the accumulator type. Integer sums and products
+// wrap (computed in 32-bit unsigned arithmetic, then narrowed); min and max
+// are std::min / std::max.
+template<class Acc, int OP>
+__device__ __forceinline__ Acc apply(Acc a, Acc b)
+{
+	if constexpr (OP == IANE_OP_SUM) {
+		if constexpr (std::is_integral_v<Acc>) {
+			return static_cast<Acc>(static_cast<uint32_t>(a) + static_cast<uint32_t>(b));
+		} else {
+			return a + b;
+		}
+	} else if constexpr (OP == IANE_OP_PRODUCT) {
+		if constexpr (std::is_integral_v<Acc>) {
+			return static_cast<Acc>(static_cast<uint32_t>(a) * static_cast<uint32_t>(b));
+		} else {
+			return a * b;
+		}
+	} else if constexpr (OP == IANE_OP_MIN) {
+		return b < a ? b : a;
+	} else {
+		return a < b ? b : a;
+	}
+}
+
+// Neutral element: padding for partial tiles and block collectives.
+template<class Acc, int OP>
+__device__ __forceinline__ Acc identity()
+{
+	if constexpr (OP == IANE_OP_SUM) {
+		return Acc(0);
+	} else if constexpr (OP == IANE_OP_PRODUCT) {
+		return Acc(1);
+	} else if constexpr (OP == IANE_OP_MIN) {
+		if constexpr (std::is_floating_point_v<Acc>) {
+			return std::numeric_limits<Acc>::infinity();
+		} else {
+			return std::numeric_limits<Acc>::max();
+		}
+	} else {
+		if constexpr (std::is_floating_point_v<Acc>) {
+			return -std::numeric_limits<Acc>::infinity();
+		} else {
+			return std::numeric_limits<Acc>::lowest();
+		}
+	}
+}
+
+template<class Acc>
+__device__ __forceinline__ Acc shfl_up(Acc aValue, int aDelta)
+{
+	if constexpr (sizeof(Acc) < 4) {
+		return static_cast<Acc>(__shfl_up(static_cast<unsigned>(aValue), aDelta, 64));
+	} else {
+		return __shfl_up(aValue, aDelta, 64);
+	}
+}
+
+// Exclusive scan over the 256 threads of a workgroup (identity for thread 0);
+// `aTotal` receives the combination of all 256 values.
+template<class Acc, int OP>
+__device__ __forceinline__ Acc block_exclusive_scan(Acc aValue, Acc& aTotal, Acc* aShared)
+{
+	const int lane = threadIdx.x & 63;
+	const int wave = threadIdx.x >> 6;
+	Acc inclusive = aValue;
+#pragma unroll
+	for (int d = 1; d < 64; d <<= 1) {
+		const Acc other = shfl_up(inclusive, d);
+		if (lane >= d) {
+			inclusive = apply<Acc, OP>(other, inclusive);
+		}
+	}
+	if (lane == 63) {
+		aShared[wave] = inclusive;
+	}
+	__syncthreads();
+	Acc before = identity<Acc, OP>();
+	for (int w = 0; w < wave; ++w) {
+		before = apply<Acc, OP>(before, aShared[w]);
+	}
+	aTotal = apply<Acc, OP>(apply<Acc, OP>(apply<Acc, OP>(aShared[0], aShared[1]), aShared[2]), aShared[3]);
+	Acc exclusive = shfl_up(inclusive, 1);
+	if (lane == 0) {
+		exclusive = identity<Acc, OP>();
+	}
+	__syncthreads();
+	return apply<Acc, OP>(before, exclusive);
+}
+
+template<class Acc>
+__device__ __forceinline__ void store_value(char* aPtr, Acc aValue)
+{
+	*reinterpret_cast<Acc*>(aPtr) = aValue;
+}
+
+// ---- row mode -------------------------------------------------------------
+constexpr int kSeqPer = 16;                       // elements per thread of a tile, at most
+constexpr int kSeqTile = kSeqThreads * kSeqPer;   // 4096
+
+template<class Acc>
+__device__ __forceinline__ Acc shfl_xor(Acc aValue, int aMask)
+{
+	if constexpr (sizeof(Acc) < 4) {
+		return static_cast<Acc>(__shfl_xor(static_cast<unsigned>(aValue), aMask, 64));
+	} else {
+		return __shfl_xor(aValue, aMask, 64);
+	}
+}
+
+// aOut[e] <- source element aFirst + e * aStep of a line (e < aPer), the
+// neutral element where that index is >= aCount. Every load is issued before
+// any is used: indices past the end are clamped to the last element (a valid
+// address) and replaced afterwards, so there is no branch between the loads.
+template<class Acc, int OP>
+__device__ __forceinline__ void load_line(const seq_map& m, long long aBase, long long aBegin, long long aCount,
+	long long aFirst, long long aStep, int aPer, Acc (&aOut)[kSeqPer])
+{
+	with_element_type(m.src_type, [&](auto aType) {
+		using T = decltype(aType);
+		T raw[kSeqPer];
+#pragma unroll
+		for (int e = 0; e < kSeqPer; ++e) {
+			const long long j = min(aFirst + e * aStep, aCount - 1);
+			raw[e] = *reinterpret_cast<const T*>(m.src + src_offset(m, aBase, aBegin + j));
+		}
+#pragma unroll
+		for (int e = 0; e < kSeqPer; ++e) {
+			const bool valid = e < aPer && aFirst + e * aStep < aCount;
+			aOut[e] = valid ? static_cast<Acc>(raw[e]) : identity<Acc, OP>();
+		}
+	});
+}
+
+// One wave per line (lines of at most a few thousand elements): lanes stride
+// through the line in batches, then a butterfly over the wave; the result
+// init (+) total goes to the destination.
+template<class Acc, int OP>
+__global__ __launch_bounds__(kSeqThreads) void row_reduce_wave_kernel(seq_map m, Acc aInit)
+{
+	const int lane = threadIdx.x & 63;
+	const long long line = static_cast<long long>(blockIdx.x) * (kSeqThreads / 64) + (threadIdx.x >> 6);
+	if (line >= m.lines) {
+		return; // uniform over the wave
+	}
+	const long long base = m.whole ? 0 : line_base(m.src_other_stride, m, line);
+	Acc acc = identity<Acc, OP>();
+	for (long long start = 0; start < m.length; start += 64 * kSeqPer) {
+		Acc v[kSeqPer];
+		load_line<Acc, OP>(m, base, start, m.length - start, lane, 64, kSeqPer, v);
+#pragma unroll
+		for (int e = 0; e < kSeqPer; ++e) {
+			acc = apply<Acc, OP>(acc, v[e]);
+		}
+	}
+#pragma unroll
+	for (int mask = 32; mask > 0; mask >>= 1) {
+		acc = apply<Acc, OP>(acc, shfl_xor(acc, mask));
+	}
+	if (lane == 0) {
+		const long long offset = m.whole ? 0 : line_base(m.dst_other_stride, m, line);
+		store_value(m.dst + offset, apply<Acc, OP>(aInit, acc));
+	}
+}
+
+// Totals of tiles of 256 * aPer elements (all tiles for a reduction; all but the last for a scan).
+template<class Acc, int OP>
+__global__ __launch_bounds__(kSeqThreads) void row_totals_kernel(seq_map m, Acc* aPartials, long long aTiles, int aPer,
+	int aSkipLast)
+{
+	__shared__ Acc shared[4];
+	const long long task = blockIdx.x;
+	const long long line = task / aTiles;
+	const long long t = task - line * aTiles;
+	if (aSkipLast && t == aTiles - 1) {
+		return; // uniform over the workgroup
+	}
+	const long long tile = static_cast<long long>(kSeqThreads) * aPer;
+	const long long begin = t * tile;
+	const long long n = min(tile, m.length - begin);
+	const long long base = m.whole ? 0 : line_base(m.src_other_stride, m, line);
+	Acc v[kSeqPer];
+	load_line<Acc, OP>(m, base, begin, n, threadIdx.x, kSeqThreads, aPer, v);
+	Acc acc = identity<Acc, OP>();
+#pragma unroll
+	for (int e = 0; e < kSeqPer; ++e) {
+		acc = apply<Acc, OP>(acc, v[e]);
+	}
+	Acc total;
+	(void)block_exclusive_scan<Acc, OP>(acc, total, shared);
+	if (threadIdx.x == 0) {
+		aPartials[task] = total;
+	}
+}
+
+// Carries of one line: slot k <- init (+) totals of tiles 0..k-1 (scans), or the
+// line's result (reduction, written to the destination). Each thread takes a
+// segment of kSeqPer consecutive totals, loaded together.
+template<class Acc, int OP, int KIND>
+__global__ __launch_bounds__(kSeqThreads) void row_carries_kernel(seq_map m, Acc* aPartials, long long aTiles, Acc aInit)
+{
+	__shared__ Acc shared[4];
+	const long long line = blockIdx.x;
+	Acc* p = aPartials + line * aTiles;
+	const long long read = KIND == IANE_SEQ_REDUCE ? aTiles : aTiles - 1;
+	Acc carry = aInit;
+	for (long long start = 0; start < aTiles; start += kSeqTile) {
+		const long long first = start + static_cast<long long>(threadIdx.x) * kSeqPer;
+		Acc v[kSeqPer];
+#pragma unroll
+		for (int e = 0; e < kSeqPer; ++e) {
+			v[e] = p[min(first + e, read - 1)];
+		}
+		Acc run = identity<Acc, OP>();
+		Acc before_local[kSeqPer];
+#pragma unroll
+		for (int e = 0; e < kSeqPer; ++e) {
+			before_local[e] = run;
+			run = apply<Acc, OP>(run, first + e < read ? v[e] : identity<Acc, OP>());
+		}
+		Acc total;
+		const Acc before = block_exclusive_scan<Acc, OP>(run, total, shared);
+		if (KIND != IANE_SEQ_REDUCE) {
+			const Acc head = apply<Acc, OP>(carry, before);
+#pragma unroll
+			for (int e = 0; e < kSeqPer; ++e) {
+				if (first + e < aTiles) {
+					p[first + e] = apply<Acc, OP>(head, before_local[e]);
+				}
+			}
+		}
+		carry = apply<Acc, OP>(carry, total);
+	}
+	if (KIND == IANE_SEQ_REDUCE && threadIdx.x == 0) {
+		const long long offset = m.whole ? 0 : line_base(m.dst_other_stride, m, line);
+		store_value(m.dst + offset, carry);
+	}
+}
+
+// Output of one tile: rescan from the tile's carry (init when the line is one tile).
+template<class Acc, int OP, int KIND>
+__global__ __launch_bounds__(kSeqThreads) void row_output_kernel(seq_map m, const Acc* aCarries, long long aTiles,
+	int aPer, Acc aInit)
+{
+	__shared__ Acc stage[kSeqTile];
+	__shared__ Acc shared[4];
+	const long long task = blockIdx.x;
+	const long long line = task / aTiles;
+	const long long t = task - line * aTiles;
+	const long long tile = static_cast<long long>(kSeqThreads) * aPer;
+	const long long begin = t * tile;
+	const int n = static_cast<int>(min(tile, m.length - begin));
+	const long long src_base = m.whole ? 0 : line_base(m.src_other_stride, m, line);
+	const long long dst_base = m.whole ? 0 : line_base(m.dst_other_stride, m, line);
+	{
+		Acc v[kSeqPer];
+		load_line<Acc, OP>(m, src_base, begin, n, threadIdx.x, kSeqThreads, aPer, v);
+#pragma unroll
+		for (int e = 0; e < kSeqPer; ++e) {
+			const int j = threadIdx.x + e * kSeqThreads;
+			if (e < aPer && j < n) {
+				stage[j] = v[e];
+			}
+		}
+	}
+	__syncthreads();
+	Acc local[kSeqPer];
+	Acc run = identity<Acc, OP>();
+#pragma unroll
+	for (int e = 0; e < kSeqPer; ++e) {
+		const int index = aPer * threadIdx.x + e;
+		const Acc x = (e < aPer && index < n) ? stage[index] : identity<Acc, OP>();
+		local[e] = run;             // exclusive within the thread
+		run = apply<Acc, OP>(run, x);
+		if constexpr (KIND == IANE_SEQ_INCLUSIVE) {
+			local[e] = run;
+		}
+	}
+	Acc total;
+	const Acc prefix = block_exclusive_scan<Acc, OP>(run, total, shared);
+	const Acc carry = aTiles > 1 ? aCarries[task] : aInit;
+	const Acc before = apply<Acc, OP>(carry, prefix);
+#pragma unroll
+	for (int e = 0; e < kSeqPer; ++e) {
+		const int index = aPer * threadIdx.x + e;
+		if (e < aPer && index < n) {
+			stage[index] = apply<Acc, OP>(before, local[e]);
+		}
+	}
+	__syncthreads();
+#pragma unroll
+	for (int e = 0; e < kSeqPer; ++e) {
+		const int j = threadIdx.x + e * kSeqThreads;
+		if (e < aPer && j < n) {
+			store_value(m.dst + dst_offset(m, dst_base, begin + j), stage[j]);
+		}
+	}
+}
+
+// ---- column mode ----------------------------------------------------------
+template<class Acc, int OP>
+__global__ __launch_bounds__(kSeqThreads) void col_totals_kernel(seq_map m, Acc* aPartials, long long aChunks,
+	long long aChunk, int aSkipLast)
+{
+	const long long index = static_cast<long long>(blockIdx.x) * kSeqThreads + threadIdx.x;
+	if (index >= m.lines * aChunks) {
+		return;
+	}
+	const long long line = index % m.lines;
+	const long long c = index / m.lines;
+	if (aSkipLast && c == aChunks - 1) {
+		return;
+	}
+	const long long begin = c * aChunk;
+	const long long end = min(begin + aChunk, m.length);
+	const long long base = line_base(m.src_other_stride, m, line);
+	Acc acc = load_value<Acc>(m.src + base + begin * m.src_step, m.src_type);
+	for (long long i = begin + 1; i < end; ++i) {
+		acc = apply<Acc, OP>(acc, load_value<Acc>(m.src + base + i * m.src_step, m.src_type));
+	}
+	aPartials[index] = acc;
+}
+
+template<class Acc, int OP, int KIND>
+__global__ __launch_bounds__(kSeqThreads) void col_carries_kernel(seq_map m, Acc* aPartials, long long aChunks, Acc aInit)
+{
+	const long long line = static_cast<long long>(blockIdx.x) * kSeqThreads + threadIdx.x;
+	if (line >= m.lines) {
+		return;
+	}
+	const long long parts = KIND == IANE_SEQ_REDUCE ? aChunks : aChunks - 1;
+	Acc total = aInit;
+	for (long long c = 0; c < parts; ++c) {
+		const Acc x = aPartials[c * m.lines + line];
+		if (KIND != IANE_SEQ_REDUCE) {
+			aPartials[c * m.lines + line] = total;
+		}
+		total = apply<Acc, OP>(total, x);
+	}
+	if (KIND == IANE_SEQ_REDUCE) {
+		store_value(m.dst + line_base(m.dst_other_stride, m, line), total);
+	} else {
+		aPartials[(aChunks - 1) * m.lines + line] = total;
+	}
+}
+
+// Output of one (chunk, line) from its carry; a reduction of unchunked lines writes the result.
+template<class Acc, int OP, int KIND>
+__global__ __launch_bounds__(kSeqThreads) void col_output_kernel(seq_map m, const Acc* aCarries, long long aChunks,
+	long long aChunk, Acc aInit)
+{
+	const long long index = static_cast<long long>(blockIdx.x) * kSeqThreads + threadIdx.x;
+	if (index >= m.lines * aChunks) {
+		return;
+	}
+	const long long line = index % m.lines;
+	const long long c = index / m.lines;
+	const long long begin = c * aChunk;
+	const long long end = min(begin + aChunk, m.length);
+	const long long src_base = line_base(m.src_other_stride, m, line);
+	const long long dst_base = line_base(m.dst_other_stride, m, line);
+	Acc total = aChunks > 1 ? aCarries[index] : aInit;
+	for (long long i = begin; i < end; ++i) {
+		const Acc x = load_value<Acc>(m.src + src_base + i * m.src_step, m.src_type);
+		if constexpr (KIND == IANE_SEQ_EXCLUSIVE) {
+			store_value(m.dst + dst_base + i * m.dst_step, total);
+		}
+		total = apply<Acc, OP>(total, x);
+		if constexpr (KIND == IANE_SEQ_INCLUSIVE) {
+			store_value(m.dst + dst_base + i * m.dst_step, total);
+		}
+	}
+	if constexpr (KIND == IANE_SEQ_REDUCE) {
+		store_value(m.dst + dst_base, total);
+	}
+}
+
+// Every destination position <- init (reductions of empty lines).
+template<class Acc>
+__global__ __launch_bounds__(kSeqThreads) void fill_init_kernel(seq_map m, Acc aInit)
+{
+	const long long line = static_cast<long long>(blockIdx.x) * kSeqThreads + threadIdx.x;
+	if (line < m.lines) {
+		store_value(m.dst + (m.whole ? 0 : line_base(m.dst_other_stride, m, line)), aInit);
+	}
+}
+
+unsigned blocks_of(long long aCount)
+{
+	const long long blocks = (aCount + kSeqThreads - 1) / kSeqThreads;
+	if (blocks > 0x7fffffffLL) {
+		fail(IANE_ERR_UNSUPPORTED, "sequence exceeds the grid size limit");
+	}
+	return static_cast<unsigned>(blocks);
+}
+
+unsigned grid_of(long long aBlocks)
+{
+	if (aBlocks > 0x7fffffffLL) {
+		fail(IANE_ERR_UNSUPPORTED, "sequence exceeds the grid size limit");
+	}
+	return static_cast<unsigned>(aBlocks);
+}
+
+template<class Acc, int OP, int KIND>
+void run_kind(const iane_sequence_plan& p, const seq_map& m, void* aWorkspace, hipStream_t s)
+{
+	Acc init;
+	std::memcpy(&init, p.init, sizeof(Acc));
+	auto* partials = static_cast<Acc*>(aWorkspace);
+	if (m.lines == 0) {
+		return;
+	}
+	if (m.length == 0) {
+		if (KIND == IANE_SEQ_REDUCE) {
+			fill_init_kernel<Acc><<<blocks_of(m.lines), kSeqThreads, 0, s>>>(m, init);
+		}
+		return;
+	}
+	if (p.row) {
+		const long long tasks = m.lines * p.tiles;
+		if (KIND == IANE_SEQ_REDUCE) {
+			if (p.wave_reduce) {
+				row_reduce_wave_kernel<Acc, OP><<<grid_of((m.lines + 3) / 4), kSeqThreads, 0, s>>>(m, init);
+				return;
+			}
+			row_totals_kernel<Acc, OP><<<grid_of(tasks), kSeqThreads, 0, s>>>(m, partials, p.tiles, p.per, 0);
+			row_carries_kernel<Acc, OP, KIND><<<grid_of(m.lines), kSeqThreads, 0, s>>>(m, partials, p.tiles, init);
+			return;
+		}
+		if (p.tiles > 1) {
+			row_totals_kernel<Acc, OP><<<grid_of(tasks), kSeqThreads, 0, s>>>(m, partials, p.tiles, p.per, 1);
+			row_carries_kernel<Acc, OP, KIND><<<grid_of(m.lines), kSeqThreads, 0, s>>>(m, partials, p.tiles, init);
+		}
+		row_output_kernel<Acc, OP, KIND><<<grid_of(tasks), kSeqThreads, 0, s>>>(m, partials, p.tiles, p.per, init);
+		return;
+	}
+	const long long tasks = m.lines * p.chunks;
+	if (p.chunks > 1) {
+		col_totals_kernel<Acc, OP><<<blocks_of(tasks), kSeqThreads, 0, s>>>(m, partials, p.chunks, p.chunk,
+			KIND == IANE_SEQ_REDUCE ? 0 : 1);
+		col_carries_kernel<Acc, OP, KIND><<<blocks_of(m.lines), kSeqThreads, 0, s>>>(m, partials, p.chunks, init);
+		if (KIND == IANE_SEQ_REDUCE) {
+			return;
+		}
+	}
+	col_output_kernel<Acc, OP, KIND><<<blocks_of(tasks), kSeqThreads, 0, s>>>(m, partials, p.chunks, p.chunk, init);
+}
+
+template<class Acc, int OP>
+void run_op(const iane_sequence_plan& p, const seq_map& m, void* w, hipStream_t s)
+{
+	switch (p.kind) {
+	case IANE_SEQ_INCLUSIVE: run_kind<Acc, OP, IANE_SEQ_INCLUSIVE>(p, m, w, s); break;
+	case IANE_SEQ_EXCLUSIVE: run_kind<Acc, OP, IANE_SEQ_EXCLUSIVE>(p, m, w, s); break;
+	default: run_kind<Acc, OP, IANE_SEQ_REDUCE>(p, m, w, s); break;
+	}
+}
+
+template<class Acc>
+void run_acc(const iane_sequence_plan& p, const seq_map& m, void* w, hipStream_t s)
+{
+	switch (p.op) {
+	case IANE_OP_SUM: run_op<Acc, IANE_OP_SUM>(p, m, w, s); break;
+	case IANE_OP_PRODUCT: run_op<Acc, IANE_OP_PRODUCT>(p, m, w, s); break;
+	case IANE_OP_MIN: run_op<Acc, IANE_OP_MIN>(p, m, w, s); break;
+	default: run_op<Acc, IANE_OP_MAX>(p, m, w, s); break;
+	}
+}
+
+long long pow2_at_least(long long aValue)
+{
+	long long p = 1;
+	while (p < aValue) {
+		p <<= 1;
+	}
+	return p;
+}
+
+} // namespace
+} // namespace iane_hip
+
+using namespace iane_hip;
+
+extern "C" int iane_hip_sequence_plan_create(iane_sequence_plan** aPlan, int aKind, int aAxis, int aOp, int aInitType,
+	const void* aInit, const iane_view_desc* aSrc, const iane_view_desc* aDst)
+{
+	return guarded([&] {
+		if (aPlan == nullptr || aInit == nullptr) {
+			invalid("sequence_plan_create: null argument");
+		}
+		*aPlan = nullptr;
+		const auto src = make_view4(aSrc, "source");
+		const auto dst = make_view4(aDst, "destination");
+		if (aKind < IANE_SEQ_INCLUSIVE || aKind > IANE_SEQ_REDUCE) {
+			invalid("unknown sequence kind");
+		}
+		if (aOp < IANE_OP_SUM || aOp > IANE_OP_MAX) {
+			invalid("unknown sequence operator");
+		}
+		if (aInitType == IANE_C64 || src.type == IANE_C64) {
+			fail(IANE_ERR_UNSUPPORTED, "scan and reduce support u8, u16, i32, f32 and f64 elements");
+		}
+		dtype_size(aInitType);
+		if (dst.type != aInitType) {
+			invalid("the destination element type must be the type of the initial value");
+		}
+		if (aAxis < -1 || aAxis >= src.rank) {
+			invalid("the sequence axis must be below the source rank");
+		}
+		// expected destination shape (sequence.hpp:276-290)
+		int expected_rank = src.rank;
+		int expected[4] = {1, 1, 1, 1};
+		if (aKind != IANE_SEQ_REDUCE) {
+			for (int i = 0; i < src.rank; ++i) {
+				expected[i] = src.extent[i];
+			}
+		} else if (aAxis < 0 || src.rank == 1) {
+			expected_rank = 1;
+		} else {
+			expected_rank = src.rank - 1;
+			for (int i = 0; i < expected_rank; ++i) {
+				expected[i] = src.extent[i + (i >= aAxis)];
+			}
+		}
+		if (dst.rank != expected_rank) {
+			invalid("the destination rank does not fit the sequence");
+		}
+		for (int i = 0; i < expected_rank; ++i) {
+			if (dst.extent[i] != expected[i]) {
+				invalid("Sequence views must match the prepared shape");
+			}
+		}
+		auto plan = new iane_sequence_plan{};
+		plan->kind = aKind;
+		plan->axis = aAxis;
+		plan->op = aOp;
+		plan->acc_type = aInitType;
+		std::memcpy(plan->init, aInit, static_cast<size_t>(dtype_size(aInitType)));
+		plan->src_type = src.type;
+		plan->src_rank = src.rank;
+		plan->dst_rank = dst.rank;
+		for (int i = 0; i < 4; ++i) {
+			plan->src_extent[i] = src.extent[i];
+			plan->dst_extent[i] = dst.extent[i];
+		}
+		const long long count = element_count(src);
+		if (aAxis < 0) {
+			plan->lines = (count == 0 && aKind != IANE_SEQ_REDUCE) ? 0 : 1;
+			plan->length = count;
+		} else {
+			plan->length = src.extent[aAxis];
+			long long others = 1;
+			for (int i = 0; i < src.rank; ++i) {
+				others *= i == aAxis ? 1 : src.extent[i];
+			}
+			plan->lines = (count != 0 || aKind == IANE_SEQ_REDUCE) ? others : 0;
+		}
+		plan->row = aAxis <= 0;
+		const size_t acc = static_cast<size_t>(dtype_size(aInitType));
+		if (plan->row) {
+			const long long per_thread = (plan->length + kSeqThreads - 1) / kSeqThreads;
+			plan->per = static_cast<int>(std::min<long long>(kSeqPer, std::max<long long>(1, pow2_at_least(per_thread))));
+			const long long tile = static_cast<long long>(kSeqThreads) * plan->per;
+			plan->tiles = plan->length == 0 ? 1 : (plan->length + tile - 1) / tile;
+			plan->wave_reduce = aKind == IANE_SEQ_REDUCE && plan->length <= kSeqTile;
+			const bool totals = (aKind == IANE_SEQ_REDUCE && !plan->wave_reduce) || plan->tiles > 1;
+			plan->workspace = totals ? static_cast<size_t>(plan->lines * plan->tiles) * acc : 0;
+		} else {
+			// enough (line, chunk) tasks to fill the GPU, chunks of at least 32 elements
+			long long chunks = 1;
+			if (plan->lines > 0 && plan->lines < 131072 && plan->length > 32) {
+				chunks = std::min((131072 + plan->lines - 1) / plan->lines, (plan->length + 31) / 32);
+			}
+			plan->chunk = std::max<long long>(1, (plan->length + chunks - 1) / chunks);
+			plan->chunks = plan->length == 0 ? 1 : (plan->length + plan->chunk - 1) / plan->chunk;
+			plan->workspace = plan->chunks > 1 ? static_cast<size_t>(plan->lines * plan->chunks) * acc : 0;
+		}
+		*aPlan = plan;
+	});
+}
+
+extern "C" int iane_hip_sequence_plan_workspace_bytes(const iane_sequence_plan* aPlan, size_t* aBytes)
+{
+	return guarded([&] {
+		if (aPlan == nullptr || aBytes == nullptr) {
+			invalid("sequence_plan_workspace_bytes: null argument");
+		}
+		*aBytes = aPlan->workspace;
+	});
+}
+
+extern "C" int iane_hip_sequence_plan_destroy(iane_sequence_plan* aPlan)
+{
+	delete aPlan;
+	return IANE_OK;
+}
+
+extern "C" int iane_hip_sequence_run(const iane_sequence_plan* aPlan, const iane_view_desc* aSrc,
+	const iane_view_desc* aDst, void* aWorkspace, void* aStream)
+{
+	return guarded([&] {
+		if (aPlan == nullptr) {
+			invalid("sequence_run: null plan");
+		}
+		const auto src = make_view4(aSrc, "source");
+		const auto dst = make_view4(aDst, "destination");
+		if (src.rank != aPlan->src_rank || dst.rank != aPlan->dst_rank || src.type != aPlan->src_type ||
+			dst.type != aPlan->acc_type) {
+			invalid("Sequence views must match the prepared shape");
+		}
+		for (int i = 0; i < 4; ++i) {
+			if (src.extent[i] != aPlan->src_extent[i] || dst.extent[i] != aPlan->dst_extent[i]) {
+				invalid("Sequence views must match the prepared shape");
+			}
+		}
+		if (aPlan->workspace && aWorkspace == nullptr) {
+			invalid("sequence_run: the plan needs a workspace");
+		}
+		seq_map m{};
+		m.src = src.ptr;
+		m.dst = dst.ptr;
+		m.src_type = src.type;
+		m.whole = aPlan->axis < 0;
+		m.src_esize = dtype_size(src.type);
+		m.dst_esize = dtype_size(dst.type);
+		m.src_packed = is_packed(src);
+		m.dst_packed = is_packed(dst);
+		m.lines = aPlan->lines;
+		m.length = aPlan->length;
+		for (int i = 0; i < 4; ++i) {
+			m.extent[i] = src.extent[i];
+			m.src_stride[i] = src.stride[i];
+			m.dst_stride[i] = dst.stride[i];
+		}
+		if (!m.whole) {
+			const int axis = aPlan->axis;
+			m.src_step = src.stride[axis];
+			m.dst_step = aPlan->kind == IANE_SEQ_REDUCE ? 0 : dst.stride[axis];
+			int k = 0;
+			for (int i = 0; i < src.rank; ++i) {
+				if (i == axis) {
+					continue;
+				}
+				m.other_extent[k] = src.extent[i];
+				m.src_other_stride[k] = src.stride[i];
+				// scans keep the axes; a reduction along `axis` drops it from the destination
+				m.dst_other_stride[k] = aPlan->kind == IANE_SEQ_REDUCE ? (src.rank == 1 ? 0 : dst.stride[k])
+				                                                        : dst.stride[i];
+				++k;
+			}
+			m.other_rank = k;
+		}
+		auto stream = static_cast<hipStream_t>(aStream);
+		switch (aPlan->acc_type) {
+		case IANE_U8: run_acc<uint8_t>(*aPlan, m, aWorkspace, stream); break;
+		case IANE_U16: run_acc<uint16_t>(*aPlan, m, aWorkspace, stream); break;
+		case IANE_I32: run_acc<int32_t>(*aPlan, m, aWorkspace, stream); break;
+		case IANE_F32: run_acc<float>(*aPlan, m, aWorkspace, stream); break;
+		default: run_acc<double>(*aPlan, m, aWorkspace, stream); break;
+		}
+		check_hip(hipGetLastError(), "Sequence launch");
+	});
+}

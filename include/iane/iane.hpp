@@ -7,3 +7,4 @@
 #include <iane/deconvolution.hpp>
 #include <iane/run_on.hpp>
 #include <iane/texture.hpp>
+#include <iane/sequence.hpp>

@@ -121,6 +121,34 @@ int iane_hip_convolve(const iane_view_desc* src, const iane_view_desc* kernel,
 int iane_hip_convolve_separable(const iane_view_desc* src, const iane_view_desc* taps,
 	int n_taps, const iane_view_desc* dst, int border, int precision, void* stream);
 
+/* ---- scan / reduce: replace the prepared reduce / reduce_along /
+ * inclusive_scan / exclusive_scan / inclusive_scan_along / exclusive_scan_along
+ * (include/iane/algorithms/reduce.hpp:7-36, scan.hpp:7-46, built on
+ * algorithms/detail/sequence.hpp:255-301) for device views ---------------
+ * axis -1: the whole view in storage order (axis 0 fastest); otherwise
+ * independent lines along `axis`. The accumulator and destination element
+ * type is `init_type` and `init` points to one value of it: it is applied once
+ * per line (before the first element) and is the result of an empty
+ * reduction. Shapes follow sequence.hpp:262-293: scans keep the source
+ * extents; a whole reduction (or one of a rank-1 view) writes a rank-1 view
+ * of extent 1; reduce along an axis removes that axis. The operator is one of
+ * IANE_OP_*, applied as op(accumulated, next) in the accumulator type
+ * (integer sums and products wrap; min and max follow std::min / std::max).
+ * Floating-point grouping may vary, as the reference allows. */
+enum { IANE_SEQ_INCLUSIVE = 0, IANE_SEQ_EXCLUSIVE = 1, IANE_SEQ_REDUCE = 2 };
+enum { IANE_OP_SUM = 0, IANE_OP_PRODUCT = 1, IANE_OP_MIN = 2, IANE_OP_MAX = 3 };
+typedef struct iane_sequence_plan iane_sequence_plan;
+/* sequence_algorithm::prepare (sequence.hpp:262-300): validates the prototype
+ * views' shapes and picks the work partition. */
+int iane_hip_sequence_plan_create(iane_sequence_plan** plan, int kind, int axis, int op, int init_type,
+	const void* init, const iane_view_desc* src, const iane_view_desc* dst);
+/* scratch_bytes() (sequence.hpp:201-211): line totals and carries. */
+int iane_hip_sequence_plan_workspace_bytes(const iane_sequence_plan* plan, size_t* bytes);
+/* run (sequence.hpp:213-235): views must match the prepared shape. */
+int iane_hip_sequence_run(const iane_sequence_plan* plan, const iane_view_desc* src, const iane_view_desc* dst,
+	void* workspace, void* stream);
+int iane_hip_sequence_plan_destroy(iane_sequence_plan* plan);
+
 /* ---- FFT: replaces the prepared fft_forward / fft_inverse / rfft / irfft
  * (include/iane/algorithms/fft/fft.hpp:196-231, rfft.hpp:107-161) ---------
  * Stockham FFT, FP32, packed buffers, rank 1..3 (the reference is 2-D only,

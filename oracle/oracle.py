@@ -18,6 +18,10 @@ Pieces and what pins each (see tests/test_oracle.py):
   * ``rl`` / ``wiener`` — the loops of examples/rl_deconvolution/main.cu:51-81
     and examples/wiener/main.cu:21-83 in float64 (numpy); pinned against the
     reference loop run through oracle/_ref and the EM / identity properties.
+  * ``sequence``      — the sequential fold of algorithms/detail/sequence.hpp:73-150
+    (reduce / scans, whole or along an axis) with sum, product, min or max in
+    the accumulator type; pinned by the closed forms of the reference's
+    test/unit/prepared/sequence_cases.hpp (tests/test_oracle_sequence.py).
 
 Array convention: numpy arrays in C order; IANE axis 0 is the LAST array axis.
 """
@@ -232,3 +236,40 @@ def pad_psf_centered(psf: np.ndarray, shape) -> np.ndarray:
     target = tuple((idx[a] - psf.shape[a] // 2) % shape[a] for a in range(psf.ndim))
     out[target] = psf.reshape(-1)
     return out
+
+
+# ------------------------------------------------------------ sequences -----
+SEQ_INCLUSIVE, SEQ_EXCLUSIVE, SEQ_REDUCE = 0, 1, 2
+_SEQ_UFUNC = {"sum": np.add, "product": np.multiply, "min": np.minimum, "max": np.maximum}
+
+
+def sequence(src: np.ndarray, kind: int, axis: int, op: str, init, acc_dtype) -> np.ndarray:
+    """The serial fold of algorithms/detail/sequence.hpp:73-150: every element is
+    converted to the accumulator type, ``total = init`` and ``total = op(total, x)``
+    in storage order along each line (axis -1: the whole array in C order, IANE
+    axis a = array axis ndim-1-a). Integer arithmetic wraps like the accumulator
+    type. Returns the scan (source shape) or the reduction (axis removed; shape
+    (1,) for a whole or 1-D reduction)."""
+    acc = np.dtype(acc_dtype)
+    ufunc = _SEQ_UFUNC[op]
+    x = np.asarray(src).astype(acc)
+    start = np.asarray(init).astype(acc)
+    if axis < 0:
+        lines = x.reshape(1, -1)
+    else:
+        lines = np.moveaxis(x, x.ndim - 1 - axis, -1)
+    shape = lines.shape
+    flat = lines.reshape(int(np.prod(shape[:-1], dtype=np.int64)), shape[-1])
+    padded = np.concatenate([np.full((flat.shape[0], 1), start, dtype=acc), flat], axis=1)
+    with np.errstate(over="ignore"):
+        folded = ufunc.accumulate(padded, axis=1, dtype=acc)
+    if kind == SEQ_REDUCE:
+        totals = folded[:, -1]
+        if axis < 0 or x.ndim == 1:
+            return totals.reshape(1)
+        return totals.reshape(shape[:-1])
+    out = (folded[:, 1:] if kind == SEQ_INCLUSIVE else folded[:, :-1]).reshape(shape)
+    if axis < 0:
+        return out.reshape(x.shape)
+    return np.moveaxis(out, -1, x.ndim - 1 - axis)
+

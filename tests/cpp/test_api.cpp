@@ -11,7 +11,8 @@
 // (device equals host exactly), test_for_each.cpp:43-56 (policy validation),
 // prepared_cases.hpp:73-123 and real_fft_cases.hpp:12-77 (FFT), and the RL loop of
 // examples/rl_deconvolution/main.cu:51-81 (device DIRECT/reference == host loop, bitwise),
-// and the texture-image cases of test_convolution_cuda.cu:10-163.
+// the texture-image cases of test_convolution_cuda.cu:10-163, and the scan /
+// reduce case of test/unit/prepared/sequence_cases.hpp:31-117.
 #include <iane/iane.hpp>
 
 #include <cmath>
@@ -527,6 +528,132 @@ void texture_3d_upload_from_strided_views()
 	check_throws<std::invalid_argument>([&] { iane::copy(strided_rows, small); }, "non-contiguous rows");
 }
 
+// Scan / reduce (test/unit/prepared/sequence_cases.hpp:31-117): a 263 x 5 u8
+// image x % 7 + y; row and column scans composed into 2-D prefix sums, flat
+// exclusive and inclusive scans seeded with 11, whole and per-row reductions,
+// exclusive column scans. The host runs the reference's u64 functor; the device
+// (whose C ABI carries operator codes and has no u64) uses std::plus<int32_t>.
+struct sequence_plus {
+	std::uint64_t operator()(std::uint64_t aLeft, std::uint64_t aRight) const { return aLeft + aRight; }
+};
+
+template<bool tDevice>
+void sequence_case()
+{
+	using coord_t = iane::math::vec2i;
+	constexpr int width = 263;
+	constexpr int height = 5;
+	const coord_t size{width, height};
+	using acc_t = std::conditional_t<tDevice, std::int32_t, std::uint64_t>;
+	using op_t = std::conditional_t<tDevice, std::plus<std::int32_t>, sequence_plus>;
+	iane::host_image<std::uint8_t, 2> input(size);
+	for (int y = 0; y < height; ++y) {
+		for (int x = 0; x < width; ++x) {
+			input.view()[coord_t{x, y}] = static_cast<std::uint8_t>(x % 7 + y);
+		}
+	}
+	iane::host_image<acc_t, 2> output(size), prefixes(size);
+	iane::host_image<acc_t, 1> total(1), totals(height);
+	constexpr acc_t seed = 11;
+	const op_t plus{};
+	const auto check_results = [&](bool aInclusive) {
+		acc_t prefix = seed;
+		for (int y = 0; y < height; ++y) {
+			acc_t row = seed;
+			for (int x = 0; x < width; ++x) {
+				const auto pixel = static_cast<acc_t>(x % 7 + y);
+				CHECK((prefixes.view()[coord_t{x, y}] == prefix + (aInclusive ? pixel : 0)));
+				prefix += pixel;
+				row += pixel;
+			}
+			CHECK(totals.view()[y] == row);
+		}
+		CHECK(total.view()[0] == prefix);
+	};
+	const auto check_columns = [&] {
+		for (int x = 0; x < width; ++x) {
+			acc_t prefix = seed;
+			for (int y = 0; y < height; ++y) {
+				CHECK((prefixes.view()[coord_t{x, y}] == prefix));
+				prefix += static_cast<acc_t>(x % 7 + y);
+			}
+		}
+	};
+	const auto check_rectangles = [&] {
+		for (int y = 0; y < height; ++y) {
+			acc_t rectangle = 0;
+			for (int x = 0; x < width; ++x) {
+				for (int row = 0; row <= y; ++row) {
+					rectangle += static_cast<acc_t>(x % 7 + row);
+				}
+				CHECK((output.const_view()[coord_t{x, y}] == rectangle));
+			}
+		}
+	};
+	if constexpr (tDevice) {
+		iane::device_image<std::uint8_t, 2> source(size);
+		iane::device_image<acc_t, 2> intermediate(coord_t{width + 2, height}), destination(size);
+		iane::device_image<acc_t, 1> scalar(1), row_totals(height);
+		iane::copy(input.const_view(), source.view()).wait();
+		auto rows = intermediate.view().subview(coord_t{1, 0}, size);
+		auto exec = iane::make_execution_context(iane::hip_policy{});
+		auto src = source.const_view();
+		auto dst = destination.view();
+		auto row_scan = exec.prepare(iane::inclusive_scan_along<0>(plus, acc_t{}), src, rows);
+		auto column_scan = exec.prepare(iane::inclusive_scan_along<1>(plus, acc_t{}), iane::const_device_image_view<acc_t, 2>(rows), dst);
+		auto flat_scan = exec.prepare(iane::exclusive_scan(plus, seed), src, dst);
+		auto full_reduce = exec.prepare(iane::reduce(plus, seed), src, scalar.view());
+		auto row_reduce = exec.prepare(iane::reduce_along<0>(plus, seed), src, row_totals.view());
+		auto exclusive_columns = exec.prepare(iane::exclusive_scan_along<1>(plus, seed), src, dst);
+		auto inclusive_flat = exec.prepare(iane::inclusive_scan(plus, seed), src, dst);
+		exec.run(full_reduce, src, scalar.view());
+		exec.run(row_reduce, src, row_totals.view());
+		iane::copy(scalar.const_view(), total.view()).wait();
+		iane::copy(row_totals.const_view(), totals.view()).wait();
+		for (bool inclusive : {false, true}) {
+			exec.run(inclusive ? inclusive_flat : flat_scan, src, dst);
+			iane::copy(destination.const_view(), prefixes.view()).wait();
+			check_results(inclusive);
+		}
+		exec.run(exclusive_columns, src, dst);
+		iane::copy(destination.const_view(), prefixes.view()).wait();
+		check_columns();
+		exec.run(row_scan, src, rows);
+		exec.run(column_scan, iane::const_device_image_view<acc_t, 2>(rows), dst).wait();
+		iane::copy(destination.const_view(), output.view()).wait();
+		check_rectangles();
+	} else {
+		iane::host_image<acc_t, 2> intermediate(coord_t{width + 2, height});
+		auto rows = intermediate.view().subview(coord_t{1, 0}, size);
+		auto exec = iane::make_execution_context(iane::cpu_policy{.threads = 8, .grain_size = 31});
+		auto src = input.const_view();
+		auto dst = output.view();
+		auto row_scan = exec.prepare(iane::inclusive_scan_along<0>(plus, acc_t{}), src, rows);
+		auto column_scan = exec.prepare(iane::inclusive_scan_along<1>(plus, acc_t{}), rows, dst);
+		auto flat_scan = exec.prepare(iane::exclusive_scan(plus, seed), src, prefixes.view());
+		auto full_reduce = exec.prepare(iane::reduce(plus, seed), src, total.view());
+		auto row_reduce = exec.prepare(iane::reduce_along<0>(plus, seed), src, totals.view());
+		auto exclusive_columns = exec.prepare(iane::exclusive_scan_along<1>(plus, seed), src, prefixes.view());
+		auto inclusive_flat = exec.prepare(iane::inclusive_scan(plus, seed), src, prefixes.view());
+		exec.run(full_reduce, src, total.view());
+		exec.run(row_reduce, src, totals.view());
+		exec.run(flat_scan, src, prefixes.view());
+		check_results(false);
+		exec.run(inclusive_flat, src, prefixes.view());
+		check_results(true);
+		exec.run(exclusive_columns, src, prefixes.view());
+		check_columns();
+		exec.run(row_scan, src, rows);
+		exec.run(column_scan, rows, dst);
+		check_rectangles();
+	}
+	check_throws<std::invalid_argument>([&] {
+		auto exec = iane::make_execution_context(iane::cpu_policy{});
+		iane::host_image<acc_t, 1> wrong(height + 1);
+		(void)exec.prepare(iane::reduce_along<0>(plus, seed), input.const_view(), wrong.view());
+	}, "reduce_along extent mismatch");
+}
+
 } // namespace
 
 int main(int argc, char** argv)
@@ -539,6 +666,7 @@ int main(int argc, char** argv)
 		scalar_coordinates();
 		crop_flip(false);
 		policy_validation();
+		sequence_case<false>();
 	} else {
 		int n = 0;
 		if (iane_hip_device_count(&n) != IANE_OK || n == 0) {
@@ -560,6 +688,7 @@ int main(int argc, char** argv)
 		empty_texture_and_move();
 		texture_1d_nested_crop();
 		texture_3d_upload_from_strided_views();
+		sequence_case<true>();
 	}
 	std::printf("%s: %d checks, %d failures\n", device ? "device" : "host", g_checks, g_failures);
 	return g_failures ? 1 : 0;
