@@ -18,7 +18,10 @@
 //     packed), each folding its line in order; lines are cut into chunks when
 //     there are too few of them to fill the GPU.
 // Everything is HBM-bound: one read of the source (plus the tile/chunk totals)
-// and one write of the destination.
+// and one write of the destination. A whole-view problem is one line with a
+// unit step; a whole view that is not packed is gathered into the plan's
+// workspace first (and a scan's result scattered back), so the kernels only
+// ever address base + index * step.
 #include "common.hpp"
 
 #include <type_traits>
@@ -42,6 +45,9 @@ struct iane_sequence_plan {
 	long long tiles;        // row mode: tiles per line
 	long long chunk;        // column mode: elements per chunk
 	long long chunks;       // column mode: chunks per line
+	size_t partials_bytes;  // tile / chunk totals and carries
+	bool stage_src;         // whole view, source not packed: gathered into the workspace
+	bool stage_dst;         // whole-view scan, destination not packed: scattered from the workspace
 	size_t workspace;
 };
 
@@ -50,60 +56,72 @@ namespace {
 
 constexpr int kSeqThreads = 256;
 
-// Element addressing of a sequence problem (source and destination byte offsets).
+// Element addressing of a sequence problem: element i of line l lives at
+// base(l) + i * step, base(l) from the line index over the other axes.
 struct seq_map {
 	const char* src;
 	char* dst;
 	int src_type;
-	int whole;                   // 1: one line = the whole view in storage order
-	int src_packed, dst_packed;  // whole mode: offset = index * element size
-	int src_esize, dst_esize;
-	int extent[4];               // whole mode: view extents
-	long long src_stride[4], dst_stride[4];
-	int other_rank;              // line mode: axes other than the line axis, in order
+	int other_rank;              // axes other than the line axis, in order (0 for a whole view)
 	int other_extent[3];
 	long long src_other_stride[3];
 	long long dst_other_stride[3];
-	long long src_step, dst_step; // line mode: byte stride along the line
+	long long src_step, dst_step; // byte stride along the line
 	long long lines, length;
 };
 
+// Line index -> byte offset of the line's first element (lines < 2^31).
 __device__ __forceinline__ long long line_base(const long long* aStrides, const seq_map& m, long long aLine)
 {
+	unsigned rest = static_cast<unsigned>(aLine);
 	long long offset = 0;
 	for (int k = 0; k < m.other_rank; ++k) {
-		const long long c = aLine % m.other_extent[k];
-		aLine /= m.other_extent[k];
-		offset += c * aStrides[k];
-	}
-	return offset;
-}
-
-__device__ __forceinline__ long long whole_offset(const long long* aStrides, const seq_map& m, long long aIndex)
-{
-	long long offset = 0;
-	for (int k = 0; k < 4; ++k) {
-		const long long c = aIndex % m.extent[k];
-		aIndex /= m.extent[k];
-		offset += c * aStrides[k];
+		const unsigned extent = static_cast<unsigned>(m.other_extent[k]);
+		const unsigned c = rest % extent;
+		rest /= extent;
+		offset += static_cast<long long>(c) * aStrides[k];
 	}
 	return offset;
 }
 
 __device__ __forceinline__ long long src_offset(const seq_map& m, long long aBase, long long aIndex)
 {
-	if (m.whole) {
-		return m.src_packed ? aIndex * m.src_esize : whole_offset(m.src_stride, m, aIndex);
-	}
 	return aBase + aIndex * m.src_step;
 }
 
 __device__ __forceinline__ long long dst_offset(const seq_map& m, long long aBase, long long aIndex)
 {
-	if (m.whole) {
-		return m.dst_packed ? aIndex * m.dst_esize : whole_offset(m.dst_stride, m, aIndex);
-	}
 	return aBase + aIndex * m.dst_step;
+}
+
+// Whole views that are not packed: element i (storage order, axis 0 fastest)
+// between the view and a packed copy.
+template<int BYTES>
+struct raw_element {
+	unsigned char b[BYTES];
+};
+
+template<int BYTES, bool GATHER>
+__global__ __launch_bounds__(kSeqThreads) void stage_kernel(view4 aView, char* aPacked, long long aCount)
+{
+	const long long i = static_cast<long long>(blockIdx.x) * kSeqThreads + threadIdx.x;
+	if (i >= aCount) {
+		return;
+	}
+	long long rest = i;
+	long long offset = 0;
+	for (int k = 0; k < 4; ++k) {
+		const long long c = rest % aView.extent[k];
+		rest /= aView.extent[k];
+		offset += c * aView.stride[k];
+	}
+	auto* packed = reinterpret_cast<raw_element<BYTES>*>(aPacked) + i;
+	auto* element = reinterpret_cast<raw_element<BYTES>*>(aView.ptr + offset);
+	if constexpr (GATHER) {
+		*packed = *element;
+	} else {
+		*element = *packed;
+	}
 }
 
 template<class Acc>
@@ -263,7 +281,7 @@ __global__ __launch_bounds__(kSeqThreads) void row_reduce_wave_kernel(seq_map m,
 	if (line >= m.lines) {
 		return; // uniform over the wave
 	}
-	const long long base = m.whole ? 0 : line_base(m.src_other_stride, m, line);
+	const long long base = line_base(m.src_other_stride, m, line);
 	Acc acc = identity<Acc, OP>();
 	for (long long start = 0; start < m.length; start += 64 * kSeqPer) {
 		Acc v[kSeqPer];
@@ -278,8 +296,7 @@ __global__ __launch_bounds__(kSeqThreads) void row_reduce_wave_kernel(seq_map m,
 		acc = apply<Acc, OP>(acc, shfl_xor(acc, mask));
 	}
 	if (lane == 0) {
-		const long long offset = m.whole ? 0 : line_base(m.dst_other_stride, m, line);
-		store_value(m.dst + offset, apply<Acc, OP>(aInit, acc));
+		store_value(m.dst + line_base(m.dst_other_stride, m, line), apply<Acc, OP>(aInit, acc));
 	}
 }
 
@@ -298,7 +315,7 @@ __global__ __launch_bounds__(kSeqThreads) void row_totals_kernel(seq_map m, Acc*
 	const long long tile = static_cast<long long>(kSeqThreads) * aPer;
 	const long long begin = t * tile;
 	const long long n = min(tile, m.length - begin);
-	const long long base = m.whole ? 0 : line_base(m.src_other_stride, m, line);
+	const long long base = line_base(m.src_other_stride, m, line);
 	Acc v[kSeqPer];
 	load_line<Acc, OP>(m, base, begin, n, threadIdx.x, kSeqThreads, aPer, v);
 	Acc acc = identity<Acc, OP>();
@@ -352,8 +369,7 @@ __global__ __launch_bounds__(kSeqThreads) void row_carries_kernel(seq_map m, Acc
 		carry = apply<Acc, OP>(carry, total);
 	}
 	if (KIND == IANE_SEQ_REDUCE && threadIdx.x == 0) {
-		const long long offset = m.whole ? 0 : line_base(m.dst_other_stride, m, line);
-		store_value(m.dst + offset, carry);
+		store_value(m.dst + line_base(m.dst_other_stride, m, line), carry);
 	}
 }
 
@@ -370,8 +386,8 @@ __global__ __launch_bounds__(kSeqThreads) void row_output_kernel(seq_map m, cons
 	const long long tile = static_cast<long long>(kSeqThreads) * aPer;
 	const long long begin = t * tile;
 	const int n = static_cast<int>(min(tile, m.length - begin));
-	const long long src_base = m.whole ? 0 : line_base(m.src_other_stride, m, line);
-	const long long dst_base = m.whole ? 0 : line_base(m.dst_other_stride, m, line);
+	const long long src_base = line_base(m.src_other_stride, m, line);
+	const long long dst_base = line_base(m.dst_other_stride, m, line);
 	{
 		Acc v[kSeqPer];
 		load_line<Acc, OP>(m, src_base, begin, n, threadIdx.x, kSeqThreads, aPer, v);
@@ -501,7 +517,7 @@ __global__ __launch_bounds__(kSeqThreads) void fill_init_kernel(seq_map m, Acc a
 {
 	const long long line = static_cast<long long>(blockIdx.x) * kSeqThreads + threadIdx.x;
 	if (line < m.lines) {
-		store_value(m.dst + (m.whole ? 0 : line_base(m.dst_other_stride, m, line)), aInit);
+		store_value(m.dst + line_base(m.dst_other_stride, m, line), aInit);
 	}
 }
 
@@ -523,36 +539,36 @@ unsigned grid_of(long long aBlocks)
 }
 
 template<class Acc, int OP, int KIND>
-void run_kind(const iane_sequence_plan& p, const seq_map& m, void* aWorkspace, hipStream_t s)
+void run_kind(const iane_sequence_plan& p, const seq_map& m, void* aPartials, hipStream_t s)
 {
 	Acc init;
 	std::memcpy(&init, p.init, sizeof(Acc));
-	auto* partials = static_cast<Acc*>(aWorkspace);
+	auto* partials = static_cast<Acc*>(aPartials);
 	if (m.lines == 0) {
 		return;
 	}
 	if (m.length == 0) {
-		if (KIND == IANE_SEQ_REDUCE) {
+		if constexpr (KIND == IANE_SEQ_REDUCE) {
 			fill_init_kernel<Acc><<<blocks_of(m.lines), kSeqThreads, 0, s>>>(m, init);
 		}
 		return;
 	}
 	if (p.row) {
 		const long long tasks = m.lines * p.tiles;
-		if (KIND == IANE_SEQ_REDUCE) {
+		if constexpr (KIND == IANE_SEQ_REDUCE) {
 			if (p.wave_reduce) {
 				row_reduce_wave_kernel<Acc, OP><<<grid_of((m.lines + 3) / 4), kSeqThreads, 0, s>>>(m, init);
 				return;
 			}
 			row_totals_kernel<Acc, OP><<<grid_of(tasks), kSeqThreads, 0, s>>>(m, partials, p.tiles, p.per, 0);
 			row_carries_kernel<Acc, OP, KIND><<<grid_of(m.lines), kSeqThreads, 0, s>>>(m, partials, p.tiles, init);
-			return;
+		} else {
+			if (p.tiles > 1) {
+				row_totals_kernel<Acc, OP><<<grid_of(tasks), kSeqThreads, 0, s>>>(m, partials, p.tiles, p.per, 1);
+				row_carries_kernel<Acc, OP, KIND><<<grid_of(m.lines), kSeqThreads, 0, s>>>(m, partials, p.tiles, init);
+			}
+			row_output_kernel<Acc, OP, KIND><<<grid_of(tasks), kSeqThreads, 0, s>>>(m, partials, p.tiles, p.per, init);
 		}
-		if (p.tiles > 1) {
-			row_totals_kernel<Acc, OP><<<grid_of(tasks), kSeqThreads, 0, s>>>(m, partials, p.tiles, p.per, 1);
-			row_carries_kernel<Acc, OP, KIND><<<grid_of(m.lines), kSeqThreads, 0, s>>>(m, partials, p.tiles, init);
-		}
-		row_output_kernel<Acc, OP, KIND><<<grid_of(tasks), kSeqThreads, 0, s>>>(m, partials, p.tiles, p.per, init);
 		return;
 	}
 	const long long tasks = m.lines * p.chunks;
@@ -560,7 +576,7 @@ void run_kind(const iane_sequence_plan& p, const seq_map& m, void* aWorkspace, h
 		col_totals_kernel<Acc, OP><<<blocks_of(tasks), kSeqThreads, 0, s>>>(m, partials, p.chunks, p.chunk,
 			KIND == IANE_SEQ_REDUCE ? 0 : 1);
 		col_carries_kernel<Acc, OP, KIND><<<blocks_of(m.lines), kSeqThreads, 0, s>>>(m, partials, p.chunks, init);
-		if (KIND == IANE_SEQ_REDUCE) {
+		if constexpr (KIND == IANE_SEQ_REDUCE) {
 			return;
 		}
 	}
@@ -676,6 +692,10 @@ extern "C" int iane_hip_sequence_plan_create(iane_sequence_plan** aPlan, int aKi
 			}
 			plan->lines = (count != 0 || aKind == IANE_SEQ_REDUCE) ? others : 0;
 		}
+		if (plan->lines > 0x7fffffffLL || plan->length > (1LL << 40)) {
+			delete plan;
+			fail(IANE_ERR_UNSUPPORTED, "sequence exceeds the supported line count");
+		}
 		plan->row = aAxis <= 0;
 		const size_t acc = static_cast<size_t>(dtype_size(aInitType));
 		if (plan->row) {
@@ -685,7 +705,7 @@ extern "C" int iane_hip_sequence_plan_create(iane_sequence_plan** aPlan, int aKi
 			plan->tiles = plan->length == 0 ? 1 : (plan->length + tile - 1) / tile;
 			plan->wave_reduce = aKind == IANE_SEQ_REDUCE && plan->length <= kSeqTile;
 			const bool totals = (aKind == IANE_SEQ_REDUCE && !plan->wave_reduce) || plan->tiles > 1;
-			plan->workspace = totals ? static_cast<size_t>(plan->lines * plan->tiles) * acc : 0;
+			plan->partials_bytes = totals ? static_cast<size_t>(plan->lines * plan->tiles) * acc : 0;
 		} else {
 			// enough (line, chunk) tasks to fill the GPU, chunks of at least 32 elements
 			long long chunks = 1;
@@ -694,8 +714,15 @@ extern "C" int iane_hip_sequence_plan_create(iane_sequence_plan** aPlan, int aKi
 			}
 			plan->chunk = std::max<long long>(1, (plan->length + chunks - 1) / chunks);
 			plan->chunks = plan->length == 0 ? 1 : (plan->length + plan->chunk - 1) / plan->chunk;
-			plan->workspace = plan->chunks > 1 ? static_cast<size_t>(plan->lines * plan->chunks) * acc : 0;
+			plan->partials_bytes = plan->chunks > 1 ? static_cast<size_t>(plan->lines * plan->chunks) * acc : 0;
 		}
+		// whole views that are not packed pass through a packed copy in the workspace
+		plan->stage_src = aAxis < 0 && count > 0 && !is_packed(src);
+		plan->stage_dst = aAxis < 0 && aKind != IANE_SEQ_REDUCE && count > 0 && !is_packed(dst);
+		const auto round = [](size_t aBytes) { return (aBytes + 255) / 256 * 256; };
+		plan->workspace = round(plan->partials_bytes) +
+			(plan->stage_src ? round(static_cast<size_t>(count) * dtype_size(src.type)) : 0) +
+			(plan->stage_dst ? round(static_cast<size_t>(count) * acc) : 0);
 		*aPlan = plan;
 	});
 }
@@ -737,23 +764,52 @@ extern "C" int iane_hip_sequence_run(const iane_sequence_plan* aPlan, const iane
 		if (aPlan->workspace && aWorkspace == nullptr) {
 			invalid("sequence_run: the plan needs a workspace");
 		}
+		const bool whole = aPlan->axis < 0;
+		if (whole && ((!aPlan->stage_src && element_count(src) > 0 && !is_packed(src)) ||
+				(!aPlan->stage_dst && aPlan->kind != IANE_SEQ_REDUCE && element_count(dst) > 0 && !is_packed(dst)))) {
+			invalid("a whole-view sequence prepared with packed views needs packed views");
+		}
+		auto stream = static_cast<hipStream_t>(aStream);
+		const auto round = [](size_t aBytes) { return (aBytes + 255) / 256 * 256; };
+		char* ws = static_cast<char*>(aWorkspace);
+		char* staged_src = ws + round(aPlan->partials_bytes);
+		char* staged_dst = staged_src + (aPlan->stage_src ? round(static_cast<size_t>(element_count(src)) *
+			dtype_size(src.type)) : 0);
+		const int src_bytes = dtype_size(src.type);
+		const int dst_bytes = dtype_size(dst.type);
+		const long long count = element_count(src);
+		const auto stage = [&](bool aGather, const view4& aView, char* aPacked, int aBytes) {
+			const unsigned blocks = blocks_of(count);
+			switch (aBytes) {
+			case 1: aGather ? stage_kernel<1, true><<<blocks, kSeqThreads, 0, stream>>>(aView, aPacked, count)
+			                : stage_kernel<1, false><<<blocks, kSeqThreads, 0, stream>>>(aView, aPacked, count); break;
+			case 2: aGather ? stage_kernel<2, true><<<blocks, kSeqThreads, 0, stream>>>(aView, aPacked, count)
+			                : stage_kernel<2, false><<<blocks, kSeqThreads, 0, stream>>>(aView, aPacked, count); break;
+			case 4: aGather ? stage_kernel<4, true><<<blocks, kSeqThreads, 0, stream>>>(aView, aPacked, count)
+			                : stage_kernel<4, false><<<blocks, kSeqThreads, 0, stream>>>(aView, aPacked, count); break;
+			default: aGather ? stage_kernel<8, true><<<blocks, kSeqThreads, 0, stream>>>(aView, aPacked, count)
+			                 : stage_kernel<8, false><<<blocks, kSeqThreads, 0, stream>>>(aView, aPacked, count); break;
+			}
+		};
 		seq_map m{};
 		m.src = src.ptr;
 		m.dst = dst.ptr;
 		m.src_type = src.type;
-		m.whole = aPlan->axis < 0;
-		m.src_esize = dtype_size(src.type);
-		m.dst_esize = dtype_size(dst.type);
-		m.src_packed = is_packed(src);
-		m.dst_packed = is_packed(dst);
 		m.lines = aPlan->lines;
 		m.length = aPlan->length;
-		for (int i = 0; i < 4; ++i) {
-			m.extent[i] = src.extent[i];
-			m.src_stride[i] = src.stride[i];
-			m.dst_stride[i] = dst.stride[i];
-		}
-		if (!m.whole) {
+		if (whole) {
+			// one line, unit step over a packed view (staged when the view is not packed)
+			if (aPlan->stage_src && !is_packed(src)) {
+				stage(true, src, staged_src, src_bytes);
+				m.src = staged_src;
+			}
+			if (aPlan->stage_dst && !is_packed(dst)) {
+				m.dst = staged_dst;
+			}
+			m.src_step = src_bytes;
+			m.dst_step = aPlan->kind == IANE_SEQ_REDUCE ? 0 : dst_bytes;
+			m.other_rank = 0;
+		} else {
 			const int axis = aPlan->axis;
 			m.src_step = src.stride[axis];
 			m.dst_step = aPlan->kind == IANE_SEQ_REDUCE ? 0 : dst.stride[axis];
@@ -771,13 +827,16 @@ extern "C" int iane_hip_sequence_run(const iane_sequence_plan* aPlan, const iane
 			}
 			m.other_rank = k;
 		}
-		auto stream = static_cast<hipStream_t>(aStream);
+		void* partials = aPlan->partials_bytes ? ws : nullptr;
 		switch (aPlan->acc_type) {
-		case IANE_U8: run_acc<uint8_t>(*aPlan, m, aWorkspace, stream); break;
-		case IANE_U16: run_acc<uint16_t>(*aPlan, m, aWorkspace, stream); break;
-		case IANE_I32: run_acc<int32_t>(*aPlan, m, aWorkspace, stream); break;
-		case IANE_F32: run_acc<float>(*aPlan, m, aWorkspace, stream); break;
-		default: run_acc<double>(*aPlan, m, aWorkspace, stream); break;
+		case IANE_U8: run_acc<uint8_t>(*aPlan, m, partials, stream); break;
+		case IANE_U16: run_acc<uint16_t>(*aPlan, m, partials, stream); break;
+		case IANE_I32: run_acc<int32_t>(*aPlan, m, partials, stream); break;
+		case IANE_F32: run_acc<float>(*aPlan, m, partials, stream); break;
+		default: run_acc<double>(*aPlan, m, partials, stream); break;
+		}
+		if (whole && m.dst == staged_dst && aPlan->stage_dst) {
+			stage(false, dst, staged_dst, dst_bytes);
 		}
 		check_hip(hipGetLastError(), "Sequence launch");
 	});

@@ -170,6 +170,10 @@ def test_errors():
         ctx.prepare(iane_amd.inclusive_scan("sum", 0), src, torch.zeros((4, 5), dtype=torch.int32, device="cuda"))
     with pytest.raises(iane_amd.invalid_argument):
         iane_amd.reduce("average", 0)
+    scan = ctx.prepare(iane_amd.inclusive_scan("sum", 0), src, torch.zeros((4, 6), dtype=torch.int32, device="cuda"))
+    wide = torch.zeros((4, 12), dtype=torch.int32, device="cuda")
+    with pytest.raises(iane_amd.invalid_argument):  # prepared on packed views, run on a strided one
+        ctx.run(scan, wide[:, ::2], torch.zeros((4, 6), dtype=torch.int32, device="cuda"))
     op = ctx.prepare(iane_amd.reduce("sum", 0), src, torch.zeros(1, dtype=torch.int32, device="cuda"))
     with pytest.raises(iane_amd.invalid_argument):
         ctx.run(op, torch.zeros((4, 7), dtype=torch.int32, device="cuda"), torch.zeros(1, dtype=torch.int32,
