@@ -27,6 +27,21 @@ struct fft_radix_plan {
 	int radix[kFftMaxStages];
 };
 
+// Division of n < 2^16 by a wave-uniform d in [1, 2^16): q = hi32(n * c) with
+// c = ceil(2^32 / d) is exact in that range (c = (2^32 + e) / d with e < d, so
+// the error term n e / (d 2^32) stays below 1 / d). One 32-bit division per
+// divisor instead of one per element.
+struct fast_div {
+	unsigned d;
+	unsigned c;
+	__device__ __forceinline__ explicit fast_div(int aD)
+		: d(static_cast<unsigned>(aD)), c(aD > 1 ? 0xFFFFFFFFu / static_cast<unsigned>(aD) + 1u : 0u) {}
+	__device__ __forceinline__ int div(int aN) const
+	{
+		return d == 1 ? aN : static_cast<int>(__umulhi(static_cast<unsigned>(aN), c));
+	}
+};
+
 __device__ __forceinline__ float2 cmul(float2 a, float2 b)
 {
 	return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -37,6 +52,26 @@ __device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y
 __device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
 // i * a
 __device__ __forceinline__ float2 cmuli(float2 a) { return make_float2(-a.y, a.x); }
+
+// cos(2 pi j / R) and sin(2 pi j / R), j = 0..R-1, for the odd prime radices
+// with their own butterfly (values of the mathematical constants, rounded to float).
+template<int R>
+struct unit_roots;
+template<>
+struct unit_roots<7> {
+	static constexpr float c[7] = {1.000000000e+00f, 6.234898019e-01f, -2.225209340e-01f, -9.009688679e-01f, -9.009688679e-01f, -2.225209340e-01f, 6.234898019e-01f};
+	static constexpr float s[7] = {0.000000000e+00f, 7.818314825e-01f, 9.749279122e-01f, 4.338837391e-01f, -4.338837391e-01f, -9.749279122e-01f, -7.818314825e-01f};
+};
+template<>
+struct unit_roots<11> {
+	static constexpr float c[11] = {1.000000000e+00f, 8.412535328e-01f, 4.154150130e-01f, -1.423148383e-01f, -6.548607339e-01f, -9.594929736e-01f, -9.594929736e-01f, -6.548607339e-01f, -1.423148383e-01f, 4.154150130e-01f, 8.412535328e-01f};
+	static constexpr float s[11] = {0.000000000e+00f, 5.406408175e-01f, 9.096319954e-01f, 9.898214419e-01f, 7.557495744e-01f, 2.817325568e-01f, -2.817325568e-01f, -7.557495744e-01f, -9.898214419e-01f, -9.096319954e-01f, -5.406408175e-01f};
+};
+template<>
+struct unit_roots<13> {
+	static constexpr float c[13] = {1.000000000e+00f, 8.854560257e-01f, 5.680647467e-01f, 1.205366803e-01f, -3.546048870e-01f, -7.485107482e-01f, -9.709418174e-01f, -9.709418174e-01f, -7.485107482e-01f, -3.546048870e-01f, 1.205366803e-01f, 5.680647467e-01f, 8.854560257e-01f};
+	static constexpr float s[13] = {0.000000000e+00f, 4.647231720e-01f, 8.229838659e-01f, 9.927088741e-01f, 9.350162427e-01f, 6.631226582e-01f, 2.393156643e-01f, -2.393156643e-01f, -6.631226582e-01f, -9.350162427e-01f, -9.927088741e-01f, -8.229838659e-01f, -4.647231720e-01f};
+};
 
 // R-point DFT in registers; sign -1 forward, +1 inverse; tw = W_m table, m = line length.
 template<int R>
@@ -97,26 +132,42 @@ __device__ __forceinline__ void dft(float2 (&v)[R], float sign, const float2* tw
 			v[k + 4] = csub(e[k], o[k]);
 		}
 	} else {
-		// odd prime radix: direct DFT, twiddles from the W_m table
+		// odd prime radix R (7, 11, 13): with a_k = v[k] + v[R-k], b_k = v[k] - v[R-k]
+		// (k = 1..h, h = (R-1)/2), X[0] = v[0] + sum a_k and, for q = 1..h,
+		// X[q] = C_q + (sign i) S_q, X[R-q] = C_q - (sign i) S_q with
+		// C_q = v[0] + sum a_k cos(2 pi qk / R), S_q = sum b_k sin(2 pi qk / R).
+		constexpr int h = (R - 1) / 2;
+		float2 a[h + 1], b[h + 1];
+		float2 sum = v[0];
+#pragma unroll
+		for (int k = 1; k <= h; ++k) {
+			a[k] = cadd(v[k], v[R - k]);
+			b[k] = csub(v[k], v[R - k]);
+			sum = cadd(sum, a[k]);
+		}
 		float2 out[R];
-		const int step = m / R;
+		out[0] = sum;
 #pragma unroll
-		for (int k = 0; k < R; ++k) {
-			float2 acc = v[0];
+		for (int q = 1; q <= h; ++q) {
+			float2 cq = v[0];
+			float2 sq = make_float2(0.0f, 0.0f);
 #pragma unroll
-			for (int r = 1; r < R; ++r) {
-				float2 w = tw[((r * k) % R) * step];
-				if (sign > 0) {
-					w = cconj(w);
-				}
-				acc = cadd(acc, cmul(v[r], w));
+			for (int k = 1; k <= h; ++k) {
+				const float c = unit_roots<R>::c[(q * k) % R];
+				const float sn = unit_roots<R>::s[(q * k) % R];
+				cq = make_float2(cq.x + c * a[k].x, cq.y + c * a[k].y);
+				sq = make_float2(sq.x + sn * b[k].x, sq.y + sn * b[k].y);
 			}
-			out[k] = acc;
+			const float2 rot = sign < 0 ? make_float2(sq.y, -sq.x) : make_float2(-sq.y, sq.x); // (sign i) * S_q
+			out[q] = cadd(cq, rot);
+			out[R - q] = csub(cq, rot);
 		}
 #pragma unroll
 		for (int k = 0; k < R; ++k) {
 			v[k] = out[k];
 		}
+		(void)tw;
+		(void)m;
 	}
 }
 
@@ -128,8 +179,9 @@ __device__ __forceinline__ void stockham_stage(const float2* src, float2* dst, i
 	const int nb = m / R;
 	const int total = bundle * nb;
 	const int step = m / (Ns * R);
+	const fast_div by_nb(nb), by_ns(Ns);
 	for (int t = threadIdx.x; t < total; t += blockDim.x) {
-		const int w = t / nb;
+		const int w = by_nb.div(t);
 		const int j = t - w * nb;
 		const float2* s = src + w * pitch;
 		float2 v[R];
@@ -137,7 +189,7 @@ __device__ __forceinline__ void stockham_stage(const float2* src, float2* dst, i
 		for (int r = 0; r < R; ++r) {
 			v[r] = s[j + r * nb];
 		}
-		const int k = j % Ns;
+		const int k = j - by_ns.div(j) * Ns;
 		if (Ns > 1) {
 #pragma unroll
 			for (int r = 1; r < R; ++r) {
@@ -287,14 +339,16 @@ __device__ inline void load_complex_lines(const float2* in, const line_geom& g, 
 	const long long base = aOuter * g.outer + aLine0 * g.line;
 	const int total = count * width;
 	if (g.elem == 1) {
+		const fast_div by_width(width);
 		for (int t = threadIdx.x; t < total; t += blockDim.x) {
-			const int w = t / width;
+			const int w = by_width.div(t);
 			const int k = t - w * width;
 			buf[w * pitch + k] = k < load_n ? in[base + w * g.line + k] : make_float2(0.0f, 0.0f);
 		}
 	} else {
+		const fast_div by_count(count);
 		for (int t = threadIdx.x; t < total; t += blockDim.x) {
-			const int k = t / count;
+			const int k = by_count.div(t);
 			const int w = t - k * count;
 			buf[w * pitch + k] = k < load_n ? in[base + w * g.line + k * g.elem] : make_float2(0.0f, 0.0f);
 		}
@@ -307,14 +361,16 @@ __device__ inline void store_complex_lines(float2* out, const line_geom& g, int 
 	const long long base = aOuter * g.outer + aLine0 * g.line;
 	const int total = count * store_n;
 	if (g.elem == 1) {
+		const fast_div by_store(store_n);
 		for (int t = threadIdx.x; t < total; t += blockDim.x) {
-			const int w = t / store_n;
+			const int w = by_store.div(t);
 			const int k = t - w * store_n;
 			out[base + w * g.line + k] = buf[w * pitch + k];
 		}
 	} else {
+		const fast_div by_count(count);
 		for (int t = threadIdx.x; t < total; t += blockDim.x) {
-			const int k = t / count;
+			const int k = by_count.div(t);
 			const int w = t - k * count;
 			out[base + w * g.line + k * g.elem] = buf[w * pitch + k];
 		}
@@ -330,13 +386,14 @@ __device__ inline void load_real_lines(const float* in, const line_geom& g, int 
 	const bool even = (n & 1) == 0;
 	const int width = even ? n / 2 : n;
 	const int total = count * width;
+	const fast_div by_width(width), by_count(count);
 	for (int t = threadIdx.x; t < total; t += blockDim.x) {
 		int w, j;
 		if (g.elem == 1) {
-			w = t / width;
+			w = by_width.div(t);
 			j = t - w * width;
 		} else {
-			j = t / count;
+			j = by_count.div(t);
 			w = t - j * count;
 		}
 		const float* line = in + base + w * g.line;
@@ -359,8 +416,9 @@ __device__ inline void r2c_post(const float2* z, float2* x, int count, int n, in
 {
 	const int m = n / 2;
 	const int total = count * (m + 1);
+	const fast_div by_line(m + 1);
 	for (int t = threadIdx.x; t < total; t += blockDim.x) {
-		const int w = t / (m + 1);
+		const int w = by_line.div(t);
 		const int k = t - w * (m + 1);
 		const float2* zl = z + w * pitch;
 		const float2 zk = zl[k == m ? 0 : k];
@@ -378,8 +436,9 @@ __device__ inline void c2r_pre(const float2* x, float2* z, int count, int n, int
 {
 	const int m = n / 2;
 	const int total = count * m;
+	const fast_div by_line(m);
 	for (int t = threadIdx.x; t < total; t += blockDim.x) {
-		const int w = t / m;
+		const int w = by_line.div(t);
 		const int k = t - w * m;
 		const float2* xl = x + w * pitch;
 		const float2 xk = xl[k];
@@ -395,8 +454,9 @@ __device__ inline void c2r_extend_odd(float2* x, int count, int n, int pitch)
 {
 	const int h = n / 2;
 	const int total = count * (n - h - 1);
+	const fast_div by_line(n - h - 1);
 	for (int t = threadIdx.x; t < total; t += blockDim.x) {
-		const int w = t / (n - h - 1);
+		const int w = by_line.div(t);
 		const int k = h + 1 + (t - w * (n - h - 1));
 		x[w * pitch + k] = cconj(x[w * pitch + n - k]);
 	}
@@ -408,13 +468,14 @@ __device__ inline void store_real_lines(float* out, const line_geom& g, int aOut
 	const long long base = aOuter * g.outer + aLine0 * g.line;
 	const bool even = (n & 1) == 0;
 	const int total = count * store_n;
+	const fast_div by_store(store_n), by_count(count);
 	for (int t = threadIdx.x; t < total; t += blockDim.x) {
 		int w, i;
 		if (g.elem == 1) {
-			w = t / store_n;
+			w = by_store.div(t);
 			i = t - w * store_n;
 		} else {
-			i = t / count;
+			i = by_count.div(t);
 			w = t - i * count;
 		}
 		const float2 v = buf[w * pitch + (even ? i / 2 : i)];
@@ -432,6 +493,8 @@ __device__ __forceinline__ void fft_pass_body(const fft_pass_args& a, unsigned a
 {
 	float2* A = lds;
 	float2* B = lds + a.lds_elems;
+	const float2* tw_m = a.tw_m;
+	const float2* tw_n = a.tw_n;
 	int o, l0, count;
 	decode_group(a, aBlock, o, l0, count);
 	const int m = a.plan.m;
@@ -439,20 +502,21 @@ __device__ __forceinline__ void fft_pass_body(const fft_pass_args& a, unsigned a
 	if constexpr (MODE == PASS_C2C) {
 		load_complex_lines(static_cast<const float2*>(a.in), a.gin, o, l0, count, a.load_n, m, A, a.pitch);
 		__syncthreads();
-		const float2* r = run_stages(a.plan, A, B, count, a.pitch, a.tw_m, a.sign);
+		const float2* r = run_stages(a.plan, A, B, count, a.pitch, tw_m, a.sign);
 		store_complex_lines(static_cast<float2*>(a.out), a.gout, o, l0, count, a.store_n, r, a.pitch);
 	} else if constexpr (MODE == PASS_C2C_MULT) {
 		load_complex_lines(static_cast<const float2*>(a.in), a.gin, o, l0, count, a.load_n, m, A, a.pitch);
 		__syncthreads();
-		float2* r = run_stages(a.plan, A, B, count, a.pitch, a.tw_m, -1.0f);
+		float2* r = run_stages(a.plan, A, B, count, a.pitch, tw_m, -1.0f);
 		const long long hbase = o * a.gmult.outer + l0 * a.gmult.line;
+		const fast_div by_m(m), by_count(count);
 		for (int t = threadIdx.x; t < count * m; t += blockDim.x) {
 			int w, k;
 			if (a.gmult.elem == 1) {
-				w = t / m;
+				w = by_m.div(t);
 				k = t - w * m;
 			} else {
-				k = t / count;
+				k = by_count.div(t);
 				w = t - k * count;
 			}
 			float2 h = a.mult[hbase + w * a.gmult.line + k * a.gmult.elem];
@@ -462,15 +526,15 @@ __device__ __forceinline__ void fft_pass_body(const fft_pass_args& a, unsigned a
 			r[w * a.pitch + k] = cscale(cmul(r[w * a.pitch + k], h), a.mult_scale);
 		}
 		__syncthreads();
-		const float2* q = run_stages(a.plan, r, r == A ? B : A, count, a.pitch, a.tw_m, 1.0f);
+		const float2* q = run_stages(a.plan, r, r == A ? B : A, count, a.pitch, tw_m, 1.0f);
 		store_complex_lines(static_cast<float2*>(a.out), a.gout, o, l0, count, a.store_n, q, a.pitch);
 	} else if constexpr (MODE == PASS_R2C) {
 		load_real_lines(static_cast<const float*>(a.in), a.gin, o, l0, count, n, a.load_n, A, a.pitch);
 		__syncthreads();
-		float2* r = run_stages(a.plan, A, B, count, a.pitch, a.tw_m, -1.0f);
+		float2* r = run_stages(a.plan, A, B, count, a.pitch, tw_m, -1.0f);
 		if ((n & 1) == 0) {
 			float2* x = r == A ? B : A;
-			r2c_post(r, x, count, n, a.pitch, a.tw_n);
+			r2c_post(r, x, count, n, a.pitch, tw_n);
 			__syncthreads();
 			r = x;
 		}
@@ -479,16 +543,16 @@ __device__ __forceinline__ void fft_pass_body(const fft_pass_args& a, unsigned a
 		if ((n & 1) == 0) {
 			load_complex_lines(static_cast<const float2*>(a.in), a.gin, o, l0, count, a.load_n, m + 1, A, a.pitch);
 			__syncthreads();
-			c2r_pre(A, B, count, n, a.pitch, a.tw_n);
+			c2r_pre(A, B, count, n, a.pitch, tw_n);
 			__syncthreads();
-			const float2* r = run_stages(a.plan, B, A, count, a.pitch, a.tw_m, 1.0f);
+			const float2* r = run_stages(a.plan, B, A, count, a.pitch, tw_m, 1.0f);
 			store_real_lines(static_cast<float*>(a.out), a.gout, o, l0, count, n, a.store_n, r, a.pitch);
 		} else {
 			load_complex_lines(static_cast<const float2*>(a.in), a.gin, o, l0, count, a.load_n, m, A, a.pitch);
 			__syncthreads();
 			c2r_extend_odd(A, count, n, a.pitch);
 			__syncthreads();
-			const float2* r = run_stages(a.plan, A, B, count, a.pitch, a.tw_m, 1.0f);
+			const float2* r = run_stages(a.plan, A, B, count, a.pitch, tw_m, 1.0f);
 			store_real_lines(static_cast<float*>(a.out), a.gout, o, l0, count, n, a.store_n, r, a.pitch);
 		}
 	}

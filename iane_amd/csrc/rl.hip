@@ -180,20 +180,24 @@ __device__ __forceinline__ void rl_row_body(const rl_row_args& a, unsigned aBloc
 	const fft_pass_args& f = a.f;
 	float2* A = lds;
 	float2* B = lds + f.lds_elems;
+	const float2* tw_m = f.tw_m;
+	const float2* tw_n = f.tw_n;
 	int o, l0, count;
 	decode_group(f, aBlock, o, l0, count);
 	const int n = f.n;
 	const int m = n / 2;
 	load_complex_lines(static_cast<const float2*>(f.in), f.gin, o, l0, count, f.load_n, m + 1, A, f.pitch);
 	__syncthreads();
-	c2r_pre(A, B, count, n, f.pitch, f.tw_n);
+	c2r_pre(A, B, count, n, f.pitch, tw_n);
 	__syncthreads();
-	float2* r = run_stages(f.plan, B, A, count, f.pitch, f.tw_m, 1.0f);
+	float2* r = run_stages(f.plan, B, A, count, f.pitch, tw_m, 1.0f);
 	// pointwise step on real samples 2j, 2j+1 of each row
 	double ll = 0.0;
 	const long long base = o * a.real_outer + l0 * a.real_line;
+	const bool want_ll = EPI == EPI_RATIO && a.partials != nullptr; // uniform
+	const fast_div by_m(m);
 	for (int t = threadIdx.x; t < count * m; t += blockDim.x) {
-		const int w = t / m;
+		const int w = by_m.div(t);
 		const int j = t - w * m;
 		const float2 v = r[w * f.pitch + j];
 		float out[2] = {0.0f, 0.0f};
@@ -208,7 +212,9 @@ __device__ __forceinline__ void rl_row_body(const rl_row_args& a, unsigned aBloc
 					const float pred = in[h];
 					const float floor = pred > a.eps ? pred : a.eps;
 					out[h] = obs / floor;
-					ll += static_cast<double>(obs) * log(static_cast<double>(floor)) - static_cast<double>(pred);
+					if (want_ll) {
+						ll += static_cast<double>(obs) * log(static_cast<double>(floor)) - static_cast<double>(pred);
+					}
 				} else {
 					const float xn = a.x[idx] * edge_factor(in[h], a.s[idx], a.eps);
 					a.x[idx] = xn;
@@ -230,9 +236,9 @@ __device__ __forceinline__ void rl_row_body(const rl_row_args& a, unsigned aBloc
 		return;
 	}
 	__syncthreads();
-	float2* z = run_stages(f.plan, r, r == A ? B : A, count, f.pitch, f.tw_m, -1.0f);
+	float2* z = run_stages(f.plan, r, r == A ? B : A, count, f.pitch, tw_m, -1.0f);
 	float2* xb = z == A ? B : A;
-	r2c_post(z, xb, count, n, f.pitch, f.tw_n);
+	r2c_post(z, xb, count, n, f.pitch, tw_n);
 	__syncthreads();
 	store_complex_lines(static_cast<float2*>(f.out), f.gout, o, l0, count, f.store_n, xb, f.pitch);
 }
