@@ -13,6 +13,16 @@
 // region with its x/y halo (border applied at staging) into LDS as the
 // accumulator type, then each thread accumulates a 4 x 2 register block with a
 // sliding register window along x (kernel width is a template parameter).
+//
+// Streaming kernel (rank <= 3, small kernels with kz >= 2): a workgroup owns a
+// 64 x 16 (x, y) tile and walks a chunk of output planes; every input plane is
+// staged once into an LDS ring of input planes (next plane prefetched into
+// registers, one barrier per plane), so the input is read from memory once per
+// chunk instead of once per kernel plane. The ring has kz + 2 slots: the plane
+// stored at step z + 1 never overwrites a plane that step z still reads, so
+// the barrier after each store is the only one. Each output keeps the same FP64 (or
+// FP32) fma chain in the reference's tap order, so both kernels produce
+// identical bits.
 #include "common.hpp"
 
 namespace iane_hip {
@@ -211,6 +221,415 @@ __global__ __launch_bounds__(kThreads) void conv_simple_kernel(conv_args a, long
 	store_converted(out, a.dst.type, sum);
 }
 
+constexpr int kStreamTileY = 16;    // output rows per streaming workgroup (16 x 16 threads, 4 x-outputs each)
+constexpr int kStreamChunk = 8;     // output planes per streaming workgroup (before grid fitting)
+constexpr int kStreamStage = 6;     // staged elements per thread and plane (at most)
+constexpr int kStreamDepth = 2;     // input planes of loads in flight per thread
+
+// LDS row length of a streamed plane, a multiple of 4 samples: every thread's
+// window (4 outputs + KX - 1 halo samples) starts 16-byte aligned.
+__host__ __device__ constexpr int stream_cols(int aKx)
+{
+	return (kTileX + aKx - 1 + 3) & ~3;
+}
+
+// Read SPAN consecutive 4-byte samples from a 16-byte aligned LDS row with
+// 16- and 8-byte reads.
+template<int SPAN, class T>
+__device__ __forceinline__ void read_window(const T* aRow, T (&aOut)[SPAN])
+{
+	static_assert(sizeof(T) == 4 && SPAN % 2 == 0);
+	using V4 = T __attribute__((ext_vector_type(4)));
+	using V2 = T __attribute__((ext_vector_type(2)));
+#pragma unroll
+	for (int j = 0; j + 4 <= SPAN; j += 4) {
+		const V4 v = *reinterpret_cast<const V4*>(aRow + j);
+		aOut[j] = v.x;
+		aOut[j + 1] = v.y;
+		aOut[j + 2] = v.z;
+		aOut[j + 3] = v.w;
+	}
+	if constexpr (SPAN % 4 == 2) {
+		const V2 v = *reinterpret_cast<const V2*>(aRow + SPAN - 2);
+		aOut[SPAN - 2] = v.x;
+		aOut[SPAN - 1] = v.y;
+	}
+}
+
+// Where a streaming workgroup works and which samples each of its threads
+// stages: a 64 x 16 output tile of planes [z0, z1), fed by input planes
+// [z0 - cz, z1 + cz) that pass through an LDS ring of kz + 2 planes. A thread
+// stages elements tid + e * 256 (e < kStreamStage) of every plane, with the x and
+// y border already applied.
+struct stream_tile {
+	int x0, y0, z0, z1, cz;
+	int rows, cols, slots, plane_elems;
+	int tx, ty, tid;
+	long long offset[kStreamStage];
+	bool inside[kStreamStage];
+
+	__device__ stream_tile(const conv_args& a, int aCols, int aChunk)
+	{
+		const int ky = a.ker.extent[1];
+		const int kz = a.ker.extent[2];
+		const unsigned block = xcd_remap(blockIdx.x, gridDim.x);
+		const unsigned tile_x = block % a.tiles_x;
+		const unsigned rest = block / a.tiles_x;
+		const unsigned tile_y = rest % a.tiles_y;
+		const unsigned chunk = rest / a.tiles_y;
+		x0 = static_cast<int>(tile_x) * kTileX;
+		y0 = static_cast<int>(tile_y) * kStreamTileY;
+		z0 = static_cast<int>(chunk) * aChunk;
+		z1 = min(z0 + aChunk, a.dst.extent[2]);
+		cz = a.center[2];
+		rows = kStreamTileY + ky - 1;
+		cols = aCols;
+		slots = kz + 2;
+		plane_elems = rows * cols;
+		tid = threadIdx.x;
+		tx = tid % kThreadsX;
+		ty = tid / kThreadsX;
+#pragma unroll
+		for (int e = 0; e < kStreamStage; ++e) {
+			const int i = tid + e * kThreads;
+			const int ly = i / cols;
+			const int lx = i - ly * cols;
+			int xi = x0 - a.center[0] + lx;
+			int yi = y0 - a.center[1] + ly;
+			inside[e] = i < plane_elems && border_coord(xi, a.src.extent[0], a.border) &&
+				border_coord(yi, a.src.extent[1], a.border);
+			offset[e] = inside[e] ? xi * a.src.stride[0] + yi * a.src.stride[1] : 0;
+		}
+	}
+
+	__device__ int slot_of(int aPlane) const { return ((aPlane % slots) + slots) % slots; }
+
+	// Issue the loads of one input plane (z border applied) into registers. The
+	// loads are unconditional (samples outside the image read the plane's first
+	// sample) and of one static type; the raw values are masked and converted
+	// only when stored to the ring, so nothing uses them before then and they
+	// stay in flight across the taps of the planes in between.
+	template<class S>
+	__device__ bool load(const conv_args& a, int aPlane, S (&aRaw)[kStreamStage]) const
+	{
+		int zi = aPlane;
+		const bool plane_in = border_coord(zi, a.src.extent[2], a.border);
+		const char* base = a.src.ptr + static_cast<long long>(plane_in ? zi : 0) * a.src.stride[2];
+#pragma unroll
+		for (int e = 0; e < kStreamStage; ++e) {
+			aRaw[e] = *reinterpret_cast<const S*>(base + offset[e]);
+		}
+		return plane_in;
+	}
+
+	// Store a staged plane to its ring slot. Elements past the plane go to a
+	// per-thread spare word after the ring (aSpare[tid]) instead of behind a
+	// branch: with no branch between the loads and this store the compiler
+	// waits for exactly the loads it needs.
+	template<class T, class S>
+	__device__ void store(T* aRing, T* aSpare, int aPlane, const S (&aRaw)[kStreamStage], bool aPlaneIn) const
+	{
+		T* slot = aRing + slot_of(aPlane) * plane_elems;
+#pragma unroll
+		for (int e = 0; e < kStreamStage; ++e) {
+			const int i = tid + e * kThreads;
+			T* at = i < plane_elems ? slot + i : aSpare + tid;
+			*at = (aPlaneIn && inside[e]) ? static_cast<T>(aRaw[e]) : T(0);
+		}
+	}
+
+	// Stream the chunk with P planes of loads in flight: each input plane is
+	// loaded P steps before it is stored to the ring; after the store of plane
+	// z + cz and a barrier, aCompute(z) reads planes z + cz - kz + 1 .. z + cz.
+	// The ring has kz + 2 slots, so the store of the next plane never lands on a
+	// slot that a thread still computing the previous output plane reads.
+	template<int P, class S, class T, class Compute>
+	__device__ void run(const conv_args& a, T* aRing, T* aSpare, Compute aCompute) const
+	{
+		const int first = z0 - cz;
+		const int count = (z1 - z0) + 2 * cz;
+		const int lead = 2 * cz; // planes stored before the first output plane
+		S staged[P][kStreamStage];
+		bool plane_in[P];
+#pragma unroll
+		for (int q = 0; q < P; ++q) {
+			plane_in[q] = load(a, first + min(q, count - 1), staged[q]);
+		}
+		for (int base = 0; base < count; base += P) {
+#pragma unroll
+			for (int q = 0; q < P; ++q) {
+				const int i = base + q;
+				if (i < count) {
+					store(aRing, aSpare, first + i, staged[q], plane_in[q]);
+					if (i + P < count) {
+						plane_in[q] = load(a, first + i + P, staged[q]);
+					}
+					if (i >= lead) {
+						__syncthreads();
+						aCompute(first + i - cz);
+					}
+				}
+			}
+		}
+	}
+};
+
+// Streaming kernel for FP32 accumulation: the same fma chain per output as the
+// tiled kernel (kernel rows from the back, x taps within a row from the back),
+// hence the same bits.
+template<class Acc, int KX, int P>
+__global__ __launch_bounds__(kThreads) void conv_stream_kernel(conv_args a, int aChunk)
+{
+	extern __shared__ __align__(16) char smem[];
+	const int ky = a.ker.extent[1];
+	const int kz = a.ker.extent[2];
+	const stream_tile t(a, stream_cols(KX), aChunk);
+	Acc* ring = reinterpret_cast<Acc*>(smem);
+	Acc* coef = ring + t.slots * t.plane_elems;
+	Acc* spare = coef + KX * ky * kz;
+	for (int i = t.tid; i < KX * ky * kz; i += kThreads) {
+		const int qx = i % KX;
+		const int qy = (i / KX) % ky;
+		const int qz = i / (KX * ky);
+		coef[i] = load_as<Acc>(a.ker.ptr + qx * a.ker.stride[0] + qy * a.ker.stride[1] + qz * a.ker.stride[2],
+			a.ker.type);
+	}
+	const auto compute = [&](int z) {
+		Acc acc[kRx];
+#pragma unroll
+		for (int rx = 0; rx < kRx; ++rx) {
+			acc[rx] = Acc(0);
+		}
+		for (int qz = 0; qz < kz; ++qz) {
+			const Acc* slot = ring + t.slot_of(z + t.cz - qz) * t.plane_elems;
+			for (int qy = 0; qy < ky; ++qy) {
+				const Acc* row = slot + (t.ty + (ky - 1 - qy)) * t.cols + t.tx * kRx;
+				const Acc* krow = coef + (qz * ky + qy) * KX;
+				constexpr int span = kRx + KX - 1;
+				Acc window[span];
+				read_window(row, window);
+#pragma unroll
+				for (int qx = 0; qx < KX; ++qx) {
+					const Acc k = krow[qx];
+#pragma unroll
+					for (int rx = 0; rx < kRx; ++rx) {
+						acc[rx] = fma(window[rx + KX - 1 - qx], k, acc[rx]);
+					}
+				}
+			}
+		}
+		const int py = t.y0 + t.ty;
+		if (py < a.dst.extent[1]) {
+			char* out = a.dst.ptr + static_cast<long long>(z) * a.dst.stride[2] + py * a.dst.stride[1];
+#pragma unroll
+			for (int rx = 0; rx < kRx; ++rx) {
+				const int px = t.x0 + t.tx * kRx + rx;
+				if (px < a.dst.extent[0]) {
+					store_converted(out + px * a.dst.stride[0], a.dst.type, acc[rx]);
+				}
+			}
+		}
+	};
+	with_element_type(a.src.type, [&](auto aTag) { t.run<P, decltype(aTag)>(a, ring, spare, compute); });
+}
+
+__device__ __forceinline__ int load_as_int(const char* aPtr, int aType)
+{
+	switch (aType) {
+	case IANE_U8: return *reinterpret_cast<const uint8_t*>(aPtr);
+	case IANE_U16: return *reinterpret_cast<const uint16_t*>(aPtr);
+	default: return *reinterpret_cast<const int32_t*>(aPtr);
+	}
+}
+
+// Streaming kernel for integer images (u8, u16) with integer kernels: the ring
+// holds int32 samples. When every partial sum provably fits in int32 (largest
+// source value x sum |k| < 2^31, and |k| < 2^23 so the 24-bit multiply is
+// exact), taps accumulate with 24-bit integer multiply-adds; the sum is the
+// exact integer the reference's FP64 chain represents (it is exact below 2^53),
+// converted the same way. Otherwise the workgroup runs the FP64 chain in the
+// tiled kernel's tap order on the same ring.
+template<int KX, int P>
+__global__ __launch_bounds__(kThreads) void conv_stream_int_kernel(conv_args a, int aChunk, int aSourceMax)
+{
+	extern __shared__ __align__(16) char smem[];
+	const int ky = a.ker.extent[1];
+	const int kz = a.ker.extent[2];
+	const stream_tile t(a, stream_cols(KX), aChunk);
+	int* ring = reinterpret_cast<int*>(smem);
+	int* coef = ring + t.slots * t.plane_elems;
+	int* spare = coef + KX * ky * kz;
+	__shared__ int exact_int;
+
+	const int taps = KX * ky * kz;
+	for (int i = t.tid; i < taps; i += kThreads) {
+		const int qx = i % KX;
+		const int qy = (i / KX) % ky;
+		const int qz = i / (KX * ky);
+		coef[i] = load_as_int(a.ker.ptr + qx * a.ker.stride[0] + qy * a.ker.stride[1] + qz * a.ker.stride[2],
+			a.ker.type);
+	}
+	__syncthreads();
+	if (t.tid == 0) {
+		double total = 0.0;
+		bool small = true;
+		for (int i = 0; i < taps; ++i) {
+			const double k = coef[i] < 0 ? -static_cast<double>(coef[i]) : static_cast<double>(coef[i]);
+			total += k;
+			small = small && k < 8388608.0;
+		}
+		exact_int = small && total * aSourceMax < 2147483648.0 ? 1 : 0;
+	}
+	// exact_int is read after the barrier that precedes the first output plane
+	const auto compute = [&](int z) {
+		constexpr int span = kRx + KX - 1;
+		double sums[kRx];
+		if (exact_int != 0) { // uniform over the workgroup
+			int acc[kRx] = {0, 0, 0, 0};
+			for (int qz = 0; qz < kz; ++qz) {
+				const int* slot = ring + t.slot_of(z + t.cz - qz) * t.plane_elems;
+				for (int qy = 0; qy < ky; ++qy) {
+					const int* row = slot + (t.ty + (ky - 1 - qy)) * t.cols + t.tx * kRx;
+					const int* krow = coef + (qz * ky + qy) * KX;
+					int window[span];
+					read_window(row, window);
+#pragma unroll
+					for (int qx = 0; qx < KX; ++qx) {
+						const int k = krow[qx];
+#pragma unroll
+						for (int rx = 0; rx < kRx; ++rx) {
+							acc[rx] += __mul24(window[rx + KX - 1 - qx], k);
+						}
+					}
+				}
+			}
+#pragma unroll
+			for (int rx = 0; rx < kRx; ++rx) {
+				sums[rx] = static_cast<double>(acc[rx]);
+			}
+		} else {
+#pragma unroll
+			for (int rx = 0; rx < kRx; ++rx) {
+				sums[rx] = 0.0;
+			}
+			for (int qz = 0; qz < kz; ++qz) {
+				const int* slot = ring + t.slot_of(z + t.cz - qz) * t.plane_elems;
+				for (int qy = 0; qy < ky; ++qy) {
+					const int* row = slot + (t.ty + (ky - 1 - qy)) * t.cols + t.tx * kRx;
+					const int* krow = coef + (qz * ky + qy) * KX;
+					int raw[span];
+					read_window(row, raw);
+					double window[span];
+#pragma unroll
+					for (int j = 0; j < span; ++j) {
+						window[j] = static_cast<double>(raw[j]);
+					}
+#pragma unroll
+					for (int qx = 0; qx < KX; ++qx) {
+						const double k = static_cast<double>(krow[qx]);
+#pragma unroll
+						for (int rx = 0; rx < kRx; ++rx) {
+							sums[rx] = fma(window[rx + KX - 1 - qx], k, sums[rx]);
+						}
+					}
+				}
+			}
+		}
+		const int py = t.y0 + t.ty;
+		if (py < a.dst.extent[1]) {
+			char* out = a.dst.ptr + static_cast<long long>(z) * a.dst.stride[2] + py * a.dst.stride[1];
+#pragma unroll
+			for (int rx = 0; rx < kRx; ++rx) {
+				const int px = t.x0 + t.tx * kRx + rx;
+				if (px < a.dst.extent[0]) {
+					store_converted(out + px * a.dst.stride[0], a.dst.type, sums[rx]);
+				}
+			}
+		}
+	};
+	if (a.src.type == IANE_U8) {
+		t.run<P, uint8_t>(a, ring, spare, compute);
+	} else {
+		t.run<P, uint16_t>(a, ring, spare, compute);
+	}
+}
+
+template<int KX, int P>
+void launch_stream_kx(const conv_args& a, bool aInteger, int aSourceMax, int aChunk, unsigned aBlocks, size_t aLds,
+	hipStream_t aStream)
+{
+	if (aInteger) {
+		conv_stream_int_kernel<KX, P><<<aBlocks, kThreads, aLds, aStream>>>(a, aChunk, aSourceMax);
+	} else {
+		conv_stream_kernel<float, KX, P><<<aBlocks, kThreads, aLds, aStream>>>(a, aChunk);
+	}
+}
+
+template<int P>
+void launch_stream_depth(const conv_args& a, bool aInteger, int aSourceMax, int aChunk, unsigned aBlocks, size_t aLds,
+	hipStream_t aStream)
+{
+	switch (a.ker.extent[0]) {
+	case 1: launch_stream_kx<1, P>(a, aInteger, aSourceMax, aChunk, aBlocks, aLds, aStream); break;
+	case 3: launch_stream_kx<3, P>(a, aInteger, aSourceMax, aChunk, aBlocks, aLds, aStream); break;
+	case 5: launch_stream_kx<5, P>(a, aInteger, aSourceMax, aChunk, aBlocks, aLds, aStream); break;
+	default: launch_stream_kx<7, P>(a, aInteger, aSourceMax, aChunk, aBlocks, aLds, aStream); break;
+	}
+}
+
+// Streaming launch for rank <= 3 with 2 <= kz <= 7, kx in {1, 3, 5, 7}, ky <= 7 and
+// a ring that fits, for FP32 accumulation or integer images with integer
+// kernels; returns false when the tiled kernel should run instead.
+template<class Acc>
+bool launch_stream(conv_args& a, hipStream_t aStream)
+{
+	const int kx = a.ker.extent[0];
+	const int ky = a.ker.extent[1];
+	const int kz = a.ker.extent[2];
+	if (a.src.extent[3] != 1 || a.ker.extent[3] != 1 || kz < 2 || kz > 7 || ky > 7 ||
+		!(kx == 1 || kx == 3 || kx == 5 || kx == 7)) {
+		return false;
+	}
+	const int rows = kStreamTileY + ky - 1;
+	const int cols = stream_cols(kx);
+	if (rows * cols > kStreamStage * kThreads) {
+		return false;
+	}
+	const bool integer = (a.src.type == IANE_U8 || a.src.type == IANE_U16) &&
+		(a.ker.type == IANE_U8 || a.ker.type == IANE_U16 || a.ker.type == IANE_I32) && sizeof(Acc) == 8;
+	if (!integer && sizeof(Acc) != sizeof(float)) {
+		return false; // FP64 window reads from LDS: the tiled kernel is faster
+	}
+	const size_t lds = 4 * (static_cast<size_t>(kz + 2) * rows * cols + static_cast<size_t>(kx) * ky * kz + kThreads);
+	if (lds > 96 * 1024) {
+		return false;
+	}
+	a.tiles_x = ceil_div(a.dst.extent[0], kTileX);
+	a.tiles_y = ceil_div(a.dst.extent[1], kStreamTileY);
+	// short chunks keep several workgroups on every CU; halo planes re-read: (kz - 1) / chunk
+	const long long xy = static_cast<long long>(a.tiles_x) * a.tiles_y;
+	int chunk = kStreamChunk;
+	while (chunk > 4 && xy * ceil_div(a.dst.extent[2], chunk) < 2048) {
+		chunk /= 2;
+	}
+	const int forced = tune_get("conv.stream_chunk", 0);
+	if (forced > 0) {
+		chunk = std::min(forced, 64);
+	}
+	const long long blocks = xy * ceil_div(a.dst.extent[2], chunk);
+	if (blocks > 0x7fffffffLL || blocks == 0) {
+		return false;
+	}
+	const unsigned nb = static_cast<unsigned>(blocks);
+	const int source_max = a.src.type == IANE_U8 ? 255 : 65535;
+	switch (tune_get("conv.stream_depth", kStreamDepth)) {
+	case 1: launch_stream_depth<1>(a, integer, source_max, chunk, nb, lds, aStream); break;
+	case 2: launch_stream_depth<2>(a, integer, source_max, chunk, nb, lds, aStream); break;
+	default: launch_stream_depth<3>(a, integer, source_max, chunk, nb, lds, aStream); break;
+	}
+	return true;
+}
+
 template<class Acc, int KX>
 void launch_tiled(const conv_args& a, size_t aLds, unsigned aBlocks, hipStream_t aStream)
 {
@@ -225,6 +644,9 @@ void launch_conv(conv_args a, hipStream_t aStream, bool aForceSimple)
 	const size_t lds = sizeof(Acc) * (static_cast<size_t>(kTileY + ky - 1) * (kTileX + kx - 1) +
 		static_cast<size_t>(kx) * ky);
 	const long long count = element_count(a.dst);
+	if (!aForceSimple && tune_get("conv.stream", 1) != 0 && launch_stream<Acc>(a, aStream)) {
+		return;
+	}
 	if (aForceSimple || lds > kMaxLds) {
 		conv_simple_kernel<Acc><<<ceil_div(count, kThreads), kThreads, 0, aStream>>>(a, count);
 		return;

@@ -297,3 +297,36 @@ def test_full_size_separable_vs_sampled_oracle():
         ref = float(np.sum(window * dense[::-1, ::-1, ::-1]))
         worst = max(worst, abs(got[z, y, x] - ref) / max(abs(ref), 1e-3))
     assert worst <= TOL_FP32
+
+
+@pytest.mark.parametrize("shape,kshape", [((40, 37, 70), (3, 3, 3)), ((9, 20, 130), (5, 3, 7)), ((5, 33, 64), (7, 5, 1)),
+                                          ((3, 17, 65), (5, 5, 5)), ((70, 16, 64), (3, 7, 3))])
+@pytest.mark.parametrize("border", [cases.CLAMP, cases.ZERO])
+def test_streaming_kernel_equals_tiled(shape, kshape, border):
+    """The z-streaming direct kernels (small 3-D kernels) and the per-plane tiled kernel
+    give identical bits in every precision and dtype pair. Integer images with integer
+    kernels accumulate in int32 when the sum provably fits, else in FP64: both cases run
+    (the last pair's coefficients up to 2^21 break the int32 bound)."""
+    from iane_amd import autotune
+    rng = np.random.default_rng(sum(shape) + sum(kshape))
+    for sdt, kdt, ddt, prec in [(np.float32, np.float32, np.float32, precision.reference),
+                                (np.float32, np.float32, np.float32, precision.fp32),
+                                (np.uint16, np.int32, np.int32, precision.reference),
+                                (np.uint8, np.uint8, np.int32, precision.reference),
+                                (np.uint8, np.float32, np.uint8, precision.reference),
+                                (np.uint16, "big", np.float64, precision.reference)]:
+        if kdt == "big":
+            src, _ = _case(rng, shape, kshape, sdt, np.int32)
+            ker = rng.integers(-(1 << 21), 1 << 21, kshape).astype(np.int32)
+        else:
+            src, ker = _case(rng, shape, kshape, sdt, kdt)
+        outs = []
+        for config in ({"conv.stream": 0}, {"conv.stream_depth": 1}, {"conv.stream_depth": 2},
+                       {"conv.stream_depth": 3, "conv.stream_chunk": 5}):
+            with autotune.applied(config):
+                outs.append(gpu_convolve(src, ker, ddt, border, prec))
+        for out in outs[1:]:
+            same_bits(out, outs[0])
+        if prec == precision.reference:
+            same_bits(outs[0], oracle.convolve(src, ker, ddt, border))
+
