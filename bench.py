@@ -342,6 +342,17 @@ def main() -> int:
                                   "GB_s": round(voxels * 6 / t / 1e6, 1),
                                   "frac_hbm": round(voxels * 6 / t / 1e6 / HBM_PEAK_GBS, 3)}
         log(f"[bench] u16 3^3: {t * 1e3:.1f} us")
+        # f32 volume, dense random 3^3 kernel, FP32 arithmetic (z-streaming direct kernel)
+        k3 = torch.from_numpy(data["dense3"]).cuda()
+        o = convolution_options(border_mode.zero, precision.fp32)
+        fn = lambda: iane_amd.convolve(vol, k3, out, o, policy)  # noqa: E731
+        fn()
+        _lib.call("iane_hip_stream_sync", stream)
+        t = timer.time(fn, k) / k
+        extra["dense3_fp32"] = {"ms": round(t, 4), "Mvoxel_s": round(voxels / t / 1e3, 1),
+                                "GB_s": round(voxels * 8 / t / 1e6, 1),
+                                "frac_hbm": round(voxels * 8 / t / 1e6 / HBM_PEAK_GBS, 3)}
+        log(f"[bench] f32 dense 3^3: {t * 1e3:.1f} us")
 
     if not args.quick:
         extra["sequence_256"] = bench_sequence(timer, stream, vol)

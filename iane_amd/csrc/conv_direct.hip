@@ -25,6 +25,8 @@
 // identical bits.
 #include "common.hpp"
 
+#include <atomic>
+
 namespace iane_hip {
 namespace {
 
@@ -44,6 +46,8 @@ struct conv_args {
 	int center[4];
 	unsigned tiles_x;
 	unsigned tiles_y;
+	int src_span; // bytes of one (x, y) plane (streaming kernels)
+	int dst_span;
 };
 
 // Sample the source at an integer coordinate with the border rule; returns
@@ -222,7 +226,6 @@ __global__ __launch_bounds__(kThreads) void conv_simple_kernel(conv_args a, long
 }
 
 constexpr int kStreamTileY = 16;    // output rows per streaming workgroup (16 x 16 threads, 4 x-outputs each)
-constexpr int kStreamChunk = 8;     // output planes per streaming workgroup (before grid fitting)
 constexpr int kStreamStage = 6;     // staged elements per thread and plane (at most)
 constexpr int kStreamDepth = 2;     // input planes of loads in flight per thread
 
@@ -256,17 +259,76 @@ __device__ __forceinline__ void read_window(const T* aRow, T (&aOut)[SPAN])
 	}
 }
 
+constexpr int kOob = 0x7fffff00; // byte offset past every plane a streaming kernel addresses
+
+// Raw buffer resource of one plane (GFX9 word 3: 32-bit data format): loads at
+// offsets at or past `aBytes` read 0 and stores there are dropped, so border
+// samples, tile overhang and a zero-border plane need no branch.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t stream_resource(const char* aBase, int aBytes)
+{
+	return __builtin_amdgcn_make_buffer_rsrc(const_cast<char*>(aBase), static_cast<short>(0), aBytes, 0x00020000);
+}
+
+template<class T>
+__device__ __forceinline__ T buffer_load_as(__amdgpu_buffer_rsrc_t aResource, int aOffset)
+{
+	if constexpr (sizeof(T) == 1) {
+		return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b8(aResource, aOffset, 0, 0));
+	} else if constexpr (sizeof(T) == 2) {
+		return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b16(aResource, aOffset, 0, 0));
+	} else if constexpr (sizeof(T) == 4) {
+		return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(aResource, aOffset, 0, 0));
+	} else {
+		return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(aResource, aOffset, 0, 0));
+	}
+}
+
+// The b8 .. b64 builtins move raw words: bit casts, not conversions.
+template<class T>
+__device__ __forceinline__ void buffer_store_as(__amdgpu_buffer_rsrc_t aResource, int aOffset, T aValue)
+{
+	if constexpr (sizeof(T) == 1) {
+		__builtin_amdgcn_raw_buffer_store_b8(__builtin_bit_cast(unsigned char, aValue), aResource, aOffset, 0, 0);
+	} else if constexpr (sizeof(T) == 2) {
+		__builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, aValue), aResource, aOffset, 0, 0);
+	} else if constexpr (sizeof(T) == 4) {
+		__builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, aValue), aResource, aOffset, 0, 0);
+	} else {
+		using v2u = unsigned __attribute__((ext_vector_type(2)));
+		__builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2u, aValue), aResource, aOffset, 0, 0);
+	}
+}
+
+// The output conversion of store_converted for a static destination type.
+template<class D, class Acc>
+__device__ __forceinline__ D convert_out(Acc aSum)
+{
+	if constexpr (std::is_same_v<D, float> && std::is_same_v<Acc, float>) {
+		return aSum;
+	} else {
+		const double sum = static_cast<double>(aSum);
+		if constexpr (std::is_same_v<D, uint8_t>) {
+			return round_byte(sum);
+		} else if constexpr (std::is_same_v<D, uint16_t> || std::is_same_v<D, int32_t>) {
+			return saturate_truncate<D>(sum);
+		} else {
+			return static_cast<D>(sum);
+		}
+	}
+}
+
 // Where a streaming workgroup works and which samples each of its threads
 // stages: a 64 x 16 output tile of planes [z0, z1), fed by input planes
 // [z0 - cz, z1 + cz) that pass through an LDS ring of kz + 2 planes. A thread
 // stages elements tid + e * 256 (e < kStreamStage) of every plane, with the x and
-// y border already applied.
+// y border applied (kOob where the zero border or the tile overhang makes the
+// sample 0), and writes outputs x0 + 4 tx + r of row y0 + ty.
 struct stream_tile {
 	int x0, y0, z0, z1, cz;
 	int rows, cols, slots, plane_elems;
 	int tx, ty, tid;
-	long long offset[kStreamStage];
-	bool inside[kStreamStage];
+	int offset[kStreamStage];
+	int out_offset[kRx];
 
 	__device__ stream_tile(const conv_args& a, int aCols, int aChunk)
 	{
@@ -296,142 +358,124 @@ struct stream_tile {
 			const int lx = i - ly * cols;
 			int xi = x0 - a.center[0] + lx;
 			int yi = y0 - a.center[1] + ly;
-			inside[e] = i < plane_elems && border_coord(xi, a.src.extent[0], a.border) &&
+			const bool inside = i < plane_elems && border_coord(xi, a.src.extent[0], a.border) &&
 				border_coord(yi, a.src.extent[1], a.border);
-			offset[e] = inside[e] ? xi * a.src.stride[0] + yi * a.src.stride[1] : 0;
+			offset[e] = inside ? static_cast<int>(xi * a.src.stride[0] + yi * a.src.stride[1]) : kOob;
+		}
+		const int py = y0 + ty;
+#pragma unroll
+		for (int r = 0; r < kRx; ++r) {
+			const int px = x0 + tx * kRx + r;
+			out_offset[r] = px < a.dst.extent[0] && py < a.dst.extent[1] ?
+				static_cast<int>(px * a.dst.stride[0] + py * a.dst.stride[1]) : kOob;
 		}
 	}
 
-	__device__ int slot_of(int aPlane) const { return ((aPlane % slots) + slots) % slots; }
+	// Ring slot of the plane qz steps before the one in slot aTop.
+	__device__ int slot_before(int aTop, int aSteps) const
+	{
+		const int slot = aTop - aSteps;
+		return slot < 0 ? slot + slots : slot;
+	}
 
-	// Issue the loads of one input plane (z border applied) into registers. The
-	// loads are unconditional (samples outside the image read the plane's first
-	// sample) and of one static type; the raw values are masked and converted
-	// only when stored to the ring, so nothing uses them before then and they
-	// stay in flight across the taps of the planes in between.
+	// Issue the loads of one input plane (z border applied) into registers;
+	// nothing uses them before the plane is stored to the ring P steps later,
+	// so they stay in flight across the taps of the planes in between.
+	// u8 / u16 samples are held one per 32-bit register: packing two into one
+	// register would use them (and wait for them) right after the load.
 	template<class S>
-	__device__ bool load(const conv_args& a, int aPlane, S (&aRaw)[kStreamStage]) const
+	using staged_t = std::conditional_t<(sizeof(S) < 4), uint32_t, S>;
+
+	// A load with aWanted false is still issued (so every path through the
+	// pipeline has the same loads in flight) but reads 0 without touching memory.
+	template<class S>
+	__device__ void load(const conv_args& a, int aPlane, bool aWanted, staged_t<S> (&aRaw)[kStreamStage]) const
 	{
 		int zi = aPlane;
-		const bool plane_in = border_coord(zi, a.src.extent[2], a.border);
-		const char* base = a.src.ptr + static_cast<long long>(plane_in ? zi : 0) * a.src.stride[2];
+		const bool plane_in = aWanted && border_coord(zi, a.src.extent[2], a.border);
+		const auto plane = stream_resource(a.src.ptr + static_cast<long long>(plane_in ? zi : 0) * a.src.stride[2],
+			plane_in ? a.src_span : 0);
 #pragma unroll
 		for (int e = 0; e < kStreamStage; ++e) {
-			aRaw[e] = *reinterpret_cast<const S*>(base + offset[e]);
+			aRaw[e] = static_cast<staged_t<S>>(buffer_load_as<S>(plane, offset[e]));
 		}
-		return plane_in;
 	}
 
 	// Store a staged plane to its ring slot. Elements past the plane go to a
 	// per-thread spare word after the ring (aSpare[tid]) instead of behind a
-	// branch: with no branch between the loads and this store the compiler
-	// waits for exactly the loads it needs.
+	// branch.
 	template<class T, class S>
-	__device__ void store(T* aRing, T* aSpare, int aPlane, const S (&aRaw)[kStreamStage], bool aPlaneIn) const
+	__device__ void store(T* aRing, T* aSpare, int aSlot, const S (&aRaw)[kStreamStage]) const
 	{
-		T* slot = aRing + slot_of(aPlane) * plane_elems;
+		T* slot = aRing + aSlot * plane_elems;
 #pragma unroll
 		for (int e = 0; e < kStreamStage; ++e) {
 			const int i = tid + e * kThreads;
 			T* at = i < plane_elems ? slot + i : aSpare + tid;
-			*at = (aPlaneIn && inside[e]) ? static_cast<T>(aRaw[e]) : T(0);
+			*at = static_cast<T>(aRaw[e]);
+		}
+	}
+
+	template<class D, class Acc>
+	__device__ void write(const conv_args& a, int aZ, const Acc (&aSums)[kRx]) const
+	{
+		const auto plane = stream_resource(a.dst.ptr + static_cast<long long>(aZ) * a.dst.stride[2], a.dst_span);
+#pragma unroll
+		for (int r = 0; r < kRx; ++r) {
+			buffer_store_as(plane, out_offset[r], convert_out<D>(aSums[r]));
 		}
 	}
 
 	// Stream the chunk with P planes of loads in flight: each input plane is
 	// loaded P steps before it is stored to the ring; after the store of plane
-	// z + cz and a barrier, aCompute(z) reads planes z + cz - kz + 1 .. z + cz.
+	// z + cz (to slot s) and a barrier, aCompute(z, s) reads planes
+	// z + cz - kz + 1 .. z + cz from slots s - kz + 1 .. s (mod kz + 2).
 	// The ring has kz + 2 slots, so the store of the next plane never lands on a
 	// slot that a thread still computing the previous output plane reads.
+	// The main loop issues a load after every store (past the chunk: a load
+	// that reads nothing), so at each store the P - 1 younger planes' loads are
+	// known to be in flight and the wait covers only the plane stored; the
+	// last count mod P planes run in a tail with no loads.
 	template<int P, class S, class T, class Compute>
 	__device__ void run(const conv_args& a, T* aRing, T* aSpare, Compute aCompute) const
 	{
 		const int first = z0 - cz;
 		const int count = (z1 - z0) + 2 * cz;
 		const int lead = 2 * cz; // planes stored before the first output plane
-		S staged[P][kStreamStage];
-		bool plane_in[P];
+		staged_t<S> staged[P][kStreamStage];
 #pragma unroll
 		for (int q = 0; q < P; ++q) {
-			plane_in[q] = load(a, first + min(q, count - 1), staged[q]);
+			load<S>(a, first + q, q < count, staged[q]);
 		}
-		for (int base = 0; base < count; base += P) {
+		int slot = 0; // ring slot of plane first + i (the ring starts empty at every chunk)
+		int base = 0;
+		for (; base + P <= count; base += P) {
 #pragma unroll
 			for (int q = 0; q < P; ++q) {
 				const int i = base + q;
-				if (i < count) {
-					store(aRing, aSpare, first + i, staged[q], plane_in[q]);
-					if (i + P < count) {
-						plane_in[q] = load(a, first + i + P, staged[q]);
-					}
-					if (i >= lead) {
-						__syncthreads();
-						aCompute(first + i - cz);
-					}
+				store(aRing, aSpare, slot, staged[q]);
+				load<S>(a, first + i + P, i + P < count, staged[q]);
+				if (i >= lead) {
+					__syncthreads();
+					aCompute(first + i - cz, slot);
 				}
+				slot = slot + 1 == slots ? 0 : slot + 1;
+			}
+		}
+#pragma unroll
+		for (int q = 0; q + 1 < P; ++q) {
+			const int i = base + q;
+			if (i < count) {
+				store(aRing, aSpare, slot, staged[q]);
+				if (i >= lead) {
+					__syncthreads();
+					aCompute(first + i - cz, slot);
+				}
+				slot = slot + 1 == slots ? 0 : slot + 1;
 			}
 		}
 	}
 };
-
-// Streaming kernel for FP32 accumulation: the same fma chain per output as the
-// tiled kernel (kernel rows from the back, x taps within a row from the back),
-// hence the same bits.
-template<class Acc, int KX, int P>
-__global__ __launch_bounds__(kThreads) void conv_stream_kernel(conv_args a, int aChunk)
-{
-	extern __shared__ __align__(16) char smem[];
-	const int ky = a.ker.extent[1];
-	const int kz = a.ker.extent[2];
-	const stream_tile t(a, stream_cols(KX), aChunk);
-	Acc* ring = reinterpret_cast<Acc*>(smem);
-	Acc* coef = ring + t.slots * t.plane_elems;
-	Acc* spare = coef + KX * ky * kz;
-	for (int i = t.tid; i < KX * ky * kz; i += kThreads) {
-		const int qx = i % KX;
-		const int qy = (i / KX) % ky;
-		const int qz = i / (KX * ky);
-		coef[i] = load_as<Acc>(a.ker.ptr + qx * a.ker.stride[0] + qy * a.ker.stride[1] + qz * a.ker.stride[2],
-			a.ker.type);
-	}
-	const auto compute = [&](int z) {
-		Acc acc[kRx];
-#pragma unroll
-		for (int rx = 0; rx < kRx; ++rx) {
-			acc[rx] = Acc(0);
-		}
-		for (int qz = 0; qz < kz; ++qz) {
-			const Acc* slot = ring + t.slot_of(z + t.cz - qz) * t.plane_elems;
-			for (int qy = 0; qy < ky; ++qy) {
-				const Acc* row = slot + (t.ty + (ky - 1 - qy)) * t.cols + t.tx * kRx;
-				const Acc* krow = coef + (qz * ky + qy) * KX;
-				constexpr int span = kRx + KX - 1;
-				Acc window[span];
-				read_window(row, window);
-#pragma unroll
-				for (int qx = 0; qx < KX; ++qx) {
-					const Acc k = krow[qx];
-#pragma unroll
-					for (int rx = 0; rx < kRx; ++rx) {
-						acc[rx] = fma(window[rx + KX - 1 - qx], k, acc[rx]);
-					}
-				}
-			}
-		}
-		const int py = t.y0 + t.ty;
-		if (py < a.dst.extent[1]) {
-			char* out = a.dst.ptr + static_cast<long long>(z) * a.dst.stride[2] + py * a.dst.stride[1];
-#pragma unroll
-			for (int rx = 0; rx < kRx; ++rx) {
-				const int px = t.x0 + t.tx * kRx + rx;
-				if (px < a.dst.extent[0]) {
-					store_converted(out + px * a.dst.stride[0], a.dst.type, acc[rx]);
-				}
-			}
-		}
-	};
-	with_element_type(a.src.type, [&](auto aTag) { t.run<P, decltype(aTag)>(a, ring, spare, compute); });
-}
 
 __device__ __forceinline__ int load_as_int(const char* aPtr, int aType)
 {
@@ -442,6 +486,107 @@ __device__ __forceinline__ int load_as_int(const char* aPtr, int aType)
 	}
 }
 
+// Kernel coefficients of a streaming workgroup: the KX x ky x kz taps in LDS
+// (x fastest); with KYZ > 0 (ky = kz = KYZ) also in registers, where the fully
+// unrolled tap loops read them.
+template<class T, int KX, int KYZ>
+struct stream_coefficients {
+	T* lds;
+	T reg[KYZ > 0 ? KX * KYZ * KYZ : 1];
+
+	// Fills the LDS copy; every thread must pass a barrier before load_registers.
+	__device__ stream_coefficients(const conv_args& a, T* aLds, int aTid) : lds(aLds)
+	{
+		const int ky = a.ker.extent[1];
+		const int taps = KX * ky * a.ker.extent[2];
+		for (int i = aTid; i < taps; i += kThreads) {
+			const int qx = i % KX;
+			const int qy = (i / KX) % ky;
+			const int qz = i / (KX * ky);
+			const char* at = a.ker.ptr + qx * a.ker.stride[0] + qy * a.ker.stride[1] + qz * a.ker.stride[2];
+			if constexpr (std::is_same_v<T, float>) {
+				lds[i] = load_as<float>(at, a.ker.type);
+			} else {
+				lds[i] = load_as_int(at, a.ker.type);
+			}
+		}
+	}
+
+	__device__ void load_registers()
+	{
+		if constexpr (KYZ > 0) {
+#pragma unroll
+			for (int i = 0; i < KX * KYZ * KYZ; ++i) {
+				reg[i] = lds[i];
+			}
+		}
+	}
+};
+
+// Accumulate the taps of the output plane whose newest input plane is in ring
+// slot aTop: aMac(acc, sample, coefficient) over kernel planes qz, rows qy (the
+// tiled kernel's order: rows from the back) and x taps from the back. UNROLL
+// false keeps runtime loops over the LDS coefficients even when KYZ > 0.
+template<int KX, int KYZ, bool UNROLL = true, class T, class Acc, class Mac>
+__device__ __forceinline__ void stream_taps(const stream_tile& t, const T* aRing,
+	const stream_coefficients<T, KX, KYZ>& aCoef, int aKy, int aKz, int aTop, Acc (&aAcc)[kRx], Mac aMac)
+{
+	constexpr int span = kRx + KX - 1;
+	const auto row_taps = [&](const T* aSlot, int aRow, const T* aK) {
+		T window[span];
+		read_window(aSlot + (t.ty + aRow) * t.cols + t.tx * kRx, window);
+#pragma unroll
+		for (int qx = 0; qx < KX; ++qx) {
+			const T k = aK[qx];
+#pragma unroll
+			for (int rx = 0; rx < kRx; ++rx) {
+				aMac(aAcc[rx], window[rx + KX - 1 - qx], k);
+			}
+		}
+	};
+	if constexpr (KYZ > 0 && UNROLL) {
+#pragma unroll
+		for (int qz = 0; qz < KYZ; ++qz) {
+			const T* slot = aRing + t.slot_before(aTop, qz) * t.plane_elems;
+#pragma unroll
+			for (int qy = 0; qy < KYZ; ++qy) {
+				row_taps(slot, KYZ - 1 - qy, aCoef.reg + (qz * KYZ + qy) * KX);
+			}
+		}
+	} else {
+		for (int qz = 0; qz < aKz; ++qz) {
+			const T* slot = aRing + t.slot_before(aTop, qz) * t.plane_elems;
+			for (int qy = 0; qy < aKy; ++qy) {
+				row_taps(slot, aKy - 1 - qy, aCoef.lds + (qz * aKy + qy) * KX);
+			}
+		}
+	}
+}
+
+// Streaming kernel for FP32 accumulation: the same fma chain per output as the
+// tiled kernel (kernel rows from the back, x taps within a row from the back),
+// hence the same bits.
+template<class S, class D, int KX, int KYZ>
+__global__ __launch_bounds__(kThreads) void conv_stream_kernel(conv_args a, int aChunk)
+{
+	extern __shared__ __align__(16) char smem[];
+	const int ky = a.ker.extent[1];
+	const int kz = a.ker.extent[2];
+	const stream_tile t(a, stream_cols(KX), aChunk);
+	float* ring = reinterpret_cast<float*>(smem);
+	stream_coefficients<float, KX, KYZ> coef(a, ring + t.slots * t.plane_elems, t.tid);
+	float* spare = coef.lds + KX * ky * kz;
+	__syncthreads();
+	coef.load_registers();
+	const auto compute = [&](int z, int aTop) {
+		float acc[kRx] = {0.0f, 0.0f, 0.0f, 0.0f};
+		stream_taps(t, ring, coef, ky, kz, aTop, acc,
+			[](float& aSum, float aSample, float aK) { aSum = fma(aSample, aK, aSum); });
+		t.write<D>(a, z, acc);
+	};
+	t.run<kStreamDepth, S>(a, ring, spare, compute);
+}
+
 // Streaming kernel for integer images (u8, u16) with integer kernels: the ring
 // holds int32 samples. When every partial sum provably fits in int32 (largest
 // source value x sum |k| < 2^31, and |k| < 2^23 so the 24-bit multiply is
@@ -449,60 +594,39 @@ __device__ __forceinline__ int load_as_int(const char* aPtr, int aType)
 // exact integer the reference's FP64 chain represents (it is exact below 2^53),
 // converted the same way. Otherwise the workgroup runs the FP64 chain in the
 // tiled kernel's tap order on the same ring.
-template<int KX, int P>
-__global__ __launch_bounds__(kThreads) void conv_stream_int_kernel(conv_args a, int aChunk, int aSourceMax)
+template<class S, class D, int KX, int KYZ>
+__global__ __launch_bounds__(kThreads) void conv_stream_int_kernel(conv_args a, int aChunk)
 {
 	extern __shared__ __align__(16) char smem[];
 	const int ky = a.ker.extent[1];
 	const int kz = a.ker.extent[2];
 	const stream_tile t(a, stream_cols(KX), aChunk);
 	int* ring = reinterpret_cast<int*>(smem);
-	int* coef = ring + t.slots * t.plane_elems;
-	int* spare = coef + KX * ky * kz;
+	stream_coefficients<int, KX, KYZ> coef(a, ring + t.slots * t.plane_elems, t.tid);
+	int* spare = coef.lds + KX * ky * kz;
 	__shared__ int exact_int;
 
-	const int taps = KX * ky * kz;
-	for (int i = t.tid; i < taps; i += kThreads) {
-		const int qx = i % KX;
-		const int qy = (i / KX) % ky;
-		const int qz = i / (KX * ky);
-		coef[i] = load_as_int(a.ker.ptr + qx * a.ker.stride[0] + qy * a.ker.stride[1] + qz * a.ker.stride[2],
-			a.ker.type);
-	}
 	__syncthreads();
+	coef.load_registers();
 	if (t.tid == 0) {
+		constexpr double source_max = sizeof(S) == 1 ? 255.0 : 65535.0;
+		const int taps = KX * ky * kz;
 		double total = 0.0;
 		bool small = true;
 		for (int i = 0; i < taps; ++i) {
-			const double k = coef[i] < 0 ? -static_cast<double>(coef[i]) : static_cast<double>(coef[i]);
+			const double k = coef.lds[i] < 0 ? -static_cast<double>(coef.lds[i]) : static_cast<double>(coef.lds[i]);
 			total += k;
 			small = small && k < 8388608.0;
 		}
-		exact_int = small && total * aSourceMax < 2147483648.0 ? 1 : 0;
+		exact_int = small && total * source_max < 2147483648.0 ? 1 : 0;
 	}
 	// exact_int is read after the barrier that precedes the first output plane
-	const auto compute = [&](int z) {
-		constexpr int span = kRx + KX - 1;
+	const auto compute = [&](int z, int aTop) {
 		double sums[kRx];
 		if (exact_int != 0) { // uniform over the workgroup
 			int acc[kRx] = {0, 0, 0, 0};
-			for (int qz = 0; qz < kz; ++qz) {
-				const int* slot = ring + t.slot_of(z + t.cz - qz) * t.plane_elems;
-				for (int qy = 0; qy < ky; ++qy) {
-					const int* row = slot + (t.ty + (ky - 1 - qy)) * t.cols + t.tx * kRx;
-					const int* krow = coef + (qz * ky + qy) * KX;
-					int window[span];
-					read_window(row, window);
-#pragma unroll
-					for (int qx = 0; qx < KX; ++qx) {
-						const int k = krow[qx];
-#pragma unroll
-						for (int rx = 0; rx < kRx; ++rx) {
-							acc[rx] += __mul24(window[rx + KX - 1 - qx], k);
-						}
-					}
-				}
-			}
+			stream_taps(t, ring, coef, ky, kz, aTop, acc,
+				[](int& aSum, int aSample, int aK) { aSum += __mul24(aSample, aK); });
 #pragma unroll
 			for (int rx = 0; rx < kRx; ++rx) {
 				sums[rx] = static_cast<double>(acc[rx]);
@@ -512,74 +636,88 @@ __global__ __launch_bounds__(kThreads) void conv_stream_int_kernel(conv_args a, 
 			for (int rx = 0; rx < kRx; ++rx) {
 				sums[rx] = 0.0;
 			}
-			for (int qz = 0; qz < kz; ++qz) {
-				const int* slot = ring + t.slot_of(z + t.cz - qz) * t.plane_elems;
-				for (int qy = 0; qy < ky; ++qy) {
-					const int* row = slot + (t.ty + (ky - 1 - qy)) * t.cols + t.tx * kRx;
-					const int* krow = coef + (qz * ky + qy) * KX;
-					int raw[span];
-					read_window(row, raw);
-					double window[span];
-#pragma unroll
-					for (int j = 0; j < span; ++j) {
-						window[j] = static_cast<double>(raw[j]);
-					}
-#pragma unroll
-					for (int qx = 0; qx < KX; ++qx) {
-						const double k = static_cast<double>(krow[qx]);
-#pragma unroll
-						for (int rx = 0; rx < kRx; ++rx) {
-							sums[rx] = fma(window[rx + KX - 1 - qx], k, sums[rx]);
-						}
-					}
-				}
-			}
+			stream_taps<KX, KYZ, false>(t, ring, coef, ky, kz, aTop, sums, [](double& aSum, int aSample, int aK) {
+				aSum = fma(static_cast<double>(aSample), static_cast<double>(aK), aSum);
+			});
 		}
-		const int py = t.y0 + t.ty;
-		if (py < a.dst.extent[1]) {
-			char* out = a.dst.ptr + static_cast<long long>(z) * a.dst.stride[2] + py * a.dst.stride[1];
-#pragma unroll
-			for (int rx = 0; rx < kRx; ++rx) {
-				const int px = t.x0 + t.tx * kRx + rx;
-				if (px < a.dst.extent[0]) {
-					store_converted(out + px * a.dst.stride[0], a.dst.type, sums[rx]);
-				}
-			}
-		}
+		t.write<D>(a, z, sums);
 	};
-	if (a.src.type == IANE_U8) {
-		t.run<P, uint8_t>(a, ring, spare, compute);
-	} else {
-		t.run<P, uint16_t>(a, ring, spare, compute);
-	}
+	t.run<kStreamDepth, S>(a, ring, spare, compute);
 }
 
-template<int KX, int P>
-void launch_stream_kx(const conv_args& a, bool aInteger, int aSourceMax, int aChunk, unsigned aBlocks, size_t aLds,
-	hipStream_t aStream)
+template<bool INTEGER, class S, class D, int KX, int KYZ>
+void launch_stream_shape(const conv_args& a, int aChunk, unsigned aBlocks, size_t aLds, hipStream_t aStream)
 {
-	if (aInteger) {
-		conv_stream_int_kernel<KX, P><<<aBlocks, kThreads, aLds, aStream>>>(a, aChunk, aSourceMax);
+	if constexpr (INTEGER) {
+		conv_stream_int_kernel<S, D, KX, KYZ><<<aBlocks, kThreads, aLds, aStream>>>(a, aChunk);
 	} else {
-		conv_stream_kernel<float, KX, P><<<aBlocks, kThreads, aLds, aStream>>>(a, aChunk);
+		conv_stream_kernel<S, D, KX, KYZ><<<aBlocks, kThreads, aLds, aStream>>>(a, aChunk);
 	}
 }
 
-template<int P>
-void launch_stream_depth(const conv_args& a, bool aInteger, int aSourceMax, int aChunk, unsigned aBlocks, size_t aLds,
-	hipStream_t aStream)
+// FP32 3^3 kernels get fully unrolled tap loops with register coefficients
+// (unrolled, the integer kernel needs about 146 VGPRs: half the occupancy).
+template<bool INTEGER, class S, class D, int KX>
+void launch_stream_kx(const conv_args& a, int aChunk, unsigned aBlocks, size_t aLds, hipStream_t aStream)
+{
+	const bool cubic = a.ker.extent[1] == KX && a.ker.extent[2] == KX && tune_get("conv.stream_unroll", 1) != 0;
+	if constexpr (KX == 3 && !INTEGER) {
+		if (cubic) {
+			launch_stream_shape<INTEGER, S, D, KX, KX>(a, aChunk, aBlocks, aLds, aStream);
+			return;
+		}
+	}
+	launch_stream_shape<INTEGER, S, D, KX, 0>(a, aChunk, aBlocks, aLds, aStream);
+}
+
+template<bool INTEGER, class S, class D>
+void launch_stream_types(const conv_args& a, int aChunk, unsigned aBlocks, size_t aLds, hipStream_t aStream)
 {
 	switch (a.ker.extent[0]) {
-	case 1: launch_stream_kx<1, P>(a, aInteger, aSourceMax, aChunk, aBlocks, aLds, aStream); break;
-	case 3: launch_stream_kx<3, P>(a, aInteger, aSourceMax, aChunk, aBlocks, aLds, aStream); break;
-	case 5: launch_stream_kx<5, P>(a, aInteger, aSourceMax, aChunk, aBlocks, aLds, aStream); break;
-	default: launch_stream_kx<7, P>(a, aInteger, aSourceMax, aChunk, aBlocks, aLds, aStream); break;
+	case 1: launch_stream_kx<INTEGER, S, D, 1>(a, aChunk, aBlocks, aLds, aStream); break;
+	case 3: launch_stream_kx<INTEGER, S, D, 3>(a, aChunk, aBlocks, aLds, aStream); break;
+	case 5: launch_stream_kx<INTEGER, S, D, 5>(a, aChunk, aBlocks, aLds, aStream); break;
+	default: launch_stream_kx<INTEGER, S, D, 7>(a, aChunk, aBlocks, aLds, aStream); break;
 	}
+}
+
+// Compute units of the current device (cached per device).
+int compute_units()
+{
+	int device = 0;
+	check_hip(hipGetDevice(&device), "hipGetDevice");
+	static std::atomic<int> cache[64];
+	if (device >= 0 && device < 64) {
+		const int cached = cache[device].load(std::memory_order_relaxed);
+		if (cached > 0) {
+			return cached;
+		}
+	}
+	int units = 0;
+	check_hip(hipDeviceGetAttribute(&units, hipDeviceAttributeMultiprocessorCount, device), "hipDeviceGetAttribute");
+	units = std::max(units, 1);
+	if (device >= 0 && device < 64) {
+		cache[device].store(units, std::memory_order_relaxed);
+	}
+	return units;
+}
+
+// Byte span of a view's (x, y) plane when both strides are non-negative and the
+// span fits a buffer resource; -1 otherwise.
+int plane_span(const view4& aView)
+{
+	if (aView.stride[0] < 0 || aView.stride[1] < 0) {
+		return -1;
+	}
+	const long long span = (aView.extent[0] - 1) * aView.stride[0] + (aView.extent[1] - 1) * aView.stride[1] +
+		dtype_size(aView.type);
+	return span < kOob - 64 ? static_cast<int>(span) : -1;
 }
 
 // Streaming launch for rank <= 3 with 2 <= kz <= 7, kx in {1, 3, 5, 7}, ky <= 7 and
-// a ring that fits, for FP32 accumulation or integer images with integer
-// kernels; returns false when the tiled kernel should run instead.
+// a ring that fits, for FP32 accumulation of f32 -> f32, u8 -> u8, u16 -> u16,
+// u8 / u16 -> f32, or integer images (u8, u16) with integer kernels at
+// reference precision; returns false when the tiled kernel should run instead.
 template<class Acc>
 bool launch_stream(conv_args& a, hipStream_t aStream)
 {
@@ -595,10 +733,20 @@ bool launch_stream(conv_args& a, hipStream_t aStream)
 	if (rows * cols > kStreamStage * kThreads) {
 		return false;
 	}
-	const bool integer = (a.src.type == IANE_U8 || a.src.type == IANE_U16) &&
-		(a.ker.type == IANE_U8 || a.ker.type == IANE_U16 || a.ker.type == IANE_I32) && sizeof(Acc) == 8;
-	if (!integer && sizeof(Acc) != sizeof(float)) {
+	const int st = a.src.type;
+	const int dt = a.dst.type;
+	const bool int_source = st == IANE_U8 || st == IANE_U16;
+	const bool integer = int_source && (a.ker.type == IANE_U8 || a.ker.type == IANE_U16 || a.ker.type == IANE_I32) &&
+		sizeof(Acc) == 8;
+	const bool fp32 = sizeof(Acc) == sizeof(float) &&
+		((st == IANE_F32 && dt == IANE_F32) || (int_source && (dt == st || dt == IANE_F32)));
+	if (!integer && !fp32) {
 		return false; // FP64 window reads from LDS: the tiled kernel is faster
+	}
+	a.src_span = plane_span(a.src);
+	a.dst_span = plane_span(a.dst);
+	if (a.src_span < 0 || a.dst_span < 0) {
+		return false;
 	}
 	const size_t lds = 4 * (static_cast<size_t>(kz + 2) * rows * cols + static_cast<size_t>(kx) * ky * kz + kThreads);
 	if (lds > 96 * 1024) {
@@ -606,12 +754,13 @@ bool launch_stream(conv_args& a, hipStream_t aStream)
 	}
 	a.tiles_x = ceil_div(a.dst.extent[0], kTileX);
 	a.tiles_y = ceil_div(a.dst.extent[1], kStreamTileY);
-	// short chunks keep several workgroups on every CU; halo planes re-read: (kz - 1) / chunk
+	// One round of workgroups: as many as fit on the chip at once (LDS bound),
+	// each streaming an equal z chunk; halo planes re-read per chunk: kz - 1.
 	const long long xy = static_cast<long long>(a.tiles_x) * a.tiles_y;
-	int chunk = kStreamChunk;
-	while (chunk > 4 && xy * ceil_div(a.dst.extent[2], chunk) < 2048) {
-		chunk /= 2;
-	}
+	const long long resident = static_cast<long long>(compute_units()) *
+		std::max<long long>(1, std::min<long long>(8, (160 * 1024) / static_cast<long long>(lds)));
+	const long long columns = std::max<long long>(1, resident / xy);
+	int chunk = static_cast<int>(std::max<long long>(4, ceil_div(a.dst.extent[2], columns)));
 	const int forced = tune_get("conv.stream_chunk", 0);
 	if (forced > 0) {
 		chunk = std::min(forced, 64);
@@ -621,11 +770,36 @@ bool launch_stream(conv_args& a, hipStream_t aStream)
 		return false;
 	}
 	const unsigned nb = static_cast<unsigned>(blocks);
-	const int source_max = a.src.type == IANE_U8 ? 255 : 65535;
-	switch (tune_get("conv.stream_depth", kStreamDepth)) {
-	case 1: launch_stream_depth<1>(a, integer, source_max, chunk, nb, lds, aStream); break;
-	case 2: launch_stream_depth<2>(a, integer, source_max, chunk, nb, lds, aStream); break;
-	default: launch_stream_depth<3>(a, integer, source_max, chunk, nb, lds, aStream); break;
+	if (!integer) {
+		if (st == IANE_F32) {
+			launch_stream_types<false, float, float>(a, chunk, nb, lds, aStream);
+		} else if (st == IANE_U8) {
+			if (dt == IANE_U8) {
+				launch_stream_types<false, uint8_t, uint8_t>(a, chunk, nb, lds, aStream);
+			} else {
+				launch_stream_types<false, uint8_t, float>(a, chunk, nb, lds, aStream);
+			}
+		} else if (dt == IANE_U16) {
+			launch_stream_types<false, uint16_t, uint16_t>(a, chunk, nb, lds, aStream);
+		} else {
+			launch_stream_types<false, uint16_t, float>(a, chunk, nb, lds, aStream);
+		}
+		return true;
+	}
+	const auto with_dst = [&](auto aSource) {
+		using S = decltype(aSource);
+		switch (dt) {
+		case IANE_U8: launch_stream_types<true, S, uint8_t>(a, chunk, nb, lds, aStream); break;
+		case IANE_U16: launch_stream_types<true, S, uint16_t>(a, chunk, nb, lds, aStream); break;
+		case IANE_I32: launch_stream_types<true, S, int32_t>(a, chunk, nb, lds, aStream); break;
+		case IANE_F32: launch_stream_types<true, S, float>(a, chunk, nb, lds, aStream); break;
+		default: launch_stream_types<true, S, double>(a, chunk, nb, lds, aStream); break;
+		}
+	};
+	if (st == IANE_U8) {
+		with_dst(uint8_t{});
+	} else {
+		with_dst(uint16_t{});
 	}
 	return true;
 }

@@ -9,7 +9,8 @@ otherwise. Every array here is a numpy array in C order (IANE axis 0 = last axis
 Workloads (named by what they are):
   * image_2d_uniform       1024^2 U[0,1), 7x7 Gaussian sigma 1.5 (clamp) / u8 + 5x5 box
   * volume_blobs           256^3 U[0,1) + 200 Gaussian blobs, 15^3 Gaussian sigma 2.5,
-                           dense 15^3 normal kernel normalised to sum 1, u16 + 3^3 binomial
+                           dense 15^3 normal kernel normalised to sum 1, u16 + 3^3 binomial,
+                           dense 3^3 normal kernel normalised to sum 1
   * rl_volume              256x256x128 truth (0.01 + 300 blobs) blurred by a 31x31x63
                            anisotropic Gaussian PSF, peak 1000 photons, Poisson noise
 """
@@ -184,8 +185,10 @@ def volume_blobs(n: int = 256, seed: int = 2, blobs: int = 200):
     vol_u16 = np.minimum(65535, np.floor(vol.astype(np.float64) * 8192)).astype(np.uint16)
     b = np.array([1, 2, 1], dtype=np.int32)
     binomial = np.multiply.outer(np.multiply.outer(b, b), b).astype(np.int32)
+    dense3 = normal(seed, 3, 27).reshape(3, 3, 3)
+    dense3 = (dense3 / float(dense3.sum())).astype(np.float32)
     return {"volume": vol, "taps": taps.astype(np.float32), "dense": dense, "dense_sum": dense_sum,
-            "volume_u16": vol_u16, "binomial": binomial}
+            "volume_u16": vol_u16, "binomial": binomial, "dense3": dense3}
 
 
 def rl_batch_data(count: int = 64, seed: int = 4, edge_range=(32, 192), sigma_range=(1.0, 4.0),

@@ -303,7 +303,8 @@ def test_full_size_separable_vs_sampled_oracle():
                                           ((3, 17, 65), (5, 5, 5)), ((70, 16, 64), (3, 7, 3))])
 @pytest.mark.parametrize("border", [cases.CLAMP, cases.ZERO])
 def test_streaming_kernel_equals_tiled(shape, kshape, border):
-    """The z-streaming direct kernels (small 3-D kernels) and the per-plane tiled kernel
+    """The z-streaming direct kernels (small 3-D kernels; unrolled or looped taps, chunk
+    lengths with and without a partial pipeline tail) and the per-plane tiled kernel
     give identical bits in every precision and dtype pair. Integer images with integer
     kernels accumulate in int32 when the sum provably fits, else in FP64: both cases run
     (the last pair's coefficients up to 2^21 break the int32 bound)."""
@@ -321,8 +322,8 @@ def test_streaming_kernel_equals_tiled(shape, kshape, border):
         else:
             src, ker = _case(rng, shape, kshape, sdt, kdt)
         outs = []
-        for config in ({"conv.stream": 0}, {"conv.stream_depth": 1}, {"conv.stream_depth": 2},
-                       {"conv.stream_depth": 3, "conv.stream_chunk": 5}):
+        for config in ({"conv.stream": 0}, {}, {"conv.stream_unroll": 0}, {"conv.stream_chunk": 5},
+                       {"conv.stream_chunk": 3}):
             with autotune.applied(config):
                 outs.append(gpu_convolve(src, ker, ddt, border, prec))
         for out in outs[1:]:
