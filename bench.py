@@ -238,6 +238,8 @@ def main() -> int:
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--quick", action="store_true", help="headline only")
+    ap.add_argument("--direct-only", action="store_true",
+                    help="headline plus the two streaming direct-convolution legs (u16 3^3, f32 dense 3^3)")
     ap.add_argument("--no-tune", action="store_true", help="default launch shapes (no autotuner launches)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also copy the 256^3 volume once (device to device, 64 MiB read + 64 MiB written): "
@@ -310,8 +312,10 @@ def main() -> int:
     if not args.quick:
         # dense random 15^3 kernel, reference (FP64, bit-exact) arithmetic: compute bound
         dense = torch.from_numpy(data["dense"]).cuda()
-        for name, prec in (("dense15_reference", precision.reference), ("dense15_fp32", precision.fp32),
-                           ("dense15_split_fp16", precision.split_fp16), ("dense15_split_bf16", precision.split_bf16)):
+        dense_legs = () if args.direct_only else (
+            ("dense15_reference", precision.reference), ("dense15_fp32", precision.fp32),
+            ("dense15_split_fp16", precision.split_fp16), ("dense15_split_bf16", precision.split_bf16))
+        for name, prec in dense_legs:
             o = convolution_options(border_mode.zero, prec)
             fn = lambda: iane_amd.convolve(vol, dense, out, o, policy)  # noqa: E731
             fn()
@@ -340,7 +344,9 @@ def main() -> int:
         t = timer.time(fn, k) / k
         extra["u16_binomial3"] = {"ms": round(t, 4), "Mvoxel_s": round(voxels / t / 1e3, 1),
                                   "GB_s": round(voxels * 6 / t / 1e6, 1),
-                                  "frac_hbm": round(voxels * 6 / t / 1e6 / HBM_PEAK_GBS, 3)}
+                                  "frac_hbm": round(voxels * 6 / t / 1e6 / HBM_PEAK_GBS, 3),
+                                  "kernel": "conv_stream_int_kernel<unsigned short, int, 3, 0>",
+                                  "traffic": traffic_from_profiles("conv_stream_u16_256")}
         log(f"[bench] u16 3^3: {t * 1e3:.1f} us")
         # f32 volume, dense random 3^3 kernel, FP32 arithmetic (z-streaming direct kernel)
         k3 = torch.from_numpy(data["dense3"]).cuda()
@@ -351,10 +357,12 @@ def main() -> int:
         t = timer.time(fn, k) / k
         extra["dense3_fp32"] = {"ms": round(t, 4), "Mvoxel_s": round(voxels / t / 1e3, 1),
                                 "GB_s": round(voxels * 8 / t / 1e6, 1),
-                                "frac_hbm": round(voxels * 8 / t / 1e6 / HBM_PEAK_GBS, 3)}
+                                "frac_hbm": round(voxels * 8 / t / 1e6 / HBM_PEAK_GBS, 3),
+                                "kernel": "conv_stream_kernel<float, float, 3, 3>",
+                                "traffic": traffic_from_profiles("conv_stream_dense3_256")}
         log(f"[bench] f32 dense 3^3: {t * 1e3:.1f} us")
 
-    if not args.quick:
+    if not args.quick and not args.direct_only:
         extra["sequence_256"] = bench_sequence(timer, stream, vol)
         extra["rl_fft_256x256x128_psf31x31x63"] = bench_rl(timer, stream)
         extra["rl_batch_64_volumes"] = bench_rl_batch(timer, stream)
