@@ -138,6 +138,33 @@ def bench_rl(timer, stream, iterations: int = 200):
             "launch_shape": tuned}
 
 
+def bench_rl_split(timer, stream, iterations: int = 2):
+    """RL with the DIRECT method on split-precision MFMA convolution (hi + lo FP16 / BF16
+    operands, FP32 accumulation) on the same 256x256x128 volume and 31x31x63 PSF: GPU time
+    per iteration, and the largest difference from the FFT method's estimate after the same
+    iterations (the FFT method is checked against the FP64 oracle in the tests)."""
+    import torch
+    from iane_amd import deconvolution, hip_policy, precision, synth
+    _, psf, y = synth.rl_volume_data()
+    dy, dpsf = torch.from_numpy(y).cuda(), torch.from_numpy(psf).cuda()
+    policy = hip_policy(stream)
+    fft_plan = deconvolution.rl_plan(y.shape, dpsf, deconvolution.FFT, policy=policy)
+    ref = torch.empty_like(dy)
+    fft_plan.run(dy, ref, iterations).wait()
+    scale = float(ref.abs().max())
+    out = {"iterations": iterations, "psf": "31x31x63", "compared_with": "FFT method, same iterations"}
+    for name, prec in (("split_fp16", precision.split_fp16), ("split_bf16", precision.split_bf16)):
+        plan = deconvolution.rl_plan(y.shape, dpsf, deconvolution.DIRECT, prec, policy=policy)
+        est = torch.empty_like(dy)
+        plan.run(dy, est, 1).wait()  # warm-up
+        ms = timer.time(lambda: plan.run(dy, est, iterations), 1)
+        _lib_sync(stream)
+        diff = float((est - ref).abs().max()) / scale
+        out[name] = {"ms_per_iteration": round(ms / iterations, 2), "max_rel_diff_vs_fft": float(f"{diff:.3g}")}
+        log(f"[bench] RL direct {name}: {ms / iterations:.1f} ms/iteration, max diff vs FFT {diff:.2e}")
+    return out
+
+
 def bench_rl_batch(timer, stream, iterations: int = 100):
     """64 RL problems (edges U{32..192}, own PSFs, seed 4), 100 iterations:
     one hipGraph launch of the batched engine vs the same volumes run one plan at a time."""
@@ -365,6 +392,7 @@ def main() -> int:
     if not args.quick and not args.direct_only:
         extra["sequence_256"] = bench_sequence(timer, stream, vol)
         extra["rl_fft_256x256x128_psf31x31x63"] = bench_rl(timer, stream)
+        extra["rl_direct_split_256x256x128_psf31x31x63"] = bench_rl_split(timer, stream)
         extra["rl_batch_64_volumes"] = bench_rl_batch(timer, stream)
         extra["rl_learned_vs_plain_256x256x128"] = bench_learned(stream)
 

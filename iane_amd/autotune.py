@@ -11,10 +11,17 @@ so tuning only trades occupancy, LDS use and tile overlap.
     sep3.threads    256 | 512              threads per workgroup (vector engine)
     sep3.tile_y     16 | 32                output rows per workgroup tile (vector engine)
 
-``sep3.engine`` (0: x/y filters on FP32 MFMA, 1: vector ALUs) is not tuned: the
-engines sum the same FP32 products in different orders, so it changes bits.
     fft.lds_elems   1024 | 2048 | 4096     LDS budget per buffer (complex elements) of the
                                            FFT line passes and RL row steps: lines per workgroup
+
+``sep3.engine`` (0: x/y filters on FP32 MFMA, 1: vector ALUs) is not tuned: the
+engines sum the same FP32 products in different orders, so it changes bits.
+Direct-convolution switches, read at launch and not searched (every setting
+gives the same bits; tests compare them):
+
+    conv.stream         1 | 0              z-streaming kernel for small 3-D kernels (0: tiled kernel)
+    conv.stream_chunk   0 (auto) | 1 .. 64 output planes per streaming workgroup
+    conv.stream_unroll  1 | 0              fully unrolled FP32 3^3 taps (0: row loop)
 
 Choices are cached per (device, workload) in a JSON file: ``$IANE_AMD_TUNE_CACHE``
 or ``~/.cache/iane_amd/tune.json``. A cached choice is applied without timing.

@@ -512,12 +512,13 @@ struct stream_coefficients {
 		}
 	}
 
+	// The coefficients are the same for every lane: read into scalar registers.
 	__device__ void load_registers()
 	{
 		if constexpr (KYZ > 0) {
 #pragma unroll
 			for (int i = 0; i < KX * KYZ * KYZ; ++i) {
-				reg[i] = lds[i];
+				reg[i] = __builtin_bit_cast(T, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, lds[i])));
 			}
 		}
 	}
@@ -655,8 +656,8 @@ void launch_stream_shape(const conv_args& a, int aChunk, unsigned aBlocks, size_
 	}
 }
 
-// FP32 3^3 kernels get fully unrolled tap loops with register coefficients
-// (unrolled, the integer kernel needs about 146 VGPRs: half the occupancy).
+// FP32 3^3 kernels get fully unrolled tap loops with scalar-register
+// coefficients (the integer kernel measured no faster unrolled: its row loop stays).
 template<bool INTEGER, class S, class D, int KX>
 void launch_stream_kx(const conv_args& a, int aChunk, unsigned aBlocks, size_t aLds, hipStream_t aStream)
 {
