@@ -138,6 +138,37 @@ def bench_rl(timer, stream, iterations: int = 200):
             "launch_shape": tuned}
 
 
+def bench_image_2d(timer, stream, repeats: int = 50):
+    """The 2-D workload: 1024x1024 U[0,1) image with the 7x7 Gaussian (sigma 1.5, clamp border)
+    in reference (FP64, bit-exact) and FP32 arithmetic, and its u8 quantisation with the
+    integer 5x5 box into i32 (bit-exact)."""
+    import torch
+    import iane_amd
+    from iane_amd import border_mode, convolution_options, hip_policy, precision, synth
+    img, kernel, img_u8, box = synth.image_2d_uniform()
+    policy = hip_policy(stream)
+    d_img, d_k = torch.from_numpy(img).cuda(), torch.from_numpy(kernel).cuda()
+    d_u8, d_box = torch.from_numpy(img_u8).cuda(), torch.from_numpy(box).cuda()
+    out = torch.empty_like(d_img)
+    out_i32 = torch.empty(img.shape, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    pixels = img.size
+    result = {}
+    for name, fn in (
+            ("f32_gauss7_reference", lambda: iane_amd.convolve(
+                d_img, d_k, out, convolution_options(border_mode.clamp, precision.reference), policy)),
+            ("f32_gauss7_fp32", lambda: iane_amd.convolve(
+                d_img, d_k, out, convolution_options(border_mode.clamp, precision.fp32), policy)),
+            ("u8_box5_i32", lambda: iane_amd.convolve(
+                d_u8, d_box, out_i32, convolution_options(border_mode.clamp, precision.reference), policy))):
+        fn()
+        _lib_sync(stream)
+        ms = timer.time(fn, repeats) / repeats
+        result[name] = {"us": round(ms * 1e3, 2), "Mpixel_s": round(pixels / ms / 1e3, 1)}
+        log(f"[bench] 2-D {name}: {ms * 1e3:.1f} us")
+    return result
+
+
 def bench_rl_split(timer, stream, iterations: int = 2):
     """RL with the DIRECT method on split-precision MFMA convolution (hi + lo FP16 / BF16
     operands, FP32 accumulation) on the same 256x256x128 volume and 31x31x63 PSF: GPU time
@@ -390,6 +421,7 @@ def main() -> int:
         log(f"[bench] f32 dense 3^3: {t * 1e3:.1f} us")
 
     if not args.quick and not args.direct_only:
+        extra["image_2d_1024"] = bench_image_2d(timer, stream)
         extra["sequence_256"] = bench_sequence(timer, stream, vol)
         extra["rl_fft_256x256x128_psf31x31x63"] = bench_rl(timer, stream)
         extra["rl_direct_split_256x256x128_psf31x31x63"] = bench_rl_split(timer, stream)

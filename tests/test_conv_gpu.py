@@ -274,6 +274,22 @@ def test_full_size_delta_shift_and_integer_linearity():
     np.testing.assert_array_equal(outs[0] + outs[1], outs[2])
 
 
+def test_full_size_workloads_bit_exact_vs_oracle():
+    """The benchmark workloads that run on the streaming and tiled direct kernels, at full
+    size, against the C oracle: the 1024x1024 image with the 7x7 Gaussian (clamp,
+    reference precision) and its u8 quantisation with the 5x5 box into i32, and the
+    u16 256^3 volume with the 3^3 binomial kernel into i32 (zero border)."""
+    from iane_amd import synth
+    img, gauss7, img_u8, box5 = synth.image_2d_uniform()
+    got = gpu_convolve(img, gauss7, np.float32, cases.CLAMP)
+    same_bits(got, oracle.convolve(img, gauss7, np.float32, oracle.CLAMP, threads=8))
+    got = gpu_convolve(img_u8, box5, np.int32, cases.CLAMP)
+    same_bits(got, oracle.convolve(img_u8, box5, np.int32, oracle.CLAMP, threads=8))
+    data = synth.volume_blobs(256, seed=2)
+    got = gpu_convolve(data["volume_u16"], data["binomial"], np.int32, cases.ZERO)
+    same_bits(got, oracle.convolve(data["volume_u16"], data["binomial"], np.int32, oracle.ZERO, threads=8))
+
+
 def test_full_size_separable_vs_sampled_oracle():
     """Separable 15^3 Gaussian at 256^3 against FP64 brute force at sampled voxels (edges included)."""
     from iane_amd import synth
