@@ -333,25 +333,71 @@ __device__ __forceinline__ void decode_group(const fft_pass_args& a, unsigned aB
 }
 
 // Load complex points k < load_n (zero up to `width`) of `count` lines into buf.
+// Element loop over t = threadIdx.x + i * blockDim.x < total with kLineBatch
+// elements per thread in flight: aFetch(t) (a global or LDS load, from an
+// in-range address) runs for the whole batch before aUse(t, value) consumes
+// any of it. Nothing branches between a load and its use, so the compiler's
+// wait before each use covers only that element's load (a load followed at
+// once by its use leaves one load per thread in flight).
+constexpr int kLineBatch = 8;
+
+template<class Fetch, class Use>
+__device__ __forceinline__ void batched_loop(int total, Fetch aFetch, Use aUse)
+{
+	const int stride = blockDim.x;
+	for (int t0 = threadIdx.x; t0 < total; t0 += kLineBatch * stride) {
+		using value_t = decltype(aFetch(0));
+		value_t v[kLineBatch];
+#pragma unroll
+		for (int u = 0; u < kLineBatch; ++u) {
+			const int t = t0 + u * stride;
+			v[u] = aFetch(t < total ? t : t0);
+		}
+#pragma unroll
+		for (int u = 0; u < kLineBatch; ++u) {
+			const int t = t0 + u * stride;
+			if (t < total) {
+				aUse(t, v[u]);
+			}
+		}
+	}
+}
+
+// Points k >= load_n of a line are zero: they read the line's point
+// min(k, load_n - 1) (in range) and store 0.
 __device__ inline void load_complex_lines(const float2* in, const line_geom& g, int aOuter, int aLine0, int count,
 	int load_n, int width, float2* buf, int pitch)
 {
 	const long long base = aOuter * g.outer + aLine0 * g.line;
 	const int total = count * width;
+	const int last = load_n > 0 ? load_n - 1 : 0;
+	const float2 zero = make_float2(0.0f, 0.0f);
 	if (g.elem == 1) {
 		const fast_div by_width(width);
-		for (int t = threadIdx.x; t < total; t += blockDim.x) {
-			const int w = by_width.div(t);
-			const int k = t - w * width;
-			buf[w * pitch + k] = k < load_n ? in[base + w * g.line + k] : make_float2(0.0f, 0.0f);
-		}
+		batched_loop(total,
+			[&](int t) {
+				const int w = by_width.div(t);
+				const int k = t - w * width;
+				return in[base + w * g.line + min(k, last)];
+			},
+			[&](int t, float2 v) {
+				const int w = by_width.div(t);
+				const int k = t - w * width;
+				buf[w * pitch + k] = k < load_n ? v : zero;
+			});
 	} else {
 		const fast_div by_count(count);
-		for (int t = threadIdx.x; t < total; t += blockDim.x) {
-			const int k = by_count.div(t);
-			const int w = t - k * count;
-			buf[w * pitch + k] = k < load_n ? in[base + w * g.line + k * g.elem] : make_float2(0.0f, 0.0f);
-		}
+		batched_loop(total,
+			[&](int t) {
+				const int k = by_count.div(t);
+				const int w = t - k * count;
+				return in[base + w * g.line + min(k, last) * g.elem];
+			},
+			[&](int t, float2 v) {
+				const int k = by_count.div(t);
+				const int w = t - k * count;
+				buf[w * pitch + k] = k < load_n ? v : zero;
+			});
 	}
 }
 
@@ -362,18 +408,26 @@ __device__ inline void store_complex_lines(float2* out, const line_geom& g, int 
 	const int total = count * store_n;
 	if (g.elem == 1) {
 		const fast_div by_store(store_n);
-		for (int t = threadIdx.x; t < total; t += blockDim.x) {
-			const int w = by_store.div(t);
-			const int k = t - w * store_n;
-			out[base + w * g.line + k] = buf[w * pitch + k];
-		}
+		batched_loop(total,
+			[&](int t) {
+				const int w = by_store.div(t);
+				return buf[w * pitch + (t - w * store_n)];
+			},
+			[&](int t, float2 v) {
+				const int w = by_store.div(t);
+				out[base + w * g.line + (t - w * store_n)] = v;
+			});
 	} else {
 		const fast_div by_count(count);
-		for (int t = threadIdx.x; t < total; t += blockDim.x) {
-			const int k = by_count.div(t);
-			const int w = t - k * count;
-			out[base + w * g.line + k * g.elem] = buf[w * pitch + k];
-		}
+		batched_loop(total,
+			[&](int t) {
+				const int k = by_count.div(t);
+				return buf[(t - k * count) * pitch + k];
+			},
+			[&](int t, float2 v) {
+				const int k = by_count.div(t);
+				out[base + (t - k * count) * g.line + k * g.elem] = v;
+			});
 	}
 }
 
@@ -386,28 +440,30 @@ __device__ inline void load_real_lines(const float* in, const line_geom& g, int 
 	const bool even = (n & 1) == 0;
 	const int width = even ? n / 2 : n;
 	const int total = count * width;
+	const int last = load_n > 0 ? load_n - 1 : 0;
 	const fast_div by_width(width), by_count(count);
-	for (int t = threadIdx.x; t < total; t += blockDim.x) {
-		int w, j;
+	// (line w, position j) of element t
+	const auto place = [&](int t) {
 		if (g.elem == 1) {
-			w = by_width.div(t);
-			j = t - w * width;
-		} else {
-			j = by_count.div(t);
-			w = t - j * count;
+			const int w = by_width.div(t);
+			return make_int2(w, t - w * width);
 		}
-		const float* line = in + base + w * g.line;
-		float2 v;
-		if (even) {
-			const int i0 = 2 * j, i1 = 2 * j + 1;
-			v.x = i0 < load_n ? line[i0 * g.elem] : 0.0f;
-			v.y = i1 < load_n ? line[i1 * g.elem] : 0.0f;
-		} else {
-			v.x = j < load_n ? line[j * g.elem] : 0.0f;
-			v.y = 0.0f;
-		}
-		buf[w * pitch + j] = v;
-	}
+		const int j = by_count.div(t);
+		return make_int2(t - j * count, j);
+	};
+	// points >= load_n read point load_n - 1 and store 0
+	batched_loop(total,
+		[&](int t) {
+			const int2 wj = place(t);
+			const float* line = in + base + wj.x * g.line;
+			const int i0 = even ? 2 * wj.y : wj.y;
+			return make_float2(line[min(i0, last) * g.elem], even ? line[min(i0 + 1, last) * g.elem] : 0.0f);
+		},
+		[&](int t, float2 v) {
+			const int2 wj = place(t);
+			const int i0 = even ? 2 * wj.y : wj.y;
+			buf[wj.x * pitch + wj.y] = make_float2(i0 < load_n ? v.x : 0.0f, even && i0 + 1 < load_n ? v.y : 0.0f);
+		});
 }
 
 // even n, after the m = n/2 forward FFT of the packed line in `z`:
@@ -469,18 +525,25 @@ __device__ inline void store_real_lines(float* out, const line_geom& g, int aOut
 	const bool even = (n & 1) == 0;
 	const int total = count * store_n;
 	const fast_div by_store(store_n), by_count(count);
-	for (int t = threadIdx.x; t < total; t += blockDim.x) {
-		int w, i;
+	// (line w, point i) of element t
+	const auto place = [&](int t) {
 		if (g.elem == 1) {
-			w = by_store.div(t);
-			i = t - w * store_n;
-		} else {
-			i = by_count.div(t);
-			w = t - i * count;
+			const int w = by_store.div(t);
+			return make_int2(w, t - w * store_n);
 		}
-		const float2 v = buf[w * pitch + (even ? i / 2 : i)];
-		out[base + w * g.line + i * g.elem] = (even && (i & 1)) ? v.y : v.x;
-	}
+		const int i = by_count.div(t);
+		return make_int2(t - i * count, i);
+	};
+	batched_loop(total,
+		[&](int t) {
+			const int2 wi = place(t);
+			const float2 v = buf[wi.x * pitch + (even ? wi.y / 2 : wi.y)];
+			return (even && (wi.y & 1)) ? v.y : v.x;
+		},
+		[&](int t, float v) {
+			const int2 wi = place(t);
+			out[base + wi.x * g.line + wi.y * g.elem] = v;
+		});
 }
 
 // Dynamic LDS of a launch running pass `a`: two buffers of a.lds_elems.
@@ -510,21 +573,27 @@ __device__ __forceinline__ void fft_pass_body(const fft_pass_args& a, unsigned a
 		float2* r = run_stages(a.plan, A, B, count, a.pitch, tw_m, -1.0f);
 		const long long hbase = o * a.gmult.outer + l0 * a.gmult.line;
 		const fast_div by_m(m), by_count(count);
-		for (int t = threadIdx.x; t < count * m; t += blockDim.x) {
-			int w, k;
+		// (line w, bin k) of element t
+		const auto place = [&](int t) {
 			if (a.gmult.elem == 1) {
-				w = by_m.div(t);
-				k = t - w * m;
-			} else {
-				k = by_count.div(t);
-				w = t - k * count;
+				const int w = by_m.div(t);
+				return make_int2(w, t - w * m);
 			}
-			float2 h = a.mult[hbase + w * a.gmult.line + k * a.gmult.elem];
-			if (a.mult_conj) {
-				h = cconj(h);
-			}
-			r[w * a.pitch + k] = cscale(cmul(r[w * a.pitch + k], h), a.mult_scale);
-		}
+			const int k = by_count.div(t);
+			return make_int2(t - k * count, k);
+		};
+		batched_loop(count * m,
+			[&](int t) {
+				const int2 wk = place(t);
+				return a.mult[hbase + wk.x * a.gmult.line + wk.y * a.gmult.elem];
+			},
+			[&](int t, float2 h) {
+				const int2 wk = place(t);
+				if (a.mult_conj) {
+					h = cconj(h);
+				}
+				r[wk.x * a.pitch + wk.y] = cscale(cmul(r[wk.x * a.pitch + wk.y], h), a.mult_scale);
+			});
 		__syncthreads();
 		const float2* q = run_stages(a.plan, r, r == A ? B : A, count, a.pitch, tw_m, 1.0f);
 		store_complex_lines(static_cast<float2*>(a.out), a.gout, o, l0, count, a.store_n, q, a.pitch);

@@ -196,34 +196,52 @@ __device__ __forceinline__ void rl_row_body(const rl_row_args& a, unsigned aBloc
 	const long long base = o * a.real_outer + l0 * a.real_line;
 	const bool want_ll = EPI == EPI_RATIO && a.partials != nullptr; // uniform
 	const fast_div by_m(m);
-	for (int t = threadIdx.x; t < count * m; t += blockDim.x) {
-		const int w = by_m.div(t);
-		const int j = t - w * m;
-		const float2 v = r[w * f.pitch + j];
-		float out[2] = {0.0f, 0.0f};
-		const float in[2] = {v.x, v.y};
+	// the image samples of a row pair are loaded for a whole batch first
+	// (samples past nx read sample nx - 1 and are not used)
+	using samples_t = std::conditional_t<EPI == EPI_RATIO, float2, float4>;
+	batched_loop(count * m,
+		[&](int t) {
+			const int w = by_m.div(t);
+			const int j = t - w * m;
+			const long long row = base + w * a.real_line;
+			const long long i0 = row + min(2 * j, a.nx - 1);
+			const long long i1 = row + min(2 * j + 1, a.nx - 1);
+			if constexpr (EPI == EPI_RATIO) {
+				return make_float2(a.y[i0], a.y[i1]);
+			} else {
+				return make_float4(a.x[i0], a.x[i1], a.s[i0], a.s[i1]);
+			}
+		},
+		[&](int t, samples_t aSamples) {
+			const int w = by_m.div(t);
+			const int j = t - w * m;
+			const float2 v = r[w * f.pitch + j];
+			float out[2] = {0.0f, 0.0f};
+			const float in[2] = {v.x, v.y};
 #pragma unroll
-		for (int h = 0; h < 2; ++h) {
-			const int i = 2 * j + h;
-			if (i < a.nx) {
-				const long long idx = base + w * a.real_line + i;
-				if constexpr (EPI == EPI_RATIO) {
-					const float obs = a.y[idx];
-					const float pred = in[h];
-					const float floor = pred > a.eps ? pred : a.eps;
-					out[h] = obs / floor;
-					if (want_ll) {
-						ll += static_cast<double>(obs) * log(static_cast<double>(floor)) - static_cast<double>(pred);
+			for (int h = 0; h < 2; ++h) {
+				const int i = 2 * j + h;
+				if (i < a.nx) {
+					const long long idx = base + w * a.real_line + i;
+					if constexpr (EPI == EPI_RATIO) {
+						const float obs = h == 0 ? aSamples.x : aSamples.y;
+						const float pred = in[h];
+						const float floor = pred > a.eps ? pred : a.eps;
+						out[h] = obs / floor;
+						if (want_ll) {
+							ll += static_cast<double>(obs) * log(static_cast<double>(floor)) - static_cast<double>(pred);
+						}
+					} else {
+						const float x = h == 0 ? aSamples.x : aSamples.y;
+						const float sens = h == 0 ? aSamples.z : aSamples.w;
+						const float xn = x * edge_factor(in[h], sens, a.eps);
+						a.x[idx] = xn;
+						out[h] = xn;
 					}
-				} else {
-					const float xn = a.x[idx] * edge_factor(in[h], a.s[idx], a.eps);
-					a.x[idx] = xn;
-					out[h] = xn;
 				}
 			}
-		}
-		r[w * f.pitch + j] = make_float2(out[0], out[1]);
-	}
+			r[w * f.pitch + j] = make_float2(out[0], out[1]);
+		});
 	if constexpr (EPI == EPI_RATIO) {
 		if (a.partials) {
 			const double total = block_sum(ll, shared);
