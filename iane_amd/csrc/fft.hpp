@@ -565,7 +565,9 @@ __device__ __forceinline__ void fft_pass_body(const fft_pass_args& a, unsigned a
 	if constexpr (MODE == PASS_C2C) {
 		load_complex_lines(static_cast<const float2*>(a.in), a.gin, o, l0, count, a.load_n, m, A, a.pitch);
 		__syncthreads();
-		const float2* r = run_stages(a.plan, A, B, count, a.pitch, tw_m, a.sign);
+		// a constant sign in each copy folds the conjugations and (sign i) rotations
+		const float2* r = a.sign < 0.0f ? run_stages(a.plan, A, B, count, a.pitch, tw_m, -1.0f)
+		                                : run_stages(a.plan, A, B, count, a.pitch, tw_m, 1.0f);
 		store_complex_lines(static_cast<float2*>(a.out), a.gout, o, l0, count, a.store_n, r, a.pitch);
 	} else if constexpr (MODE == PASS_C2C_MULT) {
 		load_complex_lines(static_cast<const float2*>(a.in), a.gin, o, l0, count, a.load_n, m, A, a.pitch);
