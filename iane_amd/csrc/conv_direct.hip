@@ -317,6 +317,22 @@ __device__ __forceinline__ D convert_out(Acc aSum)
 	}
 }
 
+// convert_out<D>(double(aSum)) for an exact integer |aSum| < 2^31: the double
+// is exact, so rounding and truncation do nothing and only the clamps remain
+// (u8 rounds half up after clamping, u16 and i32 saturate; float rounds the
+// integer once either way).
+template<class D>
+__device__ __forceinline__ D convert_exact_int(int aSum)
+{
+	if constexpr (std::is_same_v<D, uint8_t>) {
+		return static_cast<uint8_t>(min(max(aSum, 0), 255));
+	} else if constexpr (std::is_same_v<D, uint16_t>) {
+		return static_cast<uint16_t>(min(max(aSum, 0), 65535));
+	} else {
+		return static_cast<D>(aSum);
+	}
+}
+
 // Where a streaming workgroup works and which samples each of its threads
 // stages: a 64 x 16 output tile of planes [z0, z1), fed by input planes
 // [z0 - cz, z1 + cz) that pass through an LDS ring of kz + 2 planes. A thread
@@ -422,7 +438,11 @@ struct stream_tile {
 		const auto plane = stream_resource(a.dst.ptr + static_cast<long long>(aZ) * a.dst.stride[2], a.dst_span);
 #pragma unroll
 		for (int r = 0; r < kRx; ++r) {
-			buffer_store_as(plane, out_offset[r], convert_out<D>(aSums[r]));
+			if constexpr (std::is_same_v<Acc, int>) {
+				buffer_store_as(plane, out_offset[r], convert_exact_int<D>(aSums[r]));
+			} else {
+				buffer_store_as(plane, out_offset[r], convert_out<D>(aSums[r]));
+			}
 		}
 	}
 
@@ -623,25 +643,18 @@ __global__ __launch_bounds__(kThreads) void conv_stream_int_kernel(conv_args a, 
 	}
 	// exact_int is read after the barrier that precedes the first output plane
 	const auto compute = [&](int z, int aTop) {
-		double sums[kRx];
 		if (exact_int != 0) { // uniform over the workgroup
 			int acc[kRx] = {0, 0, 0, 0};
 			stream_taps(t, ring, coef, ky, kz, aTop, acc,
 				[](int& aSum, int aSample, int aK) { aSum += __mul24(aSample, aK); });
-#pragma unroll
-			for (int rx = 0; rx < kRx; ++rx) {
-				sums[rx] = static_cast<double>(acc[rx]);
-			}
+			t.write<D>(a, z, acc);
 		} else {
-#pragma unroll
-			for (int rx = 0; rx < kRx; ++rx) {
-				sums[rx] = 0.0;
-			}
+			double sums[kRx] = {0.0, 0.0, 0.0, 0.0};
 			stream_taps<KX, KYZ, false>(t, ring, coef, ky, kz, aTop, sums, [](double& aSum, int aSample, int aK) {
 				aSum = fma(static_cast<double>(aSample), static_cast<double>(aK), aSum);
 			});
+			t.write<D>(a, z, sums);
 		}
-		t.write<D>(a, z, sums);
 	};
 	t.run<kStreamDepth, S>(a, ring, spare, compute);
 }
