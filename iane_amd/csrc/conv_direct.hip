@@ -669,13 +669,12 @@ void launch_stream_shape(const conv_args& a, int aChunk, unsigned aBlocks, size_
 	}
 }
 
-// FP32 3^3 kernels get fully unrolled tap loops with scalar-register
-// coefficients (the integer kernel measured no faster unrolled: its row loop stays).
+// 3^3 kernels get fully unrolled tap loops with scalar-register coefficients.
 template<bool INTEGER, class S, class D, int KX>
 void launch_stream_kx(const conv_args& a, int aChunk, unsigned aBlocks, size_t aLds, hipStream_t aStream)
 {
 	const bool cubic = a.ker.extent[1] == KX && a.ker.extent[2] == KX && tune_get("conv.stream_unroll", 1) != 0;
-	if constexpr (KX == 3 && !INTEGER) {
+	if constexpr (KX == 3) {
 		if (cubic) {
 			launch_stream_shape<INTEGER, S, D, KX, KX>(a, aChunk, aBlocks, aLds, aStream);
 			return;
