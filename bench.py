@@ -403,7 +403,7 @@ def main() -> int:
         extra["u16_binomial3"] = {"ms": round(t, 4), "Mvoxel_s": round(voxels / t / 1e3, 1),
                                   "GB_s": round(voxels * 6 / t / 1e6, 1),
                                   "frac_hbm": round(voxels * 6 / t / 1e6 / HBM_PEAK_GBS, 3),
-                                  "kernel": "conv_stream_int_kernel<unsigned short, int, 3, 0>",
+                                  "kernel": "conv_stream_int_kernel<unsigned short, int, 3, 3>",
                                   "traffic": traffic_from_profiles("conv_stream_u16_256")}
         log(f"[bench] u16 3^3: {t * 1e3:.1f} us")
         # f32 volume, dense random 3^3 kernel, FP32 arithmetic (z-streaming direct kernel)
