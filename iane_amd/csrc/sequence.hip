@@ -250,20 +250,20 @@ __device__ __forceinline__ Acc shfl_xor(Acc aValue, int aMask)
 // neutral element where that index is >= aCount. Every load is issued before
 // any is used: indices past the end are clamped to the last element (a valid
 // address) and replaced afterwards, so there is no branch between the loads.
-template<class Acc, int OP>
+template<class Acc, int OP, int PER>
 __device__ __forceinline__ void load_line(const seq_map& m, long long aBase, long long aBegin, long long aCount,
-	long long aFirst, long long aStep, int aPer, Acc (&aOut)[kSeqPer])
+	long long aFirst, long long aStep, int aPer, Acc (&aOut)[PER])
 {
 	with_element_type(m.src_type, [&](auto aType) {
 		using T = decltype(aType);
-		T raw[kSeqPer];
+		T raw[PER];
 #pragma unroll
-		for (int e = 0; e < kSeqPer; ++e) {
+		for (int e = 0; e < PER; ++e) {
 			const long long j = min(aFirst + e * aStep, aCount - 1);
 			raw[e] = *reinterpret_cast<const T*>(m.src + src_offset(m, aBase, aBegin + j));
 		}
 #pragma unroll
-		for (int e = 0; e < kSeqPer; ++e) {
+		for (int e = 0; e < PER; ++e) {
 			const bool valid = e < aPer && aFirst + e * aStep < aCount;
 			aOut[e] = valid ? static_cast<Acc>(raw[e]) : identity<Acc, OP>();
 		}
@@ -271,9 +271,10 @@ __device__ __forceinline__ void load_line(const seq_map& m, long long aBase, lon
 }
 
 // One wave per line (lines of at most a few thousand elements): lanes stride
-// through the line in batches, then a butterfly over the wave; the result
+// through the line in batches of PER elements each (PER sized to the line, so
+// no lane issues loads past it), then a butterfly over the wave; the result
 // init (+) total goes to the destination.
-template<class Acc, int OP>
+template<class Acc, int OP, int PER>
 __global__ __launch_bounds__(kSeqThreads) void row_reduce_wave_kernel(seq_map m, Acc aInit)
 {
 	const int lane = threadIdx.x & 63;
@@ -283,11 +284,11 @@ __global__ __launch_bounds__(kSeqThreads) void row_reduce_wave_kernel(seq_map m,
 	}
 	const long long base = line_base(m.src_other_stride, m, line);
 	Acc acc = identity<Acc, OP>();
-	for (long long start = 0; start < m.length; start += 64 * kSeqPer) {
-		Acc v[kSeqPer];
-		load_line<Acc, OP>(m, base, start, m.length - start, lane, 64, kSeqPer, v);
+	for (long long start = 0; start < m.length; start += 64 * PER) {
+		Acc v[PER];
+		load_line<Acc, OP, PER>(m, base, start, m.length - start, lane, 64, PER, v);
 #pragma unroll
-		for (int e = 0; e < kSeqPer; ++e) {
+		for (int e = 0; e < PER; ++e) {
 			acc = apply<Acc, OP>(acc, v[e]);
 		}
 	}
@@ -317,7 +318,7 @@ __global__ __launch_bounds__(kSeqThreads) void row_totals_kernel(seq_map m, Acc*
 	const long long n = min(tile, m.length - begin);
 	const long long base = line_base(m.src_other_stride, m, line);
 	Acc v[kSeqPer];
-	load_line<Acc, OP>(m, base, begin, n, threadIdx.x, kSeqThreads, aPer, v);
+	load_line<Acc, OP, kSeqPer>(m, base, begin, n, threadIdx.x, kSeqThreads, aPer, v);
 	Acc acc = identity<Acc, OP>();
 #pragma unroll
 	for (int e = 0; e < kSeqPer; ++e) {
@@ -390,7 +391,7 @@ __global__ __launch_bounds__(kSeqThreads) void row_output_kernel(seq_map m, cons
 	const long long dst_base = line_base(m.dst_other_stride, m, line);
 	{
 		Acc v[kSeqPer];
-		load_line<Acc, OP>(m, src_base, begin, n, threadIdx.x, kSeqThreads, aPer, v);
+		load_line<Acc, OP, kSeqPer>(m, src_base, begin, n, threadIdx.x, kSeqThreads, aPer, v);
 #pragma unroll
 		for (int e = 0; e < kSeqPer; ++e) {
 			const int j = threadIdx.x + e * kSeqThreads;
@@ -557,7 +558,14 @@ void run_kind(const iane_sequence_plan& p, const seq_map& m, void* aPartials, hi
 		const long long tasks = m.lines * p.tiles;
 		if constexpr (KIND == IANE_SEQ_REDUCE) {
 			if (p.wave_reduce) {
-				row_reduce_wave_kernel<Acc, OP><<<grid_of((m.lines + 3) / 4), kSeqThreads, 0, s>>>(m, init);
+				const unsigned grid = grid_of((m.lines + 3) / 4);
+				if (m.length <= 64 * 4) {
+					row_reduce_wave_kernel<Acc, OP, 4><<<grid, kSeqThreads, 0, s>>>(m, init);
+				} else if (m.length <= 64 * 8) {
+					row_reduce_wave_kernel<Acc, OP, 8><<<grid, kSeqThreads, 0, s>>>(m, init);
+				} else {
+					row_reduce_wave_kernel<Acc, OP, kSeqPer><<<grid, kSeqThreads, 0, s>>>(m, init);
+				}
 				return;
 			}
 			row_totals_kernel<Acc, OP><<<grid_of(tasks), kSeqThreads, 0, s>>>(m, partials, p.tiles, p.per, 0);
