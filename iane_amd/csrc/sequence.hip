@@ -301,8 +301,9 @@ __global__ __launch_bounds__(kSeqThreads) void row_reduce_wave_kernel(seq_map m,
 	}
 }
 
-// Totals of tiles of 256 * aPer elements (all tiles for a reduction; all but the last for a scan).
-template<class Acc, int OP>
+// Totals of tiles of 256 * aPer elements (all tiles for a reduction; all but the last for a scan);
+// PER >= aPer is the number of loads a thread issues.
+template<class Acc, int OP, int PER>
 __global__ __launch_bounds__(kSeqThreads) void row_totals_kernel(seq_map m, Acc* aPartials, long long aTiles, int aPer,
 	int aSkipLast)
 {
@@ -317,11 +318,11 @@ __global__ __launch_bounds__(kSeqThreads) void row_totals_kernel(seq_map m, Acc*
 	const long long begin = t * tile;
 	const long long n = min(tile, m.length - begin);
 	const long long base = line_base(m.src_other_stride, m, line);
-	Acc v[kSeqPer];
-	load_line<Acc, OP, kSeqPer>(m, base, begin, n, threadIdx.x, kSeqThreads, aPer, v);
+	Acc v[PER];
+	load_line<Acc, OP, PER>(m, base, begin, n, threadIdx.x, kSeqThreads, aPer, v);
 	Acc acc = identity<Acc, OP>();
 #pragma unroll
-	for (int e = 0; e < kSeqPer; ++e) {
+	for (int e = 0; e < PER; ++e) {
 		acc = apply<Acc, OP>(acc, v[e]);
 	}
 	Acc total;
@@ -374,8 +375,9 @@ __global__ __launch_bounds__(kSeqThreads) void row_carries_kernel(seq_map m, Acc
 	}
 }
 
-// Output of one tile: rescan from the tile's carry (init when the line is one tile).
-template<class Acc, int OP, int KIND>
+// Output of one tile: rescan from the tile's carry (init when the line is one tile);
+// PER >= aPer elements per thread.
+template<class Acc, int OP, int KIND, int PER>
 __global__ __launch_bounds__(kSeqThreads) void row_output_kernel(seq_map m, const Acc* aCarries, long long aTiles,
 	int aPer, Acc aInit)
 {
@@ -390,10 +392,10 @@ __global__ __launch_bounds__(kSeqThreads) void row_output_kernel(seq_map m, cons
 	const long long src_base = line_base(m.src_other_stride, m, line);
 	const long long dst_base = line_base(m.dst_other_stride, m, line);
 	{
-		Acc v[kSeqPer];
-		load_line<Acc, OP, kSeqPer>(m, src_base, begin, n, threadIdx.x, kSeqThreads, aPer, v);
+		Acc v[PER];
+		load_line<Acc, OP, PER>(m, src_base, begin, n, threadIdx.x, kSeqThreads, aPer, v);
 #pragma unroll
-		for (int e = 0; e < kSeqPer; ++e) {
+		for (int e = 0; e < PER; ++e) {
 			const int j = threadIdx.x + e * kSeqThreads;
 			if (e < aPer && j < n) {
 				stage[j] = v[e];
@@ -401,10 +403,10 @@ __global__ __launch_bounds__(kSeqThreads) void row_output_kernel(seq_map m, cons
 		}
 	}
 	__syncthreads();
-	Acc local[kSeqPer];
+	Acc local[PER];
 	Acc run = identity<Acc, OP>();
 #pragma unroll
-	for (int e = 0; e < kSeqPer; ++e) {
+	for (int e = 0; e < PER; ++e) {
 		const int index = aPer * threadIdx.x + e;
 		const Acc x = (e < aPer && index < n) ? stage[index] : identity<Acc, OP>();
 		local[e] = run;             // exclusive within the thread
@@ -418,7 +420,7 @@ __global__ __launch_bounds__(kSeqThreads) void row_output_kernel(seq_map m, cons
 	const Acc carry = aTiles > 1 ? aCarries[task] : aInit;
 	const Acc before = apply<Acc, OP>(carry, prefix);
 #pragma unroll
-	for (int e = 0; e < kSeqPer; ++e) {
+	for (int e = 0; e < PER; ++e) {
 		const int index = aPer * threadIdx.x + e;
 		if (e < aPer && index < n) {
 			stage[index] = apply<Acc, OP>(before, local[e]);
@@ -426,7 +428,7 @@ __global__ __launch_bounds__(kSeqThreads) void row_output_kernel(seq_map m, cons
 	}
 	__syncthreads();
 #pragma unroll
-	for (int e = 0; e < kSeqPer; ++e) {
+	for (int e = 0; e < PER; ++e) {
 		const int j = threadIdx.x + e * kSeqThreads;
 		if (e < aPer && j < n) {
 			store_value(m.dst + dst_offset(m, dst_base, begin + j), stage[j]);
@@ -539,6 +541,20 @@ unsigned grid_of(long long aBlocks)
 	return static_cast<unsigned>(aBlocks);
 }
 
+// Tile totals with as many loads per thread as the tile needs (4 or 16).
+template<class Acc, int OP>
+void launch_row_totals(const iane_sequence_plan& p, const seq_map& m, Acc* aPartials, long long aTasks, int aSkipLast,
+	hipStream_t aStream)
+{
+	if (p.per <= 4) {
+		row_totals_kernel<Acc, OP, 4><<<grid_of(aTasks), kSeqThreads, 0, aStream>>>(m, aPartials, p.tiles, p.per,
+			aSkipLast);
+	} else {
+		row_totals_kernel<Acc, OP, kSeqPer><<<grid_of(aTasks), kSeqThreads, 0, aStream>>>(m, aPartials, p.tiles, p.per,
+			aSkipLast);
+	}
+}
+
 template<class Acc, int OP, int KIND>
 void run_kind(const iane_sequence_plan& p, const seq_map& m, void* aPartials, hipStream_t s)
 {
@@ -568,14 +584,19 @@ void run_kind(const iane_sequence_plan& p, const seq_map& m, void* aPartials, hi
 				}
 				return;
 			}
-			row_totals_kernel<Acc, OP><<<grid_of(tasks), kSeqThreads, 0, s>>>(m, partials, p.tiles, p.per, 0);
+			launch_row_totals<Acc, OP>(p, m, partials, tasks, 0, s);
 			row_carries_kernel<Acc, OP, KIND><<<grid_of(m.lines), kSeqThreads, 0, s>>>(m, partials, p.tiles, init);
 		} else {
 			if (p.tiles > 1) {
-				row_totals_kernel<Acc, OP><<<grid_of(tasks), kSeqThreads, 0, s>>>(m, partials, p.tiles, p.per, 1);
+				launch_row_totals<Acc, OP>(p, m, partials, tasks, 1, s);
 				row_carries_kernel<Acc, OP, KIND><<<grid_of(m.lines), kSeqThreads, 0, s>>>(m, partials, p.tiles, init);
 			}
-			row_output_kernel<Acc, OP, KIND><<<grid_of(tasks), kSeqThreads, 0, s>>>(m, partials, p.tiles, p.per, init);
+			if (p.per <= 4) {
+				row_output_kernel<Acc, OP, KIND, 4><<<grid_of(tasks), kSeqThreads, 0, s>>>(m, partials, p.tiles, p.per, init);
+			} else {
+				row_output_kernel<Acc, OP, KIND, kSeqPer><<<grid_of(tasks), kSeqThreads, 0, s>>>(m, partials, p.tiles, p.per,
+					init);
+			}
 		}
 		return;
 	}
