@@ -372,32 +372,31 @@ __device__ inline void load_complex_lines(const float2* in, const line_geom& g, 
 	const int total = count * width;
 	const int last = load_n > 0 ? load_n - 1 : 0;
 	const float2 zero = make_float2(0.0f, 0.0f);
+	// the fetch also returns the LDS slot (negative: a point past load_n, stored as 0)
+	struct fetched {
+		float2 v;
+		int slot;
+	};
 	if (g.elem == 1) {
 		const fast_div by_width(width);
 		batched_loop(total,
 			[&](int t) {
 				const int w = by_width.div(t);
 				const int k = t - w * width;
-				return in[base + w * g.line + min(k, last)];
+				const int slot = w * pitch + k;
+				return fetched{in[base + w * g.line + min(k, last)], k < load_n ? slot : -1 - slot};
 			},
-			[&](int t, float2 v) {
-				const int w = by_width.div(t);
-				const int k = t - w * width;
-				buf[w * pitch + k] = k < load_n ? v : zero;
-			});
+			[&](int, fetched f) { buf[f.slot < 0 ? -1 - f.slot : f.slot] = f.slot < 0 ? zero : f.v; });
 	} else {
 		const fast_div by_count(count);
 		batched_loop(total,
 			[&](int t) {
 				const int k = by_count.div(t);
 				const int w = t - k * count;
-				return in[base + w * g.line + min(k, last) * g.elem];
+				const int slot = w * pitch + k;
+				return fetched{in[base + w * g.line + min(k, last) * g.elem], k < load_n ? slot : -1 - slot};
 			},
-			[&](int t, float2 v) {
-				const int k = by_count.div(t);
-				const int w = t - k * count;
-				buf[w * pitch + k] = k < load_n ? v : zero;
-			});
+			[&](int, fetched f) { buf[f.slot < 0 ? -1 - f.slot : f.slot] = f.slot < 0 ? zero : f.v; });
 	}
 }
 
@@ -406,28 +405,29 @@ __device__ inline void store_complex_lines(float2* out, const line_geom& g, int 
 {
 	const long long base = aOuter * g.outer + aLine0 * g.line;
 	const int total = count * store_n;
+	// the fetch (an LDS read) also returns the destination index
+	struct fetched {
+		float2 v;
+		long long at;
+	};
 	if (g.elem == 1) {
 		const fast_div by_store(store_n);
 		batched_loop(total,
 			[&](int t) {
 				const int w = by_store.div(t);
-				return buf[w * pitch + (t - w * store_n)];
+				const int k = t - w * store_n;
+				return fetched{buf[w * pitch + k], base + w * g.line + k};
 			},
-			[&](int t, float2 v) {
-				const int w = by_store.div(t);
-				out[base + w * g.line + (t - w * store_n)] = v;
-			});
+			[&](int, fetched f) { out[f.at] = f.v; });
 	} else {
 		const fast_div by_count(count);
 		batched_loop(total,
 			[&](int t) {
 				const int k = by_count.div(t);
-				return buf[(t - k * count) * pitch + k];
+				const int w = t - k * count;
+				return fetched{buf[w * pitch + k], base + w * g.line + k * g.elem};
 			},
-			[&](int t, float2 v) {
-				const int k = by_count.div(t);
-				out[base + (t - k * count) * g.line + k * g.elem] = v;
-			});
+			[&](int, fetched f) { out[f.at] = f.v; });
 	}
 }
 
@@ -584,17 +584,18 @@ __device__ __forceinline__ void fft_pass_body(const fft_pass_args& a, unsigned a
 			const int k = by_count.div(t);
 			return make_int2(t - k * count, k);
 		};
+		struct fetched {
+			float2 h;
+			int slot;
+		};
 		batched_loop(count * m,
 			[&](int t) {
 				const int2 wk = place(t);
-				return a.mult[hbase + wk.x * a.gmult.line + wk.y * a.gmult.elem];
+				return fetched{a.mult[hbase + wk.x * a.gmult.line + wk.y * a.gmult.elem], wk.x * a.pitch + wk.y};
 			},
-			[&](int t, float2 h) {
-				const int2 wk = place(t);
-				if (a.mult_conj) {
-					h = cconj(h);
-				}
-				r[wk.x * a.pitch + wk.y] = cscale(cmul(r[wk.x * a.pitch + wk.y], h), a.mult_scale);
+			[&](int, fetched f) {
+				const float2 h = a.mult_conj ? cconj(f.h) : f.h;
+				r[f.slot] = cscale(cmul(r[f.slot], h), a.mult_scale);
 			});
 		__syncthreads();
 		const float2* q = run_stages(a.plan, r, r == A ? B : A, count, a.pitch, tw_m, 1.0f);
