@@ -160,6 +160,45 @@ def test_strided_and_reversed_views():
         same_bits(host(out), oracle.convolve(base[:, ::-1, :], ker, np.float32, border))
 
 
+def test_streaming_kernel_strided_and_reversed_views():
+    """The streaming kernels address planes through buffer resources: sliced and strided
+    sources, a strided destination and a source with a negative z stride give the same
+    bits as the tiled kernel (FP32 precision, and u16 with an integer kernel)."""
+    from iane_amd import _lib, autotune
+    rng = np.random.default_rng(9)
+    cases_ = [((rng.random((20, 40, 140)) * 10).astype(np.float32), (rng.random((3, 3, 3)) - 0.3).astype(np.float32),
+               np.float32, precision.fp32),
+              (rng.integers(0, 65536, (20, 40, 140)).astype(np.uint16), rng.integers(-4, 5, (3, 5, 3)).astype(np.int32),
+               np.int32, precision.reference)]
+    for base, ker, ddt, prec in cases_:
+        dbase, dker = dev(base), dev(ker)
+        for border in (cases.CLAMP, cases.ZERO):
+            for h, d in ((base[1:19, 3:35, 2:131], dbase[1:19, 3:35, 2:131]), (base[::2, :, 1::3], dbase[::2, :, 1::3])):
+                outs = []
+                for stream in (1, 0):
+                    out_base = empty((h.shape[0], h.shape[1], h.shape[2] * 2), ddt)
+                    out = out_base[:, :, ::2]  # strided destination
+                    with autotune.applied({"conv.stream": stream}):
+                        iane_amd.convolve(d, dker, out, convolution_options(border_mode(border), prec)).wait()
+                    outs.append(host(out))
+                same_bits(outs[0], outs[1])
+                if prec == precision.reference:
+                    same_bits(outs[0], oracle.convolve(h, ker, ddt, border))
+            # negative stride along IANE axis 2 (numpy axis 0): planes in reverse order
+            desc = iane_amd.view_desc(dbase)
+            desc.ptr = dbase.data_ptr() + (base.shape[0] - 1) * desc.stride_bytes[2]
+            desc.stride_bytes[2] = -desc.stride_bytes[2]
+            outs = []
+            for stream in (1, 0):
+                out = empty(base.shape, ddt)
+                with autotune.applied({"conv.stream": stream}):
+                    _lib.call("iane_hip_convolve", desc, iane_amd.view_desc(dker), iane_amd.view_desc(out), border,
+                              int(prec), 0)
+                    iane_amd.hip_execution(0).wait()
+                outs.append(host(out))
+            same_bits(outs[0], outs[1])
+
+
 def test_empty_image_is_a_no_op():
     import torch
     src = torch.zeros((0, 5), device="cuda")
