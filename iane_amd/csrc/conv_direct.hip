@@ -113,19 +113,38 @@ __global__ __launch_bounds__(kThreads) void conv_tiled_kernel(conv_args a)
 			int wi = pw + a.center[3] - qw;
 			const bool plane_in = border_coord(zi, a.src.extent[2], a.border) &&
 				border_coord(wi, a.src.extent[3], a.border);
-			const char* base = a.src.ptr + zi * a.src.stride[2] + wi * a.src.stride[3];
-			for (int i = tid; i < rows * cols; i += kThreads) {
-				const int ly = i / cols;
-				const int lx = i - ly * cols;
-				int xi = x0 - a.center[0] + lx;
-				int yi = y0 - a.center[1] + ly;
-				Acc value = Acc(0);
-				if (plane_in && border_coord(xi, a.src.extent[0], a.border) &&
-					border_coord(yi, a.src.extent[1], a.border)) {
-					value = load_as<Acc>(base + xi * a.src.stride[0] + yi * a.src.stride[1], a.src.type);
+			const char* base = a.src.ptr + (plane_in ? zi * a.src.stride[2] + wi * a.src.stride[3] : 0);
+			// Stage the tile kStage elements per thread at a time: every load of a
+			// batch is issued (a zero-border sample reads the plane's first sample)
+			// before any is used, with the element type fixed outside the loop.
+			constexpr int kStage = 8;
+			const int total = rows * cols;
+			with_element_type(a.src.type, [&](auto aType) {
+				using S = decltype(aType);
+				for (int i0 = tid; i0 < total; i0 += kStage * kThreads) {
+					S raw[kStage];
+					bool inside[kStage];
+#pragma unroll
+					for (int u = 0; u < kStage; ++u) {
+						const int i = min(i0 + u * kThreads, total - 1);
+						const int ly = i / cols;
+						const int lx = i - ly * cols;
+						int xi = x0 - a.center[0] + lx;
+						int yi = y0 - a.center[1] + ly;
+						inside[u] = plane_in && border_coord(xi, a.src.extent[0], a.border) &&
+							border_coord(yi, a.src.extent[1], a.border);
+						const long long offset = inside[u] ? xi * a.src.stride[0] + yi * a.src.stride[1] : 0;
+						raw[u] = *reinterpret_cast<const S*>(base + offset);
+					}
+#pragma unroll
+					for (int u = 0; u < kStage; ++u) {
+						const int i = i0 + u * kThreads;
+						if (i < total) {
+							tile[i] = inside[u] ? static_cast<Acc>(raw[u]) : Acc(0);
+						}
+					}
 				}
-				tile[i] = value;
-			}
+			});
 			__syncthreads();
 			for (int qy = 0; qy < ky; ++qy) {
 				const Acc* krow = coef + qy * kx;
