@@ -376,12 +376,16 @@ __global__ __launch_bounds__(kSeqThreads) void row_carries_kernel(seq_map m, Acc
 }
 
 // Output of one tile: rescan from the tile's carry (init when the line is one tile);
-// PER >= aPer elements per thread.
+// PER >= aPer elements per thread. The tile passes through LDS between the
+// coalesced order (element j to thread j mod 256) and per-thread segments
+// (aPer consecutive elements); one padding slot per 16 elements puts the
+// segment reads and writes of a wave on different banks.
 template<class Acc, int OP, int KIND, int PER>
 __global__ __launch_bounds__(kSeqThreads) void row_output_kernel(seq_map m, const Acc* aCarries, long long aTiles,
 	int aPer, Acc aInit)
 {
-	__shared__ Acc stage[kSeqTile];
+	__shared__ Acc stage[kSeqTile + kSeqTile / 16];
+	const auto slot = [](int aIndex) { return aIndex + (aIndex >> 4); };
 	__shared__ Acc shared[4];
 	const long long task = blockIdx.x;
 	const long long line = task / aTiles;
@@ -398,7 +402,7 @@ __global__ __launch_bounds__(kSeqThreads) void row_output_kernel(seq_map m, cons
 		for (int e = 0; e < PER; ++e) {
 			const int j = threadIdx.x + e * kSeqThreads;
 			if (e < aPer && j < n) {
-				stage[j] = v[e];
+				stage[slot(j)] = v[e];
 			}
 		}
 	}
@@ -408,7 +412,7 @@ __global__ __launch_bounds__(kSeqThreads) void row_output_kernel(seq_map m, cons
 #pragma unroll
 	for (int e = 0; e < PER; ++e) {
 		const int index = aPer * threadIdx.x + e;
-		const Acc x = (e < aPer && index < n) ? stage[index] : identity<Acc, OP>();
+		const Acc x = (e < aPer && index < n) ? stage[slot(index)] : identity<Acc, OP>();
 		local[e] = run;             // exclusive within the thread
 		run = apply<Acc, OP>(run, x);
 		if constexpr (KIND == IANE_SEQ_INCLUSIVE) {
@@ -423,7 +427,7 @@ __global__ __launch_bounds__(kSeqThreads) void row_output_kernel(seq_map m, cons
 	for (int e = 0; e < PER; ++e) {
 		const int index = aPer * threadIdx.x + e;
 		if (e < aPer && index < n) {
-			stage[index] = apply<Acc, OP>(before, local[e]);
+			stage[slot(index)] = apply<Acc, OP>(before, local[e]);
 		}
 	}
 	__syncthreads();
@@ -431,7 +435,7 @@ __global__ __launch_bounds__(kSeqThreads) void row_output_kernel(seq_map m, cons
 	for (int e = 0; e < PER; ++e) {
 		const int j = threadIdx.x + e * kSeqThreads;
 		if (e < aPer && j < n) {
-			store_value(m.dst + dst_offset(m, dst_base, begin + j), stage[j]);
+			store_value(m.dst + dst_offset(m, dst_base, begin + j), stage[slot(j)]);
 		}
 	}
 }
