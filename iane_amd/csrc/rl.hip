@@ -48,10 +48,43 @@ __global__ void rl_initial_kernel(const float* y, float* x, long long n, float e
 	}
 }
 
-__device__ __forceinline__ double block_sum(double v, double* shared)
+// Log-likelihood sums stay in FP32: a compensated pair (hi, lo) where lo
+// carries the rounding error of every addition into hi (Knuth's TwoSum), so
+// the sum of millions of terms keeps the resolution of the terms themselves
+// (each term y log max(p, eps) - p is formed in FP32). Same size as a double:
+// the partial buffers hold one pair per block.
+struct ll_sum {
+	float hi;
+	float lo;
+};
+
+__device__ __forceinline__ ll_sum ll_add(ll_sum aSum, float aTerm)
+{
+	const float s = aSum.hi + aTerm;
+	const float b = s - aSum.hi;
+	const float err = (aSum.hi - (s - b)) + (aTerm - b);
+	return ll_sum{s, aSum.lo + err};
+}
+
+__device__ __forceinline__ ll_sum ll_merge(ll_sum aA, ll_sum aB)
+{
+	ll_sum r = ll_add(aA, aB.hi);
+	r.lo += aB.lo;
+	return r;
+}
+
+__device__ __forceinline__ float ll_term(float aObs, float aPred, float aFloor)
+{
+	return aObs * logf(aFloor) - aPred;
+}
+
+// Deterministic block sum (fixed shuffle tree, waves in order); the result is
+// valid in thread 0.
+__device__ __forceinline__ ll_sum block_sum(ll_sum v, ll_sum* shared)
 {
 	for (int off = 32; off > 0; off >>= 1) {
-		v += __shfl_down(v, off, 64);
+		const ll_sum other{__shfl_down(v.hi, off, 64), __shfl_down(v.lo, off, 64)};
+		v = ll_merge(v, other);
 	}
 	const int lane = threadIdx.x & 63;
 	const int wave = threadIdx.x >> 6;
@@ -59,30 +92,32 @@ __device__ __forceinline__ double block_sum(double v, double* shared)
 		shared[wave] = v;
 	}
 	__syncthreads();
-	double total = 0.0;
+	ll_sum total{0.0f, 0.0f};
 	if (threadIdx.x == 0) {
 		for (int w = 0; w < (blockDim.x + 63) / 64; ++w) {
-			total += shared[w];
+			total = ll_merge(total, shared[w]);
 		}
 	}
 	return total;
 }
 
 // main.cu:22-29 plus the Poisson log-likelihood partial sum(y log max(p, eps) - p)
-__global__ void rl_ratio_kernel(const float* y, const float* p, float* r, long long n, float eps, double* partials)
+__global__ void rl_ratio_kernel(const float* y, const float* p, float* r, long long n, float eps, ll_sum* partials)
 {
-	__shared__ double shared[kPointThreads / 64];
-	double ll = 0.0;
+	__shared__ ll_sum shared[kPointThreads / 64];
+	ll_sum ll{0.0f, 0.0f};
 	for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
 		 i += static_cast<long long>(gridDim.x) * blockDim.x) {
 		const float obs = y[i];
 		const float pred = p[i];
 		const float floor = pred > eps ? pred : eps;
 		r[i] = obs / floor;
-		ll += static_cast<double>(obs) * log(static_cast<double>(floor)) - static_cast<double>(pred);
+		if (partials) {
+			ll = ll_add(ll, ll_term(obs, pred, floor));
+		}
 	}
 	if (partials) {
-		const double total = block_sum(ll, shared);
+		const ll_sum total = block_sum(ll, shared);
 		if (threadIdx.x == 0) {
 			partials[blockIdx.x] = total;
 		}
@@ -103,17 +138,18 @@ __global__ void rl_update_kernel(float* x, const float* c, const float* s, long 
 	}
 }
 
-// Deterministic final sum of the per-block partials (fixed order).
-__global__ void ll_finalize_kernel(const double* partials, int n, double* out)
+// Deterministic final sum of the per-block partials (fixed order); the pair is
+// widened to the double the interface returns.
+__global__ void ll_finalize_kernel(const ll_sum* partials, int n, double* out)
 {
-	__shared__ double shared[kPointThreads / 64];
-	double v = 0.0;
+	__shared__ ll_sum shared[kPointThreads / 64];
+	ll_sum v{0.0f, 0.0f};
 	for (int i = threadIdx.x; i < n; i += blockDim.x) {
-		v += partials[i];
+		v = ll_merge(v, partials[i]);
 	}
-	const double total = block_sum(v, shared);
+	const ll_sum total = block_sum(v, shared);
 	if (threadIdx.x == 0) {
-		*out = total;
+		*out = static_cast<double>(total.hi) + static_cast<double>(total.lo);
 	}
 }
 
@@ -166,7 +202,7 @@ struct rl_row_args {
 	int nx;
 	float eps;
 	int forward_next;       // run the R2C of the new row (0 on the last update)
-	double* partials;       // EPI_RATIO: log-likelihood partials per block (may be null)
+	ll_sum* partials;       // EPI_RATIO: log-likelihood partials per block (may be null)
 };
 
 // C2R of `count` spectrum rows, the RL pointwise step on the real row, R2C of
@@ -175,7 +211,7 @@ struct rl_row_args {
 // the group goes to a.partials[aSlot].
 template<int EPI>
 __device__ __forceinline__ void rl_row_body(const rl_row_args& a, unsigned aBlock, unsigned aSlot, float2* lds,
-	double* shared)
+	ll_sum* shared)
 {
 	const fft_pass_args& f = a.f;
 	float2* A = lds;
@@ -192,7 +228,7 @@ __device__ __forceinline__ void rl_row_body(const rl_row_args& a, unsigned aBloc
 	__syncthreads();
 	float2* r = run_stages(f.plan, B, A, count, f.pitch, tw_m, 1.0f);
 	// pointwise step on real samples 2j, 2j+1 of each row
-	double ll = 0.0;
+	ll_sum ll{0.0f, 0.0f};
 	const long long base = o * a.real_outer + l0 * a.real_line;
 	const bool want_ll = EPI == EPI_RATIO && a.partials != nullptr; // uniform
 	const fast_div by_m(m);
@@ -229,7 +265,7 @@ __device__ __forceinline__ void rl_row_body(const rl_row_args& a, unsigned aBloc
 						const float floor = pred > a.eps ? pred : a.eps;
 						out[h] = obs / floor;
 						if (want_ll) {
-							ll += static_cast<double>(obs) * log(static_cast<double>(floor)) - static_cast<double>(pred);
+							ll = ll_add(ll, ll_term(obs, pred, floor));
 						}
 					} else {
 						const float x = h == 0 ? aSamples.x : aSamples.y;
@@ -244,7 +280,7 @@ __device__ __forceinline__ void rl_row_body(const rl_row_args& a, unsigned aBloc
 		});
 	if constexpr (EPI == EPI_RATIO) {
 		if (a.partials) {
-			const double total = block_sum(ll, shared);
+			const ll_sum total = block_sum(ll, shared);
 			if (threadIdx.x == 0) {
 				a.partials[aSlot] = total;
 			}
@@ -265,7 +301,7 @@ template<int EPI>
 __global__ __launch_bounds__(kFftThreads) void rl_row_kernel(rl_row_args a)
 {
 	extern __shared__ float2 lds[];
-	__shared__ double shared[kFftThreads / 64];
+	__shared__ ll_sum shared[kFftThreads / 64];
 	// the partial of line group g goes to slot g (the batched kernel does the same)
 	const unsigned group = xcd_remap(blockIdx.x, gridDim.x);
 	rl_row_body<EPI>(a, group, group, lds, shared);
@@ -317,7 +353,7 @@ __global__ __launch_bounds__(kFftThreads) void rl_row_batched_kernel(
 	const rl_row_args* __restrict__ args, const unsigned* __restrict__ offsets, int count)
 {
 	extern __shared__ float2 lds[];
-	__shared__ double shared[kFftThreads / 64];
+	__shared__ ll_sum shared[kFftThreads / 64];
 	__shared__ rl_row_args desc;
 	const unsigned b = xcd_remap(blockIdx.x, gridDim.x);
 	const int v = __builtin_amdgcn_readfirstlane(volume_of(offsets, count, b));
@@ -326,17 +362,17 @@ __global__ __launch_bounds__(kFftThreads) void rl_row_batched_kernel(
 }
 
 // log-likelihood of volume v (partials offsets[v] .. offsets[v+1]-1) into out[v * stride]
-__global__ void ll_finalize_batched_kernel(const double* partials, const unsigned* offsets, double* out, int stride)
+__global__ void ll_finalize_batched_kernel(const ll_sum* partials, const unsigned* offsets, double* out, int stride)
 {
-	__shared__ double shared[kPointThreads / 64];
+	__shared__ ll_sum shared[kPointThreads / 64];
 	const int v = blockIdx.x;
-	double s = 0.0;
+	ll_sum s{0.0f, 0.0f};
 	for (unsigned i = offsets[v] + threadIdx.x; i < offsets[v + 1]; i += blockDim.x) {
-		s += partials[i];
+		s = ll_merge(s, partials[i]);
 	}
-	const double total = block_sum(s, shared);
+	const ll_sum total = block_sum(s, shared);
 	if (threadIdx.x == 0) {
-		out[static_cast<long long>(v) * stride] = total;
+		out[static_cast<long long>(v) * stride] = static_cast<double>(total.hi) + static_cast<double>(total.lo);
 	}
 }
 
@@ -595,7 +631,7 @@ void rl_plan_impl::build_fft(hipStream_t stream)
 	row_tpl.eps = eps;
 	partial_capacity = static_cast<int>(std::max<long long>(kPartialBlocks,
 		static_cast<long long>(row_tpl.f.groups_per_outer) * row_tpl.f.outer));
-	workspace = ((spectrum_bytes + 255) / 256) * 256 + static_cast<size_t>(partial_capacity) * sizeof(double);
+	workspace = ((spectrum_bytes + 255) / 256) * 256 + static_cast<size_t>(partial_capacity) * sizeof(ll_sum);
 }
 
 } // namespace iane_hip
@@ -657,7 +693,7 @@ std::unique_ptr<rl_plan_impl> make_rl_plan(int aRank, const int32_t* aExtent, co
 		fill_kernel<<<point_blocks(impl->count), kPointThreads, 0, stream>>>(ones, impl->count, 1.0f);
 		impl->convolve(ones, impl->flipped, impl->sensitivity, stream);
 		check_hip(hipFreeAsync(ones, stream), "Free ones");
-		impl->workspace = 2 * impl->count * sizeof(float) + static_cast<size_t>(kPartialBlocks + 64) * sizeof(double);
+		impl->workspace = 2 * impl->count * sizeof(float) + static_cast<size_t>(kPartialBlocks + 64) * sizeof(ll_sum);
 	} else {
 		impl->build_fft(stream);
 		// s = H^T 1 through the same pruned passes
@@ -724,10 +760,10 @@ int iane_hip_rl_ratio(const float* aObserved, const float* aPredicted, float* aR
 			return;
 		}
 		auto stream = static_cast<hipStream_t>(aStream);
-		double* partials = nullptr;
+		ll_sum* partials = nullptr;
 		const unsigned blocks = point_blocks(aCount);
 		if (aLlAccum) {
-			check_hip(hipMallocAsync(reinterpret_cast<void**>(&partials), blocks * sizeof(double), stream),
+			check_hip(hipMallocAsync(reinterpret_cast<void**>(&partials), blocks * sizeof(ll_sum), stream),
 				"Allocate partials");
 		}
 		rl_ratio_kernel<<<blocks, kPointThreads, 0, stream>>>(aObserved, aPredicted, aRatio, aCount, aEps, partials);
@@ -880,7 +916,7 @@ int iane_hip_rl_run(iane_rl_plan* aPlan, const float* aObserved, float* aEstimat
 		if (p.method == IANE_RL_DIRECT) {
 			float* predicted = reinterpret_cast<float*>(ws);
 			float* correction = predicted + n;
-			double* partials = reinterpret_cast<double*>(ws + ((2 * n * sizeof(float) + 255) / 256) * 256);
+			ll_sum* partials = reinterpret_cast<ll_sum*>(ws + ((2 * n * sizeof(float) + 255) / 256) * 256);
 			const unsigned blocks = point_blocks(n);
 			for (int it = 0; it < aIterations; ++it) {
 				p.convolve(aEstimate, p.psf, predicted, stream);
@@ -896,7 +932,7 @@ int iane_hip_rl_run(iane_rl_plan* aPlan, const float* aObserved, float* aEstimat
 			return;
 		}
 		float2* spec = reinterpret_cast<float2*>(ws);
-		double* partials = reinterpret_cast<double*>(ws + ((p.spectrum_bytes + 255) / 256) * 256);
+		ll_sum* partials = reinterpret_cast<ll_sum*>(ws + ((p.spectrum_bytes + 255) / 256) * 256);
 		// spectrum of the current estimate
 		fft_pass_args a = p.r2c;
 		a.in = aEstimate;
@@ -1135,7 +1171,7 @@ int iane_hip_rl_batch_create(iane_rl_batch** aBatch, int aCount, int aRank, cons
 		batch->update_next.finalize(pass_of_row);
 		batch->update_last.finalize(pass_of_row);
 		batch->partial_offset = offset;
-		batch->workspace = offset + static_cast<size_t>(batch->ratio.blocks) * sizeof(double);
+		batch->workspace = offset + static_cast<size_t>(batch->ratio.blocks) * sizeof(ll_sum);
 		check_hip(hipMalloc(reinterpret_cast<void**>(&batch->d_observed), sizeof(float*) * aCount), "Allocate tables");
 		check_hip(hipMalloc(reinterpret_cast<void**>(&batch->d_estimate), sizeof(float*) * aCount), "Allocate tables");
 		check_hip(hipMalloc(reinterpret_cast<void**>(&batch->d_counts), sizeof(long long) * aCount), "Allocate tables");
@@ -1179,7 +1215,7 @@ int iane_hip_rl_batch_run(iane_rl_batch* aBatch, const float* const* aObserved, 
 			invalid("graph capture needs a created (non-null) stream");
 		}
 		char* ws = static_cast<char*>(aWorkspace);
-		double* partials = reinterpret_cast<double*>(ws + b.partial_offset);
+		ll_sum* partials = reinterpret_cast<ll_sum*>(ws + b.partial_offset);
 		for (int v = 0; v < b.count; ++v) {
 			if (!aObserved[v] || !aEstimates[v]) {
 				invalid("RL batch buffers are null");

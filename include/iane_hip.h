@@ -173,7 +173,8 @@ int iane_hip_fft_plan_destroy(iane_fft_plan* plan);
 int iane_hip_rl_initial(const float* observed, float* estimate, int64_t count, float epsilon, void* stream);
 /* ratio = observed / max(predicted, eps) (main.cu:22-29); ratio may alias predicted.
  * ll_out != NULL: *ll_out (device double) = sum(y log max(p, eps) - p), the
- * Poisson log-likelihood without log(y!) (no reference counterpart).        */
+ * Poisson log-likelihood without log(y!) (no reference counterpart): FP32
+ * terms summed as a compensated FP32 pair, widened to double at the end.    */
 int iane_hip_rl_ratio(const float* observed, const float* predicted, float* ratio, int64_t count,
 	float epsilon, double* ll_out, void* stream);
 /* estimate *= (sens > 0 ? corr / max(sens, eps) : 0)           (main.cu:32-49,74-77) */
