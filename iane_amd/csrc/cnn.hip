@@ -196,7 +196,7 @@ __global__ __launch_bounds__(256) void conv3d_wgrad_kernel(const __bf16* in, con
 				acc[r] = s;
 			}
 		}
-		if (t == 13 && threadIdx.x < g.cout) {
+		if (t == 13 && static_cast<int>(threadIdx.x) < g.cout) {
 			for (int v = 0; v < tile; ++v) {
 				bacc += s_dy[v * g.cout + threadIdx.x];
 			}
@@ -210,7 +210,7 @@ __global__ __launch_bounds__(256) void conv3d_wgrad_kernel(const __bf16* in, con
 			atomicAdd(&dw[(static_cast<long long>(co) * 27 + t) * g.cin + ci], acc[r]);
 		}
 	}
-	if (t == 13 && threadIdx.x < g.cout && dbias) {
+	if (t == 13 && static_cast<int>(threadIdx.x) < g.cout && dbias) {
 		atomicAdd(&dbias[threadIdx.x], bacc);
 	}
 }
@@ -462,7 +462,7 @@ __global__ void max_kernel(const float* in, long long n, float* partial)
 	}
 	s[threadIdx.x] = m;
 	__syncthreads();
-	for (int off = 128; off > 0; off >>= 1) {
+	for (unsigned off = 128; off > 0; off >>= 1) {
 		if (threadIdx.x < off) {
 			s[threadIdx.x] = fmaxf(s[threadIdx.x], s[threadIdx.x + off]);
 		}

@@ -94,7 +94,8 @@ __device__ __forceinline__ ll_sum block_sum(ll_sum v, ll_sum* shared)
 	__syncthreads();
 	ll_sum total{0.0f, 0.0f};
 	if (threadIdx.x == 0) {
-		for (int w = 0; w < (blockDim.x + 63) / 64; ++w) {
+		const int waves = static_cast<int>((blockDim.x + 63) / 64);
+		for (int w = 0; w < waves; ++w) {
 			total = ll_merge(total, shared[w]);
 		}
 	}
@@ -460,10 +461,11 @@ struct rl_plan_impl {
 
 	~rl_plan_impl()
 	{
-		hipFree(sensitivity);
-		hipFree(psf);
-		hipFree(flipped);
-		hipFree(spectrum_h);
+		// destructors cannot report: a failed free is ignored
+		(void)hipFree(sensitivity);
+		(void)hipFree(psf);
+		(void)hipFree(flipped);
+		(void)hipFree(spectrum_h);
 	}
 
 	view4 image_view(float* p) const
@@ -994,8 +996,8 @@ struct batched_launch {
 	batched_launch& operator=(const batched_launch&) = delete;
 	~batched_launch()
 	{
-		hipFree(d_args);
-		hipFree(d_offsets);
+		(void)hipFree(d_args);
+		(void)hipFree(d_offsets);
 	}
 
 	// offsets from the per-volume group counts; device copies of both tables
@@ -1073,15 +1075,16 @@ struct iane_rl_batch {
 
 	~iane_rl_batch()
 	{
+		// destructors cannot report: failed releases are ignored
 		if (exec) {
-			hipGraphExecDestroy(exec);
+			(void)hipGraphExecDestroy(exec);
 		}
 		if (graph) {
-			hipGraphDestroy(graph);
+			(void)hipGraphDestroy(graph);
 		}
-		hipFree(d_observed);
-		hipFree(d_estimate);
-		hipFree(d_counts);
+		(void)hipFree(d_observed);
+		(void)hipFree(d_estimate);
+		(void)hipFree(d_counts);
 	}
 
 	// the launch sequence of `iterations` RL steps (captured into the graph)
@@ -1274,10 +1277,11 @@ int iane_hip_rl_batch_run(iane_rl_batch* aBatch, const float* const* aObserved, 
 			try {
 				b.enqueue(aIterations, aInit, aLlHistory, stream);
 			} catch (...) {
+				// end the capture before reporting the original error
 				hipGraph_t discard = nullptr;
-				hipStreamEndCapture(stream, &discard);
+				(void)hipStreamEndCapture(stream, &discard);
 				if (discard) {
-					hipGraphDestroy(discard);
+					(void)hipGraphDestroy(discard);
 				}
 				throw;
 			}
