@@ -97,7 +97,16 @@ def cpu_baseline(volume: np.ndarray, taps: np.ndarray, planes: int = 16):
         run()
         samples.append(time.perf_counter() - t0)
     median = sorted(samples)[1]
+    model = "unknown"
+    try:
+        for line in pathlib.Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     return {"value": slab.size / median / 1e6, "unit": "Mvoxel/s", "cores": cores, "kind": kind,
+            "cpu_model": model, "min_s": min(samples), "max_s": max(samples),
             "sample": f"{planes}-plane z-slab ({slab.shape[2]}x{slab.shape[1]}x{planes}) of the 256^3 workload, "
                       f"dense 15^3 outer-product kernel (the reference has no separable path), zero border, "
                       f"cpu_policy{{threads={cores}, grain=4096}}, median of 3 after 1 warm-up",
