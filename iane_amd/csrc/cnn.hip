@@ -30,6 +30,150 @@ struct conv_geom {
 
 __device__ __forceinline__ float bf2f(__bf16 v) { return static_cast<float>(v); }
 
+// Write a lane's 4 consecutive output channels co0 .. co0 + 3 of one voxel
+// (bias, optional ReLU): one 8-byte (BF16) or 16-byte (FP32) store when all 4
+// exist and cout is a multiple of 4, so a wave writes whole voxels contiguously;
+// channel by channel otherwise.
+__device__ __forceinline__ void store_outputs(void* out, int out_bf16, int relu, const float* bias, long long voxel,
+	int cout, int co0, f32x4 acc)
+{
+	float v[4];
+#pragma unroll
+	for (int i = 0; i < 4; ++i) {
+		const int co = co0 + i;
+		float t = acc[i] + ((bias && co < cout) ? bias[co] : 0.0f);
+		v[i] = relu ? (t > 0.0f ? t : 0.0f) : t;
+	}
+	const long long at = voxel * cout + co0;
+	if ((cout & 3) == 0 && co0 + 3 < cout) { // aligned: cout and co0 are multiples of 4
+		if (out_bf16) {
+			typedef __attribute__((__vector_size__(4 * sizeof(__bf16)))) __bf16 bf16x4;
+			bf16x4 packed;
+#pragma unroll
+			for (int i = 0; i < 4; ++i) {
+				packed[i] = static_cast<__bf16>(v[i]);
+			}
+			*reinterpret_cast<bf16x4*>(static_cast<__bf16*>(out) + at) = packed;
+		} else {
+			*reinterpret_cast<f32x4*>(static_cast<float*>(out) + at) = f32x4{v[0], v[1], v[2], v[3]};
+		}
+		return;
+	}
+#pragma unroll
+	for (int i = 0; i < 4; ++i) {
+		if (co0 + i < cout) {
+			if (out_bf16) {
+				static_cast<__bf16*>(out)[at + i] = static_cast<__bf16>(v[i]);
+			} else {
+				static_cast<float*>(out)[at + i] = v[i];
+			}
+		}
+	}
+}
+
+// Layers with 16 input and at most 16 output channels (the full-resolution
+// layers of the network): the same implicit GEMM, with K ordered by x shift so
+// that an input row is loaded once per wave and reused for all three x taps.
+// K = 3 shifts x 5 row pairs x 32 (two (dz, dy) rows of 16 channels; the tenth
+// row is zero padding): chunk (dx, p) holds rows 2p (lane groups 0, 1) and
+// 2p + 1 (groups 2, 3), channels 8 (g % 2) .. + 7, all at x shift dx. A lane
+// loads voxel x0 + n of its row and the two voxels just outside the wave's 16
+// (x0 - 1, x0 + 16, the same address for the 16 lanes of a group); the x - 1
+// and x + 1 operands are the centre values moved one lane within each group of
+// 16 (DPP row shifts), the end lane taking the outside voxel. The weights of
+// all 15 chunks stay in registers, loaded once for kFwd16Rows consecutive y
+// rows that the wave computes in turn.
+__device__ __forceinline__ bf16x8 shift_lanes(bf16x8 aCentre, bf16x8 aEdge, int aDx)
+{
+	using i32x4 = __attribute__((__vector_size__(4 * sizeof(int)))) int;
+	const i32x4 c = __builtin_bit_cast(i32x4, aCentre);
+	const i32x4 e = __builtin_bit_cast(i32x4, aEdge);
+	i32x4 r;
+#pragma unroll
+	for (int i = 0; i < 4; ++i) {
+		// row_shr:1 (lane n <- lane n - 1) or row_shl:1 (lane n <- lane n + 1); the lane with
+		// no source in its row keeps the edge value
+		r[i] = aDx < 0 ? __builtin_amdgcn_update_dpp(e[i], c[i], 0x111, 0xF, 0xF, false)
+		               : __builtin_amdgcn_update_dpp(e[i], c[i], 0x101, 0xF, 0xF, false);
+	}
+	return __builtin_bit_cast(bf16x8, r);
+}
+
+constexpr int kFwd16Rows = 8;
+
+__global__ __launch_bounds__(64 * kConvWaves) void conv3d_fwd16_kernel(const __bf16* __restrict__ in,
+	const __bf16* __restrict__ w, const float* __restrict__ bias, void* out, int out_bf16, int relu, conv_geom g)
+{
+	constexpr int kCin = 16;
+	const int lane = threadIdx.x & 63;
+	const long long wave = static_cast<long long>(blockIdx.x) * kConvWaves + (threadIdx.x >> 6);
+	const int xt = (g.nx + 15) / 16;
+	const int yb = (g.ny + kFwd16Rows - 1) / kFwd16Rows;
+	if (wave >= static_cast<long long>(xt) * yb * g.nz) {
+		return;
+	}
+	const int x0 = static_cast<int>(wave % xt) * 16;
+	const int y0 = static_cast<int>((wave / xt) % yb) * kFwd16Rows;
+	const int z = static_cast<int>(wave / (static_cast<long long>(xt) * yb));
+	const int n = lane & 15;
+	const int grp = lane >> 4;
+	const int x = x0 + n;
+	const int ch = 8 * (grp & 1);
+	bf16x8 zero8;
+#pragma unroll
+	for (int j = 0; j < 8; ++j) {
+		zero8[j] = static_cast<__bf16>(0.0f);
+	}
+	// A operands of all chunks: output channel n, row 2p + grp / 2, tap (row, dx)
+	bf16x8 wreg[3][5];
+#pragma unroll
+	for (int d = 0; d < 3; ++d) {
+#pragma unroll
+		for (int p = 0; p < 5; ++p) {
+			const int r = 2 * p + (grp >> 1);
+			const bool ok = n < g.cout && r < 9;
+			const long long at = ok ? (static_cast<long long>(n) * 27 + r * 3 + d) * kCin + ch : 0;
+			wreg[d][p] = *reinterpret_cast<const bf16x8*>(w + at);
+			if (!ok) {
+				wreg[d][p] = zero8;
+			}
+		}
+	}
+	for (int y = y0; y < min(y0 + kFwd16Rows, g.ny); ++y) {
+		f32x4 acc = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+		for (int p = 0; p < 5; ++p) {
+			const int r = 2 * p + (grp >> 1);
+			const int yi = y + (r % 3) - 1;
+			const int zi = z + (r / 3) - 1;
+			const bool row_ok = r < 9 && yi >= 0 && yi < g.ny && zi >= 0 && zi < g.nz;
+			const long long row = (static_cast<long long>(zi) * g.ny + yi) * g.nx;
+			const bool ok_c = row_ok && x < g.nx;
+			const bool ok_l = row_ok && x0 > 0;
+			const bool ok_r = row_ok && x0 + 16 < g.nx;
+			bf16x8 centre = *reinterpret_cast<const bf16x8*>(in + (ok_c ? (row + x) * kCin + ch : 0));
+			bf16x8 left = *reinterpret_cast<const bf16x8*>(in + (ok_l ? (row + x0 - 1) * kCin + ch : 0));
+			bf16x8 right = *reinterpret_cast<const bf16x8*>(in + (ok_r ? (row + x0 + 16) * kCin + ch : 0));
+			if (!ok_c) {
+				centre = zero8;
+			}
+			if (!ok_l) {
+				left = zero8;
+			}
+			if (!ok_r) {
+				right = zero8;
+			}
+			acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wreg[0][p], shift_lanes(centre, left, -1), acc, 0, 0, 0);
+			acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wreg[1][p], centre, acc, 0, 0, 0);
+			acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wreg[2][p], shift_lanes(centre, right, 1), acc, 0, 0, 0);
+		}
+		if (x < g.nx) {
+			const long long voxel = (static_cast<long long>(z) * g.ny + y) * g.nx + x;
+			store_outputs(out, out_bf16, relu, bias, voxel, g.cout, grp * 4, acc);
+		}
+	}
+}
+
 // One wave: 16 voxels along x at (y, z), all output channels (<= 64).
 template<int MT>
 __global__ __launch_bounds__(64 * kConvWaves) void conv3d_fwd_kernel(const __bf16* __restrict__ in,
@@ -116,21 +260,7 @@ __global__ __launch_bounds__(64 * kConvWaves) void conv3d_fwd_kernel(const __bf1
 	const long long voxel = (static_cast<long long>(z) * g.ny + y) * g.nx + x;
 #pragma unroll
 	for (int mt = 0; mt < MT; ++mt) {
-#pragma unroll
-		for (int i = 0; i < 4; ++i) {
-			const int co = mt * 16 + grp * 4 + i;
-			if (co < g.cout) {
-				float v = acc[mt][i] + (bias ? bias[co] : 0.0f);
-				if (relu) {
-					v = v > 0.0f ? v : 0.0f;
-				}
-				if (out_bf16) {
-					static_cast<__bf16*>(out)[voxel * g.cout + co] = static_cast<__bf16>(v);
-				} else {
-					static_cast<float*>(out)[voxel * g.cout + co] = v;
-				}
-			}
-		}
+		store_outputs(out, out_bf16, relu, bias, voxel, g.cout, mt * 16 + grp * 4, acc[mt]);
 	}
 }
 
@@ -513,6 +643,13 @@ int iane_hip_cnn_conv3d(const void* in, const void* weights, const float* bias, 
 		auto s = static_cast<hipStream_t>(stream);
 		const auto* x = static_cast<const __bf16*>(in);
 		const auto* w = static_cast<const __bf16*>(weights);
+		if (cin == 16 && cout <= 16) {
+			const long long waves16 = static_cast<long long>((nx + 15) / 16) * ((ny + kFwd16Rows - 1) / kFwd16Rows) * nz;
+			const unsigned blocks16 = static_cast<unsigned>((waves16 + kConvWaves - 1) / kConvWaves);
+			conv3d_fwd16_kernel<<<blocks16, 64 * kConvWaves, 0, s>>>(x, w, bias, out, out_bf16, relu, g);
+			check_hip(hipGetLastError(), "conv3d launch");
+			return;
+		}
 		switch ((cout + 15) / 16) {
 		case 1: conv3d_fwd_kernel<1><<<blocks, 64 * kConvWaves, 0, s>>>(x, w, bias, out, out_bf16, relu, g); break;
 		case 2: conv3d_fwd_kernel<2><<<blocks, 64 * kConvWaves, 0, s>>>(x, w, bias, out, out_bf16, relu, g); break;
