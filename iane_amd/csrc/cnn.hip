@@ -390,6 +390,63 @@ __global__ void pool2_backward_kernel(const float* dy, float* dx, int nx, int ny
 }
 
 // out[fine] = up2(coarse)[fine] + skip[fine] (nearest upsampling), bf16
+// Channel counts that are multiples of 8: one thread per 8 channels (16 bytes)
+// of a voxel, y and z from the grid (no 64-bit division per element); the
+// same arithmetic, in the same order, as the kernels below.
+__global__ void up2_add8_kernel(const __bf16* coarse, const __bf16* skip, __bf16* out, int nx, int ny, int c)
+{
+	const int c8 = c / 8;
+	const int t = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+	if (t >= nx * c8) {
+		return;
+	}
+	const int x = t / c8;
+	const int g = t - x * c8;
+	const int y = static_cast<int>(blockIdx.y);
+	const int z = static_cast<int>(blockIdx.z);
+	const long long fine = ((static_cast<long long>(z) * ny + y) * nx + x) * c + 8 * g;
+	const long long crs = ((static_cast<long long>(z / 2) * (ny / 2) + y / 2) * (nx / 2) + x / 2) * c + 8 * g;
+	const bf16x8 u = *reinterpret_cast<const bf16x8*>(coarse + crs);
+	const bf16x8 v = *reinterpret_cast<const bf16x8*>(skip + fine);
+	bf16x8 o;
+#pragma unroll
+	for (int j = 0; j < 8; ++j) {
+		o[j] = static_cast<__bf16>(bf2f(u[j]) + bf2f(v[j]));
+	}
+	*reinterpret_cast<bf16x8*>(out + fine) = o;
+}
+
+// FINE extents nx, ny; grid over the coarse voxels
+__global__ void pool2_8_kernel(const __bf16* in, __bf16* out, int nx, int ny, int c)
+{
+	const int ox = nx / 2, oy = ny / 2;
+	const int c8 = c / 8;
+	const int t = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+	if (t >= ox * c8) {
+		return;
+	}
+	const int x = t / c8;
+	const int g = t - x * c8;
+	const int y = static_cast<int>(blockIdx.y);
+	const int z = static_cast<int>(blockIdx.z);
+	float sum[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+	for (int d = 0; d < 8; ++d) {
+		const int xi = 2 * x + (d & 1), yi = 2 * y + ((d >> 1) & 1), zi = 2 * z + (d >> 2);
+		const bf16x8 v = *reinterpret_cast<const bf16x8*>(in + ((static_cast<long long>(zi) * ny + yi) * nx + xi) * c + 8 * g);
+#pragma unroll
+		for (int j = 0; j < 8; ++j) {
+			sum[j] += bf2f(v[j]);
+		}
+	}
+	bf16x8 o;
+#pragma unroll
+	for (int j = 0; j < 8; ++j) {
+		o[j] = static_cast<__bf16>(sum[j] * 0.125f);
+	}
+	*reinterpret_cast<bf16x8*>(out + ((static_cast<long long>(z) * oy + y) * ox + x) * c + 8 * g) = o;
+}
+
 __global__ void up2_add_kernel(const __bf16* coarse, const __bf16* skip, __bf16* out, int nx, int ny, int nz, int c)
 {
 	const long long total = static_cast<long long>(nx) * ny * nz * c;
@@ -702,8 +759,17 @@ int iane_hip_cnn_pool2(const void* in, void* out, int nx, int ny, int nz, int c,
 		if (nx % 2 || ny % 2 || nz % 2) {
 			invalid("pool2 needs even extents");
 		}
-		pool2_kernel<<<grid_for(static_cast<long long>(nx) * ny * nz * c / 8), 256, 0, static_cast<hipStream_t>(stream)>>>(
-			static_cast<const __bf16*>(in), static_cast<__bf16*>(out), nx, ny, nz, c);
+		if (c % 8 == 0 && ny / 2 <= 65535 && nz / 2 <= 65535 && static_cast<long long>(nx / 2) * (c / 8) < (1LL << 30)) {
+			const dim3 grid(static_cast<unsigned>((nx / 2 * (c / 8) + 255) / 256), static_cast<unsigned>(ny / 2),
+				static_cast<unsigned>(nz / 2));
+			if (grid.x && grid.y && grid.z) {
+				pool2_8_kernel<<<grid, 256, 0, static_cast<hipStream_t>(stream)>>>(static_cast<const __bf16*>(in),
+					static_cast<__bf16*>(out), nx, ny, c);
+			}
+		} else {
+			pool2_kernel<<<grid_for(static_cast<long long>(nx) * ny * nz * c / 8), 256, 0, static_cast<hipStream_t>(stream)>>>(
+				static_cast<const __bf16*>(in), static_cast<__bf16*>(out), nx, ny, nz, c);
+		}
 		check_hip(hipGetLastError(), "pool2 launch");
 	});
 }
@@ -722,8 +788,17 @@ int iane_hip_cnn_pool2_backward(const float* dy, float* dx, int nx, int ny, int 
 int iane_hip_cnn_up2_add(const void* coarse, const void* skip, void* out, int nx, int ny, int nz, int c, void* stream)
 {
 	return guarded([&] {
-		up2_add_kernel<<<grid_for(static_cast<long long>(nx) * ny * nz * c), 256, 0, static_cast<hipStream_t>(stream)>>>(
-			static_cast<const __bf16*>(coarse), static_cast<const __bf16*>(skip), static_cast<__bf16*>(out), nx, ny, nz, c);
+		if (c % 8 == 0 && ny <= 65535 && nz <= 65535 && static_cast<long long>(nx) * (c / 8) < (1LL << 30)) {
+			const dim3 grid(static_cast<unsigned>((nx * (c / 8) + 255) / 256), static_cast<unsigned>(ny),
+				static_cast<unsigned>(nz));
+			if (grid.x && grid.y && grid.z) {
+				up2_add8_kernel<<<grid, 256, 0, static_cast<hipStream_t>(stream)>>>(static_cast<const __bf16*>(coarse),
+					static_cast<const __bf16*>(skip), static_cast<__bf16*>(out), nx, ny, c);
+			}
+		} else {
+			up2_add_kernel<<<grid_for(static_cast<long long>(nx) * ny * nz * c), 256, 0, static_cast<hipStream_t>(stream)>>>(
+				static_cast<const __bf16*>(coarse), static_cast<const __bf16*>(skip), static_cast<__bf16*>(out), nx, ny, nz, c);
+		}
 		check_hip(hipGetLastError(), "up2 launch");
 	});
 }
