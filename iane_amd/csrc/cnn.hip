@@ -174,6 +174,51 @@ __global__ __launch_bounds__(64 * kConvWaves) void conv3d_fwd16_kernel(const __b
 	}
 }
 
+// The input layer (one input channel, at most 16 output channels): one thread
+// per voxel, the 27 neighbours loaded once (zero outside the volume) and the
+// FP32 products summed over taps into all output channels; the BF16 weights,
+// widened to FP32, are read from LDS (the same address for every lane).
+__global__ __launch_bounds__(256) void conv3d_cin1_kernel(const __bf16* __restrict__ in,
+	const __bf16* __restrict__ w, const float* __restrict__ bias, void* out, int out_bf16, int relu, conv_geom g)
+{
+	__shared__ f32x4 wl[27][4]; // [tap][output channels 4q .. 4q + 3]
+	for (int i = threadIdx.x; i < 27 * 16; i += blockDim.x) {
+		const int co = i % 16;
+		const int t = i / 16;
+		wl[t][co / 4][co % 4] = co < g.cout ? bf2f(w[co * 27 + t]) : 0.0f;
+	}
+	__syncthreads();
+	const int x = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+	const int y = static_cast<int>(blockIdx.y);
+	const int z = static_cast<int>(blockIdx.z);
+	if (x >= g.nx) {
+		return;
+	}
+	float v[27];
+#pragma unroll
+	for (int t = 0; t < 27; ++t) {
+		const int xi = x + t % 3 - 1, yi = y + (t / 3) % 3 - 1, zi = z + t / 9 - 1;
+		const bool ok = xi >= 0 && xi < g.nx && yi >= 0 && yi < g.ny && zi >= 0 && zi < g.nz;
+		const float s = bf2f(in[ok ? (static_cast<long long>(zi) * g.ny + yi) * g.nx + xi : 0]);
+		v[t] = ok ? s : 0.0f;
+	}
+	const long long voxel = (static_cast<long long>(z) * g.ny + y) * g.nx + x;
+	// one group of 4 output channels at a time (the 27 samples stay in registers)
+#pragma unroll 1
+	for (int q = 0; 4 * q < g.cout; ++q) {
+		f32x4 acc = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+		for (int t = 0; t < 27; ++t) {
+			const f32x4 k = wl[t][q];
+#pragma unroll
+			for (int i = 0; i < 4; ++i) {
+				acc[i] = fmaf(k[i], v[t], acc[i]);
+			}
+		}
+		store_outputs(out, out_bf16, relu, bias, voxel, g.cout, 4 * q, acc);
+	}
+}
+
 // One wave: 16 voxels along x at (y, z), all output channels (<= 64).
 template<int MT>
 __global__ __launch_bounds__(64 * kConvWaves) void conv3d_fwd_kernel(const __bf16* __restrict__ in,
@@ -700,6 +745,12 @@ int iane_hip_cnn_conv3d(const void* in, const void* weights, const float* bias, 
 		auto s = static_cast<hipStream_t>(stream);
 		const auto* x = static_cast<const __bf16*>(in);
 		const auto* w = static_cast<const __bf16*>(weights);
+		if (cin == 1 && cout <= 16 && ny <= 65535 && nz <= 65535) {
+			const dim3 grid(static_cast<unsigned>((nx + 255) / 256), static_cast<unsigned>(ny), static_cast<unsigned>(nz));
+			conv3d_cin1_kernel<<<grid, 256, 0, s>>>(x, w, bias, out, out_bf16, relu, g);
+			check_hip(hipGetLastError(), "conv3d launch");
+			return;
+		}
 		if (cin == 16 && cout <= 16) {
 			const long long waves16 = static_cast<long long>((nx + 15) / 16) * ((ny + kFwd16Rows - 1) / kFwd16Rows) * nz;
 			const unsigned blocks16 = static_cast<unsigned>((waves16 + kConvWaves - 1) / kConvWaves);
