@@ -308,6 +308,9 @@ def main() -> int:
     ap.add_argument("--direct-only", action="store_true",
                     help="headline plus the two streaming direct-convolution legs (u16 3^3, f32 dense 3^3)")
     ap.add_argument("--no-tune", action="store_true", help="default launch shapes (no autotuner launches)")
+    ap.add_argument("--launch-shape", default=None, metavar="WAVES,CHUNK",
+                    help="fix the separable kernel's launch shape (sep3.waves, sep3.chunk) instead of searching: "
+                         "profiles of one autotuned shape without the candidates' launches")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also copy the 256^3 volume once (device to device, 64 MiB read + 64 MiB written): "
                          "the reference dispatch for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE")
@@ -343,10 +346,25 @@ def main() -> int:
         iane_amd.convolve_separable(vol, [taps, taps, taps], out, opts, policy)
 
     from iane_amd import autotune
-    tuned = {} if args.no_tune else autotune.tune_separable(vol, [taps, taps, taps], out, opts, policy)
+    if args.launch_shape:
+        waves, chunk = (int(v) for v in args.launch_shape.split(","))
+        tuned = {"sep3.waves": waves, "sep3.chunk": chunk}
+        for key, value in tuned.items():
+            autotune.set(key, value)
+    else:
+        tuned = {} if args.no_tune else autotune.tune_separable(vol, [taps, taps, taps], out, opts, policy)
     log(f"[bench] autotuned launch shape: {tuned}")
 
-    log(f"[bench] separable 15^3 fp32: warmup {args.warmup}")
+    # Steady clocks: the GPU needs tens of milliseconds of sustained work before
+    # its clocks settle (measured on this kernel: 74 us per launch after 5 launches,
+    # 68 us after 300). The autotuner's candidate launches usually provide that; a
+    # cached or fixed launch shape does not, so run the step untimed for at least
+    # 30 ms of GPU time first. The timed region below is still exactly `steps` steps.
+    settle_ms, settle_steps = 0.0, 0
+    while settle_ms < 30.0 and settle_steps < 2000:
+        settle_ms += timer.time(step, 50)
+        settle_steps += 50
+    log(f"[bench] separable 15^3 fp32: {settle_steps} settling steps ({settle_ms:.1f} ms), warmup {args.warmup}")
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
