@@ -30,18 +30,28 @@ struct conv_geom {
 
 __device__ __forceinline__ float bf2f(__bf16 v) { return static_cast<float>(v); }
 
+// The biases of output channels co0 .. co0 + 3 (0 for a missing bias or channel).
+__device__ __forceinline__ f32x4 bias4(const float* bias, int cout, int co0)
+{
+	f32x4 b;
+#pragma unroll
+	for (int i = 0; i < 4; ++i) {
+		b[i] = (bias && co0 + i < cout) ? bias[co0 + i] : 0.0f;
+	}
+	return b;
+}
+
 // Write a lane's 4 consecutive output channels co0 .. co0 + 3 of one voxel
-// (bias, optional ReLU): one 8-byte (BF16) or 16-byte (FP32) store when all 4
-// exist and cout is a multiple of 4, so a wave writes whole voxels contiguously;
-// channel by channel otherwise.
-__device__ __forceinline__ void store_outputs(void* out, int out_bf16, int relu, const float* bias, long long voxel,
+// (sum + bias, optional ReLU): one 8-byte (BF16) or 16-byte (FP32) store when
+// all 4 exist and cout is a multiple of 4, so a wave writes whole voxels
+// contiguously; channel by channel otherwise.
+__device__ __forceinline__ void store_outputs(void* out, int out_bf16, int relu, f32x4 bias, long long voxel,
 	int cout, int co0, f32x4 acc)
 {
 	float v[4];
 #pragma unroll
 	for (int i = 0; i < 4; ++i) {
-		const int co = co0 + i;
-		float t = acc[i] + ((bias && co < cout) ? bias[co] : 0.0f);
+		const float t = acc[i] + bias[i];
 		v[i] = relu ? (t > 0.0f ? t : 0.0f) : t;
 	}
 	const long long at = voxel * cout + co0;
@@ -77,12 +87,13 @@ __device__ __forceinline__ void store_outputs(void* out, int out_bf16, int relu,
 // K = 3 shifts x 5 row pairs x 32 (two (dz, dy) rows of 16 channels; the tenth
 // row is zero padding): chunk (dx, p) holds rows 2p (lane groups 0, 1) and
 // 2p + 1 (groups 2, 3), channels 8 (g % 2) .. + 7, all at x shift dx. A lane
-// loads voxel x0 + n of its row and the two voxels just outside the wave's 16
-// (x0 - 1, x0 + 16, the same address for the 16 lanes of a group); the x - 1
-// and x + 1 operands are the centre values moved one lane within each group of
-// 16 (DPP row shifts), the end lane taking the outside voxel. The weights of
-// all 15 chunks stay in registers, loaded once for kFwd16Rows consecutive y
-// rows that the wave computes in turn.
+// loads voxel x0 + n of its row; lane 0 of each group of 16 also loads voxel
+// x0 - 1 and lane 15 voxel x0 + 16 (the "edge" value, zero elsewhere). The
+// x - 1 and x + 1 operands are the centre values moved one lane within each
+// group of 16 (DPP row shifts), the lane left without a source (0 or 15)
+// taking its edge value. All ten loads of a row are issued before its first
+// MFMA. The weights of all 15 chunks stay in registers, loaded once for
+// kFwd16Rows consecutive y rows that the wave computes in turn.
 __device__ __forceinline__ bf16x8 shift_lanes(bf16x8 aCentre, bf16x8 aEdge, int aDx)
 {
 	using i32x4 = __attribute__((__vector_size__(4 * sizeof(int)))) int;
@@ -100,6 +111,7 @@ __device__ __forceinline__ bf16x8 shift_lanes(bf16x8 aCentre, bf16x8 aEdge, int 
 }
 
 constexpr int kFwd16Rows = 8;
+constexpr int kOobOffset = 0x7ffffff0; // past every buffer the 16-channel kernel addresses
 
 __global__ __launch_bounds__(64 * kConvWaves) void conv3d_fwd16_kernel(const __bf16* __restrict__ in,
 	const __bf16* __restrict__ w, const float* __restrict__ bias, void* out, int out_bf16, int relu, conv_geom g)
@@ -139,37 +151,41 @@ __global__ __launch_bounds__(64 * kConvWaves) void conv3d_fwd16_kernel(const __b
 			}
 		}
 	}
+	// the whole input volume as one buffer resource (the launch checks that its
+	// byte size fits the 32-bit offsets)
+	const auto volume = __builtin_amdgcn_make_buffer_rsrc(const_cast<__bf16*>(in), static_cast<short>(0),
+		g.nx * g.ny * g.nz * kCin * 2, 0x00020000);
+	// the edge voxel of lanes 0 and 15 of each group (x0 - 1, x0 + 16), if inside
+	const int edge_x = n == 0 ? x0 - 1 : (n == 15 ? x0 + 16 : -1);
+	const bool edge_ok = edge_x >= 0 && edge_x < g.nx;
+	const f32x4 bv = bias4(bias, g.cout, grp * 4);
 	for (int y = y0; y < min(y0 + kFwd16Rows, g.ny); ++y) {
-		f32x4 acc = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+		bf16x8 centre[5], edge[5];
 #pragma unroll
 		for (int p = 0; p < 5; ++p) {
 			const int r = 2 * p + (grp >> 1);
 			const int yi = y + (r % 3) - 1;
 			const int zi = z + (r / 3) - 1;
 			const bool row_ok = r < 9 && yi >= 0 && yi < g.ny && zi >= 0 && zi < g.nz;
-			const long long row = (static_cast<long long>(zi) * g.ny + yi) * g.nx;
-			const bool ok_c = row_ok && x < g.nx;
-			const bool ok_l = row_ok && x0 > 0;
-			const bool ok_r = row_ok && x0 + 16 < g.nx;
-			bf16x8 centre = *reinterpret_cast<const bf16x8*>(in + (ok_c ? (row + x) * kCin + ch : 0));
-			bf16x8 left = *reinterpret_cast<const bf16x8*>(in + (ok_l ? (row + x0 - 1) * kCin + ch : 0));
-			bf16x8 right = *reinterpret_cast<const bf16x8*>(in + (ok_r ? (row + x0 + 16) * kCin + ch : 0));
-			if (!ok_c) {
-				centre = zero8;
-			}
-			if (!ok_l) {
-				left = zero8;
-			}
-			if (!ok_r) {
-				right = zero8;
-			}
-			acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wreg[0][p], shift_lanes(centre, left, -1), acc, 0, 0, 0);
-			acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wreg[1][p], centre, acc, 0, 0, 0);
-			acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wreg[2][p], shift_lanes(centre, right, 1), acc, 0, 0, 0);
+			// byte offsets into the volume's buffer resource; past its end reads 0
+			const int row = ((zi * g.ny + yi) * g.nx) * (kCin * 2) + ch * 2;
+			const int centre_at = row_ok && x < g.nx ? row + x * (kCin * 2) : kOobOffset;
+			const int edge_at = row_ok && edge_ok ? row + edge_x * (kCin * 2) : kOobOffset;
+			centre[p] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(volume, centre_at, 0, 0));
+			edge[p] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(volume, edge_at, 0, 0));
+		}
+		// keep the loads ahead of the MFMAs (each waits only for its own operands)
+		__builtin_amdgcn_sched_barrier(0);
+		f32x4 acc = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+		for (int p = 0; p < 5; ++p) {
+			acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wreg[0][p], shift_lanes(centre[p], edge[p], -1), acc, 0, 0, 0);
+			acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wreg[1][p], centre[p], acc, 0, 0, 0);
+			acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wreg[2][p], shift_lanes(centre[p], edge[p], 1), acc, 0, 0, 0);
 		}
 		if (x < g.nx) {
 			const long long voxel = (static_cast<long long>(z) * g.ny + y) * g.nx + x;
-			store_outputs(out, out_bf16, relu, bias, voxel, g.cout, grp * 4, acc);
+			store_outputs(out, out_bf16, relu, bv, voxel, g.cout, grp * 4, acc);
 		}
 	}
 }
@@ -215,7 +231,7 @@ __global__ __launch_bounds__(256) void conv3d_cin1_kernel(const __bf16* __restri
 				acc[i] = fmaf(k[i], v[t], acc[i]);
 			}
 		}
-		store_outputs(out, out_bf16, relu, bias, voxel, g.cout, 4 * q, acc);
+		store_outputs(out, out_bf16, relu, bias4(bias, g.cout, 4 * q), voxel, g.cout, 4 * q, acc);
 	}
 }
 
@@ -305,7 +321,7 @@ __global__ __launch_bounds__(64 * kConvWaves) void conv3d_fwd_kernel(const __bf1
 	const long long voxel = (static_cast<long long>(z) * g.ny + y) * g.nx + x;
 #pragma unroll
 	for (int mt = 0; mt < MT; ++mt) {
-		store_outputs(out, out_bf16, relu, bias, voxel, g.cout, mt * 16 + grp * 4, acc[mt]);
+		store_outputs(out, out_bf16, relu, bias4(bias, g.cout, mt * 16 + grp * 4), voxel, g.cout, mt * 16 + grp * 4, acc[mt]);
 	}
 }
 
@@ -751,7 +767,7 @@ int iane_hip_cnn_conv3d(const void* in, const void* weights, const float* bias, 
 			check_hip(hipGetLastError(), "conv3d launch");
 			return;
 		}
-		if (cin == 16 && cout <= 16) {
+		if (cin == 16 && cout <= 16 && static_cast<long long>(nx) * ny * nz * 32 < kOobOffset - 64) {
 			const long long waves16 = static_cast<long long>((nx + 15) / 16) * ((ny + kFwd16Rows - 1) / kFwd16Rows) * nz;
 			const unsigned blocks16 = static_cast<unsigned>((waves16 + kConvWaves - 1) / kConvWaves);
 			conv3d_fwd16_kernel<<<blocks16, 64 * kConvWaves, 0, s>>>(x, w, bias, out, out_bf16, relu, g);
