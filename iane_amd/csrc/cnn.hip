@@ -41,6 +41,23 @@ __device__ __forceinline__ f32x4 bias4(const float* bias, int cout, int co0)
 	return b;
 }
 
+// bias4 for co0 = 4 grp with cout <= 16, read as 16 uniform (scalar) loads and
+// selected by lane group, so that no vector-memory wait is ever needed for it.
+__device__ __forceinline__ f32x4 bias4_uniform(const float* __restrict__ bias, int cout, int grp)
+{
+	float all[16];
+#pragma unroll
+	for (int co = 0; co < 16; ++co) {
+		all[co] = (bias && co < cout) ? bias[co] : 0.0f;
+	}
+	f32x4 b;
+#pragma unroll
+	for (int i = 0; i < 4; ++i) {
+		b[i] = grp == 0 ? all[i] : (grp == 1 ? all[4 + i] : (grp == 2 ? all[8 + i] : all[12 + i]));
+	}
+	return b;
+}
+
 // Write a lane's 4 consecutive output channels co0 .. co0 + 3 of one voxel
 // (sum + bias, optional ReLU): one 8-byte (BF16) or 16-byte (FP32) store when
 // all 4 exist and cout is a multiple of 4, so a wave writes whole voxels
@@ -111,7 +128,9 @@ __device__ __forceinline__ bf16x8 shift_lanes(bf16x8 aCentre, bf16x8 aEdge, int 
 }
 
 constexpr int kFwd16Rows = 8;
-constexpr int kOobOffset = 0x7ffffff0; // past every buffer the 16-channel kernel addresses
+// A byte offset past the end of every buffer resource made here (the launches
+// check that the buffers are smaller): a buffer load there returns 0.
+constexpr int kOobOffset = 0x7ffffff0;
 
 __global__ __launch_bounds__(64 * kConvWaves) void conv3d_fwd16_kernel(const __bf16* __restrict__ in,
 	const __bf16* __restrict__ w, const float* __restrict__ bias, void* out, int out_bf16, int relu, conv_geom g)
@@ -190,55 +209,84 @@ __global__ __launch_bounds__(64 * kConvWaves) void conv3d_fwd16_kernel(const __b
 	}
 }
 
-// The input layer (one input channel, at most 16 output channels): one thread
-// per voxel, the 27 neighbours loaded once (zero outside the volume) and the
-// FP32 products summed over taps into all output channels; the BF16 weights,
-// widened to FP32, are read from LDS (the same address for every lane).
-__global__ __launch_bounds__(256) void conv3d_cin1_kernel(const __bf16* __restrict__ in,
+// The input layer (one input channel, at most 16 output channels) on the same
+// MFMA: K = the 27 taps (zero-padded to 32), so lane group g holds taps
+// 8g .. 8g + 7 of its voxel x0 + n and the weights of output channel n for
+// those taps; one MFMA per row gives all output channels of 16 voxels. A wave
+// computes kCin1Rows consecutive y rows and issues the samples of all of them
+// (one 2-byte buffer load per tap, zero outside the volume) before its first
+// MFMA.
+constexpr int kCin1Rows = 8;
+
+__global__ __launch_bounds__(64 * kConvWaves) void conv3d_cin1_kernel(const __bf16* __restrict__ in,
 	const __bf16* __restrict__ w, const float* __restrict__ bias, void* out, int out_bf16, int relu, conv_geom g)
 {
-	__shared__ f32x4 wl[27][4]; // [tap][output channels 4q .. 4q + 3]
-	for (int i = threadIdx.x; i < 27 * 16; i += blockDim.x) {
-		const int co = i % 16;
-		const int t = i / 16;
-		wl[t][co / 4][co % 4] = co < g.cout ? bf2f(w[co * 27 + t]) : 0.0f;
-	}
-	__syncthreads();
-	const int x = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-	const int y = static_cast<int>(blockIdx.y);
-	const int z = static_cast<int>(blockIdx.z);
-	if (x >= g.nx) {
+	const int lane = threadIdx.x & 63;
+	const long long wave = static_cast<long long>(blockIdx.x) * kConvWaves + (threadIdx.x >> 6);
+	const int xt = (g.nx + 15) / 16;
+	const int yb = (g.ny + kCin1Rows - 1) / kCin1Rows;
+	if (wave >= static_cast<long long>(xt) * yb * g.nz) {
 		return;
 	}
-	float v[27];
+	const int x0 = static_cast<int>(wave % xt) * 16;
+	const int y0 = static_cast<int>((wave / xt) % yb) * kCin1Rows;
+	const int z = static_cast<int>(wave / (static_cast<long long>(xt) * yb));
+	const int n = lane & 15;
+	const int grp = lane >> 4;
+	const int x = x0 + n;
+	const f32x4 bv = bias4_uniform(bias, g.cout, grp);
+	bf16x8 a;
 #pragma unroll
-	for (int t = 0; t < 27; ++t) {
-		const int xi = x + t % 3 - 1, yi = y + (t / 3) % 3 - 1, zi = z + t / 9 - 1;
-		const bool ok = xi >= 0 && xi < g.nx && yi >= 0 && yi < g.ny && zi >= 0 && zi < g.nz;
-		const float s = bf2f(in[ok ? (static_cast<long long>(zi) * g.ny + yi) * g.nx + xi : 0]);
-		v[t] = ok ? s : 0.0f;
+	for (int j = 0; j < 8; ++j) {
+		const int t = grp * 8 + j;
+		a[j] = n < g.cout && t < 27 ? w[n * 27 + t] : static_cast<__bf16>(0.0f);
 	}
-	const long long voxel = (static_cast<long long>(z) * g.ny + y) * g.nx + x;
-	// one group of 4 output channels at a time (the 27 samples stay in registers)
-#pragma unroll 1
-	for (int q = 0; 4 * q < g.cout; ++q) {
-		f32x4 acc = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+	const auto volume = __builtin_amdgcn_make_buffer_rsrc(const_cast<__bf16*>(in), static_cast<short>(0),
+		g.nx * g.ny * g.nz * 2, 0x00020000);
+	// samples[row][j]: tap 8 grp + j of voxel (x, y0 + row, z), one per 32-bit register
+	unsigned samples[kCin1Rows][8];
 #pragma unroll
-		for (int t = 0; t < 27; ++t) {
-			const f32x4 k = wl[t][q];
+	for (int r = 0; r < kCin1Rows; ++r) {
 #pragma unroll
-			for (int i = 0; i < 4; ++i) {
-				acc[i] = fmaf(k[i], v[t], acc[i]);
-			}
+		for (int j = 0; j < 8; ++j) {
+			const int t = grp * 8 + j;
+			const int xi = x + t % 3 - 1, yi = y0 + r + (t / 3) % 3 - 1, zi = z + t / 9 - 1;
+			const bool ok = t < 27 && x < g.nx && y0 + r < g.ny && xi >= 0 && xi < g.nx && yi >= 0 && yi < g.ny
+				&& zi >= 0 && zi < g.nz;
+			const int at = ok ? ((zi * g.ny + yi) * g.nx + xi) * 2 : kOobOffset;
+			samples[r][j] = __builtin_amdgcn_raw_buffer_load_b16(volume, at, 0, 0);
 		}
-		store_outputs(out, out_bf16, relu, bias4(bias, g.cout, 4 * q), voxel, g.cout, 4 * q, acc);
+	}
+	// keep the loads ahead of the MFMAs
+	__builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+	for (int r = 0; r < kCin1Rows; ++r) {
+		bf16x8 b;
+#pragma unroll
+		for (int j = 0; j < 8; ++j) {
+			b[j] = __builtin_bit_cast(__bf16, static_cast<unsigned short>(samples[r][j]));
+		}
+		const f32x4 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, f32x4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
+		if (x < g.nx && y0 + r < g.ny) {
+			const long long voxel = (static_cast<long long>(z) * g.ny + y0 + r) * g.nx + x;
+			store_outputs(out, out_bf16, relu, bv, voxel, g.cout, grp * 4, acc);
+		}
 	}
 }
 
 // One wave: 16 voxels along x at (y, z), all output channels (<= 64).
+// When cin is a multiple of 8 and the input and weights fit 32-bit byte
+// offsets (grouped != 0), K is walked kFwdGroup<MT> chunks at a time with all
+// of a group's loads issued before its MFMAs (buffer loads: an offset past the
+// end reads 0, so padding and out-of-volume taps need no branch); the chunks
+// left over, and other cin, take one chunk at a time.
+template<int MT>
+constexpr int kFwdGroup = MT == 1 ? 8 : (MT == 2 ? 6 : 4);
+
 template<int MT>
 __global__ __launch_bounds__(64 * kConvWaves) void conv3d_fwd_kernel(const __bf16* __restrict__ in,
-	const __bf16* __restrict__ w, const float* __restrict__ bias, void* out, int out_bf16, int relu, conv_geom g)
+	const __bf16* __restrict__ w, const float* __restrict__ bias, void* out, int out_bf16, int relu, conv_geom g,
+	int grouped)
 {
 	const int lane = threadIdx.x & 63;
 	const long long wave = static_cast<long long>(blockIdx.x) * kConvWaves + (threadIdx.x >> 6);
@@ -261,7 +309,45 @@ __global__ __launch_bounds__(64 * kConvWaves) void conv3d_fwd_kernel(const __bf1
 		acc[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 	}
 	const bool vec = (g.cin & 7) == 0;
-	for (int kc = 0; kc < chunks; ++kc) {
+	int kc = 0;
+	if (grouped) {
+		constexpr int U = kFwdGroup<MT>;
+		const auto volume = __builtin_amdgcn_make_buffer_rsrc(const_cast<__bf16*>(in), static_cast<short>(0),
+			g.nx * g.ny * g.nz * g.cin * 2, 0x00020000);
+		const auto weights = __builtin_amdgcn_make_buffer_rsrc(const_cast<__bf16*>(w), static_cast<short>(0),
+			g.cout * K * 2, 0x00020000);
+		for (; kc + U <= chunks; kc += U) {
+			bf16x8 b[U];
+			bf16x8 a[U][MT];
+#pragma unroll
+			for (int u = 0; u < U; ++u) {
+				const int k0 = (kc + u) * 32 + grp * 8;
+				// 8 consecutive input channels of one tap (cin is a multiple of 8)
+				const int t = k0 / g.cin;
+				const int c = k0 - t * g.cin;
+				const int xi = x + t % 3 - 1, yi = y + (t / 3) % 3 - 1, zi = z + t / 9 - 1;
+				const bool ok = t < 27 && x < g.nx && xi >= 0 && xi < g.nx && yi >= 0 && yi < g.ny && zi >= 0 && zi < g.nz;
+				const int at = ok ? (((zi * g.ny + yi) * g.nx + xi) * g.cin + c) * 2 : kOobOffset;
+				b[u] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(volume, at, 0, 0));
+#pragma unroll
+				for (int mt = 0; mt < MT; ++mt) {
+					const int m = mt * 16 + n;
+					const int wat = m < g.cout && t < 27 ? (m * K + k0) * 2 : kOobOffset;
+					a[u][mt] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(weights, wat, 0, 0));
+				}
+			}
+			// keep the group's loads ahead of its MFMAs
+			__builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+			for (int u = 0; u < U; ++u) {
+#pragma unroll
+				for (int mt = 0; mt < MT; ++mt) {
+					acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[u][mt], b[u], acc[mt], 0, 0, 0);
+				}
+			}
+		}
+	}
+	for (; kc < chunks; ++kc) {
 		const int k0 = kc * 32 + grp * 8;
 		bf16x8 b;
 		if (vec) {
@@ -752,18 +838,16 @@ int iane_hip_cnn_conv3d(const void* in, const void* weights, const float* bias, 
 		if (nx <= 0 || ny <= 0 || nz <= 0 || cin <= 0 || cout <= 0 || cout > 64 || cin > 256) {
 			invalid("conv3d extents or channel counts out of range (cout <= 64, cin <= 256)");
 		}
-		if ((cin & 7) && cin != 1 && cin % 8) {
-			// any cin works through the scalar gather path
-		}
 		const conv_geom g{nx, ny, nz, cin, cout};
 		const long long waves = static_cast<long long>((nx + 15) / 16) * ny * nz;
 		const unsigned blocks = static_cast<unsigned>((waves + kConvWaves - 1) / kConvWaves);
 		auto s = static_cast<hipStream_t>(stream);
 		const auto* x = static_cast<const __bf16*>(in);
 		const auto* w = static_cast<const __bf16*>(weights);
-		if (cin == 1 && cout <= 16 && ny <= 65535 && nz <= 65535) {
-			const dim3 grid(static_cast<unsigned>((nx + 255) / 256), static_cast<unsigned>(ny), static_cast<unsigned>(nz));
-			conv3d_cin1_kernel<<<grid, 256, 0, s>>>(x, w, bias, out, out_bf16, relu, g);
+		if (cin == 1 && cout <= 16 && static_cast<long long>(nx) * ny * nz * 2 < kOobOffset - 64) {
+			const long long waves1 = static_cast<long long>((nx + 15) / 16) * ((ny + kCin1Rows - 1) / kCin1Rows) * nz;
+			const unsigned blocks1 = static_cast<unsigned>((waves1 + kConvWaves - 1) / kConvWaves);
+			conv3d_cin1_kernel<<<blocks1, 64 * kConvWaves, 0, s>>>(x, w, bias, out, out_bf16, relu, g);
 			check_hip(hipGetLastError(), "conv3d launch");
 			return;
 		}
@@ -774,11 +858,13 @@ int iane_hip_cnn_conv3d(const void* in, const void* weights, const float* bias, 
 			check_hip(hipGetLastError(), "conv3d launch");
 			return;
 		}
+		// grouped buffer loads need cin % 8 == 0 and byte sizes below the out-of-range offset
+		const int grouped = cin % 8 == 0 && static_cast<long long>(nx) * ny * nz * cin * 2 < kOobOffset - 64 ? 1 : 0;
 		switch ((cout + 15) / 16) {
-		case 1: conv3d_fwd_kernel<1><<<blocks, 64 * kConvWaves, 0, s>>>(x, w, bias, out, out_bf16, relu, g); break;
-		case 2: conv3d_fwd_kernel<2><<<blocks, 64 * kConvWaves, 0, s>>>(x, w, bias, out, out_bf16, relu, g); break;
-		case 3: conv3d_fwd_kernel<3><<<blocks, 64 * kConvWaves, 0, s>>>(x, w, bias, out, out_bf16, relu, g); break;
-		default: conv3d_fwd_kernel<4><<<blocks, 64 * kConvWaves, 0, s>>>(x, w, bias, out, out_bf16, relu, g); break;
+		case 1: conv3d_fwd_kernel<1><<<blocks, 64 * kConvWaves, 0, s>>>(x, w, bias, out, out_bf16, relu, g, grouped); break;
+		case 2: conv3d_fwd_kernel<2><<<blocks, 64 * kConvWaves, 0, s>>>(x, w, bias, out, out_bf16, relu, g, grouped); break;
+		case 3: conv3d_fwd_kernel<3><<<blocks, 64 * kConvWaves, 0, s>>>(x, w, bias, out, out_bf16, relu, g, grouped); break;
+		default: conv3d_fwd_kernel<4><<<blocks, 64 * kConvWaves, 0, s>>>(x, w, bias, out, out_bf16, relu, g, grouped); break;
 		}
 		check_hip(hipGetLastError(), "conv3d launch");
 	});

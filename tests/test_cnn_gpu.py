@@ -37,7 +37,11 @@ def rel(got, ref):
     return float(np.max(np.abs(got - ref)) / max(np.max(np.abs(ref)), 1e-30))
 
 
-@pytest.mark.parametrize("cin,cout", [(1, 16), (16, 16), (16, 32), (32, 64), (64, 32), (32, 16), (16, 1), (8, 3)])
+# (16, 32), (32, 64), (32, 16), (24, 40): whole load groups plus leftover K chunks;
+# (64, 32): whole groups only; (8, 3), (8, 48): fewer chunks than one group;
+# (12, 20): cin not a multiple of 8 (one chunk at a time)
+@pytest.mark.parametrize("cin,cout", [(1, 16), (16, 16), (16, 32), (32, 64), (64, 32), (32, 16), (16, 1), (8, 3),
+                                      (24, 40), (8, 48), (12, 20)])
 def test_conv3d_layer(cin, cout):
     import torch
     rng = np.random.default_rng(cin * 100 + cout)
