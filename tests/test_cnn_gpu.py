@@ -39,9 +39,10 @@ def rel(got, ref):
 
 # (16, 32), (32, 64), (32, 16), (24, 40): whole load groups plus leftover K chunks;
 # (64, 32): whole groups only; (8, 3), (8, 48): fewer chunks than one group;
-# (12, 20): cin not a multiple of 8 (one chunk at a time)
+# (12, 20): cin not a multiple of 8 (one chunk at a time); (1, 5): input layer
+# with a partial channel group (channel-by-channel stores)
 @pytest.mark.parametrize("cin,cout", [(1, 16), (16, 16), (16, 32), (32, 64), (64, 32), (32, 16), (16, 1), (8, 3),
-                                      (24, 40), (8, 48), (12, 20)])
+                                      (24, 40), (8, 48), (12, 20), (1, 5)])
 def test_conv3d_layer(cin, cout):
     import torch
     rng = np.random.default_rng(cin * 100 + cout)
