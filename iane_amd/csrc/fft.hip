@@ -12,6 +12,8 @@
 // fft_pass_kernel (lines of the axis held in LDS, all radix stages in one kernel).
 #include "fft.hpp"
 
+#include <atomic>
+
 #include <map>
 #include <memory>
 #include <mutex>
@@ -106,6 +108,27 @@ __global__ __launch_bounds__(kFftThreads) void fft_pass_kernel(fft_pass_args a)
 
 } // namespace
 
+// LDS bytes one workgroup may use on the current device (cached per device).
+size_t max_lds_per_block()
+{
+	int device = 0;
+	check_hip(hipGetDevice(&device), "hipGetDevice");
+	static std::atomic<int> cache[64];
+	if (device >= 0 && device < 64) {
+		const int cached = cache[device].load(std::memory_order_relaxed);
+		if (cached > 0) {
+			return static_cast<size_t>(cached);
+		}
+	}
+	int bytes = 0;
+	check_hip(hipDeviceGetAttribute(&bytes, hipDeviceAttributeMaxSharedMemoryPerBlock, device), "hipDeviceGetAttribute");
+	bytes = std::max(bytes, 1);
+	if (device >= 0 && device < 64) {
+		cache[device].store(bytes, std::memory_order_relaxed);
+	}
+	return static_cast<size_t>(bytes);
+}
+
 // Fill the radix plan, tables, LDS pitch and bundle of a pass for a transform
 // of logical length n along lines of `lines` x `outer`.
 void setup_pass(fft_pass_args& a, int aMode, int aN, int aLines, int aOuter)
@@ -130,7 +153,19 @@ void setup_pass(fft_pass_args& a, int aMode, int aN, int aLines, int aOuter)
 	// less LDS per workgroup and more resident workgroups per CU. Every line is
 	// transformed the same way whatever the bundle, so results do not depend on it.
 	const int budget = std::clamp(tune_get("fft.lds_elems", kFftLdsElems), a.pitch, kFftLdsElems);
-	a.bundle = std::max(1, std::min(aLines, budget / a.pitch));
+	// the W_m table (m elements) comes out of the same budget of 2 * budget
+	// elements, so the workgroups resident per CU do not change with it
+	a.bundle = std::max(1, std::min(aLines, (2 * budget - m) / (2 * a.pitch)));
+	// the W_m table follows the two line buffers in LDS: fewer lines if the
+	// workgroup would exceed the device's LDS (1 KiB kept for the kernels'
+	// static LDS: batched descriptors, reduction slots)
+	const long long limit = (static_cast<long long>(max_lds_per_block()) - 1024) / static_cast<long long>(sizeof(float2));
+	while (a.bundle > 1 && 2LL * a.bundle * a.pitch + m > limit) {
+		--a.bundle;
+	}
+	if (2LL * a.bundle * a.pitch + m > limit) {
+		fail(IANE_ERR_UNSUPPORTED, "FFT line length " + std::to_string(aN) + " exceeds the LDS line limit");
+	}
 	a.lds_elems = a.bundle * a.pitch;
 	a.groups_per_outer = ceil_div(aLines, a.bundle);
 }

@@ -546,24 +546,68 @@ __device__ inline void store_real_lines(float* out, const line_geom& g, int aOut
 		});
 }
 
-// Dynamic LDS of a launch running pass `a`: two buffers of a.lds_elems.
-inline size_t pass_lds_bytes(const fft_pass_args& a) { return 2 * sizeof(float2) * static_cast<size_t>(a.lds_elems); }
+// Dynamic LDS of a launch running pass `a`: two buffers of a.lds_elems, then
+// the W_m twiddle table (a.plan.m elements).
+inline size_t pass_lds_bytes(const fft_pass_args& a)
+{
+	return sizeof(float2) * (2 * static_cast<size_t>(a.lds_elems) + static_cast<size_t>(a.plan.m));
+}
+
+// The W_m table is copied into LDS after the two line buffers; the stages read
+// their twiddles from there (an LDS read instead of a global one per butterfly
+// input). fetch_twiddles loads this thread's first two entries (all of a table
+// of up to 2 * blockDim.x) before the line loads are issued, store_twiddles
+// writes them (and any further entries) once the lines are loaded, so the
+// table costs no extra memory round trip; the caller's next barrier, before
+// the first stage, publishes it.
+struct twiddle_fetch {
+	float2 v[2];
+};
+
+__device__ __forceinline__ twiddle_fetch fetch_twiddles(const float2* aTable, int aM)
+{
+	twiddle_fetch f;
+#pragma unroll
+	for (int u = 0; u < 2; ++u) {
+		f.v[u] = aTable[min(static_cast<int>(threadIdx.x + u * blockDim.x), aM - 1)];
+	}
+	return f;
+}
+
+__device__ __forceinline__ const float2* store_twiddles(const twiddle_fetch& aFetched, const float2* aTable, int aM,
+	float2* aLds)
+{
+#pragma unroll
+	for (int u = 0; u < 2; ++u) {
+		const int i = static_cast<int>(threadIdx.x + u * blockDim.x);
+		if (i < aM) {
+			aLds[i] = aFetched.v[u];
+		}
+	}
+	for (int i = static_cast<int>(threadIdx.x + 2 * blockDim.x); i < aM; i += blockDim.x) {
+		aLds[i] = aTable[i];
+	}
+	return aLds;
+}
 
 // One workgroup's share of a line pass: the lines of group `aBlock` of `a`.
-// `lds` holds two buffers of a.lds_elems complex elements.
+// `lds` holds two buffers of a.lds_elems complex elements and the W_m table.
 template<int MODE>
 __device__ __forceinline__ void fft_pass_body(const fft_pass_args& a, unsigned aBlock, float2* lds)
 {
 	float2* A = lds;
 	float2* B = lds + a.lds_elems;
-	const float2* tw_m = a.tw_m;
+	const int m = a.plan.m;
+	const twiddle_fetch fetched = fetch_twiddles(a.tw_m, m);
+	float2* const tw_lds = lds + 2 * a.lds_elems;
+	const float2* tw_m = tw_lds;
 	const float2* tw_n = a.tw_n;
 	int o, l0, count;
 	decode_group(a, aBlock, o, l0, count);
-	const int m = a.plan.m;
 	const int n = a.n;
 	if constexpr (MODE == PASS_C2C) {
 		load_complex_lines(static_cast<const float2*>(a.in), a.gin, o, l0, count, a.load_n, m, A, a.pitch);
+		store_twiddles(fetched, a.tw_m, m, tw_lds);
 		__syncthreads();
 		// a constant sign in each copy folds the conjugations and (sign i) rotations
 		const float2* r = a.sign < 0.0f ? run_stages(a.plan, A, B, count, a.pitch, tw_m, -1.0f)
@@ -571,6 +615,7 @@ __device__ __forceinline__ void fft_pass_body(const fft_pass_args& a, unsigned a
 		store_complex_lines(static_cast<float2*>(a.out), a.gout, o, l0, count, a.store_n, r, a.pitch);
 	} else if constexpr (MODE == PASS_C2C_MULT) {
 		load_complex_lines(static_cast<const float2*>(a.in), a.gin, o, l0, count, a.load_n, m, A, a.pitch);
+		store_twiddles(fetched, a.tw_m, m, tw_lds);
 		__syncthreads();
 		float2* r = run_stages(a.plan, A, B, count, a.pitch, tw_m, -1.0f);
 		const long long hbase = o * a.gmult.outer + l0 * a.gmult.line;
@@ -602,6 +647,7 @@ __device__ __forceinline__ void fft_pass_body(const fft_pass_args& a, unsigned a
 		store_complex_lines(static_cast<float2*>(a.out), a.gout, o, l0, count, a.store_n, q, a.pitch);
 	} else if constexpr (MODE == PASS_R2C) {
 		load_real_lines(static_cast<const float*>(a.in), a.gin, o, l0, count, n, a.load_n, A, a.pitch);
+		store_twiddles(fetched, a.tw_m, m, tw_lds);
 		__syncthreads();
 		float2* r = run_stages(a.plan, A, B, count, a.pitch, tw_m, -1.0f);
 		if ((n & 1) == 0) {
@@ -614,6 +660,7 @@ __device__ __forceinline__ void fft_pass_body(const fft_pass_args& a, unsigned a
 	} else { // PASS_C2R
 		if ((n & 1) == 0) {
 			load_complex_lines(static_cast<const float2*>(a.in), a.gin, o, l0, count, a.load_n, m + 1, A, a.pitch);
+			store_twiddles(fetched, a.tw_m, m, tw_lds);
 			__syncthreads();
 			c2r_pre(A, B, count, n, a.pitch, tw_n);
 			__syncthreads();
@@ -621,6 +668,7 @@ __device__ __forceinline__ void fft_pass_body(const fft_pass_args& a, unsigned a
 			store_real_lines(static_cast<float*>(a.out), a.gout, o, l0, count, n, a.store_n, r, a.pitch);
 		} else {
 			load_complex_lines(static_cast<const float2*>(a.in), a.gin, o, l0, count, a.load_n, m, A, a.pitch);
+			store_twiddles(fetched, a.tw_m, m, tw_lds);
 			__syncthreads();
 			c2r_extend_odd(A, count, n, a.pitch);
 			__syncthreads();
