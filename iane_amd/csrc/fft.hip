@@ -153,17 +153,18 @@ void setup_pass(fft_pass_args& a, int aMode, int aN, int aLines, int aOuter)
 	// less LDS per workgroup and more resident workgroups per CU. Every line is
 	// transformed the same way whatever the bundle, so results do not depend on it.
 	const int budget = std::clamp(tune_get("fft.lds_elems", kFftLdsElems), a.pitch, kFftLdsElems);
-	// the W_m table (m elements) comes out of the same budget of 2 * budget
-	// elements, so the workgroups resident per CU do not change with it
-	a.bundle = std::max(1, std::min(aLines, (2 * budget - m) / (2 * a.pitch)));
-	// the W_m table follows the two line buffers in LDS: fewer lines if the
+	// the twiddle tables come out of the same budget of 2 * budget elements, so
+	// the workgroups resident per CU do not change with them
+	const int tables = pass_table_elems(m, half);
+	a.bundle = std::max(1, std::min(aLines, (2 * budget - tables) / (2 * a.pitch)));
+	// the tables follow the two line buffers in LDS: fewer lines if the
 	// workgroup would exceed the device's LDS (1 KiB kept for the kernels'
 	// static LDS: batched descriptors, reduction slots)
 	const long long limit = (static_cast<long long>(max_lds_per_block()) - 1024) / static_cast<long long>(sizeof(float2));
-	while (a.bundle > 1 && 2LL * a.bundle * a.pitch + m > limit) {
+	while (a.bundle > 1 && 2LL * a.bundle * a.pitch + tables > limit) {
 		--a.bundle;
 	}
-	if (2LL * a.bundle * a.pitch + m > limit) {
+	if (2LL * a.bundle * a.pitch + tables > limit) {
 		fail(IANE_ERR_UNSUPPORTED, "FFT line length " + std::to_string(aN) + " exceeds the LDS line limit");
 	}
 	a.lds_elems = a.bundle * a.pitch;

@@ -546,48 +546,70 @@ __device__ inline void store_real_lines(float* out, const line_geom& g, int aOut
 		});
 }
 
+// Twiddle entries a pass keeps in LDS: W_m (m entries) and, for a half-length
+// real transform, W_n^k for k = 0 .. m (the only ones its pre/post step reads).
+__host__ __device__ inline int pass_table_elems(int aM, bool aHalf) { return aM + (aHalf ? aM + 1 : 0); }
+
 // Dynamic LDS of a launch running pass `a`: two buffers of a.lds_elems, then
-// the W_m twiddle table (a.plan.m elements).
+// the twiddle tables.
 inline size_t pass_lds_bytes(const fft_pass_args& a)
 {
-	return sizeof(float2) * (2 * static_cast<size_t>(a.lds_elems) + static_cast<size_t>(a.plan.m));
+	return sizeof(float2) *
+		(2 * static_cast<size_t>(a.lds_elems) + static_cast<size_t>(pass_table_elems(a.plan.m, a.tw_n != nullptr)));
 }
 
-// The W_m table is copied into LDS after the two line buffers; the stages read
-// their twiddles from there (an LDS read instead of a global one per butterfly
-// input). fetch_twiddles loads this thread's first two entries (all of a table
-// of up to 2 * blockDim.x) before the line loads are issued, store_twiddles
-// writes them (and any further entries) once the lines are loaded, so the
-// table costs no extra memory round trip; the caller's next barrier, before
-// the first stage, publishes it.
+// The twiddle tables are copied into LDS after the two line buffers (W_m, then
+// W_n^0 .. W_n^m for a half-length real transform); the stages and the real
+// pre/post steps read them there (an LDS read instead of a global one per
+// butterfly input or bin). fetch_twiddles loads this thread's first two
+// entries of each table (all of a table of up to 2 * blockDim.x) before the
+// line loads are issued, store_twiddles writes them (and any further entries)
+// once the lines are loaded, so the tables cost no extra memory round trip;
+// the caller's next barrier, before the first stage, publishes them.
 struct twiddle_fetch {
-	float2 v[2];
+	float2 m[2];
+	float2 n[2];
 };
 
-__device__ __forceinline__ twiddle_fetch fetch_twiddles(const float2* aTable, int aM)
+__device__ __forceinline__ twiddle_fetch fetch_twiddles(const fft_pass_args& a)
 {
 	twiddle_fetch f;
+	const int m = a.plan.m;
+	// unconditional loads (no branch before their use): without a W_n table the
+	// second pair re-reads W_m and is never stored
+	const float2* tn = a.tw_n ? a.tw_n : a.tw_m;
+	const int last_n = a.tw_n ? m : m - 1;
 #pragma unroll
 	for (int u = 0; u < 2; ++u) {
-		f.v[u] = aTable[min(static_cast<int>(threadIdx.x + u * blockDim.x), aM - 1)];
+		const int i = static_cast<int>(threadIdx.x + u * blockDim.x);
+		f.m[u] = a.tw_m[min(i, m - 1)];
+		f.n[u] = tn[min(i, last_n)];
 	}
 	return f;
 }
 
-__device__ __forceinline__ const float2* store_twiddles(const twiddle_fetch& aFetched, const float2* aTable, int aM,
-	float2* aLds)
+__device__ __forceinline__ void store_table(const float2 (&aFetched)[2], const float2* aTable, int aCount, float2* aLds)
 {
 #pragma unroll
 	for (int u = 0; u < 2; ++u) {
 		const int i = static_cast<int>(threadIdx.x + u * blockDim.x);
-		if (i < aM) {
-			aLds[i] = aFetched.v[u];
+		if (i < aCount) {
+			aLds[i] = aFetched[u];
 		}
 	}
-	for (int i = static_cast<int>(threadIdx.x + 2 * blockDim.x); i < aM; i += blockDim.x) {
+	for (int i = static_cast<int>(threadIdx.x + 2 * blockDim.x); i < aCount; i += blockDim.x) {
 		aLds[i] = aTable[i];
 	}
-	return aLds;
+}
+
+// aLds: the table area (W_m at aLds, W_n at aLds + m).
+__device__ __forceinline__ void store_twiddles(const twiddle_fetch& aFetched, const fft_pass_args& a, float2* aLds)
+{
+	const int m = a.plan.m;
+	store_table(aFetched.m, a.tw_m, m, aLds);
+	if (a.tw_n) {
+		store_table(aFetched.n, a.tw_n, m + 1, aLds + m);
+	}
 }
 
 // One workgroup's share of a line pass: the lines of group `aBlock` of `a`.
@@ -598,16 +620,16 @@ __device__ __forceinline__ void fft_pass_body(const fft_pass_args& a, unsigned a
 	float2* A = lds;
 	float2* B = lds + a.lds_elems;
 	const int m = a.plan.m;
-	const twiddle_fetch fetched = fetch_twiddles(a.tw_m, m);
+	const twiddle_fetch fetched = fetch_twiddles(a);
 	float2* const tw_lds = lds + 2 * a.lds_elems;
 	const float2* tw_m = tw_lds;
-	const float2* tw_n = a.tw_n;
+	const float2* tw_n = tw_lds + m; // read only by the half-length real modes, which stage it
 	int o, l0, count;
 	decode_group(a, aBlock, o, l0, count);
 	const int n = a.n;
 	if constexpr (MODE == PASS_C2C) {
 		load_complex_lines(static_cast<const float2*>(a.in), a.gin, o, l0, count, a.load_n, m, A, a.pitch);
-		store_twiddles(fetched, a.tw_m, m, tw_lds);
+		store_twiddles(fetched, a, tw_lds);
 		__syncthreads();
 		// a constant sign in each copy folds the conjugations and (sign i) rotations
 		const float2* r = a.sign < 0.0f ? run_stages(a.plan, A, B, count, a.pitch, tw_m, -1.0f)
@@ -615,7 +637,7 @@ __device__ __forceinline__ void fft_pass_body(const fft_pass_args& a, unsigned a
 		store_complex_lines(static_cast<float2*>(a.out), a.gout, o, l0, count, a.store_n, r, a.pitch);
 	} else if constexpr (MODE == PASS_C2C_MULT) {
 		load_complex_lines(static_cast<const float2*>(a.in), a.gin, o, l0, count, a.load_n, m, A, a.pitch);
-		store_twiddles(fetched, a.tw_m, m, tw_lds);
+		store_twiddles(fetched, a, tw_lds);
 		__syncthreads();
 		float2* r = run_stages(a.plan, A, B, count, a.pitch, tw_m, -1.0f);
 		const long long hbase = o * a.gmult.outer + l0 * a.gmult.line;
@@ -647,7 +669,7 @@ __device__ __forceinline__ void fft_pass_body(const fft_pass_args& a, unsigned a
 		store_complex_lines(static_cast<float2*>(a.out), a.gout, o, l0, count, a.store_n, q, a.pitch);
 	} else if constexpr (MODE == PASS_R2C) {
 		load_real_lines(static_cast<const float*>(a.in), a.gin, o, l0, count, n, a.load_n, A, a.pitch);
-		store_twiddles(fetched, a.tw_m, m, tw_lds);
+		store_twiddles(fetched, a, tw_lds);
 		__syncthreads();
 		float2* r = run_stages(a.plan, A, B, count, a.pitch, tw_m, -1.0f);
 		if ((n & 1) == 0) {
@@ -660,7 +682,7 @@ __device__ __forceinline__ void fft_pass_body(const fft_pass_args& a, unsigned a
 	} else { // PASS_C2R
 		if ((n & 1) == 0) {
 			load_complex_lines(static_cast<const float2*>(a.in), a.gin, o, l0, count, a.load_n, m + 1, A, a.pitch);
-			store_twiddles(fetched, a.tw_m, m, tw_lds);
+			store_twiddles(fetched, a, tw_lds);
 			__syncthreads();
 			c2r_pre(A, B, count, n, a.pitch, tw_n);
 			__syncthreads();
@@ -668,7 +690,7 @@ __device__ __forceinline__ void fft_pass_body(const fft_pass_args& a, unsigned a
 			store_real_lines(static_cast<float*>(a.out), a.gout, o, l0, count, n, a.store_n, r, a.pitch);
 		} else {
 			load_complex_lines(static_cast<const float2*>(a.in), a.gin, o, l0, count, a.load_n, m, A, a.pitch);
-			store_twiddles(fetched, a.tw_m, m, tw_lds);
+			store_twiddles(fetched, a, tw_lds);
 			__syncthreads();
 			c2r_extend_odd(A, count, n, a.pitch);
 			__syncthreads();

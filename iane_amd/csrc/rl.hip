@@ -217,15 +217,15 @@ __device__ __forceinline__ void rl_row_body(const rl_row_args& a, unsigned aBloc
 	const fft_pass_args& f = a.f;
 	float2* A = lds;
 	float2* B = lds + f.lds_elems;
-	const twiddle_fetch fetched = fetch_twiddles(f.tw_m, f.plan.m);
+	const twiddle_fetch fetched = fetch_twiddles(f);
 	const float2* tw_m = lds + 2 * f.lds_elems;
-	const float2* tw_n = f.tw_n;
+	const float2* tw_n = tw_m + f.plan.m; // W_n^0 .. W_n^m (the rows are half-length real transforms)
 	int o, l0, count;
 	decode_group(f, aBlock, o, l0, count);
 	const int n = f.n;
 	const int m = n / 2;
 	load_complex_lines(static_cast<const float2*>(f.in), f.gin, o, l0, count, f.load_n, m + 1, A, f.pitch);
-	store_twiddles(fetched, f.tw_m, f.plan.m, lds + 2 * f.lds_elems);
+	store_twiddles(fetched, f, lds + 2 * f.lds_elems);
 	__syncthreads();
 	c2r_pre(A, B, count, n, f.pitch, tw_n);
 	__syncthreads();
