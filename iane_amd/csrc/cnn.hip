@@ -274,12 +274,15 @@ __global__ __launch_bounds__(64 * kConvWaves) void conv3d_cin1_kernel(const __bf
 	}
 }
 
-// One wave: 16 voxels along x at (y, z), all output channels (<= 64).
+// One wave: 16 voxels along x in kFwdRows consecutive rows y0, y0 + 1 at z, all
+// output channels (<= 64); the rows share each K chunk's weight loads.
 // When cin is a multiple of 8 and the input and weights fit 32-bit byte
 // offsets (grouped != 0), K is walked kFwdGroup<MT> chunks at a time with all
 // of a group's loads issued before its MFMAs (buffer loads: an offset past the
 // end reads 0, so padding and out-of-volume taps need no branch); the chunks
 // left over, and other cin, take one chunk at a time.
+constexpr int kFwdRows = 2;
+
 template<int MT>
 constexpr int kFwdGroup = MT == 1 ? 8 : (MT == 2 ? 6 : 4);
 
@@ -291,22 +294,25 @@ __global__ __launch_bounds__(64 * kConvWaves) void conv3d_fwd_kernel(const __bf1
 	const int lane = threadIdx.x & 63;
 	const long long wave = static_cast<long long>(blockIdx.x) * kConvWaves + (threadIdx.x >> 6);
 	const int xt = (g.nx + 15) / 16;
-	const long long rows = static_cast<long long>(g.ny) * g.nz;
-	if (wave >= rows * xt) {
+	const int yb = (g.ny + kFwdRows - 1) / kFwdRows;
+	if (wave >= static_cast<long long>(xt) * yb * g.nz) {
 		return;
 	}
 	const int x0 = static_cast<int>(wave % xt) * 16;
-	const int y = static_cast<int>((wave / xt) % g.ny);
-	const int z = static_cast<int>(wave / (static_cast<long long>(xt) * g.ny));
+	const int y0 = static_cast<int>((wave / xt) % yb) * kFwdRows;
+	const int z = static_cast<int>(wave / (static_cast<long long>(xt) * yb));
 	const int n = lane & 15;
 	const int grp = lane >> 4;
 	const int x = x0 + n;
 	const int K = 27 * g.cin;
 	const int chunks = (K + 31) / 32;
-	f32x4 acc[MT];
+	f32x4 acc[kFwdRows][MT];
 #pragma unroll
-	for (int mt = 0; mt < MT; ++mt) {
-		acc[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+	for (int r = 0; r < kFwdRows; ++r) {
+#pragma unroll
+		for (int mt = 0; mt < MT; ++mt) {
+			acc[r][mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+		}
 	}
 	const bool vec = (g.cin & 7) == 0;
 	int kc = 0;
@@ -317,7 +323,7 @@ __global__ __launch_bounds__(64 * kConvWaves) void conv3d_fwd_kernel(const __bf1
 		const auto weights = __builtin_amdgcn_make_buffer_rsrc(const_cast<__bf16*>(w), static_cast<short>(0),
 			g.cout * K * 2, 0x00020000);
 		for (; kc + U <= chunks; kc += U) {
-			bf16x8 b[U];
+			bf16x8 b[U][kFwdRows];
 			bf16x8 a[U][MT];
 #pragma unroll
 			for (int u = 0; u < U; ++u) {
@@ -325,10 +331,15 @@ __global__ __launch_bounds__(64 * kConvWaves) void conv3d_fwd_kernel(const __bf1
 				// 8 consecutive input channels of one tap (cin is a multiple of 8)
 				const int t = k0 / g.cin;
 				const int c = k0 - t * g.cin;
-				const int xi = x + t % 3 - 1, yi = y + (t / 3) % 3 - 1, zi = z + t / 9 - 1;
-				const bool ok = t < 27 && x < g.nx && xi >= 0 && xi < g.nx && yi >= 0 && yi < g.ny && zi >= 0 && zi < g.nz;
-				const int at = ok ? (((zi * g.ny + yi) * g.nx + xi) * g.cin + c) * 2 : kOobOffset;
-				b[u] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(volume, at, 0, 0));
+				const int xi = x + t % 3 - 1, zi = z + t / 9 - 1;
+				const bool ok_xz = t < 27 && x < g.nx && xi >= 0 && xi < g.nx && zi >= 0 && zi < g.nz;
+#pragma unroll
+				for (int r = 0; r < kFwdRows; ++r) {
+					const int yi = y0 + r + (t / 3) % 3 - 1;
+					const bool ok = ok_xz && yi >= 0 && yi < g.ny;
+					const int at = ok ? (((zi * g.ny + yi) * g.nx + xi) * g.cin + c) * 2 : kOobOffset;
+					b[u][r] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(volume, at, 0, 0));
+				}
 #pragma unroll
 				for (int mt = 0; mt < MT; ++mt) {
 					const int m = mt * 16 + n;
@@ -341,41 +352,48 @@ __global__ __launch_bounds__(64 * kConvWaves) void conv3d_fwd_kernel(const __bf1
 #pragma unroll
 			for (int u = 0; u < U; ++u) {
 #pragma unroll
-				for (int mt = 0; mt < MT; ++mt) {
-					acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[u][mt], b[u], acc[mt], 0, 0, 0);
+				for (int r = 0; r < kFwdRows; ++r) {
+#pragma unroll
+					for (int mt = 0; mt < MT; ++mt) {
+						acc[r][mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[u][mt], b[u][r], acc[r][mt], 0, 0, 0);
+					}
 				}
 			}
 		}
 	}
 	for (; kc < chunks; ++kc) {
 		const int k0 = kc * 32 + grp * 8;
-		bf16x8 b;
-		if (vec) {
-			// 8 consecutive input channels of one tap
-			const int t = k0 / g.cin;
-			const int c = k0 - t * g.cin;
-			const int dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = t / 9 - 1;
-			const int xi = x + dx, yi = y + dy, zi = z + dz;
-			const bool ok = t < 27 && x < g.nx && xi >= 0 && xi < g.nx && yi >= 0 && yi < g.ny && zi >= 0 && zi < g.nz;
-			if (ok) {
-				const long long voxel = (static_cast<long long>(zi) * g.ny + yi) * g.nx + xi;
-				b = *reinterpret_cast<const bf16x8*>(in + voxel * g.cin + c);
-			} else {
-				for (int j = 0; j < 8; ++j) {
-					b[j] = static_cast<__bf16>(0.0f);
-				}
-			}
-		} else {
+		bf16x8 b[kFwdRows];
 #pragma unroll
-			for (int j = 0; j < 8; ++j) {
-				const int k = k0 + j;
-				const int t = k / g.cin;
-				const int c = k - t * g.cin;
+		for (int r = 0; r < kFwdRows; ++r) {
+			const int y = y0 + r; // a row past the volume computes values that are never stored
+			if (vec) {
+				// 8 consecutive input channels of one tap
+				const int t = k0 / g.cin;
+				const int c = k0 - t * g.cin;
 				const int dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = t / 9 - 1;
 				const int xi = x + dx, yi = y + dy, zi = z + dz;
-				const bool ok = k < K && x < g.nx && xi >= 0 && xi < g.nx && yi >= 0 && yi < g.ny && zi >= 0 && zi < g.nz;
-				const long long voxel = (static_cast<long long>(zi) * g.ny + yi) * g.nx + xi;
-				b[j] = ok ? in[voxel * g.cin + c] : static_cast<__bf16>(0.0f);
+				const bool ok = t < 27 && x < g.nx && xi >= 0 && xi < g.nx && yi >= 0 && yi < g.ny && zi >= 0 && zi < g.nz;
+				if (ok) {
+					const long long voxel = (static_cast<long long>(zi) * g.ny + yi) * g.nx + xi;
+					b[r] = *reinterpret_cast<const bf16x8*>(in + voxel * g.cin + c);
+				} else {
+					for (int j = 0; j < 8; ++j) {
+						b[r][j] = static_cast<__bf16>(0.0f);
+					}
+				}
+			} else {
+#pragma unroll
+				for (int j = 0; j < 8; ++j) {
+					const int k = k0 + j;
+					const int t = k / g.cin;
+					const int c = k - t * g.cin;
+					const int dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = t / 9 - 1;
+					const int xi = x + dx, yi = y + dy, zi = z + dz;
+					const bool ok = k < K && x < g.nx && xi >= 0 && xi < g.nx && yi >= 0 && yi < g.ny && zi >= 0 && zi < g.nz;
+					const long long voxel = (static_cast<long long>(zi) * g.ny + yi) * g.nx + xi;
+					b[r][j] = ok ? in[voxel * g.cin + c] : static_cast<__bf16>(0.0f);
+				}
 			}
 		}
 #pragma unroll
@@ -398,16 +416,25 @@ __global__ __launch_bounds__(64 * kConvWaves) void conv3d_fwd_kernel(const __bf1
 					a[j] = (m < g.cout && k < K) ? w[static_cast<long long>(m) * K + k] : static_cast<__bf16>(0.0f);
 				}
 			}
-			acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[mt], 0, 0, 0);
+#pragma unroll
+			for (int r = 0; r < kFwdRows; ++r) {
+				acc[r][mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[r], acc[r][mt], 0, 0, 0);
+			}
 		}
 	}
 	if (x >= g.nx) {
 		return;
 	}
-	const long long voxel = (static_cast<long long>(z) * g.ny + y) * g.nx + x;
 #pragma unroll
-	for (int mt = 0; mt < MT; ++mt) {
-		store_outputs(out, out_bf16, relu, bias4(bias, g.cout, mt * 16 + grp * 4), voxel, g.cout, mt * 16 + grp * 4, acc[mt]);
+	for (int r = 0; r < kFwdRows; ++r) {
+		if (y0 + r < g.ny) {
+			const long long voxel = (static_cast<long long>(z) * g.ny + y0 + r) * g.nx + x;
+#pragma unroll
+			for (int mt = 0; mt < MT; ++mt) {
+				store_outputs(out, out_bf16, relu, bias4(bias, g.cout, mt * 16 + grp * 4), voxel, g.cout,
+					mt * 16 + grp * 4, acc[r][mt]);
+			}
+		}
 	}
 }
 
@@ -839,7 +866,7 @@ int iane_hip_cnn_conv3d(const void* in, const void* weights, const float* bias, 
 			invalid("conv3d extents or channel counts out of range (cout <= 64, cin <= 256)");
 		}
 		const conv_geom g{nx, ny, nz, cin, cout};
-		const long long waves = static_cast<long long>((nx + 15) / 16) * ny * nz;
+		const long long waves = static_cast<long long>((nx + 15) / 16) * ((ny + kFwdRows - 1) / kFwdRows) * nz;
 		const unsigned blocks = static_cast<unsigned>((waves + kConvWaves - 1) / kConvWaves);
 		auto s = static_cast<hipStream_t>(stream);
 		const auto* x = static_cast<const __bf16*>(in);
