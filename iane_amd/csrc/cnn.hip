@@ -274,17 +274,17 @@ __global__ __launch_bounds__(64 * kConvWaves) void conv3d_cin1_kernel(const __bf
 	}
 }
 
-// One wave: 16 voxels along x in kFwdRows consecutive rows y0, y0 + 1 at z, all
-// output channels (<= 64); the rows share each K chunk's weight loads.
+// One wave: 16 voxels along x in kFwdRows consecutive rows y0 .. y0 + kFwdRows - 1
+// at z, all output channels (<= 64); the rows share each K chunk's weight loads.
 // When cin is a multiple of 8 and the input and weights fit 32-bit byte
 // offsets (grouped != 0), K is walked kFwdGroup<MT> chunks at a time with all
 // of a group's loads issued before its MFMAs (buffer loads: an offset past the
 // end reads 0, so padding and out-of-volume taps need no branch); the chunks
 // left over, and other cin, take one chunk at a time.
-constexpr int kFwdRows = 2;
+constexpr int kFwdRows = 4;
 
 template<int MT>
-constexpr int kFwdGroup = MT == 1 ? 8 : (MT == 2 ? 6 : 4);
+constexpr int kFwdGroup = MT == 1 ? 5 : (MT == 2 ? 4 : 3);
 
 template<int MT>
 __global__ __launch_bounds__(64 * kConvWaves) void conv3d_fwd_kernel(const __bf16* __restrict__ in,

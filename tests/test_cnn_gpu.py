@@ -46,8 +46,8 @@ def rel(got, ref):
 def test_conv3d_layer(cin, cout):
     import torch
     rng = np.random.default_rng(cin * 100 + cout)
-    # odd y extent: the last row pair of the generic kernel (and the last
-    # 8-row block of the others) is partial
+    # odd y extent: the last row group of the generic kernel (4 rows) and the
+    # last 8-row block of the others are partial
     shape = (6, 11, 37)
     x = cnn.bf16(rng.standard_normal(shape + (cin,)))
     w = cnn.bf16(rng.standard_normal((cout, 27, cin)) * 0.2)
