@@ -11,7 +11,7 @@ so tuning only trades occupancy, LDS use and tile overlap.
     sep3.threads    256 | 512              threads per workgroup (vector engine)
     sep3.tile_y     16 | 32                output rows per workgroup tile (vector engine)
 
-    fft.lds_elems   1024 | 2048 | 4096     LDS budget per buffer (complex elements) of the
+    fft.lds_elems   1024 .. 4096           LDS budget per buffer (complex elements) of the
                                            FFT line passes and RL row steps: lines per workgroup
 
 ``sep3.engine`` (0: x/y filters on FP32 MFMA, 1: vector ALUs) is not tuned: the
@@ -39,7 +39,7 @@ from . import _lib
 
 SEPARABLE_SPACE = {"sep3.waves": [2, 3], "sep3.chunk": [0, 16, 21, 24, 32, 43, 64]}  # matrix-core engine (default)
 VECTOR_SEPARABLE_SPACE = {"sep3.threads": [512, 256], "sep3.tile_y": [16, 32], "sep3.chunk": [0, 16, 32, 64]}
-FFT_SPACE = {"fft.lds_elems": [4096, 2048, 1024]}
+FFT_SPACE = {"fft.lds_elems": [4096, 3072, 2048, 1536, 1024]}
 RL_BATCH_SPACE = FFT_SPACE
 
 

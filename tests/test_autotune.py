@@ -41,12 +41,12 @@ def test_search_picks_fastest_and_caches(cache):
     def measure(config):
         seen.append(dict(config))
         assert autotune.get("fft.lds_elems") == config["fft.lds_elems"]  # applied while measured
-        return {4096: 3.0, 2048: 1.0, 1024: 2.0}[config["fft.lds_elems"]]
+        return {4096: 3.0, 3072: 2.5, 2048: 1.0, 1536: 1.5, 1024: 2.0}[config["fft.lds_elems"]]
 
     best = autotune.search("k", autotune.FFT_SPACE, measure)
     assert best == {"fft.lds_elems": 2048}
     assert autotune.get("fft.lds_elems") == 2048
-    assert len(seen) == 3
+    assert len(seen) == len(autotune.FFT_SPACE["fft.lds_elems"])
     stored = json.loads(cache.read_text())["k"]
     assert stored["config"] == {"fft.lds_elems": 2048}
     # cached: applied without measuring
