@@ -24,7 +24,7 @@ from ._lib import invalid_argument, length_error, overflow_error, runtime_error,
 
 __all__ = [
     "border_mode", "precision", "convolution_options", "hip_policy", "hip_execution",
-    "convolve", "convolve_separable", "view_desc", "library",
+    "convolve", "convolve_workspace_bytes", "convolve_separable", "view_desc", "library",
 ]
 
 
@@ -127,33 +127,69 @@ def _stream(policy) -> int:
     return policy.handle()
 
 
-def convolve(src, kernel, dst, options: convolution_options | None = None, policy=None) -> hip_execution:
+def _workspace(nbytes: int, workspace):
+    """Caller-side scratch for an entry point that needs ``nbytes`` (torch's
+    caching allocator stands in for the caller's execution_context)."""
+    if nbytes == 0:
+        return 0
+    if workspace is None:
+        import torch
+        workspace = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    elif workspace.device.type != "cuda" or workspace.numel() * workspace.element_size() < nbytes:
+        raise length_error("Prepared operation exceeded its scratch reservation")
+    return workspace
+
+
+def convolve_workspace_bytes(src, kernel, dst, options: convolution_options | None = None) -> int:
+    """Scratch ``convolve`` needs for these shapes (nonzero only on the split-precision MFMA path)."""
+    import ctypes
+    options = options or convolution_options()
+    if isinstance(src, texture_image_view):
+        src = src.texels()
+    nbytes = ctypes.c_size_t()
+    _lib.call("iane_hip_convolve_workspace_bytes", view_desc(src), view_desc(kernel), view_desc(dst),
+              int(options.border), int(options.precision), ctypes.byref(nbytes))
+    return nbytes.value
+
+
+def convolve(src, kernel, dst, options: convolution_options | None = None, policy=None,
+             workspace=None) -> hip_execution:
     """iane::convolve for device images (convolution.hpp:97-117); ``src`` may be a
     texture view (docs/cuda_texture_images.md:5-17).
 
     ``dst[p] = sum_q kernel[q] * src[p + c - q]`` with ``c = extent // 2`` per axis.
     Raises ``invalid_argument`` for even/non-positive kernel extents, rank or
-    extent mismatch, exactly where the reference throws.
+    extent mismatch, exactly where the reference throws. ``workspace`` (a device
+    tensor of at least ``convolve_workspace_bytes`` bytes) is used by the
+    split-precision path; without one it is taken from torch's allocator.
     """
     options = options or convolution_options()
     stream = _stream(policy)
     if isinstance(src, texture_image_view):
         src = src.texels()  # texture source: its texels, border rule at the view boundary
     s, k, d = view_desc(src), view_desc(kernel), view_desc(dst)
-    _lib.call("iane_hip_convolve", s, k, d, int(options.border), int(options.precision), stream)
+    ws = _workspace(convolve_workspace_bytes(src, kernel, dst, options), workspace)
+    ptr, nbytes = (0, 0) if isinstance(ws, int) else (ws.data_ptr(), ws.numel() * ws.element_size())
+    _lib.call("iane_hip_convolve", s, k, d, int(options.border), int(options.precision), ptr, nbytes, stream)
     return hip_execution(stream)
 
 
 def convolve_separable(src, taps, dst, options: convolution_options | None = None, policy=None,
-                       *, fused: bool = True) -> hip_execution:
+                       *, fused: bool = True, workspace=None) -> hip_execution:
     """Convolve with the outer product of ``taps`` (one 1-D tensor per IANE axis,
     ``taps[0]`` along axis 0 = the last tensor dimension)."""
+    import ctypes
     options = options or convolution_options(precision=precision.fp32)
     stream = _stream(policy)
     arr = (_lib.iane_view_desc * len(taps))(*[view_desc(t) for t in taps])
     name = "iane_hip_convolve_separable" if fused else "iane_hip_convolve_separable_passes"
-    _lib.call(name, view_desc(src), arr, len(taps), view_desc(dst), int(options.border),
-              int(options.precision), stream)
+    s, d = view_desc(src), view_desc(dst)
+    nbytes = ctypes.c_size_t()
+    _lib.call(name + "_workspace_bytes", s, arr, len(taps), d, int(options.border), int(options.precision),
+              ctypes.byref(nbytes))
+    ws = _workspace(nbytes.value, workspace)
+    ptr, size = (0, 0) if isinstance(ws, int) else (ws.data_ptr(), ws.numel() * ws.element_size())
+    _lib.call(name, s, arr, len(taps), d, int(options.border), int(options.precision), ptr, size, stream)
     return hip_execution(stream)
 
 

@@ -83,16 +83,23 @@ SIGNATURES = {
     "iane_hip_event_destroy": [vp],
     "iane_hip_event_record": [vp, vp],
     "iane_hip_event_elapsed_ms": [ctypes.POINTER(f32), vp, vp],
-    "iane_hip_convolve": [desc_p, desc_p, desc_p, ctypes.c_int, ctypes.c_int, vp],
+    "iane_hip_convolve_workspace_bytes": [desc_p, desc_p, desc_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(sz)],
+    "iane_hip_convolve": [desc_p, desc_p, desc_p, ctypes.c_int, ctypes.c_int, vp, sz, vp],
     "iane_hip_convolve_untiled": [desc_p, desc_p, desc_p, ctypes.c_int, ctypes.c_int, vp],
-    "iane_hip_convolve_separable": [desc_p, desc_p, ctypes.c_int, desc_p, ctypes.c_int, ctypes.c_int, vp],
-    "iane_hip_convolve_separable_passes": [desc_p, desc_p, ctypes.c_int, desc_p, ctypes.c_int, ctypes.c_int, vp],
+    "iane_hip_convolve_separable_workspace_bytes": [desc_p, desc_p, ctypes.c_int, desc_p, ctypes.c_int, ctypes.c_int,
+                                                    ctypes.POINTER(sz)],
+    "iane_hip_convolve_separable": [desc_p, desc_p, ctypes.c_int, desc_p, ctypes.c_int, ctypes.c_int, vp, sz, vp],
+    "iane_hip_convolve_separable_passes_workspace_bytes": [desc_p, desc_p, ctypes.c_int, desc_p, ctypes.c_int,
+                                                           ctypes.c_int, ctypes.POINTER(sz)],
+    "iane_hip_convolve_separable_passes": [desc_p, desc_p, ctypes.c_int, desc_p, ctypes.c_int, ctypes.c_int, vp, sz,
+                                           vp],
     "iane_hip_fft_plan_create": [ctypes.POINTER(vp), ctypes.c_int, ctypes.POINTER(i32), ctypes.c_int],
     "iane_hip_fft_plan_workspace_bytes": [vp, ctypes.POINTER(sz)],
     "iane_hip_fft_exec": [vp, vp, vp, vp, vp],
     "iane_hip_fft_plan_destroy": [vp],
     "iane_hip_rl_initial": [vp, vp, i64, f32, vp],
-    "iane_hip_rl_ratio": [vp, vp, vp, i64, f32, vp, vp],
+    "iane_hip_rl_ratio_workspace_bytes": [i64, ctypes.POINTER(sz)],
+    "iane_hip_rl_ratio": [vp, vp, vp, i64, f32, vp, vp, sz, vp],
     "iane_hip_rl_update": [vp, vp, vp, i64, f32, vp],
     "iane_hip_wiener_apply": [vp, vp, i64, f32, vp],
     "iane_hip_real_to_complex": [vp, vp, i64, vp],

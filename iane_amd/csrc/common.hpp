@@ -258,6 +258,36 @@ __device__ __forceinline__ unsigned xcd_remap(unsigned aBlock, unsigned aCount)
 
 inline unsigned ceil_div(long long a, long long b) { return static_cast<unsigned>((a + b - 1) / b); }
 
+// A caller-supplied workspace: it must hold `aNeeded` bytes (the size the
+// matching *_workspace_bytes query reported) and start on a 256-byte boundary.
+inline void check_workspace(const void* aWorkspace, size_t aBytes, size_t aNeeded)
+{
+	if (aNeeded == 0) {
+		return;
+	}
+	if (!aWorkspace || aBytes < aNeeded) {
+		fail(IANE_ERR_LENGTH, "Prepared operation exceeded its scratch reservation");
+	}
+	if (reinterpret_cast<uintptr_t>(aWorkspace) % 256 != 0) {
+		invalid("workspace must be 256-byte aligned");
+	}
+}
+
+// Split-precision MFMA convolution (conv_mfma.hip). The kernel part (scale and
+// Toeplitz table, split_kernel_bytes) can be built once per kernel; the source
+// part (scale and padded hi/lo planes, split_source_bytes) is scratch per call.
+bool split_applicable(const view4& aSrc, const view4& aKer, const view4& aDst);
+size_t split_kernel_bytes(const view4& aKer);
+size_t split_source_bytes(const view4& aSrc, const view4& aKer);
+void split_prepare_kernel(const view4& aKer, int aPrecision, void* aKernelPart, hipStream_t aStream);
+void split_convolve_prepared(const view4& aSrc, const view4& aKer, const view4& aDst, int aBorder, int aPrecision,
+	const void* aKernelPart, void* aSourcePart, hipStream_t aStream);
+
+// Direct convolution (conv_direct.hip): tiled, streaming or untiled kernels.
+void convolve_direct(const view4& aSrc, const view4& aKer, const view4& aDst, int aBorder, int aPrecision,
+	hipStream_t aStream, bool aForceSimple);
+void validate_convolution(const view4& aSrc, const view4& aKer, const view4& aDst);
+
 // Workgroup barrier for LDS traffic only: waits for this wave's LDS operations
 // (lgkmcnt) and then for the other waves, but not for outstanding global loads
 // (vmcnt), so register prefetches of the next tile stay in flight across it.

@@ -900,10 +900,6 @@ void validate_convolution(const view4& aSrc, const view4& aKer, const view4& aDs
 	}
 }
 
-// Implemented in conv_mfma.hip: split-precision MFMA path for large float kernels.
-int convolve_split(const view4& aSrc, const view4& aKer, const view4& aDst, int aBorder, int aPrecision,
-	hipStream_t aStream);
-
 void convolve_direct(const view4& aSrc, const view4& aKer, const view4& aDst, int aBorder, int aPrecision,
 	hipStream_t aStream, bool aForceSimple)
 {
@@ -930,32 +926,74 @@ void convolve_direct(const view4& aSrc, const view4& aKer, const view4& aDst, in
 
 using namespace iane_hip;
 
-extern "C" int iane_hip_convolve(const iane_view_desc* aSrc, const iane_view_desc* aKernel,
-	const iane_view_desc* aDst, int aBorder, int aPrecision, void* aStream)
+namespace {
+
+struct convolve_call {
+	view4 src, ker, dst;
+	int precision;
+	bool split;  // runs on the split-precision MFMA path
+};
+
+convolve_call check_convolve(const iane_view_desc* aSrc, const iane_view_desc* aKernel, const iane_view_desc* aDst,
+	int aBorder, int aPrecision)
+{
+	convolve_call c{make_view4(aSrc, "source"), make_view4(aKernel, "kernel"), make_view4(aDst, "destination"),
+		aPrecision, false};
+	validate_convolution(c.src, c.ker, c.dst);
+	if (aBorder != IANE_BORDER_CLAMP && aBorder != IANE_BORDER_ZERO) {
+		invalid("unknown border mode");
+	}
+	if (aPrecision < IANE_PREC_REFERENCE || aPrecision > IANE_PREC_SPLIT_FP16) {
+		invalid("unknown precision mode");
+	}
+	if (aPrecision == IANE_PREC_SPLIT_BF16 || aPrecision == IANE_PREC_SPLIT_FP16) {
+		c.split = split_applicable(c.src, c.ker, c.dst);
+		if (!c.split) {
+			c.precision = IANE_PREC_FP32;  // shapes the MFMA path does not tile: FP32 accumulation
+		}
+	}
+	return c;
+}
+
+size_t convolve_workspace(const convolve_call& c)
+{
+	if (!c.split || element_count(c.dst) == 0) {
+		return 0;
+	}
+	return split_kernel_bytes(c.ker) + split_source_bytes(c.src, c.ker);
+}
+
+} // namespace
+
+extern "C" int iane_hip_convolve_workspace_bytes(const iane_view_desc* aSrc, const iane_view_desc* aKernel,
+	const iane_view_desc* aDst, int aBorder, int aPrecision, size_t* aBytes)
 {
 	return guarded([&] {
-		const auto src = make_view4(aSrc, "source");
-		const auto ker = make_view4(aKernel, "kernel");
-		const auto dst = make_view4(aDst, "destination");
-		validate_convolution(src, ker, dst);
-		if (aBorder != IANE_BORDER_CLAMP && aBorder != IANE_BORDER_ZERO) {
-			invalid("unknown border mode");
+		if (!aBytes) {
+			invalid("workspace size output is null");
 		}
-		if (aPrecision < IANE_PREC_REFERENCE || aPrecision > IANE_PREC_SPLIT_FP16) {
-			invalid("unknown precision mode");
-		}
-		if (element_count(dst) == 0) {
+		*aBytes = convolve_workspace(check_convolve(aSrc, aKernel, aDst, aBorder, aPrecision));
+	});
+}
+
+extern "C" int iane_hip_convolve(const iane_view_desc* aSrc, const iane_view_desc* aKernel,
+	const iane_view_desc* aDst, int aBorder, int aPrecision, void* aWorkspace, size_t aWorkspaceBytes, void* aStream)
+{
+	return guarded([&] {
+		const convolve_call c = check_convolve(aSrc, aKernel, aDst, aBorder, aPrecision);
+		if (element_count(c.dst) == 0) {
 			return;
 		}
 		auto stream = static_cast<hipStream_t>(aStream);
-		if (aPrecision == IANE_PREC_SPLIT_BF16 || aPrecision == IANE_PREC_SPLIT_FP16) {
-			if (convolve_split(src, ker, dst, aBorder, aPrecision, stream)) {
-				return;
-			}
-			// shapes the MFMA path does not tile fall back to FP32 accumulation
-			aPrecision = IANE_PREC_FP32;
+		if (c.split) {
+			check_workspace(aWorkspace, aWorkspaceBytes, convolve_workspace(c));
+			char* kernel_part = static_cast<char*>(aWorkspace);
+			split_prepare_kernel(c.ker, c.precision, kernel_part, stream);
+			split_convolve_prepared(c.src, c.ker, c.dst, aBorder, c.precision, kernel_part,
+				kernel_part + split_kernel_bytes(c.ker), stream);
+			return;
 		}
-		convolve_direct(src, ker, dst, aBorder, aPrecision, stream, false);
+		convolve_direct(c.src, c.ker, c.dst, aBorder, c.precision, stream, false);
 	});
 }
 

@@ -102,8 +102,8 @@ __global__ void absmax_partial_kernel(view4 v, float* partial)
 	}
 }
 
-// scale[slot] = 2^(14 - e) with max < 2^e (1 for a zero maximum, or when `aUnit`).
-__global__ void scale_from_partials_kernel(const float* partial, int count, float* scale, int slot, int unit)
+// *scale = 2^(14 - e) with max < 2^e (1 for a zero maximum, or when `aUnit`).
+__global__ void scale_from_partials_kernel(const float* partial, int count, float* scale, int unit)
 {
 	__shared__ float shared[4];
 	float m = 0.0f;
@@ -124,7 +124,7 @@ __global__ void scale_from_partials_kernel(const float* partial, int count, floa
 		}
 		int e = 0;
 		frexpf(r, &e);
-		scale[slot] = (unit || r == 0.0f) ? 1.0f : ldexpf(1.0f, 14 - e);
+		*scale = (unit || r == 0.0f) ? 1.0f : ldexpf(1.0f, 14 - e);
 	}
 }
 
@@ -132,10 +132,10 @@ __global__ void scale_from_partials_kernel(const float* partial, int count, floa
 // hi/lo planes of the padded source: padded (xp, yp, zp) holds the sample at
 // (xp - hx, yp - hy, zp - hz) under the border rule.
 template<class T16>
-__global__ void split_pad_kernel(view4 src, int border, split_geom g, const float* scale, T16* out)
+__global__ void split_pad_kernel(view4 src, int border, split_geom g, const float* src_scale, T16* out)
 {
 	using S = split16<T16>;
-	const float sd = scale[0];
+	const float sd = *src_scale;
 	const float lo_scale = static_cast<float>(1 << S::lo_shift);
 	const long long total = g.part;
 	for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < total;
@@ -163,10 +163,10 @@ __global__ void split_pad_kernel(view4 src, int border, split_geom g, const floa
 // (hi or lo part) of the Toeplitz matrix of kernel row (qz, qy), K-step s, in
 // the MFMA lane layout (lane l holds A[l % 16][32 s + 8 (l / 16) + j]).
 template<class T16>
-__global__ void toeplitz_table_kernel(view4 ker, int hx, int qy_padded, int steps, const float* scale, T16* table)
+__global__ void toeplitz_table_kernel(view4 ker, int hx, int qy_padded, int steps, const float* ker_scale, T16* table)
 {
 	using S = split16<T16>;
-	const float sk = scale[1];
+	const float sk = *ker_scale;
 	const float lo_scale = static_cast<float>(1 << S::lo_shift);
 	const int kx = ker.extent[0], ky = ker.extent[1], kz = ker.extent[2];
 	const long long entries = static_cast<long long>(kz) * qy_padded * steps * 64 * 8;
@@ -196,8 +196,8 @@ __global__ void toeplitz_table_kernel(view4 ker, int hx, int qy_padded, int step
 // ------------------------------------------------------------- main kernel --
 template<class T16, int QC, int STEPS>
 __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(1, 1)))
-void conv_split_kernel(const T16* __restrict__ padded, const T16* __restrict__ table, const float* __restrict__ scale,
-	view4 dst, split_geom g, unsigned tasks)
+void conv_split_kernel(const T16* __restrict__ padded, const T16* __restrict__ table,
+	const float* __restrict__ src_scale, const float* __restrict__ ker_scale, view4 dst, split_geom g, unsigned tasks)
 {
 	using S = split16<T16>;
 	using vec = typename S::vec;
@@ -277,7 +277,7 @@ void conv_split_kernel(const T16* __restrict__ padded, const T16* __restrict__ t
 		}
 	}
 
-	const float unscale = 1.0f / (scale[0] * scale[1]);
+	const float unscale = 1.0f / (*src_scale * *ker_scale);
 	const float lo_unscale = 1.0f / static_cast<float>(1 << S::lo_shift);
 	const int xs = x0 + 16 * n + 4 * grp;
 #pragma unroll
@@ -303,9 +303,25 @@ void conv_split_kernel(const T16* __restrict__ padded, const T16* __restrict__ t
 	}
 }
 
-template<class T16, int QC, int STEPS>
-void launch_split(const view4& aSrc, const view4& aKer, const view4& aDst, int aBorder, hipStream_t aStream)
+// Tile variants: QC rows of A kept in registers, STEPS K-steps of 32 along x.
+enum split_variant { SPLIT_NONE = 0, SPLIT_QC8_S1, SPLIT_QC15_S1, SPLIT_QC8_S2 };
+
+split_variant pick_variant(const view4& aKer)
 {
+	const int kx = aKer.extent[0], ky = aKer.extent[1];
+	const int steps = (15 + kx + 31) / 32;
+	if (steps == 1) {
+		return ky <= 8 ? SPLIT_QC8_S1 : SPLIT_QC15_S1;
+	}
+	return steps == 2 ? SPLIT_QC8_S2 : SPLIT_NONE;
+}
+
+int variant_qc(split_variant aV) { return aV == SPLIT_QC15_S1 ? 15 : 8; }
+int variant_steps(split_variant aV) { return aV == SPLIT_QC8_S2 ? 2 : 1; }
+
+split_geom make_geom(const view4& aSrc, const view4& aKer, split_variant aV)
+{
+	const int qc = variant_qc(aV), steps = variant_steps(aV);
 	split_geom g{};
 	g.nx = aSrc.extent[0];
 	g.ny = aSrc.extent[1];
@@ -314,78 +330,128 @@ void launch_split(const view4& aSrc, const view4& aKer, const view4& aDst, int a
 	g.hy = aKer.extent[1] / 2;
 	g.hz = aKer.extent[2] / 2;
 	g.kz = aKer.extent[2];
-	g.qy_padded = ((aKer.extent[1] + QC - 1) / QC) * QC;
+	g.qy_padded = ((aKer.extent[1] + qc - 1) / qc) * qc;
 	g.strips = (g.nx + kStrip - 1) / kStrip;
 	g.yblocks = (g.ny + kRows - 1) / kRows;
-	g.px = g.strips * kStrip + 32 * STEPS;
+	g.px = g.strips * kStrip + 32 * steps;
 	g.py = g.yblocks * kRows + 2 * g.hy;
 	g.pz = g.nz + 2 * g.hz;
 	g.plane = static_cast<long long>(g.px) * g.py;
 	g.part = g.plane * g.pz;
+	return g;
+}
 
-	const long long table_halves = static_cast<long long>(g.kz) * g.qy_padded * STEPS * 2 * 64 * 8;
-	constexpr int partial_blocks = 1024;
-	const size_t bytes_scale = 256;
-	const size_t bytes_partial = sizeof(float) * partial_blocks;
-	const size_t bytes_table = sizeof(T16) * static_cast<size_t>(table_halves);
-	const size_t bytes_padded = sizeof(T16) * static_cast<size_t>(2 * g.part);
-	char* work = nullptr;
-	check_hip(hipMallocAsync(reinterpret_cast<void**>(&work), bytes_scale + bytes_partial + bytes_table + bytes_padded,
-		aStream), "split convolution workspace");
-	auto* scale = reinterpret_cast<float*>(work);
-	auto* partial = reinterpret_cast<float*>(work + bytes_scale);
-	auto* table = reinterpret_cast<T16*>(work + bytes_scale + bytes_partial);
-	auto* padded = reinterpret_cast<T16*>(work + bytes_scale + bytes_partial + bytes_table);
-	const int unit = std::is_same_v<T16, __bf16> ? 1 : 0;
+constexpr int kPartialBlocks = 1024;
+constexpr size_t kScaleBytes = 256;                                  // one float, padded
+constexpr size_t kPartialBytes = sizeof(float) * kPartialBlocks;      // block maxima
 
-	absmax_partial_kernel<<<partial_blocks, 256, 0, aStream>>>(aSrc, partial);
-	scale_from_partials_kernel<<<1, 256, 0, aStream>>>(partial, partial_blocks, scale, 0, unit);
-	absmax_partial_kernel<<<64, 256, 0, aStream>>>(aKer, partial);
-	scale_from_partials_kernel<<<1, 256, 0, aStream>>>(partial, 64, scale, 1, unit);
-	split_pad_kernel<T16><<<ceil_div(g.part, 256 * 8), 256, 0, aStream>>>(aSrc, aBorder, g, scale, padded);
-	toeplitz_table_kernel<T16><<<ceil_div(table_halves / 2, 256), 256, 0, aStream>>>(aKer, g.hx, g.qy_padded, STEPS,
-		scale, table);
-	const unsigned tasks = static_cast<unsigned>(g.strips * g.yblocks) * ceil_div(g.nz, kWaves);
-	conv_split_kernel<T16, QC, STEPS><<<tasks, 64 * kWaves, 0, aStream>>>(padded, table, scale, aDst, g, tasks);
-	check_hip(hipGetLastError(), "split convolution launch");
-	check_hip(hipFreeAsync(work, aStream), "split convolution workspace");
+size_t round_up(size_t aBytes) { return (aBytes + 255) / 256 * 256; }
+
+// Halves of the Toeplitz table of one kernel (hi and lo parts, every K-step).
+long long table_halves(const view4& aKer, split_variant aV)
+{
+	const int qc = variant_qc(aV);
+	const long long qy_padded = (aKer.extent[1] + qc - 1) / qc * qc;
+	return static_cast<long long>(aKer.extent[2]) * qy_padded * variant_steps(aV) * 2 * 64 * 8;
 }
 
 template<class T16>
-bool dispatch_split(const view4& aSrc, const view4& aKer, const view4& aDst, int aBorder, hipStream_t aStream)
+void build_kernel_part(const view4& aKer, split_variant aV, int aUnit, float* aKerScale, float* aPartial,
+	T16* aTable, hipStream_t aStream)
 {
-	const int kx = aKer.extent[0], ky = aKer.extent[1];
-	const int steps = (15 + kx + 31) / 32;
-	if (steps == 1 && ky <= 8) {
-		launch_split<T16, 8, 1>(aSrc, aKer, aDst, aBorder, aStream);
-	} else if (steps == 1) {
-		launch_split<T16, 15, 1>(aSrc, aKer, aDst, aBorder, aStream);
-	} else if (steps == 2) {
-		launch_split<T16, 8, 2>(aSrc, aKer, aDst, aBorder, aStream);
-	} else {
-		return false;
+	const int qc = variant_qc(aV);
+	const int qy_padded = (aKer.extent[1] + qc - 1) / qc * qc;
+	absmax_partial_kernel<<<64, 256, 0, aStream>>>(aKer, aPartial);
+	scale_from_partials_kernel<<<1, 256, 0, aStream>>>(aPartial, 64, aKerScale, aUnit);
+	toeplitz_table_kernel<T16><<<ceil_div(table_halves(aKer, aV) / 2, 256), 256, 0, aStream>>>(aKer, aKer.extent[0] / 2,
+		qy_padded, variant_steps(aV), aKerScale, aTable);
+	check_hip(hipGetLastError(), "split convolution kernel table");
+}
+
+template<class T16, int QC, int STEPS>
+void run_source_part(const view4& aSrc, const split_geom& aGeom, const view4& aDst, int aBorder, int aUnit,
+	char* aWork, const T16* aTable, const float* aKerScale, hipStream_t aStream)
+{
+	auto* src_scale = reinterpret_cast<float*>(aWork);
+	auto* partial = reinterpret_cast<float*>(aWork + kScaleBytes);
+	auto* padded = reinterpret_cast<T16*>(aWork + kScaleBytes + kPartialBytes);
+	absmax_partial_kernel<<<kPartialBlocks, 256, 0, aStream>>>(aSrc, partial);
+	scale_from_partials_kernel<<<1, 256, 0, aStream>>>(partial, kPartialBlocks, src_scale, aUnit);
+	split_pad_kernel<T16><<<ceil_div(aGeom.part, 256 * 8), 256, 0, aStream>>>(aSrc, aBorder, aGeom, src_scale, padded);
+	const unsigned tasks = static_cast<unsigned>(aGeom.strips * aGeom.yblocks) * ceil_div(aGeom.nz, kWaves);
+	conv_split_kernel<T16, QC, STEPS><<<tasks, 64 * kWaves, 0, aStream>>>(padded, aTable, src_scale, aKerScale, aDst,
+		aGeom, tasks);
+	check_hip(hipGetLastError(), "split convolution launch");
+}
+
+template<class T16>
+void run_source_part(const view4& aSrc, const view4& aKer, const view4& aDst, int aBorder, split_variant aV,
+	char* aWork, const void* aTable, const float* aKerScale, hipStream_t aStream)
+{
+	const int unit = std::is_same_v<T16, __bf16> ? 1 : 0;
+	const split_geom g = make_geom(aSrc, aKer, aV);
+	const auto* table = static_cast<const T16*>(aTable);
+	switch (aV) {
+	case SPLIT_QC8_S1: run_source_part<T16, 8, 1>(aSrc, g, aDst, aBorder, unit, aWork, table, aKerScale, aStream); break;
+	case SPLIT_QC15_S1: run_source_part<T16, 15, 1>(aSrc, g, aDst, aBorder, unit, aWork, table, aKerScale, aStream); break;
+	default: run_source_part<T16, 8, 2>(aSrc, g, aDst, aBorder, unit, aWork, table, aKerScale, aStream); break;
 	}
-	return true;
 }
 
 } // namespace
 
-// Returns false for shapes this path does not tile (the caller then runs the
-// FP32 direct kernel): rank-4 images, kernels wider than 49 along x, or
-// non-float destinations.
-int convolve_split(const view4& aSrc, const view4& aKer, const view4& aDst, int aBorder, int aPrecision,
-	hipStream_t aStream)
+// ------------------------------------------------------------ entry points --
+// The MFMA path tiles rank <= 3 images with float destinations and kernels up
+// to 49 wide along x; other shapes run the FP32 direct kernel instead.
+bool split_applicable(const view4& aSrc, const view4& aKer, const view4& aDst)
 {
-	if (aSrc.extent[3] != 1 || aKer.extent[3] != 1 || (aDst.type != IANE_F32 && aDst.type != IANE_F64)) {
-		return 0;
-	}
-	if (aSrc.extent[0] >= (1 << 24) || aSrc.extent[1] >= (1 << 24)) {
-		return 0;
-	}
+	return aSrc.extent[3] == 1 && aKer.extent[3] == 1 && (aDst.type == IANE_F32 || aDst.type == IANE_F64) &&
+		aSrc.extent[0] < (1 << 24) && aSrc.extent[1] < (1 << 24) && pick_variant(aKer) != SPLIT_NONE;
+}
+
+size_t split_kernel_bytes(const view4& aKer)
+{
+	// a kernel scale, the block maxima of the kernel, and the table (2-byte halves)
+	return kScaleBytes + kPartialBytes + round_up(2 * static_cast<size_t>(table_halves(aKer, pick_variant(aKer))));
+}
+
+size_t split_source_bytes(const view4& aSrc, const view4& aKer)
+{
+	const split_geom g = make_geom(aSrc, aKer, pick_variant(aKer));
+	return kScaleBytes + kPartialBytes + round_up(2 * sizeof(uint16_t) * static_cast<size_t>(g.part));
+}
+
+// Build the kernel-side data (scale and Toeplitz table) into `aKernelPart`
+// (split_kernel_bytes of memory); it stays valid while the kernel values do.
+void split_prepare_kernel(const view4& aKer, int aPrecision, void* aKernelPart, hipStream_t aStream)
+{
+	char* base = static_cast<char*>(aKernelPart);
+	auto* ker_scale = reinterpret_cast<float*>(base);
+	auto* partial = reinterpret_cast<float*>(base + kScaleBytes);
+	void* table = base + kScaleBytes + kPartialBytes;
+	const split_variant v = pick_variant(aKer);
 	if (aPrecision == IANE_PREC_SPLIT_FP16) {
-		return dispatch_split<_Float16>(aSrc, aKer, aDst, aBorder, aStream) ? 1 : 0;
+		build_kernel_part<_Float16>(aKer, v, 0, ker_scale, partial, static_cast<_Float16*>(table), aStream);
+	} else {
+		build_kernel_part<__bf16>(aKer, v, 1, ker_scale, partial, static_cast<__bf16*>(table), aStream);
 	}
-	return dispatch_split<__bf16>(aSrc, aKer, aDst, aBorder, aStream) ? 1 : 0;
+}
+
+// Convolve with a kernel prepared by split_prepare_kernel; `aSourcePart` is
+// split_source_bytes of scratch (the source's scale and padded hi/lo planes).
+void split_convolve_prepared(const view4& aSrc, const view4& aKer, const view4& aDst, int aBorder, int aPrecision,
+	const void* aKernelPart, void* aSourcePart, hipStream_t aStream)
+{
+	const char* base = static_cast<const char*>(aKernelPart);
+	const auto* ker_scale = reinterpret_cast<const float*>(base);
+	const void* table = base + kScaleBytes + kPartialBytes;
+	const split_variant v = pick_variant(aKer);
+	char* work = static_cast<char*>(aSourcePart);
+	if (aPrecision == IANE_PREC_SPLIT_FP16) {
+		run_source_part<_Float16>(aSrc, aKer, aDst, aBorder, v, work, table, ker_scale, aStream);
+	} else {
+		run_source_part<__bf16>(aSrc, aKer, aDst, aBorder, v, work, table, ker_scale, aStream);
+	}
 }
 
 } // namespace iane_hip

@@ -611,20 +611,39 @@ void launch_fused(sep_args& a, hipStream_t aStream, int aThreads)
 
 } // namespace
 
-void convolve_separable(const view4& aSrc, const view4* aTaps, int aN, const view4& aDst, int aBorder,
-	int aPrecision, hipStream_t aStream, bool aAllowFused)
+// The one-kernel path: rank 3, FP32 images and arithmetic, equal tap lengths
+// up to 15, staged offsets within a plane kept in 32-bit registers.
+bool separable_fused(const view4& aSrc, const view4* aTaps, int aN, const view4& aDst, int aPrecision)
 {
-	const int rank = aSrc.rank;
-	bool fused = aAllowFused && rank == 3 && aPrecision == IANE_PREC_FP32 && aSrc.type == IANE_F32 &&
-		aDst.type == IANE_F32;
+	bool fused = aSrc.rank == 3 && aPrecision == IANE_PREC_FP32 && aSrc.type == IANE_F32 && aDst.type == IANE_F32;
 	const int k = aTaps[0].extent[0];
 	for (int i = 0; i < aN; ++i) {
 		fused = fused && aTaps[i].extent[0] == k;
 	}
 	fused = fused && k <= 15;
-	// staged offsets within one plane are kept in 32-bit registers
-	fused = fused && (static_cast<long long>(aSrc.extent[0]) + 64) * std::llabs(aSrc.stride[0]) +
+	return fused && (static_cast<long long>(aSrc.extent[0]) + 64) * std::llabs(aSrc.stride[0]) +
 		(static_cast<long long>(aSrc.extent[1]) + 64) * std::llabs(aSrc.stride[1]) < (1LL << 31);
+}
+
+// Scratch of the per-axis path: up to two packed intermediate images (FP64 in
+// REFERENCE precision, FP32 otherwise), each on a 256-byte boundary.
+size_t separable_workspace(const view4& aSrc, const view4* aTaps, int aN, const view4& aDst, int aPrecision,
+	bool aAllowFused)
+{
+	if (element_count(aDst) == 0 || (aAllowFused && separable_fused(aSrc, aTaps, aN, aDst, aPrecision))) {
+		return 0;
+	}
+	const int temps = std::min(2, aSrc.rank - 1);
+	const size_t one = static_cast<size_t>(element_count(aDst)) * (aPrecision != IANE_PREC_FP32 ? 8 : 4);
+	return static_cast<size_t>(temps) * ((one + 255) / 256 * 256);
+}
+
+void convolve_separable(const view4& aSrc, const view4* aTaps, int aN, const view4& aDst, int aBorder,
+	int aPrecision, void* aWorkspace, hipStream_t aStream, bool aAllowFused)
+{
+	const int rank = aSrc.rank;
+	const int k = aTaps[0].extent[0];
+	const bool fused = aAllowFused && separable_fused(aSrc, aTaps, aN, aDst, aPrecision);
 	if (fused) {
 		sep_args a{};
 		for (int axis = 0; axis < 3; ++axis) {
@@ -687,12 +706,9 @@ void convolve_separable(const view4& aSrc, const view4* aTaps, int aN, const vie
 	const bool wide = aPrecision != IANE_PREC_FP32;
 	const int temp_type = wide ? IANE_F64 : IANE_F32;
 	const long long count = element_count(aDst);
-	void* temps[2] = {nullptr, nullptr};
+	const size_t temp_bytes = (static_cast<size_t>(count) * dtype_size(temp_type) + 255) / 256 * 256;
+	void* temps[2] = {aWorkspace, static_cast<char*>(aWorkspace) + temp_bytes};
 	const int passes = rank;
-	for (int t = 0; t < 2 && t < passes - 1; ++t) {
-		check_hip(hipMallocAsync(&temps[t], static_cast<size_t>(count) * dtype_size(temp_type), aStream),
-			"Allocate separable temporary");
-	}
 	view4 in = aSrc;
 	for (int axis = 0; axis < passes; ++axis) {
 		const bool last = axis == passes - 1;
@@ -707,70 +723,102 @@ void convolve_separable(const view4& aSrc, const view4* aTaps, int aN, const vie
 		check_hip(hipGetLastError(), "Separable pass launch");
 		in = out;
 	}
-	for (int t = 0; t < 2; ++t) {
-		if (temps[t]) {
-			check_hip(hipFreeAsync(temps[t], aStream), "Free separable temporary");
-		}
-	}
 }
 
 } // namespace iane_hip
 
 using namespace iane_hip;
 
-extern "C" int iane_hip_convolve_separable(const iane_view_desc* aSrc, const iane_view_desc* aTaps, int aN,
-	const iane_view_desc* aDst, int aBorder, int aPrecision, void* aStream)
+namespace {
+
+struct separable_call {
+	view4 src, dst;
+	view4 taps[IANE_MAX_RANK];
+	int n;
+};
+
+separable_call check_separable(const iane_view_desc* aSrc, const iane_view_desc* aTaps, int aN,
+	const iane_view_desc* aDst, int aBorder, int aPrecision)
+{
+	separable_call c{};
+	c.src = make_view4(aSrc, "source");
+	c.dst = make_view4(aDst, "destination");
+	if (!aTaps || aN != c.src.rank) {
+		invalid("separable convolution needs one tap vector per axis");
+	}
+	c.n = aN;
+	view4 shape = c.src;
+	for (int i = 0; i < aN; ++i) {
+		c.taps[i] = make_view4(&aTaps[i], "taps");
+		if (c.taps[i].rank != 1 || (c.taps[i].type != IANE_F32 && c.taps[i].type != IANE_F64)) {
+			invalid("separable taps must be rank-1 f32 or f64 views");
+		}
+		shape.extent[i] = c.taps[i].extent[0];
+	}
+	for (int i = aN; i < 4; ++i) {
+		shape.extent[i] = 1;
+	}
+	shape.type = c.taps[0].type;
+	validate_convolution(c.src, shape, c.dst);
+	if (aBorder != IANE_BORDER_CLAMP && aBorder != IANE_BORDER_ZERO) {
+		invalid("unknown border mode");
+	}
+	if (aPrecision != IANE_PREC_REFERENCE && aPrecision != IANE_PREC_FP32) {
+		fail(IANE_ERR_UNSUPPORTED, "separable convolution supports REFERENCE and FP32 precision");
+	}
+	return c;
+}
+
+int run_separable(const iane_view_desc* aSrc, const iane_view_desc* aTaps, int aN, const iane_view_desc* aDst,
+	int aBorder, int aPrecision, void* aWorkspace, size_t aWorkspaceBytes, void* aStream, bool aAllowFused)
 {
 	return guarded([&] {
-		const auto src = make_view4(aSrc, "source");
-		const auto dst = make_view4(aDst, "destination");
-		if (!aTaps || aN != src.rank) {
-			invalid("separable convolution needs one tap vector per axis");
-		}
-		view4 taps[IANE_MAX_RANK];
-		view4 shape = src;
-		for (int i = 0; i < aN; ++i) {
-			taps[i] = make_view4(&aTaps[i], "taps");
-			if (taps[i].rank != 1 || (taps[i].type != IANE_F32 && taps[i].type != IANE_F64)) {
-				invalid("separable taps must be rank-1 f32 or f64 views");
-			}
-			shape.extent[i] = taps[i].extent[0];
-		}
-		for (int i = aN; i < 4; ++i) {
-			shape.extent[i] = 1;
-		}
-		shape.type = taps[0].type;
-		validate_convolution(src, shape, dst);
-		if (aBorder != IANE_BORDER_CLAMP && aBorder != IANE_BORDER_ZERO) {
-			invalid("unknown border mode");
-		}
-		if (aPrecision != IANE_PREC_REFERENCE && aPrecision != IANE_PREC_FP32) {
-			fail(IANE_ERR_UNSUPPORTED, "separable convolution supports REFERENCE and FP32 precision");
-		}
-		if (element_count(dst) == 0) {
+		const separable_call c = check_separable(aSrc, aTaps, aN, aDst, aBorder, aPrecision);
+		if (element_count(c.dst) == 0) {
 			return;
 		}
-		convolve_separable(src, taps, aN, dst, aBorder, aPrecision, static_cast<hipStream_t>(aStream), true);
+		check_workspace(aWorkspace, aWorkspaceBytes,
+			separable_workspace(c.src, c.taps, c.n, c.dst, aPrecision, aAllowFused));
+		convolve_separable(c.src, c.taps, c.n, c.dst, aBorder, aPrecision, aWorkspace,
+			static_cast<hipStream_t>(aStream), aAllowFused);
 	});
 }
 
-// Test hook: force the per-axis pass path.
-extern "C" int iane_hip_convolve_separable_passes(const iane_view_desc* aSrc, const iane_view_desc* aTaps, int aN,
-	const iane_view_desc* aDst, int aBorder, int aPrecision, void* aStream)
+int separable_bytes(const iane_view_desc* aSrc, const iane_view_desc* aTaps, int aN, const iane_view_desc* aDst,
+	int aBorder, int aPrecision, size_t* aBytes, bool aAllowFused)
 {
 	return guarded([&] {
-		const auto src = make_view4(aSrc, "source");
-		const auto dst = make_view4(aDst, "destination");
-		if (!aTaps || aN != src.rank) {
-			invalid("separable convolution needs one tap vector per axis");
+		if (!aBytes) {
+			invalid("workspace size output is null");
 		}
-		view4 taps[IANE_MAX_RANK];
-		for (int i = 0; i < aN; ++i) {
-			taps[i] = make_view4(&aTaps[i], "taps");
-		}
-		if (element_count(dst) == 0) {
-			return;
-		}
-		convolve_separable(src, taps, aN, dst, aBorder, aPrecision, static_cast<hipStream_t>(aStream), false);
+		const separable_call c = check_separable(aSrc, aTaps, aN, aDst, aBorder, aPrecision);
+		*aBytes = separable_workspace(c.src, c.taps, c.n, c.dst, aPrecision, aAllowFused);
 	});
+}
+
+} // namespace
+
+extern "C" int iane_hip_convolve_separable_workspace_bytes(const iane_view_desc* aSrc, const iane_view_desc* aTaps,
+	int aN, const iane_view_desc* aDst, int aBorder, int aPrecision, size_t* aBytes)
+{
+	return separable_bytes(aSrc, aTaps, aN, aDst, aBorder, aPrecision, aBytes, true);
+}
+
+extern "C" int iane_hip_convolve_separable(const iane_view_desc* aSrc, const iane_view_desc* aTaps, int aN,
+	const iane_view_desc* aDst, int aBorder, int aPrecision, void* aWorkspace, size_t aWorkspaceBytes, void* aStream)
+{
+	return run_separable(aSrc, aTaps, aN, aDst, aBorder, aPrecision, aWorkspace, aWorkspaceBytes, aStream, true);
+}
+
+// Test hooks: the per-axis pass path forced, with its own workspace query.
+extern "C" int iane_hip_convolve_separable_passes_workspace_bytes(const iane_view_desc* aSrc,
+	const iane_view_desc* aTaps, int aN, const iane_view_desc* aDst, int aBorder, int aPrecision, size_t* aBytes)
+{
+	return separable_bytes(aSrc, aTaps, aN, aDst, aBorder, aPrecision, aBytes, false);
+}
+
+extern "C" int iane_hip_convolve_separable_passes(const iane_view_desc* aSrc, const iane_view_desc* aTaps, int aN,
+	const iane_view_desc* aDst, int aBorder, int aPrecision, void* aWorkspace, size_t aWorkspaceBytes, void* aStream)
+{
+	return run_separable(aSrc, aTaps, aN, aDst, aBorder, aPrecision, aWorkspace, aWorkspaceBytes, aStream, false);
 }

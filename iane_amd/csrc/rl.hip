@@ -25,10 +25,6 @@
 
 namespace iane_hip {
 
-void convolve_direct(const view4& aSrc, const view4& aKer, const view4& aDst, int aBorder, int aPrecision,
-	hipStream_t aStream, bool aForceSimple);
-int convolve_split(const view4& aSrc, const view4& aKer, const view4& aDst, int aBorder, int aPrecision,
-	hipStream_t aStream);
 void setup_pass(fft_pass_args& a, int aMode, int aN, int aLines, int aOuter);
 void launch_pass(const fft_pass_args& a, hipStream_t aStream);
 
@@ -449,6 +445,12 @@ struct rl_plan_impl {
 	float* psf = nullptr;       // packed float PSF (direct)
 	float* flipped = nullptr;   // packed flipped PSF (direct)
 	float2* spectrum_h = nullptr; // PSF spectrum scaled by 1/(n0 n1 n2) (fft)
+	// split-precision direct method: kernel parts (scale + Toeplitz table) of
+	// the PSF and of its flip, built once; the source part is run scratch
+	char* split_psf = nullptr;
+	char* split_flipped = nullptr;
+	size_t split_scratch = 0;     // bytes of the per-call source part (0: no split path)
+	size_t split_offset = 0;      // where the source part starts in the run workspace
 	// fft geometry
 	int pad[3] = {1, 1, 1};
 	int h0 = 1;
@@ -468,6 +470,8 @@ struct rl_plan_impl {
 		(void)hipFree(psf);
 		(void)hipFree(flipped);
 		(void)hipFree(spectrum_h);
+		(void)hipFree(split_psf);
+		(void)hipFree(split_flipped);
 	}
 
 	view4 image_view(float* p) const
@@ -500,13 +504,16 @@ struct rl_plan_impl {
 		return v;
 	}
 
-	void convolve(const float* in, const float* kernel, float* out, hipStream_t stream) const
+	// out = in (*) kernel, zero border; `scratch` holds split_scratch bytes when
+	// the split path runs (kernel is psf or flipped).
+	void convolve(const float* in, const float* kernel, float* out, void* scratch, hipStream_t stream) const
 	{
 		const view4 src = image_view(const_cast<float*>(in));
 		const view4 ker = kernel_view(const_cast<float*>(kernel));
 		const view4 dst = image_view(out);
-		if ((precision == IANE_PREC_SPLIT_BF16 || precision == IANE_PREC_SPLIT_FP16) &&
-			convolve_split(src, ker, dst, IANE_BORDER_ZERO, precision, stream)) {
+		if (split_scratch) {
+			split_convolve_prepared(src, ker, dst, IANE_BORDER_ZERO, precision,
+				kernel == psf ? split_psf : split_flipped, scratch, stream);
 			return;
 		}
 		convolve_direct(src, ker, dst, IANE_BORDER_ZERO,
@@ -691,13 +698,38 @@ std::unique_ptr<rl_plan_impl> make_rl_plan(int aRank, const int32_t* aExtent, co
 		check_hip(hipGetLastError(), "PSF copy launch");
 	}
 	if (aMethod == IANE_RL_DIRECT) {
+		const view4 image = impl->image_view(nullptr);
+		const view4 kernel = impl->kernel_view(impl->psf);
+		const bool split = (aPrecision == IANE_PREC_SPLIT_BF16 || aPrecision == IANE_PREC_SPLIT_FP16) &&
+			split_applicable(image, kernel, image);
+		if (split) {
+			// the PSF's and its flip's scale and Toeplitz table, once per plan
+			const size_t kernel_bytes = split_kernel_bytes(kernel);
+			check_hip(hipMalloc(&impl->split_psf, kernel_bytes), "Allocate split PSF table");
+			check_hip(hipMalloc(&impl->split_flipped, kernel_bytes), "Allocate split PSF table");
+			split_prepare_kernel(kernel, aPrecision, impl->split_psf, stream);
+			split_prepare_kernel(impl->kernel_view(impl->flipped), aPrecision, impl->split_flipped, stream);
+			impl->split_scratch = split_source_bytes(image, kernel);
+		}
+		// run workspace: predicted, correction, log-likelihood partials, split source part
+		const size_t images = (2 * impl->count * sizeof(float) + 255) / 256 * 256;
+		const size_t partials = (static_cast<size_t>(kPartialBlocks + 64) * sizeof(ll_sum) + 255) / 256 * 256;
+		impl->split_offset = images + partials;
+		impl->workspace = impl->split_offset + impl->split_scratch;
 		// s = H^T 1 (main.cu:62-63): the flipped PSF over an image of ones, zero border
 		float* ones = nullptr;
+		char* scratch = nullptr;
 		check_hip(hipMallocAsync(reinterpret_cast<void**>(&ones), impl->count * sizeof(float), stream), "Allocate ones");
+		if (impl->split_scratch) {
+			check_hip(hipMallocAsync(reinterpret_cast<void**>(&scratch), impl->split_scratch, stream),
+				"Allocate split scratch");
+		}
 		fill_kernel<<<point_blocks(impl->count), kPointThreads, 0, stream>>>(ones, impl->count, 1.0f);
-		impl->convolve(ones, impl->flipped, impl->sensitivity, stream);
+		impl->convolve(ones, impl->flipped, impl->sensitivity, scratch, stream);
 		check_hip(hipFreeAsync(ones, stream), "Free ones");
-		impl->workspace = 2 * impl->count * sizeof(float) + static_cast<size_t>(kPartialBlocks + 64) * sizeof(ll_sum);
+		if (scratch) {
+			check_hip(hipFreeAsync(scratch, stream), "Free split scratch");
+		}
 	} else {
 		impl->build_fft(stream);
 		// s = H^T 1 through the same pruned passes
@@ -753,8 +785,19 @@ int iane_hip_rl_initial(const float* aObserved, float* aEstimate, int64_t aCount
 	});
 }
 
+int iane_hip_rl_ratio_workspace_bytes(int64_t aCount, size_t* aBytes)
+{
+	return guarded([&] {
+		if (!aBytes || aCount < 0) {
+			invalid("RL ratio workspace query needs a non-negative count and an output");
+		}
+		// one compensated partial sum per block of the ratio kernel
+		*aBytes = aCount ? static_cast<size_t>(point_blocks(aCount)) * sizeof(ll_sum) : 0;
+	});
+}
+
 int iane_hip_rl_ratio(const float* aObserved, const float* aPredicted, float* aRatio, int64_t aCount, float aEps,
-	double* aLlAccum, void* aStream)
+	double* aLlAccum, void* aWorkspace, size_t aWorkspaceBytes, void* aStream)
 {
 	return guarded([&] {
 		if (aCount < 0 || (aCount && (!aObserved || !aPredicted || !aRatio))) {
@@ -764,17 +807,17 @@ int iane_hip_rl_ratio(const float* aObserved, const float* aPredicted, float* aR
 			return;
 		}
 		auto stream = static_cast<hipStream_t>(aStream);
-		ll_sum* partials = nullptr;
 		const unsigned blocks = point_blocks(aCount);
+		ll_sum* partials = nullptr;
 		if (aLlAccum) {
-			check_hip(hipMallocAsync(reinterpret_cast<void**>(&partials), blocks * sizeof(ll_sum), stream),
-				"Allocate partials");
+			check_workspace(aWorkspace, aWorkspaceBytes, blocks * sizeof(ll_sum));
+			partials = static_cast<ll_sum*>(aWorkspace);
 		}
 		rl_ratio_kernel<<<blocks, kPointThreads, 0, stream>>>(aObserved, aPredicted, aRatio, aCount, aEps, partials);
 		check_hip(hipGetLastError(), "RL ratio launch");
 		if (aLlAccum) {
 			ll_finalize_kernel<<<1, kPointThreads, 0, stream>>>(partials, blocks, aLlAccum);
-			check_hip(hipFreeAsync(partials, stream), "Free partials");
+			check_hip(hipGetLastError(), "RL log-likelihood launch");
 		}
 	});
 }
@@ -874,7 +917,7 @@ int iane_hip_rl_plan_blur(iane_rl_plan* aPlan, const float* aIn, float* aOut, vo
 		const rl_plan_impl& p = *aPlan->impl;
 		auto stream = static_cast<hipStream_t>(aStream);
 		if (p.method == IANE_RL_DIRECT) {
-			p.convolve(aIn, p.psf, aOut, stream);
+			p.convolve(aIn, p.psf, aOut, static_cast<char*>(aWorkspace) + p.split_offset, stream);
 			return;
 		}
 		float2* spec = static_cast<float2*>(aWorkspace);
@@ -921,15 +964,16 @@ int iane_hip_rl_run(iane_rl_plan* aPlan, const float* aObserved, float* aEstimat
 			float* predicted = reinterpret_cast<float*>(ws);
 			float* correction = predicted + n;
 			ll_sum* partials = reinterpret_cast<ll_sum*>(ws + ((2 * n * sizeof(float) + 255) / 256) * 256);
+			char* split = ws + p.split_offset;
 			const unsigned blocks = point_blocks(n);
 			for (int it = 0; it < aIterations; ++it) {
-				p.convolve(aEstimate, p.psf, predicted, stream);
+				p.convolve(aEstimate, p.psf, predicted, split, stream);
 				rl_ratio_kernel<<<blocks, kPointThreads, 0, stream>>>(aObserved, predicted, predicted, n, p.eps,
 					aLlHistory ? partials : nullptr);
 				if (aLlHistory) {
 					ll_finalize_kernel<<<1, kPointThreads, 0, stream>>>(partials, blocks, aLlHistory + it);
 				}
-				p.convolve(predicted, p.flipped, correction, stream);
+				p.convolve(predicted, p.flipped, correction, split, stream);
 				rl_update_kernel<<<blocks, kPointThreads, 0, stream>>>(aEstimate, correction, p.sensitivity, n, p.eps);
 			}
 			check_hip(hipGetLastError(), "RL iteration launch");

@@ -68,6 +68,8 @@ class rl_plan:
                 raise invalid_argument("ll_history must be a float64 device tensor with one entry per iteration")
             ll_ptr = ll_history.data_ptr()
         ws = workspace if workspace is not None else self._workspace
+        if ws.device.type != "cuda" or ws.numel() * ws.element_size() < self.workspace_bytes:
+            raise _lib.length_error("Prepared operation exceeded its scratch reservation")
         _lib.call("iane_hip_rl_run", self._handle, observed.data_ptr(), estimate.data_ptr(), int(iterations),
                   1 if init else 0, ws.data_ptr(), ll_ptr, self.stream)
         return hip_execution(self.stream)
@@ -129,7 +131,7 @@ class rl_batch:
         ll_ptr = 0
         if ll_history is not None:
             if ll_history.dtype != torch.float64 or tuple(ll_history.shape) != (n, iterations) \
-                    or not ll_history.is_contiguous():
+                    or not ll_history.is_contiguous() or ll_history.device.type != "cuda":
                 raise invalid_argument("ll_history must be a contiguous float64 (count, iterations) tensor")
             ll_ptr = ll_history.data_ptr()
         self._keep = (obs, est)
@@ -160,24 +162,54 @@ def richardson_lucy(observed, psf, iterations: int = 20, epsilon: float = 1e-6, 
     return (estimate, ll[:iterations]) if return_ll else estimate
 
 
+def _check_pointwise(*images):
+    """Pointwise steps take packed float32 device images of one element count."""
+    import torch
+    count = images[0][0].numel()
+    for t, name in images:
+        if t.device.type != "cuda" or t.dtype != torch.float32 or not t.is_contiguous():
+            raise invalid_argument(f"{name} must be a packed float32 device image")
+        if t.numel() != count:
+            raise invalid_argument("pointwise images require matching element counts")
+
+
 def rl_initial(observed, estimate, epsilon: float = 1e-6, policy=None) -> hip_execution:
     """estimate = max(observed, eps) (main.cu:12-19)."""
+    _check_pointwise((observed, "observed"), (estimate, "estimate"))
     s = _stream(policy)
     _lib.call("iane_hip_rl_initial", observed.data_ptr(), estimate.data_ptr(), observed.numel(),
               ctypes.c_float(epsilon), s)
     return hip_execution(s)
 
 
-def rl_ratio(observed, predicted, ratio, epsilon: float = 1e-6, ll_out=None, policy=None) -> hip_execution:
-    """ratio = observed / max(predicted, eps) (main.cu:22-29); ``ll_out`` gets the log-likelihood."""
+def rl_ratio(observed, predicted, ratio, epsilon: float = 1e-6, ll_out=None, policy=None,
+             workspace=None) -> hip_execution:
+    """ratio = observed / max(predicted, eps) (main.cu:22-29); ``ll_out`` (a float64
+    device tensor) gets the log-likelihood, summed through ``workspace``."""
+    import torch
+    _check_pointwise((observed, "observed"), (predicted, "predicted"), (ratio, "ratio"))
     s = _stream(policy)
+    ll_ptr, ws_ptr, ws_bytes = 0, 0, 0
+    if ll_out is not None:
+        if ll_out.device.type != "cuda" or ll_out.dtype != torch.float64 or ll_out.numel() < 1:
+            raise invalid_argument("ll_out must be a float64 device tensor with at least one element")
+        ll_ptr = ll_out.data_ptr()
+        nbytes = ctypes.c_size_t()
+        _lib.call("iane_hip_rl_ratio_workspace_bytes", observed.numel(), ctypes.byref(nbytes))
+        if workspace is None:
+            workspace = torch.empty(max(nbytes.value, 1), dtype=torch.uint8, device="cuda")
+        ws_bytes = workspace.numel() * workspace.element_size()
+        if workspace.device.type != "cuda" or ws_bytes < nbytes.value:
+            raise _lib.length_error("Prepared operation exceeded its scratch reservation")
+        ws_ptr = workspace.data_ptr()
     _lib.call("iane_hip_rl_ratio", observed.data_ptr(), predicted.data_ptr(), ratio.data_ptr(), observed.numel(),
-              ctypes.c_float(epsilon), ll_out.data_ptr() if ll_out is not None else 0, s)
+              ctypes.c_float(epsilon), ll_ptr, ws_ptr, ws_bytes, s)
     return hip_execution(s)
 
 
 def rl_update(estimate, correction, sensitivity, epsilon: float = 1e-6, policy=None) -> hip_execution:
     """estimate *= (s > 0 ? c / max(s, eps) : 0) (main.cu:32-49)."""
+    _check_pointwise((estimate, "estimate"), (correction, "correction"), (sensitivity, "sensitivity"))
     s = _stream(policy)
     _lib.call("iane_hip_rl_update", estimate.data_ptr(), correction.data_ptr(), sensitivity.data_ptr(),
               estimate.numel(), ctypes.c_float(epsilon), s)

@@ -9,7 +9,7 @@
 // product type on the host path only.
 #pragma once
 
-#include <iane/core.hpp>
+#include <iane/prepared.hpp>
 
 #include <cmath>
 #include <exception>
@@ -225,8 +225,15 @@ auto convolve(TSrc aSource, TKernel aKernel, TDst aDestination, convolution_opti
 		const auto s = detail::describe(aSource);
 		const auto k = detail::describe(aKernel);
 		const auto d = detail::describe(aDestination);
+		std::size_t scratch = 0;
+		detail::check_status(iane_hip_convolve_workspace_bytes(&s, &k, &d, static_cast<int>(aOptions.border),
+			static_cast<int>(aOptions.precision), &scratch));
+		if (scratch != 0) {
+			throw std::invalid_argument(
+				"Split-precision convolution needs scratch: prepare iane::convolution in an execution_context");
+		}
 		detail::check_status(iane_hip_convolve(&s, &k, &d, static_cast<int>(aOptions.border),
-			static_cast<int>(aOptions.precision), aPolicy.stream));
+			static_cast<int>(aOptions.precision), nullptr, 0, aPolicy.stream));
 		return hip_execution{aPolicy.stream};
 	} else {
 		detail::visit_host(aDestination, detail::convolve_visitor<TSrc, TKernel, TDst>{aSource, aKernel, aDestination,
@@ -235,14 +242,118 @@ auto convolve(TSrc aSource, TKernel aKernel, TDst aDestination, convolution_opti
 	}
 }
 
+namespace detail {
+
+// Prepared convolution with a fixed kernel (no reference counterpart: the
+// reference's convolve needs no scratch). The split-precision MFMA path takes
+// its padded planes and Toeplitz table from the context's workspace.
+template<class TKernel>
+struct convolution_state {
+	TKernel kernel;
+	convolution_options options;
+	std::size_t workspace;
+	iane_view_desc source_shape;
+
+	std::size_t scratch_bytes() const { return workspace; }
+
+	template<class TSrc, class TDst, class TPolicy>
+	auto run(TSrc aSource, TDst aDestination, scratch_space<TPolicy>& aScratch, TPolicy aPolicy)
+	{
+		if constexpr (std::same_as<TPolicy, hip_policy>) {
+			const auto s = describe(aSource);
+			const auto k = describe(kernel);
+			const auto d = describe(aDestination);
+			for (int i = 0; i < TSrc::dimension; ++i) {
+				if (s.extent[i] != source_shape.extent[i]) {
+					throw std::invalid_argument("Convolution views must match the prepared shape");
+				}
+			}
+			// the region is reserved 256 bytes long so its start can move to a 256-byte boundary
+			void* ws = nullptr;
+			std::size_t usable = 0;
+			if (workspace) {
+				const auto raw = reinterpret_cast<std::uintptr_t>(aScratch.bytes(workspace));
+				ws = reinterpret_cast<void*>((raw + 255) & ~static_cast<std::uintptr_t>(255));
+				usable = workspace - 256;
+			}
+			check_status(iane_hip_convolve(&s, &k, &d, static_cast<int>(options.border),
+				static_cast<int>(options.precision), ws, usable, aPolicy.stream));
+			return hip_execution{aPolicy.stream};
+		} else {
+			return convolve(aSource, kernel, aDestination, options, aPolicy);
+		}
+	}
+};
+
+template<class TKernel>
+struct convolution_algorithm {
+	TKernel kernel;
+	convolution_options options;
+
+	template<class TSrc, class TDst, class TPolicy>
+	auto prepare(TSrc aSource, TDst aDestination, TPolicy) const
+	{
+		check_convolution(aSource, kernel, aDestination);
+		std::size_t bytes = 0;
+		iane_view_desc shape{};
+		if constexpr (concepts::DeviceView<TDst>) {
+			const auto s = describe(aSource);
+			const auto k = describe(kernel);
+			const auto d = describe(aDestination);
+			check_status(iane_hip_convolve_workspace_bytes(&s, &k, &d, static_cast<int>(options.border),
+				static_cast<int>(options.precision), &bytes));
+			shape = s;
+		}
+		// the scratch region is carved at max_align_t; the library wants 256-byte starts
+		return convolution_state<TKernel>{kernel, options, bytes ? bytes + 256 : 0, shape};
+	}
+};
+
+} // namespace detail
+
+/**
+ * A convolution with a fixed kernel as a prepared algorithm:
+ * `context.prepare(iane::convolution(kernel, options), src, dst)` reserves the
+ * scratch the chosen arithmetic needs (split-precision MFMA only) in the
+ * context's workspace, `context.run(op, src, dst)` convolves views of the
+ * prepared shape. No reference counterpart.
+ */
+template<class TKernel>
+requires(concepts::AnyView<TKernel>)
+auto convolution(TKernel aKernel, convolution_options aOptions = {})
+{
+	return detail::convolution_algorithm<TKernel>{aKernel, aOptions};
+}
+
 /**
  * Separable device convolution: the kernel is the outer product of one
- * odd-length f32/f64 tap vector per axis (no reference counterpart).
+ * odd-length f32/f64 tap vector per axis (no reference counterpart). The
+ * single-kernel path needs no scratch; shapes it does not cover run one pass
+ * per axis through two intermediate images taken from `aScratch` (a region of
+ * at least separable_scratch_bytes), or, when none is given, refused.
  */
 template<class TSrc, class TTaps, class TDst, std::size_t R>
 requires(concepts::DeviceView<TSrc> && concepts::DeviceView<TDst> && TTaps::dimension == 1 && R == TSrc::dimension)
+std::size_t separable_scratch_bytes(TSrc aSource, const TTaps (&aTaps)[R], TDst aDestination,
+	convolution_options aOptions = {border_mode::clamp, convolution_precision::fp32})
+{
+	const auto s = detail::describe(aSource);
+	const auto d = detail::describe(aDestination);
+	iane_view_desc taps[R];
+	for (std::size_t i = 0; i < R; ++i) {
+		taps[i] = detail::describe(aTaps[i]);
+	}
+	std::size_t bytes = 0;
+	detail::check_status(iane_hip_convolve_separable_workspace_bytes(&s, taps, static_cast<int>(R), &d,
+		static_cast<int>(aOptions.border), static_cast<int>(aOptions.precision), &bytes));
+	return bytes;
+}
+
+template<class TSrc, class TTaps, class TDst, std::size_t R>
+requires(concepts::DeviceView<TSrc> && concepts::DeviceView<TDst> && TTaps::dimension == 1 && R == TSrc::dimension)
 hip_execution convolve_separable(TSrc aSource, const TTaps (&aTaps)[R], TDst aDestination,
-	convolution_options aOptions = {border_mode::clamp, convolution_precision::fp32}, hip_policy aPolicy = {})
+	convolution_options aOptions = {border_mode::clamp, convolution_precision::fp32}, hip_policy aPolicy = {},
+	void* aScratch = nullptr, std::size_t aScratchBytes = 0)
 {
 	const auto s = detail::describe(aSource);
 	const auto d = detail::describe(aDestination);
@@ -251,7 +362,8 @@ hip_execution convolve_separable(TSrc aSource, const TTaps (&aTaps)[R], TDst aDe
 		taps[i] = detail::describe(aTaps[i]);
 	}
 	detail::check_status(iane_hip_convolve_separable(&s, taps, static_cast<int>(R), &d,
-		static_cast<int>(aOptions.border), static_cast<int>(aOptions.precision), aPolicy.stream));
+		static_cast<int>(aOptions.border), static_cast<int>(aOptions.precision), aScratch, aScratchBytes,
+		aPolicy.stream));
 	return hip_execution{aPolicy.stream};
 }
 

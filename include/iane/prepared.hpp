@@ -1,9 +1,11 @@
-// Prepared algorithms with one shared scratch allocation and the FFT drop-ins.
+// Prepared algorithms with one shared workspace and the FFT drop-ins.
 //
-// execution_context / scratch_space / make_execution_context mirror
-// include/iane/execution/prepared.hpp:18-180 of the reference (prepare reserves
-// the op's scratch, the context grows one allocation, operations are bound to
-// their context, growth and destruction wait for submitted device work).
+// execution_context / prepared_operation / scratch_space / make_execution_context
+// keep the interface of include/iane/execution/prepared.hpp:18-180 of the
+// reference: prepare() reserves the operation's scratch in the context's one
+// workspace, run() hands the operation that workspace, an operation runs only
+// on its own context, and growth and destruction wait for the last device
+// submission.
 // fft_forward / fft_inverse / rfft / irfft / rfft_extent / fft_complex mirror
 // include/iane/algorithms/fft/{fft.hpp:196-231, rfft.hpp:107-161, layout.hpp:15-21,
 // complex.hpp:6-9} for device views of rank 1..3, on the Stockham FFT of
@@ -13,7 +15,9 @@
 
 #include <iane/core.hpp>
 
+#include <atomic>
 #include <cstring>
+#include <optional>
 
 namespace iane {
 
@@ -34,125 +38,190 @@ inline math::vec2i rfft_extent(math::vec2i aRealExtent)
 
 namespace detail {
 
-// Carve regions out of the context's current allocation (prepared.hpp:18-57).
+/**
+ * Bump allocator over one borrowed byte range: the view of the context's
+ * workspace that one prepared operation sees while it runs. Regions are
+ * handed out front to back; nothing is freed (the range is reused by the next
+ * submission on the same stream).
+ */
 template<class TPolicy>
 class scratch_space {
 public:
-	scratch_space(void* aData, std::size_t aSize) : mData(static_cast<char*>(aData)), mSize(aSize) {}
+	scratch_space(void* aBase, std::size_t aLength)
+		: mBase(reinterpret_cast<std::uintptr_t>(aBase)), mCursor(0), mLength(aBase ? aLength : 0) {}
+
+	/// A region of `aSize` bytes aligned to the power of two `aAlignment`
+	/// (at most alignof(max_align_t)); empty requests return null.
 	void* bytes(std::size_t aSize, std::size_t aAlignment = alignof(std::max_align_t))
 	{
 		if (aSize == 0) {
 			return nullptr;
 		}
-		if (aAlignment == 0 || aAlignment > alignof(std::max_align_t) || (aAlignment & (aAlignment - 1)) != 0) {
+		const bool power_of_two = aAlignment != 0 && (aAlignment & (aAlignment - 1)) == 0;
+		if (!power_of_two || aAlignment > alignof(std::max_align_t)) {
 			throw std::invalid_argument("Unsupported scratch alignment");
 		}
-		const auto address = reinterpret_cast<std::uintptr_t>(mData);
-		const auto pad = (aAlignment - address % aAlignment) % aAlignment;
-		if (mData == nullptr || pad + aSize > mSize) {
+		const std::uintptr_t first = (mBase + mCursor + aAlignment - 1) & ~static_cast<std::uintptr_t>(aAlignment - 1);
+		const std::size_t offset = static_cast<std::size_t>(first - mBase);
+		if (mBase == 0 || offset > mLength || mLength - offset < aSize) {
 			throw std::length_error("Prepared operation exceeded its scratch reservation");
 		}
-		char* result = mData + pad;
-		mData = result + aSize;
-		mSize -= pad + aSize;
-		return result;
+		mCursor = offset + aSize;
+		return reinterpret_cast<void*>(first);
 	}
 
-private:
-	char* mData;
-	std::size_t mSize;
-};
-
-template<class TPolicy>
-struct scratch_deleter {
-	void operator()(void* aPointer) const noexcept
+	/// A packed image of the backend's kind carved out of the range.
+	template<class TElement, int tDim>
+	auto image(const typename math::dim_traits<tDim>::extents_t& aSize)
 	{
+		static_assert(std::is_trivially_copyable_v<TElement>);
+		const auto count = resource::detail::packed_layout<TElement, tDim>(aSize).first;
+		auto* first = static_cast<TElement*>(bytes(count * sizeof(TElement), alignof(TElement)));
 		if constexpr (std::same_as<TPolicy, hip_policy>) {
-			iane_hip_free(aPointer);
+			return device::make_view<TElement, tDim>(first, aSize);
 		} else {
-			::operator delete(aPointer);
+			return host::make_view<TElement, tDim>(first, aSize);
 		}
 	}
+
+	std::size_t remaining() const noexcept { return mLength - mCursor; }
+
+private:
+	std::uintptr_t mBase;
+	std::size_t mCursor;
+	std::size_t mLength;
 };
+
+// Where a context's workspace comes from: host heap or device memory.
+template<class TPolicy>
+struct workspace_allocator {
+	static void* acquire(std::size_t aBytes) { return ::operator new(aBytes); }
+	static void release(void* aBlock) noexcept { ::operator delete(aBlock); }
+};
+
+template<>
+struct workspace_allocator<hip_policy> {
+	static void* acquire(std::size_t aBytes)
+	{
+		void* block = nullptr;
+		check_status(iane_hip_malloc(&block, aBytes));
+		return block;
+	}
+	static void release(void* aBlock) noexcept { iane_hip_free(aBlock); }
+};
+
+// Identity of a context, so an operation can refuse to run on another one.
+inline std::uint64_t next_context_serial() noexcept
+{
+	static std::atomic<std::uint64_t> counter{0};
+	return ++counter;
+}
 
 } // namespace detail
 
 /**
- * Prepare and run algorithms with one shared scratch allocation
- * (prepared.hpp:84-180). Calls on a context must be serialised by the
- * caller; a HIP context borrows a stream that must outlive it.
+ * Prepared algorithm state bound to the execution_context that reserved its
+ * scratch (the reference's prepared_operation, include/iane/execution/prepared.hpp:88-98).
+ * Only its context can run it.
+ */
+template<class TState>
+class prepared_operation {
+public:
+	prepared_operation(prepared_operation&&) noexcept = default;
+	prepared_operation& operator=(prepared_operation&&) noexcept = default;
+
+private:
+	template<class>
+	friend class execution_context;
+	prepared_operation(std::uint64_t aContext, TState&& aState) : mContext(aContext), mPrepared(std::move(aState)) {}
+	std::uint64_t mContext;
+	TState mPrepared;
+};
+
+/**
+ * Prepares algorithms and runs them with one workspace shared by all of them
+ * (the reference's execution_context, include/iane/execution/prepared.hpp:84-180).
+ *
+ * The workspace only grows: preparing an algorithm that needs more than the
+ * current capacity replaces it. A HIP context enqueues on the borrowed stream
+ * and reuses the workspace by stream order; before it replaces or frees the
+ * workspace it waits for the last submission it made. Submissions must come
+ * from one host thread at a time; overlapping work needs separate contexts.
  */
 template<class TPolicy>
 class execution_context {
-	template<class TState>
-	class prepared_operation {
-		friend class execution_context;
-		prepared_operation(const execution_context* aOwner, TState&& aState)
-			: mOwner(aOwner), mState(std::move(aState)) {}
-		const execution_context* mOwner;
-		TState mState;
-	};
+	using allocator = detail::workspace_allocator<TPolicy>;
+	static constexpr bool device_backend = std::same_as<TPolicy, hip_policy>;
 
 public:
-	explicit execution_context(TPolicy aPolicy) : mPolicy(aPolicy) {}
+	explicit execution_context(TPolicy aPolicy) : mPolicy(aPolicy), mSerial(detail::next_context_serial()) {}
 	execution_context(const execution_context&) = delete;
 	execution_context& operator=(const execution_context&) = delete;
 	~execution_context()
 	{
-		if constexpr (std::same_as<TPolicy, hip_policy>) {
-			if (mSubmitted) {
-				iane_hip_stream_sync(mPolicy.stream);
-			}
-		}
+		drain_noexcept();
+		allocator::release(mWorkspace);
 	}
 
+	/// Prepare `aAlgorithm` for views shaped like the prototypes and make room for its scratch.
 	template<class TAlgorithm, class TSrc, class TDst>
 	auto prepare(TAlgorithm aAlgorithm, TSrc aSource, TDst aDestination)
 	{
 		auto state = aAlgorithm.prepare(aSource, aDestination, mPolicy);
-		reserve(state.scratch_bytes());
-		return prepared_operation<decltype(state)>{this, std::move(state)};
+		grow(state.scratch_bytes());
+		return prepared_operation<decltype(state)>(mSerial, std::move(state));
 	}
 
+	/// Run a prepared operation on views of its prepared shape; returns the completion handle.
 	template<class TState, class TSrc, class TDst>
 	auto run(prepared_operation<TState>& aOperation, TSrc aSource, TDst aDestination)
 	{
-		if (aOperation.mOwner != this) {
+		if (aOperation.mContext != mSerial) {
 			throw std::invalid_argument("Prepared operation belongs to a different execution context");
 		}
-		if constexpr (std::same_as<TPolicy, hip_policy>) {
-			mSubmitted = true;
+		detail::scratch_space<TPolicy> scratch(mWorkspace, mBytes);
+		auto done = aOperation.mPrepared.run(aSource, aDestination, scratch, mPolicy);
+		if constexpr (device_backend) {
+			mInFlight = done;
 		}
-		detail::scratch_space<TPolicy> scratch(mScratch.get(), mCapacity);
-		return aOperation.mState.run(aSource, aDestination, scratch, mPolicy);
+		return done;
 	}
 
-	std::size_t scratch_capacity() const noexcept { return mCapacity; }
+	/// Bytes of the shared workspace currently held.
+	std::size_t scratch_capacity() const noexcept { return mBytes; }
 
 private:
-	void reserve(std::size_t aSize)
+	void grow(std::size_t aBytes)
 	{
-		if (aSize <= mCapacity) {
+		if (aBytes <= mBytes) {
 			return;
 		}
-		void* storage = nullptr;
-		if constexpr (std::same_as<TPolicy, hip_policy>) {
-			if (mSubmitted) {
-				hip_execution{mPolicy.stream}.wait();
-				mSubmitted = false;
+		if constexpr (device_backend) {
+			if (mInFlight) {
+				mInFlight->wait();
+				mInFlight.reset();
 			}
-			detail::check_status(iane_hip_malloc(&storage, aSize));
-		} else {
-			storage = ::operator new(aSize);
 		}
-		mScratch.reset(storage);
-		mCapacity = aSize;
+		void* larger = allocator::acquire(aBytes);
+		allocator::release(mWorkspace);
+		mWorkspace = larger;
+		mBytes = aBytes;
+	}
+
+	void drain_noexcept() noexcept
+	{
+		if constexpr (device_backend) {
+			if (mInFlight) {
+				iane_hip_stream_sync(mInFlight->stream);
+			}
+		}
 	}
 
 	TPolicy mPolicy;
-	std::unique_ptr<void, detail::scratch_deleter<TPolicy>> mScratch;
-	std::size_t mCapacity{};
-	bool mSubmitted{};
+	std::uint64_t mSerial;
+	void* mWorkspace = nullptr;
+	std::size_t mBytes = 0;
+	std::optional<hip_execution> mInFlight; // last device submission (HIP contexts only)
 };
 
 inline auto make_execution_context(cpu_policy aPolicy = {}) { return execution_context<cpu_policy>(aPolicy); }

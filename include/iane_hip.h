@@ -111,15 +111,31 @@ int iane_hip_event_elapsed_ms(float* ms, void* start, void* stop);
  * Kernel extents must be positive and odd (:107-113), ranks equal (:100-101),
  * dst extents equal src extents (include/iane/algorithms/transform.hpp:51-63).
  * Element types: src/kernel/dst each one of U8, U16, I32, F32, F64.     */
+/* Workspace: the split-precision MFMA path (SPLIT_BF16 / SPLIT_FP16 on rank <= 3
+ * images with float destinations and kernels up to 49 wide along x) needs
+ * caller scratch for the source's padded hi/lo planes, the kernel's Toeplitz
+ * table and their scales; every other path needs none (0 bytes; workspace may
+ * be NULL). The size depends only on the views' shapes and the mode. The
+ * workspace must start on a 256-byte boundary; too small a workspace fails
+ * with IANE_ERR_LENGTH. The library never allocates on this call (the
+ * reference's device work area comes from execution_context scratch,
+ * include/iane/execution/prepared.hpp:120-146, algorithms/fft/fft.hpp:136-144). */
+int iane_hip_convolve_workspace_bytes(const iane_view_desc* src, const iane_view_desc* kernel,
+	const iane_view_desc* dst, int border, int precision, size_t* bytes);
 int iane_hip_convolve(const iane_view_desc* src, const iane_view_desc* kernel,
-	const iane_view_desc* dst, int border, int precision, void* stream);
+	const iane_view_desc* dst, int border, int precision, void* workspace, size_t workspace_bytes, void* stream);
 
 /* Separable convolution: the kernel is the outer product taps[0] (axis 0)
  * x taps[1] x ... (each a rank-1 F32/F64 view of odd extent). Equal, up to
  * rounding, to iane_hip_convolve with the dense outer-product kernel.
  * No reference counterpart: the reference always runs the dense kernel. */
+int iane_hip_convolve_separable_workspace_bytes(const iane_view_desc* src, const iane_view_desc* taps,
+	int n_taps, const iane_view_desc* dst, int border, int precision, size_t* bytes);
+/* workspace: as for iane_hip_convolve; only the per-axis path (shapes the
+ * single-kernel path does not cover) needs it, for two intermediate images. */
 int iane_hip_convolve_separable(const iane_view_desc* src, const iane_view_desc* taps,
-	int n_taps, const iane_view_desc* dst, int border, int precision, void* stream);
+	int n_taps, const iane_view_desc* dst, int border, int precision, void* workspace, size_t workspace_bytes,
+	void* stream);
 
 /* ---- scan / reduce: replace the prepared reduce / reduce_along /
  * inclusive_scan / exclusive_scan / inclusive_scan_along / exclusive_scan_along
@@ -174,9 +190,12 @@ int iane_hip_rl_initial(const float* observed, float* estimate, int64_t count, f
 /* ratio = observed / max(predicted, eps) (main.cu:22-29); ratio may alias predicted.
  * ll_out != NULL: *ll_out (device double) = sum(y log max(p, eps) - p), the
  * Poisson log-likelihood without log(y!) (no reference counterpart): FP32
- * terms summed as a compensated FP32 pair, widened to double at the end.    */
+ * terms summed as a compensated FP32 pair, widened to double at the end; the
+ * per-block partial sums go to `workspace` (iane_hip_rl_ratio_workspace_bytes,
+ * 256-byte aligned; unused and may be NULL when ll_out is NULL).            */
+int iane_hip_rl_ratio_workspace_bytes(int64_t count, size_t* bytes);
 int iane_hip_rl_ratio(const float* observed, const float* predicted, float* ratio, int64_t count,
-	float epsilon, double* ll_out, void* stream);
+	float epsilon, double* ll_out, void* workspace, size_t workspace_bytes, void* stream);
 /* estimate *= (sens > 0 ? corr / max(sens, eps) : 0)           (main.cu:32-49,74-77) */
 int iane_hip_rl_update(float* estimate, const float* correction, const float* sensitivity,
 	int64_t count, float epsilon, void* stream);
@@ -263,8 +282,11 @@ int iane_hip_scale(float* x, int64_t n, float scale, void* stream);
  * separable passes, so tests can compare every path against the oracle. */
 int iane_hip_convolve_untiled(const iane_view_desc* src, const iane_view_desc* kernel,
 	const iane_view_desc* dst, int border, int precision, void* stream);
+int iane_hip_convolve_separable_passes_workspace_bytes(const iane_view_desc* src, const iane_view_desc* taps,
+	int n_taps, const iane_view_desc* dst, int border, int precision, size_t* bytes);
 int iane_hip_convolve_separable_passes(const iane_view_desc* src, const iane_view_desc* taps,
-	int n_taps, const iane_view_desc* dst, int border, int precision, void* stream);
+	int n_taps, const iane_view_desc* dst, int border, int precision, void* workspace, size_t workspace_bytes,
+	void* stream);
 
 #ifdef __cplusplus
 }

@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstring>
 #include <numbers>
+#include <numeric>
 #include <vector>
 
 namespace {
@@ -368,164 +369,164 @@ void rl_device_equals_host_loop()
 	CHECK(std::memcmp(result.pointer(), estimate.pointer(), sizeof(float) * size[0] * size[1]) == 0);
 }
 
-// Texture images (test_convolution_cuda.cu:10-163): uploads from strided host and
-// device views, convolution of a cropped texture view equal to the host
-// convolve for both borders, empty textures and moves, nested 1-D crops, and
-// 3-D uploads whose slice stride is not a whole number of rows.
+// Texture images: the cases of test_convolution_cuda.cu:10-163 as data.
 static_assert(iane::concepts::DeviceView<iane::texture_image_view<float>>);
 static_assert(!iane::concepts::MemoryView<iane::texture_image_view<float>>);
 static_assert(!iane::concepts::HostView<iane::texture_image_view<float>>);
 
-void texture_upload_from_strided_views()
+// One upload: the texels come from storage.subview(corner, size) of an int
+// image whose value at s is bias + sum_a weight[a] * s[a] (host storage with
+// the given byte pitch, 0 = packed, or a device copy of packed storage); the
+// texture is read back through texture.subview(outer).subview(inner).
+template<int N>
+struct texture_upload_case {
+	using ext = typename iane::math::dim_traits<N>::extents_t;
+	const char* name;
+	ext storage, pitch, corner, size;
+	bool from_device;
+	int bias;
+	ext weight;
+	ext outer_corner, outer_size, inner_corner, inner_size;
+};
+
+template<int N, class F>
+void for_each_coord(const typename iane::math::dim_traits<N>::extents_t& aSize, F&& aVisit)
 {
-	using coord_t = iane::math::vec2i;
-	const coord_t size{2, 2};
-	iane::host_image<int, 2> host(coord_t{4, 3});
-	for (int i = 0; i < 12; ++i) {
-		host.pointer()[i] = i + 101;
+	if (iane::math::detail::element_count<N>(aSize) == 0) {
+		return;
 	}
-	iane::host_image<int, 2> output(size);
-	iane::texture_image<int> texture(size);
-	iane::device_image<int, 2> linear(host.dimensions());
-	iane::device_image<int, 2> materialized(size);
-	void* stream = nullptr;
-	iane::detail::check_status(iane_hip_stream_create(&stream));
-	const iane::hip_policy policy{stream};
-	check_throws<std::invalid_argument>([&] { iane::copy(host.const_view(), texture, policy); }, "upload extent mismatch");
-	const auto verify = [&](int aOffset) {
-		iane::copy(texture.const_view(), materialized.view(), policy);
-		iane::copy(materialized.const_view(), output.view(), policy).wait();
-		const int expected[]{106, 107, 110, 111};
-		for (int i = 0; i < 4; ++i) {
-			CHECK(output.pointer()[i] == expected[i] + aOffset);
-		}
-	};
-	const coord_t origin{1, 1};
-	iane::copy(host.const_view().subview(origin, size), texture, policy);
-	verify(0);
-	for (int i = 0; i < 12; ++i) {
-		host.pointer()[i] += 100;
-	}
-	iane::copy(host.const_view(), linear.view(), policy);
-	iane::copy(linear.const_view().subview(origin, size), texture, policy);
-	verify(100);
-	iane::detail::check_status(iane_hip_stream_destroy(stream));
+	typename iane::math::dim_traits<N>::coord_t c{};
+	do {
+		aVisit(c);
+	} while (iane::math::detail::advance<N>(c, aSize));
 }
 
+template<int N>
+void run_texture_upload(const texture_upload_case<N>& aCase)
+{
+	using iane::math::detail::axis;
+	using coord_t = typename iane::math::dim_traits<N>::coord_t;
+	const auto value_at = [&](const coord_t& aStorage) {
+		int v = aCase.bias;
+		for (int a = 0; a < N; ++a) {
+			v += axis(aCase.weight, a) * axis(aStorage, a);
+		}
+		return v;
+	};
+	// host storage: packed or with the case's pitch
+	std::size_t span = sizeof(int);
+	bool packed = true;
+	for (int a = 0; a < N; ++a) {
+		packed = packed && axis(aCase.pitch, a) == 0;
+		span += static_cast<std::size_t>(axis(aCase.pitch, a)) * (axis(aCase.storage, a) - 1);
+	}
+	const auto layout = iane::resource::detail::packed_layout<int, N>(aCase.storage);
+	std::vector<int> bytes((packed ? layout.first * sizeof(int) : span) / sizeof(int) + 1, -7);
+	const iane::host_image_view<int, N> storage(bytes.data(), aCase.storage, packed ? layout.second : aCase.pitch);
+	for_each_coord<N>(aCase.storage, [&](const coord_t& c) { storage[c] = value_at(c); });
+
+	iane::texture_image<int, N> texture(aCase.size);
+	iane::device_image<int, N> linear(aCase.storage);
+	if (aCase.from_device) {
+		iane::copy(iane::const_host_image_view<int, N>(storage), linear.view()).wait();
+		iane::copy(linear.const_view().subview(aCase.corner, aCase.size), texture).wait();
+	} else {
+		iane::copy(iane::const_host_image_view<int, N>(storage).subview(aCase.corner, aCase.size), texture).wait();
+	}
+	iane::device_image<int, N> window(aCase.inner_size);
+	iane::host_image<int, N> result(aCase.inner_size);
+	iane::copy(texture.const_view().subview(aCase.outer_corner, aCase.outer_size).subview(aCase.inner_corner,
+		aCase.inner_size), window.view());
+	iane::copy(window.const_view(), result.view()).wait();
+	for_each_coord<N>(aCase.inner_size, [&](const coord_t& w) {
+		coord_t s{};
+		for (int a = 0; a < N; ++a) {
+			axis(s, a) = axis(aCase.corner, a) + axis(aCase.outer_corner, a) + axis(aCase.inner_corner, a) + axis(w, a);
+		}
+		++g_checks;
+		if (result.view()[w] != value_at(s)) {
+			++g_failures;
+			std::fprintf(stderr, "texture upload %s: wrong texel\n", aCase.name);
+		}
+	});
+}
+
+void texture_uploads()
+{
+	using v2 = iane::math::vec2i;
+	using v3 = iane::math::vec3i;
+	// test_convolution_cuda.cu:14-51 (values i + 101 on a 4x3 image, a 2x2 crop at (1, 1);
+	// once from the host view, once from a device copy shifted by 100)
+	const texture_upload_case<2> planar[] = {
+		{"2-D host crop", {4, 3}, {0, 0}, {1, 1}, {2, 2}, false, 101, {1, 4}, {0, 0}, {2, 2}, {0, 0}, {2, 2}},
+		{"2-D device crop", {4, 3}, {0, 0}, {1, 1}, {2, 2}, true, 201, {1, 4}, {0, 0}, {2, 2}, {0, 0}, {2, 2}},
+	};
+	// test_convolution_cuda.cu:110-163 (100 z + 10 y + x; host rows padded to 16 bytes and
+	// slices to 44, which is not a whole number of rows; a 5x4x4 device crop at (1, 1, 1)
+	// shifted by 1000; read back through the 2x2x2 window at (1, 0, 1))
+	const texture_upload_case<3> volumes[] = {
+		{"3-D padded host", {3, 2, 3}, {4, 16, 44}, {0, 0, 0}, {3, 2, 3}, false, 0, {1, 10, 100}, {0, 0, 0},
+			{3, 2, 3}, {1, 0, 1}, {2, 2, 2}},
+		{"3-D device crop", {5, 4, 4}, {0, 0, 0}, {1, 1, 1}, {3, 2, 3}, true, 1000 - 111, {1, 10, 100}, {0, 0, 0},
+			{3, 2, 3}, {1, 0, 1}, {2, 2, 2}},
+	};
+	// test_convolution_cuda.cu:98-108 (100 + i, a 4-sample crop at 1, read through
+	// subview(1, 3).subview(1, 2))
+	const texture_upload_case<1> lines[] = {
+		{"1-D nested crop", 6, 0, 1, 4, false, 100, 1, 1, 3, 1, 2},
+	};
+	for (const auto& c : planar) {
+		run_texture_upload(c);
+	}
+	for (const auto& c : volumes) {
+		run_texture_upload(c);
+	}
+	for (const auto& c : lines) {
+		run_texture_upload(c);
+	}
+
+	// rejected uploads and extents
+	iane::host_image<int, 2> wide(v2{4, 3});
+	iane::texture_image<int> small(v2{2, 2});
+	check_throws<std::invalid_argument>([&] { iane::copy(wide.const_view(), small); }, "upload extent mismatch");
+	int padded[33]{};
+	const iane::host_image_view<int, 3> gapped_rows(padded, v3{2, 2, 2}, v3{8, 16, 44});
+	iane::texture_image<int, 3> cube2(v3{2, 2, 2});
+	check_throws<std::invalid_argument>([&] { iane::copy(gapped_rows, cube2); }, "non-contiguous rows");
+	check_throws<std::invalid_argument>([] { iane::texture_image<float> bad(v2{-1, 2}); }, "negative extent");
+
+	// an empty texture allocates nothing, uploads nothing and survives a move
+	iane::texture_image<float> empty(v2{0, 3});
+	auto moved = std::move(empty);
+	iane::host_image<float, 2> nothing(v2{0, 3});
+	iane::copy(nothing.const_view(), moved).wait();
+	CHECK(moved.const_view().dimensions() == (v2{0, 3}));
+	CHECK(moved.array() == nullptr);
+}
+
+// test_convolution_cuda.cu:53-86 with a texture source: the 5x4 image i - 6, the
+// 3x3 kernel (i - 3) / 8, the 3x3 crop at (1, 1); equal to the host convolve for
+// both borders.
 void texture_convolution_matches_cpu()
 {
-	using coord_t = iane::math::vec2i;
-	const coord_t size{5, 4};
-	const coord_t crop_size{3, 3};
-	const coord_t origin{1, 1};
-	iane::host_image<float, 2> source(size), kernel(coord_t{3, 3}), reference(crop_size), output(crop_size);
-	for (int i = 0; i < 20; ++i) {
-		source.pointer()[i] = static_cast<float>(i - 6);
-	}
+	using v2 = iane::math::vec2i;
+	iane::host_image<float, 2> source(v2{5, 4}), kernel(v2{3, 3}), expected(v2{3, 3}), got(v2{3, 3});
+	std::iota(source.pointer(), source.pointer() + 20, -6.0f);
 	for (int i = 0; i < 9; ++i) {
 		kernel.pointer()[i] = static_cast<float>(i - 3) / 8.0f;
 	}
-	iane::texture_image<float> texture(size);
-	iane::device_image<float, 2> device_kernel(kernel.dimensions()), device_output(crop_size);
+	iane::texture_image<float> texture(source.dimensions());
+	iane::device_image<float, 2> dkernel(kernel.dimensions()), dout(got.dimensions());
 	iane::copy(source.const_view(), texture).wait();
-	iane::copy(kernel.const_view(), device_kernel.view()).wait();
+	iane::copy(kernel.const_view(), dkernel.view()).wait();
 	for (const auto border : {iane::border_mode::clamp, iane::border_mode::zero}) {
-		iane::convolve(source.const_view().subview(origin, crop_size), kernel.const_view(), reference.view(),
+		iane::convolve(source.const_view().subview(v2{1, 1}, v2{3, 3}), kernel.const_view(), expected.view(),
 			{.border = border});
-		iane::convolve(texture.const_view().subview(origin, crop_size), device_kernel.const_view(), device_output.view(),
+		iane::convolve(texture.const_view().subview(v2{1, 1}, v2{3, 3}), dkernel.const_view(), dout.view(),
 			{.border = border});
-		iane::copy(device_output.const_view(), output.view()).wait();
-		for (int i = 0; i < 9; ++i) {
-			CHECK(output.pointer()[i] == reference.pointer()[i]);
-		}
+		iane::copy(dout.const_view(), got.view()).wait();
+		CHECK(std::memcmp(got.pointer(), expected.pointer(), 9 * sizeof(float)) == 0);
 	}
-}
-
-void empty_texture_and_move()
-{
-	using coord_t = iane::math::vec2i;
-	const coord_t size{0, 3};
-	iane::texture_image<float> empty(size);
-	auto moved = std::move(empty);
-	iane::host_image<float, 2> source(size);
-	iane::copy(source.const_view(), moved).wait();
-	CHECK(moved.const_view().dimensions()[0] == size[0]);
-	CHECK(moved.const_view().dimensions()[1] == size[1]);
-	CHECK(moved.array() == nullptr);
-	check_throws<std::invalid_argument>([] { iane::texture_image<float> bad(coord_t{-1, 2}); }, "negative extent");
-}
-
-void texture_1d_nested_crop()
-{
-	iane::host_image<int, 1> source(6);
-	for (int i = 0; i < 6; ++i) {
-		source.view()[i] = 100 + i;
-	}
-	iane::texture_image<int, 1> texture(4);
-	iane::device_image<int, 1> materialized(2);
-	iane::host_image<int, 1> output(2);
-	iane::copy(source.const_view().subview(1, 4), texture);
-	iane::copy(texture.const_view().subview(1, 3).subview(1, 2), materialized.view());
-	iane::copy(materialized.const_view(), output.view()).wait();
-	CHECK(output.view()[0] == 103);
-	CHECK(output.view()[1] == 104);
-}
-
-void texture_3d_upload_from_strided_views()
-{
-	using coord_t = iane::math::vec3i;
-	const coord_t size{3, 2, 3};
-	const coord_t crop_size{2, 2, 2};
-	const coord_t origin{1, 0, 1};
-	iane::texture_image<int, 3> texture(size);
-	iane::device_image<int, 3> materialized(crop_size);
-	iane::host_image<int, 3> output(crop_size);
-	const auto verify = [&](int aOffset) {
-		iane::copy(texture.const_view().subview(origin, crop_size), materialized.view());
-		iane::copy(materialized.const_view(), output.view()).wait();
-		for (int z = 0; z < 2; ++z) {
-			for (int y = 0; y < 2; ++y) {
-				for (int x = 0; x < 2; ++x) {
-					CHECK((output.view()[coord_t{x, y, z}] == 100 * (z + 1) + 10 * y + x + 1 + aOffset));
-				}
-			}
-		}
-	};
-	// a slice stride (44 bytes) that is not a whole number of row pitches (16 bytes)
-	int padded[33]{};
-	const iane::host_image_view<int, 3> host(padded, size, coord_t{4, 16, 44});
-	for (int z = 0; z < 3; ++z) {
-		for (int y = 0; y < 2; ++y) {
-			for (int x = 0; x < 3; ++x) {
-				host[coord_t{x, y, z}] = 100 * z + 10 * y + x;
-			}
-		}
-	}
-	iane::copy(host, texture);
-	verify(0);
-	// a cropped device view as the source
-	const coord_t storage_size{5, 4, 4};
-	const coord_t storage_origin{1, 1, 1};
-	iane::host_image<int, 3> packed(storage_size);
-	for (int i = 0; i < 5 * 4 * 4; ++i) {
-		packed.pointer()[i] = -1;
-	}
-	auto crop = packed.view().subview(storage_origin, size);
-	for (int z = 0; z < 3; ++z) {
-		for (int y = 0; y < 2; ++y) {
-			for (int x = 0; x < 3; ++x) {
-				crop[coord_t{x, y, z}] = host[coord_t{x, y, z}] + 1000;
-			}
-		}
-	}
-	iane::device_image<int, 3> linear(storage_size);
-	iane::copy(packed.const_view(), linear.view());
-	iane::copy(linear.const_view().subview(storage_origin, size), texture);
-	verify(1000);
-	// rows must be contiguous
-	const iane::host_image_view<int, 3> strided_rows(padded, coord_t{2, 2, 2}, coord_t{8, 16, 44});
-	iane::texture_image<int, 3> small(coord_t{2, 2, 2});
-	check_throws<std::invalid_argument>([&] { iane::copy(strided_rows, small); }, "non-contiguous rows");
 }
 
 // Scan / reduce (test/unit/prepared/sequence_cases.hpp:31-117): a 263 x 5 u8
@@ -683,11 +684,8 @@ int main(int argc, char** argv)
 		real_fft_known_answers({5, 3});
 		real_fft_known_answers({6, 4});
 		rl_device_equals_host_loop();
-		texture_upload_from_strided_views();
+		texture_uploads();
 		texture_convolution_matches_cpu();
-		empty_texture_and_move();
-		texture_1d_nested_crop();
-		texture_3d_upload_from_strided_views();
 		sequence_case<true>();
 	}
 	std::printf("%s: %d checks, %d failures\n", device ? "device" : "host", g_checks, g_failures);

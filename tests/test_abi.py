@@ -40,7 +40,7 @@ def test_invalid_descriptor_reports_invalid_argument():
     import ctypes
     from iane_amd import _lib
     lib = _lib.load()
-    status = lib.iane_hip_convolve(None, None, None, 0, 0, None)
+    status = lib.iane_hip_convolve(None, None, None, 0, 0, None, 0, None)
     assert status == _lib.ERR_INVALID_ARGUMENT
     assert b"null" in lib.iane_hip_last_error()
     d = _lib.iane_view_desc()
@@ -51,7 +51,7 @@ def test_invalid_descriptor_reports_invalid_argument():
     k.rank = 2
     k.dtype = _lib.F32
     k.extent[0], k.extent[1] = 2, 3  # even kernel extent
-    status = lib.iane_hip_convolve(ctypes.byref(d), ctypes.byref(k), ctypes.byref(d), 0, 0, None)
+    status = lib.iane_hip_convolve(ctypes.byref(d), ctypes.byref(k), ctypes.byref(d), 0, 0, None, 0, None)
     assert status == _lib.ERR_INVALID_ARGUMENT
     assert b"odd" in lib.iane_hip_last_error()
     with pytest.raises(_lib.invalid_argument):

@@ -155,7 +155,7 @@ def test_strided_and_reversed_views():
         desc.ptr = dbase.data_ptr() + (base.shape[1] - 1) * desc.stride_bytes[1]
         desc.stride_bytes[1] = -desc.stride_bytes[1]
         out = empty(base.shape, np.float32)
-        _lib.call("iane_hip_convolve", desc, iane_amd.view_desc(dker), iane_amd.view_desc(out), border, 0, 0)
+        _lib.call("iane_hip_convolve", desc, iane_amd.view_desc(dker), iane_amd.view_desc(out), border, 0, 0, 0, 0)
         iane_amd.hip_execution(0).wait()
         same_bits(host(out), oracle.convolve(base[:, ::-1, :], ker, np.float32, border))
 
@@ -193,7 +193,7 @@ def test_streaming_kernel_strided_and_reversed_views():
                 out = empty(base.shape, ddt)
                 with autotune.applied({"conv.stream": stream}):
                     _lib.call("iane_hip_convolve", desc, iane_amd.view_desc(dker), iane_amd.view_desc(out), border,
-                              int(prec), 0)
+                              int(prec), 0, 0, 0)
                     iane_amd.hip_execution(0).wait()
                 outs.append(host(out))
             same_bits(outs[0], outs[1])

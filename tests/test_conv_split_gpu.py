@@ -10,6 +10,8 @@ derived from max|src| and max|kernel|).
 """
 from __future__ import annotations
 
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -110,8 +112,17 @@ def test_strided_views_and_double_destination():
     src = base[::2, 3:35, ::-1]
     out = torch.full((12, 32, 90), 77.0, dtype=torch.float64, device="cuda")
     dst_t = out[::2]                                # strided double destination
+    nbytes = ctypes.c_size_t()
+    _lib.call("iane_hip_convolve_workspace_bytes", desc, iane_amd.view_desc(dker), iane_amd.view_desc(dst_t),
+              cases.CLAMP, int(precision.split_fp16), ctypes.byref(nbytes))
+    assert nbytes.value > 0
+    ws = torch.empty(nbytes.value, dtype=torch.uint8, device="cuda")
+    # too small a workspace is refused before anything runs
+    with pytest.raises(_lib.length_error):
+        _lib.call("iane_hip_convolve", desc, iane_amd.view_desc(dker), iane_amd.view_desc(dst_t), cases.CLAMP,
+                  int(precision.split_fp16), ws.data_ptr(), nbytes.value - 1, 0)
     _lib.call("iane_hip_convolve", desc, iane_amd.view_desc(dker), iane_amd.view_desc(dst_t), cases.CLAMP,
-              int(precision.split_fp16), 0)
+              int(precision.split_fp16), ws.data_ptr(), nbytes.value, 0)
     iane_amd.hip_execution(0).wait()
     ref = oracle.convolve(np.ascontiguousarray(src), ker, np.float64, cases.CLAMP)
     emax, _ = errors(dst_t.cpu().numpy(), ref)

@@ -128,7 +128,7 @@ def test_pointwise_steps_match_float32_semantics():
     # against the FP64 oracle within 1e-6 relative (the terms' own log rounding), and
     # against the exact sum of FP32 terms within a few ulp of log per term (the sum
     # itself adds no error a plain FP32 sum of 10^4 terms would)
-    ll_ref = np.sum(y.astype(np.float64) * np.log(floor.astype(np.float64)) - p.astype(np.float64))
+    ll_ref = oracle.poisson_ll(y, p, 1e-6)
     got = host(ll)[0]
     assert abs(got - ll_ref) <= 1e-6 * abs(ll_ref)
     terms32 = (y * np.log(floor) - p).astype(np.float32)
@@ -189,3 +189,35 @@ def test_wiener_matches_oracle_and_delta_identity():
     ident = host(deconvolution.wiener(dev(img), dev(oracle.pad_psf_centered(delta, img.shape).astype(np.complex64)),
                                       1e-9))
     np.testing.assert_allclose(ident, img, rtol=1e-5, atol=1e-3)
+
+
+@pytest.mark.parametrize("d", [(1, -1), (-1, 0), (0, 2)])
+def test_wiener_off_centre_tap_shifts_back(d):
+    """A unit tap at offset d (5x5 PSF): with noise -> 0 the output is the
+    observation circularly shifted by -d (a missing conj or a reversed
+    centring would shift by +d)."""
+    rng = np.random.default_rng(3)
+    img = (rng.random((24, 30)) * 100).astype(np.float32)
+    psf = np.zeros((5, 5))
+    psf[2 + d[0], 2 + d[1]] = 1.0
+    padded = oracle.pad_psf_centered(psf, img.shape).astype(np.complex64)
+    got = host(deconvolution.wiener(dev(img), dev(padded), 1e-9))
+    np.testing.assert_allclose(got, np.roll(img, (-d[0], -d[1]), axis=(0, 1)), rtol=1e-5, atol=1e-3)
+
+
+def test_wiener_single_frequency_closed_form():
+    """A two-tap asymmetric PSF on one spatial frequency: amplitude |G| and phase
+    arg G with G = conj(H) / (|H|^2 + noise) at that frequency (|H|^2 ~ 0.04, so
+    the noise term changes the amplitude by ~5%)."""
+    shape, k, phase = (12, 16), (5, 7), 0.3
+    ii, jj = np.indices(shape)
+    theta = 2 * np.pi * (k[0] * ii / shape[0] + k[1] * jj / shape[1])
+    img = (100 * np.cos(theta + phase)).astype(np.float32)
+    psf = np.zeros((3, 3))
+    psf[1, 1], psf[1, 2] = 0.5, 0.45
+    h = 0.5 + 0.45 * np.exp(-2j * np.pi * k[1] / shape[1])
+    g = np.conj(h) / (abs(h) ** 2 + 0.002)
+    expect = 100 * abs(g) * np.cos(theta + phase + np.angle(g))
+    padded = oracle.pad_psf_centered(psf, shape).astype(np.complex64)
+    got = host(deconvolution.wiener(dev(img), dev(padded), 0.002))
+    assert rel(got, expect) <= 1e-5
