@@ -100,6 +100,7 @@ SIGNATURES = {
     "iane_hip_rl_initial": [vp, vp, i64, f32, vp],
     "iane_hip_rl_ratio_workspace_bytes": [i64, ctypes.POINTER(sz)],
     "iane_hip_rl_ratio": [vp, vp, vp, i64, f32, vp, vp, sz, vp],
+    "iane_hip_rl_ratio_noalloc": [vp, vp, vp, i64, f32, vp, vp],
     "iane_hip_rl_update": [vp, vp, vp, i64, f32, vp],
     "iane_hip_wiener_apply": [vp, vp, i64, f32, vp],
     "iane_hip_real_to_complex": [vp, vp, i64, vp],

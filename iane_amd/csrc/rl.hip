@@ -21,6 +21,7 @@
 #include "fft.hpp"
 
 #include <memory>
+#include <mutex>
 #include <vector>
 
 namespace iane_hip {
@@ -813,6 +814,75 @@ int iane_hip_rl_ratio(const float* aObserved, const float* aPredicted, float* aR
 			check_workspace(aWorkspace, aWorkspaceBytes, blocks * sizeof(ll_sum));
 			partials = static_cast<ll_sum*>(aWorkspace);
 		}
+		rl_ratio_kernel<<<blocks, kPointThreads, 0, stream>>>(aObserved, aPredicted, aRatio, aCount, aEps, partials);
+		check_hip(hipGetLastError(), "RL ratio launch");
+		if (aLlAccum) {
+			ll_finalize_kernel<<<1, kPointThreads, 0, stream>>>(partials, blocks, aLlAccum);
+			check_hip(hipGetLastError(), "RL log-likelihood launch");
+		}
+	});
+}
+
+} // extern "C"
+
+// ------------------------------------------------ allocation-free LL ratio --
+// iane_hip_rl_ratio_noalloc keeps the per-block log-likelihood partial sums in
+// this file-scope device array instead of a caller workspace. The ratio
+// kernel's grid is capped at kPartialBlocks blocks (point_blocks), so
+// kPartialBlocks compensated sums (8 KiB) serve every element count. Each GPU
+// holds its own copy for the life of the process; calls that overlap on one
+// GPU share it and race.
+namespace iane_hip {
+namespace {
+
+constexpr int kStaticPartials = kPartialBlocks;
+static_assert(sizeof(ll_sum) * kStaticPartials <= 8192, "the static scratch stays small");
+static __device__ __align__(256) ll_sum scratch[kStaticPartials];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_address[kMaxDevices] = {};
+
+// The address of `scratch` on the current device, looked up once per device
+// (host code must never pass the symbol itself to a kernel).
+ll_sum* device_scratch()
+{
+	int device = 0;
+	check_hip(hipGetDevice(&device), "Query current device");
+	if (device < 0 || device >= kMaxDevices) {
+		fail(IANE_ERR_UNSUPPORTED, "device index beyond the static scratch table");
+	}
+	std::lock_guard<std::mutex> lock(g_scratch_mutex);
+	if (!g_scratch_address[device]) {
+		void* address = nullptr;
+		check_hip(hipGetSymbolAddress(&address, HIP_SYMBOL(scratch)), "Locate static log-likelihood scratch");
+		g_scratch_address[device] = address;
+	}
+	return static_cast<ll_sum*>(g_scratch_address[device]);
+}
+
+} // namespace
+} // namespace iane_hip
+
+extern "C" {
+
+int iane_hip_rl_ratio_noalloc(const float* aObserved, const float* aPredicted, float* aRatio, int64_t aCount,
+	float aEps, double* aLlAccum, void* aStream)
+{
+	return guarded([&] {
+		if (aCount < 0 || (aCount && (!aObserved || !aPredicted || !aRatio))) {
+			invalid("RL buffers are null");
+		}
+		if (aCount == 0) {
+			return;
+		}
+		const unsigned blocks = point_blocks(aCount);
+		if (blocks > static_cast<unsigned>(kStaticPartials)) {
+			fail(IANE_ERR_LENGTH, "RL ratio needs more partial sums than the static scratch holds");
+		}
+		auto stream = static_cast<hipStream_t>(aStream);
+		ll_sum* partials = aLlAccum ? device_scratch() : nullptr;
+		// every partial the finalize step reads is written by this launch: nothing to clear
 		rl_ratio_kernel<<<blocks, kPointThreads, 0, stream>>>(aObserved, aPredicted, aRatio, aCount, aEps, partials);
 		check_hip(hipGetLastError(), "RL ratio launch");
 		if (aLlAccum) {

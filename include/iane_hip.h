@@ -196,6 +196,16 @@ int iane_hip_rl_initial(const float* observed, float* estimate, int64_t count, f
 int iane_hip_rl_ratio_workspace_bytes(int64_t count, size_t* bytes);
 int iane_hip_rl_ratio(const float* observed, const float* predicted, float* ratio, int64_t count,
 	float epsilon, double* ll_out, void* workspace, size_t workspace_bytes, void* stream);
+/* iane_hip_rl_ratio without a workspace argument: the log-likelihood partial
+ * sums live in a static device array of the library (1024 compensated sums,
+ * 8 KiB per GPU, held until the process ends), so the call allocates nothing.
+ * No reference counterpart: it is for callers that cannot allocate (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc). Results
+ * are the bits of iane_hip_rl_ratio. Every element count fits (the ratio
+ * kernel never uses more than 1024 blocks). All calls on one GPU share the
+ * array: calls that overlap on one GPU race.                                */
+int iane_hip_rl_ratio_noalloc(const float* observed, const float* predicted, float* ratio, int64_t count,
+	float epsilon, double* ll_out, void* stream);
 /* estimate *= (sens > 0 ? corr / max(sens, eps) : 0)           (main.cu:32-49,74-77) */
 int iane_hip_rl_update(float* estimate, const float* correction, const float* sensitivity,
 	int64_t count, float epsilon, void* stream);
