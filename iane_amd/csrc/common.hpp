@@ -79,6 +79,12 @@ inline int dtype_size(int aType)
 
 inline bool dtype_integral(int aType) { return aType == IANE_U8 || aType == IANE_U16 || aType == IANE_I32; }
 
+// Element bytes in device code (the host-side dtype_size also validates).
+__device__ __forceinline__ int dtype_size_device(int aType)
+{
+	return aType == IANE_U8 ? 1 : aType == IANE_U16 ? 2 : (aType == IANE_F64 || aType == IANE_C64) ? 8 : 4;
+}
+
 // Read one element of runtime type `aType` as double (exact for every type).
 __device__ __forceinline__ double load_as_double(const char* aPtr, int aType)
 {

@@ -23,6 +23,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <type_traits>
 #include <utility>
 
@@ -231,6 +232,28 @@ public:
 	T* pointer() const { return mPtr; }
 
 protected:
+	// Pointer, extents and strides of the rank N-1 view with axis tI fixed at aIndex.
+	template<int tI>
+	auto slice_parts(int aIndex) const
+	{
+		using lower = typename math::dim_traits<N - 1>::extents_t;
+		lower size{};
+		lower strides{};
+		int j = 0;
+		for (int i = 0; i < N; ++i) {
+			if (i == tI) {
+				continue;
+			}
+			math::detail::axis(size, j) = math::detail::axis(mSize, i);
+			math::detail::axis(strides, j) = math::detail::axis(mStrides, i);
+			++j;
+		}
+		using byte_t = std::conditional_t<std::is_const_v<T>, const char, char>;
+		T* first = reinterpret_cast<T*>(reinterpret_cast<byte_t*>(mPtr) +
+			static_cast<std::ptrdiff_t>(math::detail::axis(mStrides, tI)) * aIndex);
+		return std::tuple{first, size, strides};
+	}
+
 	T* at(const coord_t& aCoord) const
 	{
 		const auto offset = math::offset_in_strided_memory(mStrides, aCoord);
@@ -259,6 +282,14 @@ public:
 	{
 		return {this->at(aCorner), aSize, this->mStrides};
 	}
+	/// The rank N-1 view with axis tI fixed at aIndex, sharing the storage (host_image_view.hpp:59-75).
+	template<int tI>
+	requires(N > 1 && tI >= 0 && tI < N)
+	host_image_view<T, N - 1> slice(int aIndex) const
+	{
+		const auto [first, size, strides] = this->template slice_parts<tI>(aIndex);
+		return {first, size, strides};
+	}
 };
 
 template<class T, int N>
@@ -276,6 +307,14 @@ public:
 	{
 		return {this->at(aCorner), aSize, this->mStrides};
 	}
+	/// The rank N-1 view with axis tI fixed at aIndex, sharing the storage (host_image_view.hpp:59-75).
+	template<int tI>
+	requires(N > 1 && tI >= 0 && tI < N)
+	const_host_image_view<T, N - 1> slice(int aIndex) const
+	{
+		const auto [first, size, strides] = this->template slice_parts<tI>(aIndex);
+		return {first, size, strides};
+	}
 };
 
 /// Device view: device_image_view.hpp:11-60 (no host element access)
@@ -290,6 +329,14 @@ public:
 	device_image_view subview(const coord_t& aCorner, const extents_t& aSize) const
 	{
 		return {this->at(aCorner), aSize, this->mStrides};
+	}
+	/// The rank N-1 view with axis tI fixed at aIndex, sharing the storage (host_image_view.hpp:59-75).
+	template<int tI>
+	requires(N > 1 && tI >= 0 && tI < N)
+	device_image_view<T, N - 1> slice(int aIndex) const
+	{
+		const auto [first, size, strides] = this->template slice_parts<tI>(aIndex);
+		return {first, size, strides};
 	}
 };
 
@@ -306,6 +353,14 @@ public:
 	const_device_image_view subview(const coord_t& aCorner, const extents_t& aSize) const
 	{
 		return {this->at(aCorner), aSize, this->mStrides};
+	}
+	/// The rank N-1 view with axis tI fixed at aIndex, sharing the storage (host_image_view.hpp:59-75).
+	template<int tI>
+	requires(N > 1 && tI >= 0 && tI < N)
+	const_device_image_view<T, N - 1> slice(int aIndex) const
+	{
+		const auto [first, size, strides] = this->template slice_parts<tI>(aIndex);
+		return {first, size, strides};
 	}
 };
 
@@ -391,6 +446,15 @@ public:
 		return *this;
 	}
 	~basic_image() { Storage::release(mPtr); }
+
+	/// Release the storage and become empty (image.hpp:139-145).
+	void reset() noexcept
+	{
+		Storage::release(mPtr);
+		mPtr = nullptr;
+		mSize = extents_t{};
+		mStrides = extents_t{};
+	}
 
 	void resize(extents_t aSize)
 	{

@@ -6,5 +6,6 @@
 #include <iane/prepared.hpp>
 #include <iane/deconvolution.hpp>
 #include <iane/run_on.hpp>
+#include <iane/expression.hpp>
 #include <iane/texture.hpp>
 #include <iane/sequence.hpp>

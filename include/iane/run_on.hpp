@@ -22,11 +22,11 @@
 namespace iane {
 
 template<class TSrc, class TDst>
-requires(concepts::MemoryView<TSrc> && concepts::MemoryView<TDst>)
+requires(concepts::MemoryView<TSrc> && concepts::MemoryView<TDst> &&
+	(concepts::DeviceView<TSrc> || concepts::DeviceView<TDst>))
 auto copy(TSrc aSource, TDst aDestination, hip_policy aPolicy = {})
 {
 	static_assert(std::same_as<typename TSrc::value_type, typename TDst::value_type>);
-	static_assert(concepts::DeviceView<TSrc> || concepts::DeviceView<TDst>, "host-to-host copy is a plain loop");
 	constexpr int kind = concepts::DeviceView<TSrc> ? (concepts::DeviceView<TDst> ? 3 : 2) : 1;
 	using T = typename TSrc::value_type;
 	constexpr int code = detail::abi_element<T> ? detail::element_code<T>() :

@@ -137,6 +137,54 @@ int iane_hip_convolve_separable(const iane_view_desc* src, const iane_view_desc*
 	int n_taps, const iane_view_desc* dst, int border, int precision, void* workspace, size_t workspace_bytes,
 	void* stream);
 
+/* ---- pointwise expressions: the same-backend copy / transform of lazy views
+ * (include/iane/algorithms/copy.hpp:28-47 and transform.hpp:72-92 over the
+ * procedural views and view algebra of view/procedural.hpp:93-150 and
+ * view/algebra.hpp:52-151) for device images. The caller has no device
+ * compiler, so the per-pixel function is a register program: operation k
+ * writes register ops[k].dst from registers a, b, c (or loads input a, or
+ * sets `value`); registers are read only after an earlier operation wrote
+ * them; `result` is stored to dst. Every input has dst's extents; values are
+ * FP32 (inputs converted on load; the result assigned to the destination
+ * element, integers truncated toward zero and saturated, NaN -> 0).       */
+#define IANE_PW_MAX_INPUTS 8
+#define IANE_PW_MAX_OPS 32
+enum {
+	IANE_PW_INPUT = 0,  /* r[dst] = float(inputs[a][p]) */
+	IANE_PW_CONST = 1,  /* r[dst] = value */
+	IANE_PW_ADD = 2, IANE_PW_SUB = 3, IANE_PW_MUL = 4, IANE_PW_DIV = 5,
+	IANE_PW_MIN = 6,    /* std::min(a, b): b < a ? b : a */
+	IANE_PW_MAX = 7,    /* std::max(a, b): a < b ? b : a */
+	IANE_PW_NEG = 8, IANE_PW_ABS = 9, IANE_PW_SQRT = 10, IANE_PW_EXP = 11, IANE_PW_LOG = 12,
+	IANE_PW_SIN = 13, IANE_PW_COS = 14, IANE_PW_LOG1P = 15, IANE_PW_LOG2 = 16, IANE_PW_LOG10 = 17,
+	IANE_PW_EXP2 = 18,  /* unary: r[dst] = f(r[a]) */
+	IANE_PW_POW = 19,   /* powf(a, b) */
+	IANE_PW_LT = 20, IANE_PW_LE = 21, IANE_PW_GT = 22, IANE_PW_GE = 23, IANE_PW_EQ = 24,
+	IANE_PW_NE = 25,    /* 1.0f if the comparison holds, else 0.0f */
+	IANE_PW_SELECT = 26, /* r[a] != 0 ? r[b] : r[c] */
+	IANE_PW_FMA = 27     /* fmaf(r[a], r[b], r[c]) */
+};
+typedef struct iane_pw_op {
+	int32_t code;
+	int32_t dst, a, b, c;
+	float value;
+} iane_pw_op;
+typedef struct iane_pw_program {
+	int32_t n_inputs, n_ops, result;
+	iane_view_desc inputs[IANE_PW_MAX_INPUTS];
+	iane_pw_op ops[IANE_PW_MAX_OPS];
+} iane_pw_program;
+int iane_hip_pointwise(const iane_pw_program* program, const iane_view_desc* dst, void* stream);
+
+/* convolve(constant_view(dst, value), kernel, dst, {border}) — the sensitivity
+ * step of examples/rl_deconvolution/main.cu:62-63 — without materialising the
+ * constant image: every in-range (or clamped) sample is `value`, a zero-border
+ * sample outside dst's extent is 0; taps and arithmetic as iane_hip_convolve
+ * (precision REFERENCE: the reference's FP64 chain and output conversion;
+ * other modes accumulate in FP32). One thread per output: a prepare-time step. */
+int iane_hip_convolve_constant(double value, const iane_view_desc* kernel, const iane_view_desc* dst, int border,
+	int precision, void* stream);
+
 /* ---- scan / reduce: replace the prepared reduce / reduce_along /
  * inclusive_scan / exclusive_scan / inclusive_scan_along / exclusive_scan_along
  * (include/iane/algorithms/reduce.hpp:7-36, scan.hpp:7-46, built on

@@ -20,6 +20,9 @@
 #include <cstring>
 #include <numbers>
 #include <numeric>
+#include <set>
+#include <string>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -27,13 +30,13 @@ namespace {
 int g_failures = 0;
 int g_checks = 0;
 
-#define CHECK(cond)                                                                     \
-	do {                                                                                \
-		++g_checks;                                                                     \
-		if (!(cond)) {                                                                  \
-			++g_failures;                                                               \
-			std::fprintf(stderr, "%s:%d: check failed: %s\n", __FILE__, __LINE__, #cond); \
-		}                                                                               \
+#define CHECK(...)                                                                             \
+	do {                                                                                       \
+		++g_checks;                                                                            \
+		if (!(__VA_ARGS__)) {                                                                  \
+			++g_failures;                                                                      \
+			std::fprintf(stderr, "%s:%d: check failed: %s\n", __FILE__, __LINE__, #__VA_ARGS__); \
+		}                                                                                      \
 	} while (0)
 
 template<class E, class F>
@@ -655,12 +658,379 @@ void sequence_case()
 	}, "reduce_along extent mismatch");
 }
 
+// ---------------------------------------------------- expressions ------
+// copy_cases.hpp:38-84 (procedural copy over a crop, view algebra),
+// test_copy.cpp (laziness, zip extent check), transform_cases.hpp:37-70
+// (value / position / locator transforms), test_for_each.cpp:12-56 (empty
+// domain, the cpu_policy partition over a strided crop, errors reaching the caller).
+void host_expressions()
+{
+	using v2 = iane::math::vec2i;
+	{
+		iane::host_image<int, 2> input(v2{3, 2}), output(v2{2, 2});
+		std::iota(input.pointer(), input.pointer() + 6, 1);
+		const auto source = input.const_view().subview(v2{1, 0}, v2{2, 2});
+		const auto mapped = iane::map_view(source, [](int a) { return 2 * a; });
+		const auto column = iane::generate_view(source, [](const v2& p) { return p[0]; });
+		const auto bias = iane::zip_view(column, iane::constant_view(source, 1), std::plus<int>{});
+		static_assert(!iane::concepts::MemoryView<decltype(bias)>);
+		iane::copy(iane::zip_view(mapped, bias, std::plus<int>{}), output.view()).wait();
+		const int expected[] = {5, 8, 11, 14};
+		CHECK(std::equal(expected, expected + 4, output.pointer()));
+	}
+	{
+		iane::host_image<int, 1> input(3), output(3);
+		input.pointer()[0] = 2;
+		input.pointer()[1] = 3;
+		input.pointer()[2] = 5;
+		const auto s = input.const_view();
+		iane::copy((s * 2 + 1) + (20 / s) + -(10 - s), output.view());
+		const int expected[] = {7, 6, 10};
+		CHECK(std::equal(expected, expected + 3, output.pointer()));
+		// lazy: the map sees later writes; zip checks extents
+		const auto doubled = iane::map_view(s, [](int a) { return 2 * a; });
+		input.pointer()[1] = 9;
+		CHECK(doubled[1] == 18);
+		iane::host_image<int, 1> shorter(2);
+		check_throws<std::invalid_argument>([&] {
+			(void)iane::zip_view(doubled, iane::constant_view(shorter.const_view(), 1), std::plus<int>{});
+		}, "zip extents");
+		// the math functions of the algebra follow std:: on the pixel type
+		iane::host_image<float, 1> f(3), g(3);
+		f.pointer()[0] = 1.0f;
+		f.pointer()[1] = 4.0f;
+		f.pointer()[2] = 9.0f;
+		iane::copy(iane::sqrt(f.const_view()) + iane::log(f.const_view()), g.view());
+		CHECK(g.pointer()[1] == std::sqrt(4.0f) + std::log(4.0f));
+	}
+	{
+		// transform_cases.hpp:37-70 on a 3 x 3 x 3 pixel pattern
+		using v3 = iane::math::vec3i;
+		iane::host_image<pixel, 3> input(v3{3, 3, 3}), output(v3{3, 3, 3});
+		for (int i = 0; i < 27; ++i) {
+			input.pointer()[i] = {i + 1, -i - 1};
+		}
+		iane::transform(input.const_view(), output.view(), [](pixel a) { return a + pixel{1, -1}; });
+		iane::transform(output.const_view(), output.view(), [](pixel a) { return a + pixel{1, -1}; });
+		for (int i = 0; i < 27; ++i) {
+			CHECK(output.pointer()[i].first == i + 3 && output.pointer()[i].second == -i - 3);
+		}
+		iane::transform_position(input.const_view(), output.view(), [](pixel a, const v3& p) { return a + pixel{p[2], 0}; });
+		for (int i = 0; i < 27; ++i) {
+			CHECK(output.pointer()[i].first == i + 1 + i / 9);
+		}
+		iane::transform_locator(input.const_view(), output.view(), [](auto aLocator) {
+			return aLocator[v3{0, 0, 1}];
+		}, iane::border_mode::clamp);
+		for (int i = 0; i < 27; ++i) {
+			const int next = i / 9 == 2 ? i + 1 : i + 10;
+			CHECK(output.pointer()[i].first == next);
+		}
+		iane::transform_locator(input.const_view(), output.view(), [](auto aLocator) {
+			return aLocator[v3{0, 0, 1}];
+		}, iane::border_mode::zero);
+		CHECK(output.pointer()[26].first == 0 && output.pointer()[0].first == 10);
+		// image_locator on a crop: the crop's extent is the boundary
+		const auto crop = input.const_view().subview(v3{1, 1, 1}, v3{2, 2, 2});
+		const iane::image_locator<decltype(crop)> at_corner(crop, v3{0, 0, 0}, iane::border_mode::clamp);
+		CHECK(at_corner[v3{-1, 0, 0}].first == 14 && at_corner.position() == (v3{0, 0, 0}));
+		const iane::image_locator<decltype(crop)> zeroed(crop, v3{0, 0, 0}, iane::border_mode::zero);
+		CHECK(zeroed[v3{-1, 0, 0}].first == 0 && zeroed[v3{1, 1, 1}].first == 27);
+	}
+	{
+		// test_for_each.cpp:12-19, 21-41, 43-56
+		iane::host_image<int, 4> empty(iane::math::vec4i{2, 2, 0, 3});
+		int visits = 0;
+		iane::for_each(empty.view(), [&](int&) { ++visits; });
+		CHECK(visits == 0);
+		iane::host_image<int, 2> image(v2{7, 5});
+		iane::for_each(image.view(), [](int& a) { a = 0; });
+		const auto crop = image.view().subview(v2{1, 1}, v2{5, 3});
+		std::thread::id workers[15];
+		iane::for_each_position(crop, [&](int& a, const v2& p) {
+			++a;
+			workers[p[0] + 5 * p[1]] = std::this_thread::get_id();
+		}, iane::cpu_policy{.threads = 3, .grain_size = 4}).wait();
+		for (int y = 0; y < 5; ++y) {
+			for (int x = 0; x < 7; ++x) {
+				CHECK(image.const_view()[v2{x, y}] == (x >= 1 && x <= 5 && y >= 1 && y <= 3 ? 1 : 0));
+			}
+		}
+		CHECK(std::set<std::thread::id>(std::begin(workers), std::end(workers)).size() == 3u);
+		iane::host_image<int, 1> line(17), copy_out(17);
+		check_throws<std::invalid_argument>([&] {
+			iane::copy(line.const_view(), copy_out.view(), iane::cpu_policy{.threads = 0});
+		}, "threads 0");
+		check_throws<std::invalid_argument>([&] {
+			iane::for_each(line.view(), [](int&) {}, iane::cpu_policy{.grain_size = 0});
+		}, "grain 0");
+		check_throws<std::logic_error>([&] {
+			iane::for_each_position(line.view(), [](int&, int p) {
+				if (p == 16) {
+					throw std::logic_error("worker failure");
+				}
+			}, iane::cpu_policy{.threads = 3, .grain_size = 4});
+		}, "worker exception");
+		std::iota(line.pointer(), line.pointer() + 17, 1);
+		int located = 0;
+		iane::for_each_locator(line.const_view(), [&](auto aLocator) {
+			located += aLocator[1] == (aLocator.position() == 16 ? 0 : aLocator.position() + 2);
+		}, iane::border_mode::zero);
+		CHECK(located == 17);
+	}
+	{
+		// slice<tI>() and image reset()
+		iane::host_image<int, 3> volume(iane::math::vec3i{4, 3, 2});
+		std::iota(volume.pointer(), volume.pointer() + 24, 0);
+		const auto plane = volume.const_view().slice<2>(1);
+		CHECK(plane.dimensions() == (v2{4, 3}) && plane[v2{2, 1}] == 12 + 4 + 2);
+		const auto columns = volume.view().slice<0>(3);
+		CHECK(columns.dimensions() == (v2{3, 2}) && columns[v2{1, 1}] == 12 + 4 + 3);
+		const auto row = plane.slice<1>(2);
+		CHECK(row.dimensions() == 4 && row[1] == 12 + 8 + 1);
+		volume.reset();
+		CHECK(volume.pointer() == nullptr && volume.dimensions() == (iane::math::vec3i{0, 0, 0}));
+	}
+}
+
+// The pointwise functions of the RL loop (examples/rl_deconvolution/main.cu:11-49)
+// as iane::pointwise expressions: valid on host and device views alike.
+struct rl_functions {
+	float eps;
+	auto initial() const { using namespace iane::pointwise; return select(_1 > eps, _1, eps); }
+	auto ratio() const { using namespace iane::pointwise; return _1 / select(_2 > eps, _2, eps); }
+	auto edge() const { using namespace iane::pointwise; return select(_2 > 0.0f, _1 / select(_2 > eps, _2, eps), 0.0f); }
+	auto multiply() const { using namespace iane::pointwise; return _1 * _2; }
+};
+
+template<class TImage, int N>
+auto make_image(typename iane::math::dim_traits<N>::extents_t aSize)
+{
+	return TImage(aSize);
+}
+
+// prepare_rl of examples/rl_deconvolution/main.cu:51-81, stated with the same
+// calls: constant_view sensitivity, transform for the initial estimate, and
+// copy(zip_view(...)) updates between the two convolutions per iteration.
+template<class TImage, class TSrc, class TPsf, class TDst>
+void rl_as_the_reference_writes_it(TSrc aSource, TPsf aPsf, TPsf aAdjoint, TDst aDestination,
+	iane::view_policy<TDst> aPolicy, int aIterations, float aEpsilon)
+{
+	const auto size = aDestination.dimensions();
+	TImage estimate(size), predicted(size), correction(size), sensitivity(size);
+	const rl_functions f{aEpsilon};
+	const iane::convolution_options options{.border = iane::border_mode::zero};
+	iane::convolve(iane::constant_view(sensitivity.view(), 1.0f), aAdjoint, sensitivity.view(), options, aPolicy).wait();
+	iane::transform(aSource, estimate.view(), f.initial(), aPolicy);
+	for (int it = 0; it < aIterations; ++it) {
+		iane::convolve(estimate.const_view(), aPsf, predicted.view(), options, aPolicy);
+		iane::copy(iane::zip_view(aSource, predicted.const_view(), f.ratio()), predicted.view(), aPolicy);
+		iane::convolve(predicted.const_view(), aAdjoint, correction.view(), options, aPolicy);
+		const auto factor = iane::zip_view(correction.const_view(), sensitivity.const_view(), f.edge());
+		iane::copy(iane::zip_view(estimate.const_view(), factor, f.multiply()), estimate.view(), aPolicy);
+	}
+	iane::copy(estimate.const_view(), aDestination, aPolicy).wait();
+}
+
+std::vector<float> read_floats(const std::string& aPath, std::size_t aCount)
+{
+	std::vector<float> data(aCount);
+	FILE* f = std::fopen(aPath.c_str(), "rb");
+	const bool ok = f && std::fread(data.data(), sizeof(float), aCount, f) == aCount;
+	if (f) {
+		std::fclose(f);
+	}
+	if (!ok) {
+		throw std::runtime_error("cannot read " + aPath);
+	}
+	return data;
+}
+
+// The golden RL fixture in aDir (observed, psf, expected as raw float32 in
+// numpy order; meta.txt: rank, extents axis 0 first, psf extents, iterations,
+// epsilon) run through rl_as_the_reference_writes_it on host and device views.
+template<int N>
+void rl_golden_case(const std::string& aDir, const int* aExtent, const int* aPsfExtent, int aIterations, float aEps,
+	bool aDevice)
+{
+	using ext = typename iane::math::dim_traits<N>::extents_t;
+	ext size{}, ksize{};
+	std::size_t count = 1, kcount = 1;
+	for (int a = 0; a < N; ++a) {
+		iane::math::detail::axis(size, a) = aExtent[a];
+		iane::math::detail::axis(ksize, a) = aPsfExtent[a];
+		count *= aExtent[a];
+		kcount *= aPsfExtent[a];
+	}
+	const auto observed = read_floats(aDir + "/observed.f32", count);
+	const auto psf = read_floats(aDir + "/psf.f32", kcount);
+	const auto expected = read_floats(aDir + "/expected.f32", count);
+	std::vector<float> flipped(psf.rbegin(), psf.rend());
+	iane::host_image<float, N> result(size);
+	if (!aDevice) {
+		const auto obs = iane::host::make_const_view<float, N>(observed.data(), size);
+		const auto h = iane::host::make_const_view<float, N>(psf.data(), ksize);
+		const auto ht = iane::host::make_const_view<float, N>(flipped.data(), ksize);
+		rl_as_the_reference_writes_it<iane::host_image<float, N>>(obs, h, ht, result.view(), iane::cpu_policy{},
+			aIterations, aEps);
+	} else {
+		iane::device_image<float, N> dobs(size), dpsf(ksize), dflip(ksize), dout(size);
+		iane::copy(iane::host::make_const_view<float, N>(observed.data(), size), dobs.view()).wait();
+		iane::copy(iane::host::make_const_view<float, N>(psf.data(), ksize), dpsf.view()).wait();
+		iane::copy(iane::host::make_const_view<float, N>(flipped.data(), ksize), dflip.view()).wait();
+		rl_as_the_reference_writes_it<iane::device_image<float, N>>(dobs.const_view(), dpsf.const_view(),
+			dflip.const_view(), dout.view(), iane::hip_policy{}, aIterations, aEps);
+		iane::copy(dout.const_view(), result.view()).wait();
+	}
+	CHECK(std::memcmp(result.pointer(), expected.data(), count * sizeof(float)) == 0);
+}
+
+int rl_golden(const std::string& aDir, bool aDevice)
+{
+	FILE* f = std::fopen((aDir + "/meta.txt").c_str(), "r");
+	if (!f) {
+		std::fprintf(stderr, "no meta.txt in %s\n", aDir.c_str());
+		return 2;
+	}
+	int rank = 0, extent[4] = {}, kext[4] = {}, iterations = 0;
+	float eps = 0.0f;
+	int ok = std::fscanf(f, "%d", &rank) == 1;
+	for (int a = 0; ok && a < rank; ++a) {
+		ok = std::fscanf(f, "%d", &extent[a]) == 1;
+	}
+	for (int a = 0; ok && a < rank; ++a) {
+		ok = std::fscanf(f, "%d", &kext[a]) == 1;
+	}
+	ok = ok && std::fscanf(f, "%d %f", &iterations, &eps) == 2;
+	std::fclose(f);
+	if (!ok) {
+		return 2;
+	}
+	if (rank == 2) {
+		rl_golden_case<2>(aDir, extent, kext, iterations, eps, aDevice);
+	} else {
+		rl_golden_case<3>(aDir, extent, kext, iterations, eps, aDevice);
+	}
+	return 0;
+}
+
+// The device side of the expression API: the pointwise programs against the
+// same functions evaluated on the host (bit-exact for + - * / select and the
+// comparisons), the procedural copy and view algebra of copy_cases.hpp on
+// device views, device for_each on the stream it was given
+// (test_for_each_cuda.cu:19-36), and errors.
+void device_expressions()
+{
+	using v2 = iane::math::vec2i;
+	using namespace iane::pointwise;
+	const v2 size{37, 23};
+	iane::host_image<float, 2> a(size), b(size), got(size);
+	iane::host_image<std::uint16_t, 2> c(size);
+	for (int i = 0; i < size[0] * size[1]; ++i) {
+		a.pointer()[i] = static_cast<float>((i * 7919) % 1000) / 37.0f - 9.0f;
+		b.pointer()[i] = static_cast<float>((i * 104729) % 997) / 13.0f;
+		c.pointer()[i] = static_cast<std::uint16_t>((i * 31) % 65536);
+	}
+	iane::device_image<float, 2> da(size), db(size), dout(size);
+	iane::device_image<std::uint16_t, 2> dc(size);
+	iane::copy(a.const_view(), da.view()).wait();
+	iane::copy(b.const_view(), db.view()).wait();
+	iane::copy(c.const_view(), dc.view()).wait();
+	const auto f = select(_1 > _2, (_1 - _2) / (_2 + 1.5f), min(_1 * _3, max(_2, 0.25f))) + sqrt(abs(_1)) * 0.5f;
+	const auto ea = iane::device_expression<2>::load(da.const_view());
+	const auto eb = iane::device_expression<2>::load(db.const_view());
+	const auto ec = iane::device_expression<2>::load(dc.const_view());
+	const auto dev = iane::device_expression<2>::apply(size, f, {&ea, &eb, &ec});
+	iane::copy(dev, dout.view()).wait();
+	iane::copy(dout.const_view(), got.view()).wait();
+	bool same = true;
+	for (int i = 0; i < size[0] * size[1]; ++i) {
+		same = same && got.pointer()[i] == f(a.pointer()[i], b.pointer()[i], c.pointer()[i]);
+	}
+	CHECK(same);
+	// strided (every other column) destination and a reversed source through zip_view + algebra
+	iane::device_image<float, 2> wide(v2{2 * size[0], size[1]});
+	const iane::device_image_view<float, 2> every_other(wide.pointer(), size, v2{8, 8 * size[0]});
+	iane::copy((da.const_view() + db.const_view()) * 2.0f - 1.0f, every_other).wait();
+	iane::host_image<float, 2> wide_host(v2{2 * size[0], size[1]});
+	iane::copy(wide.const_view(), wide_host.view()).wait();
+	same = true;
+	for (int y = 0; y < size[1]; ++y) {
+		for (int x = 0; x < size[0]; ++x) {
+			const float expect = (a.const_view()[v2{x, y}] + b.const_view()[v2{x, y}]) * 2.0f - 1.0f;
+			same = same && wide_host.const_view()[v2{2 * x, y}] == expect;
+		}
+	}
+	CHECK(same);
+	// copy_cases.hpp:38-84 on device views (generate_view aside: no device compiler here)
+	{
+		iane::host_image<int, 2> input(v2{3, 2}), output(v2{2, 2});
+		std::iota(input.pointer(), input.pointer() + 6, 1);
+		iane::device_image<int, 2> din(input.dimensions()), dres(output.dimensions());
+		iane::copy(input.const_view(), din.view()).wait();
+		const auto source = din.const_view().subview(v2{1, 0}, v2{2, 2});
+		const auto mapped = iane::map_view(source, _1 * 2);
+		iane::copy(iane::zip_view(mapped, iane::constant_view(source, 1), _1 + _2), dres.view()).wait();
+		iane::copy(dres.const_view(), output.view()).wait();
+		const int expected[] = {5, 7, 11, 13};
+		CHECK(std::equal(expected, expected + 4, output.pointer()));
+		iane::host_image<int, 1> s(3), r(3);
+		s.pointer()[0] = 2;
+		s.pointer()[1] = 3;
+		s.pointer()[2] = 5;
+		iane::device_image<int, 1> ds(3), dr(3);
+		iane::copy(s.const_view(), ds.view()).wait();
+		const auto v = ds.const_view();
+		iane::copy((v * 2 + 1) + (20 / v) + -(10 - v), dr.view()).wait();
+		iane::copy(dr.const_view(), r.view()).wait();
+		const int algebra[] = {7, 6, 10};
+		CHECK(std::equal(algebra, algebra + 3, r.pointer()));
+	}
+	// device for_each on the caller's stream
+	{
+		iane::host_image<int, 1> output(17);
+		iane::run_on(iane::run_options{.backend = iane::backend::hip}, output.view(), [](auto aDst, auto aExecution) {
+			if constexpr (iane::concepts::DeviceView<decltype(aDst)>) {
+				const iane::hip_policy policy{.stream = aExecution.stream};
+				const auto result = iane::for_each(aDst, value(42.0f), policy);
+				CHECK(result.stream == aExecution.stream);
+				return result;
+			} else {
+				return aExecution;
+			}
+		});
+		CHECK(std::all_of(output.pointer(), output.pointer() + 17, [](int x) { return x == 42; }));
+	}
+	check_throws<std::invalid_argument>([&] {
+		iane::device_image<float, 2> other(v2{3, 3});
+		(void)iane::zip_view(da.const_view(), other.const_view(), _1 + _2);
+	}, "device zip extents");
+	check_throws<std::invalid_argument>([&] {
+		iane::device_image<float, 2> k(v2{3, 3});
+		iane::convolve(da.const_view() * 2.0f, k.const_view(), dout.view()).wait();
+	}, "expression convolve source");
+}
+
 } // namespace
 
 int main(int argc, char** argv)
 {
+	if (argc > 2 && std::strcmp(argv[1], "rl-golden") == 0) {
+		const bool on_device = argc > 3 && std::strcmp(argv[3], "device") == 0;
+		if (on_device) {
+			int n = 0;
+			if (iane_hip_device_count(&n) != IANE_OK || n == 0) {
+				std::puts("no HIP device: skipped");
+				return 77;
+			}
+		}
+		const int status = rl_golden(argv[2], on_device);
+		std::printf("rl-golden %s: %d checks, %d failures\n", on_device ? "device" : "host", g_checks, g_failures);
+		return status ? status : (g_failures ? 1 : 0);
+	}
 	const bool device = argc > 1 && std::strcmp(argv[1], "device") == 0;
 	if (!device) {
+		host_expressions();
 		pattern_host<2>();
 		pattern_host<3>();
 		pattern_host<4>();
@@ -686,6 +1056,7 @@ int main(int argc, char** argv)
 		rl_device_equals_host_loop();
 		texture_uploads();
 		texture_convolution_matches_cpu();
+		device_expressions();
 		sequence_case<true>();
 	}
 	std::printf("%s: %d checks, %d failures\n", device ? "device" : "host", g_checks, g_failures);
