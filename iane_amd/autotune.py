@@ -6,16 +6,19 @@ changes a result: every candidate computes each output sample with the same
 operations in the same order (tests check bit equality across candidates),
 so tuning only trades occupancy, LDS use and tile overlap.
 
-    sep3.waves      2 | 3                  waves per SIMD the matrix engine's registers are fitted to
-    sep3.chunk      0 (auto) | 16 .. 64    z planes streamed per workgroup (both engines)
+    sep3.waves      1 | 2 (engine 2)       waves per SIMD the matrix engines' registers are fitted to
+                    2 | 3 (engine 0)
+    sep3.chunk      0 (auto) | 16 .. 80    z planes streamed per workgroup (every engine)
     sep3.threads    256 | 512              threads per workgroup (vector engine)
     sep3.tile_y     16 | 32                output rows per workgroup tile (vector engine)
 
     fft.lds_elems   1024 .. 4096           LDS budget per buffer (complex elements) of the
                                            FFT line passes and RL row steps: lines per workgroup
 
-``sep3.engine`` (0: x/y filters on FP32 MFMA, 1: vector ALUs) is not tuned: the
-engines sum the same FP32 products in different orders, so it changes bits.
+``sep3.engine`` (2: x/y filters on FP32 MFMA with per-wave operand loads, the
+default for zero borders and 4-aligned rows; 0: the same on LDS-staged planes;
+1: vector ALUs) is not tuned: the engines sum the same FP32 products in
+different orders, so it changes bits.
 Direct-convolution switches, read at launch and not searched (every setting
 gives the same bits; tests compare them):
 
@@ -37,7 +40,8 @@ import pathlib
 
 from . import _lib
 
-SEPARABLE_SPACE = {"sep3.waves": [2, 3], "sep3.chunk": [0, 16, 21, 24, 32, 43, 64]}  # matrix-core engine (default)
+SEPARABLE_SPACE = {"sep3.waves": [1, 2], "sep3.chunk": [0, 32, 48, 64, 80]}  # barrier-free matrix engine (default)
+STAGED_SEPARABLE_SPACE = {"sep3.waves": [2, 3], "sep3.chunk": [0, 16, 21, 24, 32, 43, 64]}  # LDS-staged matrix engine
 VECTOR_SEPARABLE_SPACE = {"sep3.threads": [512, 256], "sep3.tile_y": [16, 32], "sep3.chunk": [0, 16, 32, 64]}
 FFT_SPACE = {"fft.lds_elems": [4096, 3072, 2048, 1536, 1024]}
 RL_BATCH_SPACE = FFT_SPACE
@@ -165,8 +169,9 @@ def tune_separable(src, taps, dst, options=None, policy=None, repeats: int = 10,
     import iane_amd
     policy = policy or iane_amd.hip_policy()
     stream = policy.handle()
-    engine = max(get("sep3.engine"), 0)
-    space = VECTOR_SEPARABLE_SPACE if engine == 1 else SEPARABLE_SPACE
+    engine = get("sep3.engine")
+    engine = 2 if engine < 0 else engine
+    space = {0: STAGED_SEPARABLE_SPACE, 1: VECTOR_SEPARABLE_SPACE}.get(engine, SEPARABLE_SPACE)
     key = f"{device_name()}|sep3.engine{engine}|{tuple(src.shape)}|{tuple(int(t.numel()) for t in taps)}"
     time, close = _event_timer(stream)
     try:

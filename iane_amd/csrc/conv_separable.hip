@@ -11,11 +11,13 @@
 // K-1 halo in x and y into LDS, filters it along x and y, and pushes the
 // filtered plane into a per-thread register ring of K planes from which the z
 // filter produces one output plane per input plane. Input is read once per
-// chunk (plus the z halo), output written once. Two engines share this plan:
-// sep3_mfma_kernel runs the x and y filters as FP32 Toeplitz products on the
-// matrix cores (default), sep3_fused_kernel on the vector ALUs.
+// chunk (plus the z halo), output written once. Three engines share this plan:
+// sep3_direct_kernel (default for zero borders and 4-aligned rows) and
+// sep3_mfma_kernel run the x and y filters as FP32 Toeplitz products on the
+// matrix cores, the first with per-wave operand loads and no LDS, the second
+// with the plane staged in LDS; sep3_fused_kernel uses the vector ALUs only.
 //
-// General path: one 1-D pass per axis through stream-ordered temporaries.
+// General path: one 1-D pass per axis through temporaries in the caller's workspace.
 #include "common.hpp"
 
 namespace iane_hip {
@@ -450,7 +452,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))) voi
 				break;
 			}
 			// buffer p & 1 is staged, and every wave is past plane p - 1, whose
-			// buffer (p + 1) & 1 is restaged below
+			// buffer (p + 1) & 1 is restaged next
 			lds_barrier();
 			const float* S = stage[p & 1];
 			// A operands of both row tiles first (four 128-bit reads), then the MFMA chains
@@ -461,6 +463,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))) voi
 				av[mt][0] = *reinterpret_cast<const mx_f32x4*>(rowp);       // K-steps 0..3 (block w)
 				av[mt][1] = *reinterpret_cast<const mx_f32x4*>(rowp + 16);  // K-steps 4..7 (block w + 1)
 			}
+			// buffer (p + 1) & 1 was last read in iteration p - 1, which every wave
+			// has left: stage plane p + 1 and fetch plane p + 1 + P before the MFMAs
+			const int next_slot = (j + 1) % P;
+			stage_plane(p + 1, next_slot);
+			fetch(p + 1 + P, next_slot);
 			// A wave issues in order, so the z filter of plane p - 1 (vector ALU)
 			// is written between the MFMAs of plane p: two taps per x-pass step,
 			// while the matrix pipe works on the previous MFMA.
@@ -490,18 +497,183 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))) voi
 					}
 				}
 			}
-			mx_f32x4 u = mx_f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+			// y filter: two accumulation chains (row tiles), summed at the end
+			mx_f32x4 u0 = mx_f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+			mx_f32x4 u1 = mx_f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-			for (int s = 0; s < 8; ++s) {
-				u = mfma_f32_16x16x4(ay[s], t[s / 4][s % 4], u);
+			for (int s = 0; s < 4; ++s) {
+				u0 = mfma_f32_16x16x4(ay[s], t[0][s], u0);
+				u1 = mfma_f32_16x16x4(ay[s + 4], t[1][s], u1);
 			}
 			z_store(p - 1, zout);
-			const int next_slot = (j + 1) % P;
-			stage_plane(p + 1, next_slot);
-			fetch(p + 1 + P, next_slot);
 #pragma unroll
 			for (int r = 0; r < 4; ++r) {
-				prev[r] = u[r];
+				prev[r] = u0[r] + u1[r];
+			}
+		}
+	}
+}
+
+// ---------------------------------------------------------------------------
+// Barrier-free matrix-core variant (engine 2) for zero borders and widths that
+// are multiples of 4: no LDS staging. Each wave owns a 16 x 16 output tile of
+// every plane of its z chunk and loads its own x-pass operands from memory:
+// lane (g, m) of row tile mt needs, for K-step s, window column
+// c(s, g) = 16 (s / 4) + 4 g + s % 4 of staged row 16 mt + m (the window starts
+// 8 columns and 8 rows before the tile), i.e. two 16-byte groups per row tile,
+// which arrive as two aligned buffer_load_dwordx4 (a group lies wholly inside
+// or wholly outside the image, and outside reads 0). The Toeplitz operands use
+// the same column order, so no sample moves between lanes.
+//
+// The loop is software-pipelined over planes so that four independent MFMA
+// chains interleave: iteration p runs the x filter of plane p (two row tiles)
+// and the y filter of plane p - 1 (two partial sums), while the vector ALUs run
+// the z filter that completes output plane p - 2 - (K - 1) from the ring.
+template<int K, int W>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))) void sep3_direct_kernel(sep_args a)
+{
+	constexpr int C = K / 2;
+	constexpr int R = K + 1;                                    // ring slots = iterations per unrolled group
+	constexpr int P = (R % 4 == 0) ? 4 : ((R % 3 == 0) ? 3 : 2);  // planes of loads in flight (P divides R)
+	const unsigned block = xcd_remap(blockIdx.x, gridDim.x);
+	const unsigned tile_x = block % a.tiles_x;
+	const unsigned rest = block / a.tiles_x;
+	const unsigned tile_y = rest % a.tiles_y;
+	const unsigned chunk = rest / a.tiles_y;
+	const int lane = threadIdx.x & 63;
+	const int wave = threadIdx.x >> 6;
+	const int g = lane >> 4;
+	const int n = lane & 15;
+	const int x0 = static_cast<int>(tile_x) * kMxTileX + 16 * wave;  // this wave's 16 output columns
+	const int y0 = static_cast<int>(tile_y) * kMxTileY;
+	const int z0 = static_cast<int>(chunk) * a.chunk;
+	const int zend = min(z0 + a.chunk, a.nz);
+
+	const auto tap = [&](int aAxis, int aIndex) {
+		return (aIndex >= 0 && aIndex < K) ? load_as_float(a.taps[aAxis] + aIndex * a.tap_stride[aAxis], a.tap_type[aAxis])
+		                                   : 0.0f;
+	};
+	// window position of K-step s for lane group g (columns for x, rows for y)
+	const auto window = [&](int aStep) { return 16 * (aStep / 4) + 4 * g + aStep % 4; };
+	float bx[8], ay[8], kz[K];
+#pragma unroll
+	for (int s = 0; s < 8; ++s) {
+		bx[s] = tap(0, n + 8 + C - window(s));
+		ay[s] = tap(1, n + 8 + C - window(s));
+	}
+#pragma unroll
+	for (int i = 0; i < K; ++i) {
+		kz[i] = uniform_float(load_as_float(a.taps[2] + i * a.tap_stride[2], a.tap_type[2]));
+	}
+	// the lane's four 16-byte groups per plane: row tile mt, half h (columns 16 h + 4 g .. + 3)
+	unsigned offset[2][2];
+#pragma unroll
+	for (int mt = 0; mt < 2; ++mt) {
+		const int yi = y0 - 8 + 16 * mt + n;
+#pragma unroll
+		for (int h = 0; h < 2; ++h) {
+			const int xi = x0 - 8 + 16 * h + 4 * g;
+			const bool valid = yi >= 0 && yi < a.ny && xi >= 0 && xi + 3 < a.nx;
+			offset[mt][h] = valid ? static_cast<unsigned>(xi * a.sx + yi * a.sy) : static_cast<unsigned>(a.src_span);
+		}
+	}
+	const int ox = x0 + n;
+	const int oy = y0 + 4 * g;
+	unsigned store_offset[4];
+#pragma unroll
+	for (int r = 0; r < 4; ++r) {
+		store_offset[r] = (ox < a.nx && oy + r < a.ny) ? static_cast<unsigned>(ox * a.dx + (oy + r) * a.dy)
+		                                               : static_cast<unsigned>(a.dst_span);
+	}
+
+	const int planes = (zend - z0) + K - 1;  // input planes z0 - C .. zend - 1 + C
+	mx_f32x4 pre[P][2][2];
+	auto fetch = [&](int aPlane, int aSlot) {
+		const int zi = z0 - C + aPlane;
+		const bool in = aPlane < planes && zi >= 0 && zi < a.nz;
+		const auto plane = plane_resource(a.src + (in ? zi : 0) * a.sz, in ? a.src_span : 0);
+#pragma unroll
+		for (int mt = 0; mt < 2; ++mt) {
+#pragma unroll
+			for (int h = 0; h < 2; ++h) {
+				pre[aSlot][mt][h] = __builtin_bit_cast(mx_f32x4,
+					__builtin_amdgcn_raw_buffer_load_b128(plane, static_cast<int>(offset[mt][h]), 0, 0));
+			}
+		}
+	};
+	auto z_store = [&](int aPlane, const float* aOut) {
+		const int z = z0 + aPlane - (K - 1);
+		const bool ok = z >= z0 && z < zend;
+		const auto plane = plane_resource(a.dst + (ok ? z : 0) * a.dz, ok ? a.dst_span : 0);
+#pragma unroll
+		for (int r = 0; r < 4; ++r) {
+			__builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, aOut[r]), plane,
+				static_cast<int>(store_offset[r]), 0, 0);
+		}
+	};
+	float ring[R][4];  // y-filtered planes by plane index mod R (K of them feed an output)
+#pragma unroll
+	for (int j = 0; j < R; ++j) {
+#pragma unroll
+		for (int r = 0; r < 4; ++r) {
+			ring[j][r] = 0.0f;
+		}
+	}
+#pragma unroll
+	for (int q = 0; q < P; ++q) {
+		fetch(q, q);
+	}
+	const mx_f32x4 zero4 = mx_f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+	mx_f32x4 t_prev[2] = {zero4, zero4};  // x filter of plane p - 1
+	float u_prev[4] = {0.0f, 0.0f, 0.0f, 0.0f};  // y filter of plane p - 2
+	// iteration p: x of plane p, y of plane p - 1, z completing plane p - 2
+	// Whole groups of R iterations: the group is fully unrolled (prefetch and
+	// ring slots are compile-time indices), which an early exit inside it would
+	// prevent. Iterations past planes + 1 load empty planes and store nothing;
+	// the host picks z chunks that are multiples of R, for which there are none
+	// (a chunk of c planes runs c + R iterations).
+	for (int base = 0; base <= planes + 1; base += R) {
+#pragma unroll
+		for (int j = 0; j < R; ++j) {
+			const int p = base + j;
+			const int slot = j % P;
+			const int zr = (j + R - 2) % R;  // ring slot of plane p - 2
+#pragma unroll
+			for (int r = 0; r < 4; ++r) {
+				ring[zr][r] = u_prev[r];
+			}
+			mx_f32x4 t[2] = {zero4, zero4};
+			mx_f32x4 u0 = zero4;
+			mx_f32x4 u1 = zero4;
+			float zout[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+			for (int s = 0; s < 8; ++s) {
+#pragma unroll
+				for (int mt = 0; mt < 2; ++mt) {
+					t[mt] = mfma_f32_16x16x4(pre[slot][mt][s / 4][s % 4], bx[s], t[mt]);
+				}
+				if (s < 4) {
+					u0 = mfma_f32_16x16x4(ay[s], t_prev[0][s], u0);
+				} else {
+					u1 = mfma_f32_16x16x4(ay[s], t_prev[1][s - 4], u1);
+				}
+#pragma unroll
+				for (int q = 2 * s; q < 2 * s + 2 && q < K; ++q) {
+					const int ring_slot = ((zr - q) % R + R) % R;
+#pragma unroll
+					for (int r = 0; r < 4; ++r) {
+						zout[r] = fmaf(ring[ring_slot][r], kz[q], zout[r]);
+					}
+				}
+			}
+			// this slot's operands are consumed: load plane p + P into it
+			fetch(p + P, slot);
+			z_store(p - 2, zout);
+			t_prev[0] = t[0];
+			t_prev[1] = t[1];
+#pragma unroll
+			for (int r = 0; r < 4; ++r) {
+				u_prev[r] = u0[r] + u1[r];
 			}
 		}
 	}
@@ -590,6 +762,31 @@ void launch_mfma(sep_args& a, hipStream_t aStream)
 }
 
 template<int K>
+void launch_direct(sep_args& a, hipStream_t aStream)
+{
+	unsigned nb = plan_tiles(a, kMxTileX, kMxTileY);
+	if (tune_get("sep3.chunk", 0) <= 0) {
+		// Long z chunks cut the halo (each chunk filters K - 1 extra planes in x
+		// and y); one workgroup per CU (256 of them, one wave per SIMD) is
+		// enough for this kernel. The chunk is a multiple of K + 1, so the
+		// unrolled groups of K + 1 iterations have no idle iterations.
+		constexpr int R = K + 1;
+		const long long tiles = static_cast<long long>(a.tiles_x) * a.tiles_y;
+		const long long chunks = std::max<long long>(1, (256 + tiles - 1) / tiles);
+		int chunk = static_cast<int>(ceil_div(a.nz, chunks));
+		chunk = (chunk + R - 1) / R * R;
+		a.chunk = chunk;
+		a.chunks_z = ceil_div(a.nz, chunk);
+		nb = static_cast<unsigned>(tiles * a.chunks_z);
+	}
+	if (tune_get("sep3.waves", 1) == 2) {
+		sep3_direct_kernel<K, 2><<<nb, 256, 0, aStream>>>(a);
+	} else {
+		sep3_direct_kernel<K, 1><<<nb, 256, 0, aStream>>>(a);
+	}
+}
+
+template<int K>
 void launch_fused(sep_args& a, hipStream_t aStream, int aThreads)
 {
 	const int tile_y = tune_get("sep3.tile_y", 16) == 32 ? 32 : 16;
@@ -665,13 +862,37 @@ void convolve_separable(const view4& aSrc, const view4* aTaps, int aN, const vie
 		a.dy = aDst.stride[1];
 		a.dz = aDst.stride[2];
 		a.border = aBorder;
-		// engine 0: x and y filters on the matrix cores (FP32 MFMA); engine 1: vector ALUs.
-		// Both compute FP32 products with FP32 sums, in different orders. The
-		// matrix engine addresses a plane with unsigned 32-bit lane offsets.
+		// Engines (all FP32 products with FP32 sums, in different orders):
+		//   2 (default where it applies): barrier-free, x and y filters on FP32
+		//     MFMA with operands loaded per wave, long z chunks;
+		//   0: the same filters on operands staged in LDS (any border, stride);
+		//   1: vector ALUs only.
+		// The matrix engines address a plane with unsigned 32-bit lane offsets.
 		const long long src_span = (static_cast<long long>(a.nx) - 1) * a.sx + (static_cast<long long>(a.ny) - 1) * a.sy + 4;
 		const long long dst_span = (static_cast<long long>(a.nx) - 1) * a.dx + (static_cast<long long>(a.ny) - 1) * a.dy + 4;
-		const bool matrix = tune_get("sep3.engine", 0) != 1 && a.sx >= 0 && a.sy >= 0 && a.dx >= 0 && a.dy >= 0 &&
+		const int engine = tune_get("sep3.engine", 2);
+		const bool matrix = engine != 1 && a.sx >= 0 && a.sy >= 0 && a.dx >= 0 && a.dy >= 0 &&
 			src_span < (1LL << 31) && dst_span < (1LL << 31);
+		// engine 2 reads 16-byte groups of 4 samples: packed rows, widths and
+		// row pitches that are multiples of 4, a 16-byte aligned source, zero border
+		const bool grouped = a.sx == 4 && a.nx % 4 == 0 && a.sy % 16 == 0 && a.sz % 16 == 0 &&
+			reinterpret_cast<uintptr_t>(a.src) % 16 == 0 && a.border == IANE_BORDER_ZERO;
+		if (matrix && engine == 2 && grouped) {
+			a.src_span = static_cast<int>(src_span);
+			a.dst_span = static_cast<int>(dst_span);
+			switch (k) {
+			case 1: launch_direct<1>(a, aStream); break;
+			case 3: launch_direct<3>(a, aStream); break;
+			case 5: launch_direct<5>(a, aStream); break;
+			case 7: launch_direct<7>(a, aStream); break;
+			case 9: launch_direct<9>(a, aStream); break;
+			case 11: launch_direct<11>(a, aStream); break;
+			case 13: launch_direct<13>(a, aStream); break;
+			default: launch_direct<15>(a, aStream); break;
+			}
+			check_hip(hipGetLastError(), "Separable convolution launch");
+			return;
+		}
 		if (matrix) {
 			a.src_span = static_cast<int>(src_span);
 			a.dst_span = static_cast<int>(dst_span);

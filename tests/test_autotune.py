@@ -21,7 +21,8 @@ def test_candidates_cover_the_space_default_first():
     cands = autotune.candidates(autotune.VECTOR_SEPARABLE_SPACE)
     assert len(cands) == 2 * 2 * 4
     assert cands[0] == {"sep3.threads": 512, "sep3.tile_y": 16, "sep3.chunk": 0}
-    assert autotune.candidates(autotune.SEPARABLE_SPACE)[0] == {"sep3.waves": 2, "sep3.chunk": 0}
+    assert autotune.candidates(autotune.SEPARABLE_SPACE)[0] == {"sep3.waves": 1, "sep3.chunk": 0}
+    assert autotune.candidates(autotune.STAGED_SEPARABLE_SPACE)[0] == {"sep3.waves": 2, "sep3.chunk": 0}
     assert len({tuple(sorted(c.items())) for c in cands}) == len(cands)
 
 

@@ -14,8 +14,8 @@ pytestmark = pytest.mark.gpu
 def reset_knobs(tmp_path, monkeypatch):
     monkeypatch.setenv("IANE_AMD_TUNE_CACHE", str(tmp_path / "tune.json"))
     yield
-    for key in (list(autotune.VECTOR_SEPARABLE_SPACE) + list(autotune.SEPARABLE_SPACE) + list(autotune.RL_BATCH_SPACE)
-                + ["sep3.engine"]):
+    for key in (list(autotune.VECTOR_SEPARABLE_SPACE) + list(autotune.SEPARABLE_SPACE)
+                + list(autotune.STAGED_SEPARABLE_SPACE) + list(autotune.RL_BATCH_SPACE) + ["sep3.engine"]):
         autotune.set(key, -1)
 
 
@@ -25,29 +25,27 @@ def dev(a):
 
 
 @pytest.mark.parametrize("border", [border_mode.zero, border_mode.clamp])
-def test_separable_candidates_bit_identical(border):
+@pytest.mark.parametrize("width", [132, 130], ids=["x4", "ragged"])
+def test_separable_candidates_bit_identical(border, width):
+    """Every launch shape of each engine gives the same bits (a width that is a
+    multiple of 4 with the zero border runs the barrier-free engine)."""
     import torch
     rng = np.random.default_rng(0)
-    vol = dev(rng.random((70, 45, 130)).astype(np.float32))
+    vol = dev(rng.random((70, 45, width)).astype(np.float32))
     taps = [dev(synth.gaussian_1d(s, 15).astype(np.float32)) for s in (2.0, 2.5, 3.0)]
     opts = convolution_options(border, precision.fp32)
-    ref = torch.empty_like(vol)
-    iane_amd.convolve_separable(vol, taps, ref, opts).wait()
-    ref = ref.cpu().numpy()
-    for config in autotune.candidates(autotune.SEPARABLE_SPACE):
-        out = torch.empty_like(vol)
-        with autotune.applied(config):
-            iane_amd.convolve_separable(vol, taps, out, opts).wait()
-        np.testing.assert_array_equal(out.cpu().numpy().view(np.uint32), ref.view(np.uint32), err_msg=str(config))
-    with autotune.applied({"sep3.engine": 1}):
-        ref = torch.empty_like(vol)
-        iane_amd.convolve_separable(vol, taps, ref, opts).wait()
-        ref = ref.cpu().numpy()
-        for config in autotune.candidates(autotune.VECTOR_SEPARABLE_SPACE):
-            out = torch.empty_like(vol)
-            with autotune.applied(config):
-                iane_amd.convolve_separable(vol, taps, out, opts).wait()
-            np.testing.assert_array_equal(out.cpu().numpy().view(np.uint32), ref.view(np.uint32), err_msg=str(config))
+    for engine, space in ((2, autotune.SEPARABLE_SPACE), (0, autotune.STAGED_SEPARABLE_SPACE),
+                          (1, autotune.VECTOR_SEPARABLE_SPACE)):
+        with autotune.applied({"sep3.engine": engine}):
+            ref = torch.empty_like(vol)
+            iane_amd.convolve_separable(vol, taps, ref, opts).wait()
+            ref = ref.cpu().numpy()
+            for config in autotune.candidates(space):
+                out = torch.empty_like(vol)
+                with autotune.applied(config):
+                    iane_amd.convolve_separable(vol, taps, out, opts).wait()
+                np.testing.assert_array_equal(out.cpu().numpy().view(np.uint32), ref.view(np.uint32),
+                                              err_msg=f"engine {engine} {config}")
 
 
 def _rl_case():

@@ -235,12 +235,14 @@ def test_fp32_precision_dense15(border):
 # ---------------------------------------------------- separable -----------
 @pytest.mark.parametrize("k", [1, 3, 5, 7, 9, 11, 13, 15])
 @pytest.mark.parametrize("border", [cases.CLAMP, cases.ZERO])
-@pytest.mark.parametrize("engine", [0, 1], ids=["matrix", "vector"])
+@pytest.mark.parametrize("engine", [2, 0, 1], ids=["direct", "staged", "vector"])
 def test_separable_fused_fp32(k, border, engine):
-    """Both fused engines (x/y filters on FP32 MFMA, or on the vector ALUs)."""
+    """Every fused engine (x/y filters on FP32 MFMA with per-wave loads or LDS
+    staging, or on the vector ALUs); the width is a multiple of 4, so the zero
+    border runs the barrier-free engine."""
     from iane_amd import autotune
     rng = np.random.default_rng(k)
-    src = rng.random((37, 45, 70)).astype(np.float32)
+    src = rng.random((37, 45, 72)).astype(np.float32)
     taps = [rng.random(k).astype(np.float32) for _ in range(3)]
     t64 = [t.astype(np.float64) for t in taps]
     dense = np.multiply.outer(np.multiply.outer(t64[2], t64[1]), t64[0])
@@ -252,9 +254,11 @@ def test_separable_fused_fp32(k, border, engine):
     assert rel_err(host(out), ref64) <= TOL_FP32
 
 
-@pytest.mark.parametrize("shape", [(5, 16, 64), (40, 17, 65), (3, 100, 130), (70, 33, 257)])
+@pytest.mark.parametrize("shape", [(5, 16, 64), (40, 17, 65), (3, 100, 130), (70, 33, 257), (97, 33, 260),
+                                   (130, 7, 4)])
 def test_separable_matrix_engine_ragged_tiles(shape):
-    """Tile edges in x, y and z (partial 64 x 16 tiles, short and long z chunks)."""
+    """Tile edges in x, y and z (partial 64 x 16 tiles, short and long z chunks);
+    widths that are multiples of 4 take the barrier-free engine for the zero border."""
     rng = np.random.default_rng(sum(shape))
     src = (rng.random(shape) - 0.3).astype(np.float32)
     taps = [rng.standard_normal(7).astype(np.float32) for _ in range(3)]
@@ -329,8 +333,11 @@ def test_full_size_workloads_bit_exact_vs_oracle():
     same_bits(got, oracle.convolve(data["volume_u16"], data["binomial"], np.int32, oracle.ZERO, threads=8))
 
 
-def test_full_size_separable_vs_sampled_oracle():
-    """Separable 15^3 Gaussian at 256^3 against FP64 brute force at sampled voxels (edges included)."""
+def test_full_size_separable_vs_fp64_oracle():
+    """The headline workload (separable 15^3 Gaussian on the 256^3 blob volume,
+    zero border, FP32) over the whole volume against three FP64 per-axis passes
+    of the C oracle (a separable kernel with the zero border factors over axes):
+    max|gpu - ref| <= TOL_FP32 * max|ref| (measured: ~2.2e-7; normwise ~7e-8)."""
     from iane_amd import synth
     vol = synth.volume_blobs(256, seed=2)
     v = vol["volume"]
@@ -338,20 +345,15 @@ def test_full_size_separable_vs_sampled_oracle():
     out = empty(v.shape, np.float32)
     iane_amd.convolve_separable(dev(v), [dev(taps)] * 3, out,
                                 convolution_options(border_mode.zero, precision.fp32)).wait()
-    got = host(out)
+    got = host(out).astype(np.float64)
+    ref = v.astype(np.float64)
     t = taps.astype(np.float64)
-    dense = np.multiply.outer(np.multiply.outer(t, t), t)
-    rng = np.random.default_rng(0)
-    pts = rng.integers(0, 256, (300, 3))
-    pts[:20] = [[0, 0, 0]] * 20
-    pts[20:40, 0] = 255
-    vp = np.pad(v.astype(np.float64), 7)
-    worst = 0.0
-    for z, y, x in pts:
-        window = vp[z:z + 15, y:y + 15, x:x + 15]
-        ref = float(np.sum(window * dense[::-1, ::-1, ::-1]))
-        worst = max(worst, abs(got[z, y, x] - ref) / max(abs(ref), 1e-3))
-    assert worst <= TOL_FP32
+    for shape in ((1, 1, 15), (1, 15, 1), (15, 1, 1)):
+        ref = oracle.convolve(ref, t.reshape(shape), np.float64, oracle.ZERO, threads=16)
+    err = rel_err(got, ref)
+    l2 = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+    print(f"separable 15^3 at 256^3 vs FP64 oracle: max {err:.2e}, normwise {l2:.2e}")
+    assert err <= TOL_FP32
 
 
 @pytest.mark.parametrize("shape,kshape", [((40, 37, 70), (3, 3, 3)), ((9, 20, 130), (5, 3, 7)), ((5, 33, 64), (7, 5, 1)),

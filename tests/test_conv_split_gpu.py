@@ -87,6 +87,21 @@ def test_dense15_blob_volume(prec):
     assert emax <= TOL[prec]
 
 
+@pytest.mark.parametrize("prec,tol", [(precision.fp32, 1e-5), (precision.split_fp16, TOL_FP16),
+                                      (precision.split_bf16, TOL_BF16)], ids=["fp32", "split_fp16", "split_bf16"])
+def test_dense15_full_size_vs_fft_oracle(prec, tol):
+    """The dense random 15^3 kernel (normal, normalised to sum 1) on the 256^3
+    blob volume, zero border, against the FP64 zero-padded FFT convolution of
+    the oracle (itself pinned to the direct FP64 convolve)."""
+    from iane_amd import synth
+    data = synth.volume_blobs(256, seed=2)
+    got = run(data["volume"], data["dense"], cases.ZERO, prec)
+    ref = oracle.conv_zero_fft(data["volume"].astype(np.float64), data["dense"].astype(np.float64))
+    emax, el2 = errors(got, ref)
+    print(f"dense 15^3 at 256^3, precision code {int(prec)}: max {emax:.2e}, normwise {el2:.2e}")
+    assert emax <= tol
+
+
 @pytest.mark.parametrize("border", [cases.CLAMP, cases.ZERO])
 def test_two_dimensional_image(border):
     rng = np.random.default_rng(3)

@@ -2,16 +2,25 @@
 
 * DIRECT method, REFERENCE precision: bit-identical to the reference RL loop
   (golden fixtures written by the reference's own convolve/zip/copy).
-* FFT method: against the FP64 oracle (numpy FFT restatement of the loop,
-  itself pinned in tests/test_oracle_fft.py); tolerance on the estimate
-      max|gpu - oracle64| <= TOL_RL * max|oracle64|   (TOL_RL = 1e-4 after <= 10 iterations)
-  and on the log-likelihood |ll_gpu - ll64| <= 1e-6 |ll64|.
+* FFT method: against the FP64 oracle (FFT restatement of the loop, itself
+  pinned in tests/test_oracle_fft.py). The FP32 FFT's rounding is absolute (a
+  fraction of the signal's norm), so the per-voxel relative error grows with
+  the iteration count, linearly in the measurements (about 1.1-2.2e-7 per
+  iteration: 1.0-1.6e-6 after 10 iterations, 1.4-4.3e-5 after 200; the
+  reference's own FP32 loop with FP64-accumulated direct convolutions measures
+  2.2-4.1e-6 after 200; 6.4e-7 after 2 iterations at 256x256x128). The bound is
+      max|gpu - oracle64| <= tol_rl(i) * max|oracle64|,  tol_rl(i) = 1e-6 + 2.5e-7 * i,
+  and |ll_gpu - ll64| <= 1e-6 |ll64| for the log-likelihood.
 * DIRECT method with split-precision MFMA convolution (hi + lo FP16 / BF16 operands,
   FP32 accumulation): against the FP64 oracle, max|gpu - oracle64| <= 2e-5 (FP16) and
   <= 1e-4 (BF16) of max|oracle64| after 5 iterations with a 9x7x21 PSF (measured:
-  about 3e-6 for both).
+  about 3e-6 for both); after 200 iterations with the 31x31x63 PSF on a 64^3
+  volume, <= 3e-4 (measured 1.6e-4 FP16, 2.2e-4 BF16).
 * Pointwise steps: bit-identical to the same float32 operations on the host.
-* Full size (256x256x128, PSF 31x31x63): FFT RL vs the FP64 oracle for 3 iterations,
+* 200 iterations with the 31x31x63 PSF (sigma 1.5, 1.5, 3.0) on a 64^3 volume of
+  the same construction as the 256x256x128 workload: FFT method and split FP16 / BF16
+  direct method against the FP64 oracle.
+* Full size (256x256x128, PSF 31x31x63): FFT RL vs the FP64 oracle for 2 iterations,
   and the EM property (non-decreasing log-likelihood) over 20 iterations.
 """
 from __future__ import annotations
@@ -26,7 +35,10 @@ from oracle import oracle
 
 pytestmark = pytest.mark.gpu
 GOLDEN = pathlib.Path(__file__).resolve().parent / "golden"
-TOL_RL = 1e-4
+
+
+def tol_rl(iterations: int) -> float:
+    return 1e-6 + 2.5e-7 * iterations
 
 
 def dev(a):
@@ -67,7 +79,7 @@ def test_fft_method_vs_fp64_oracle(shape, psf_shape):
     iters = 10
     est, ll = deconvolution.richardson_lucy(dev(y), dev(psf), iters, method=deconvolution.FFT, return_ll=True)
     ref, ll_ref = oracle.rl(y, psf.astype(np.float64), iters)
-    assert rel(host(est), ref) <= TOL_RL
+    assert rel(host(est), ref) <= tol_rl(iters)
     np.testing.assert_allclose(host(ll), ll_ref, rtol=1e-6)
 
 
@@ -77,8 +89,8 @@ def test_direct_fp32_and_fft_agree(shape, psf_shape):
     a = deconvolution.richardson_lucy(dev(y), dev(psf), 8, method=deconvolution.DIRECT, precision=precision.fp32)
     b = deconvolution.richardson_lucy(dev(y), dev(psf), 8, method=deconvolution.FFT)
     ref, _ = oracle.rl(y, psf.astype(np.float64), 8)
-    assert rel(host(a), ref) <= TOL_RL
-    assert rel(host(b), ref) <= TOL_RL
+    assert rel(host(a), ref) <= tol_rl(8)
+    assert rel(host(b), ref) <= tol_rl(8)
 
 
 @pytest.mark.parametrize("prec,tol", [(precision.split_fp16, 2e-5), (precision.split_bf16, 1e-4)],
@@ -91,6 +103,34 @@ def test_direct_split_precision_vs_fp64_oracle(prec, tol):
     err = rel(host(est), ref)
     print(f"RL direct, precision code {prec}: max error {err:.2e} of max|oracle64|")
     assert err <= tol
+
+
+def _c3_like(shape, seed=3):
+    """0.01 + blobs, blurred by the 31x31x63 anisotropic Gaussian PSF, scaled to
+    peak 1000 photons, Poisson noise (the construction of the 256x256x128 workload)."""
+    psf = synth.gaussian_kernel((3.0, 1.5, 1.5), (63, 31, 31)).astype(np.float32)
+    truth = np.full(shape, 0.01)
+    synth.add_blobs(truth, 40, (1.0, 4.0), (0.1, 1.0), seed, 1)
+    blurred = oracle.conv_zero_fft(truth, psf.astype(np.float64))
+    blurred *= 1000.0 / blurred.max()
+    y = synth.poisson(np.maximum(blurred, 0), seed, 2).reshape(shape).astype(np.float32)
+    return y, psf
+
+
+def test_rl_200_iterations_real_psf():
+    """200 iterations, FFT method and split-precision direct method, against the FP64 oracle."""
+    y, psf = _c3_like((64, 64, 64))
+    ref, ll_ref = oracle.rl(y, psf.astype(np.float64), 200)
+    est, ll = deconvolution.richardson_lucy(dev(y), dev(psf), 200, method=deconvolution.FFT, return_ll=True)
+    err = rel(host(est), ref)
+    print(f"FFT RL, 200 iterations, 64^3, PSF 31x31x63: max {err:.2e} of max|oracle64|")
+    assert err <= tol_rl(200)
+    np.testing.assert_allclose(host(ll), ll_ref, rtol=1e-6)
+    for prec in (precision.split_fp16, precision.split_bf16):
+        est = deconvolution.richardson_lucy(dev(y), dev(psf), 200, method=deconvolution.DIRECT, precision=prec)
+        err = rel(host(est), ref)
+        print(f"direct RL, precision code {int(prec)}, 200 iterations: max {err:.2e} of max|oracle64|")
+        assert err <= 3e-4
 
 
 def test_warm_start_continues_the_iteration():
@@ -168,7 +208,7 @@ def test_full_size_fft_rl_vs_oracle_and_em_property():
     ll = torch.zeros(20, dtype=torch.float64, device="cuda")
     plan.run(dy, est, 2, ll_history=ll).wait()
     ref, ll_ref = oracle.rl(y, psf.astype(np.float64), 2)
-    assert rel(host(est), ref) <= TOL_RL
+    assert rel(host(est), ref) <= tol_rl(2)
     np.testing.assert_allclose(host(ll)[:2], ll_ref, rtol=1e-6)
     plan.run(dy, est, 20, ll_history=ll).wait()
     history = host(ll)
