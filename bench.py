@@ -9,13 +9,14 @@ iane_hip_convolve_separable call over the whole volume, inputs resident in HBM.
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
 
 Prints ONE JSON line. ``roofline`` is for the dominant kernel
-(sep3_mfma_kernel<15, W>: x/y filters on FP32 matrix cores, z filter on the
-vector ALUs; ``sep3.engine=1`` selects the all-vector sep3_fused_kernel):
+(sep3_direct_kernel<15, W>: x/y filters on FP32 matrix cores with operands
+loaded per wave, z filter on the vector ALUs; ``sep3.engine=0`` selects the
+LDS-staged sep3_mfma_kernel, ``sep3.engine=1`` the all-vector sep3_fused_kernel):
 achieved = algorithmic bytes per launch
 (256^3 voxels x (4 B read + 4 B written)) / average launch time measured
 with HIP events on the stream the kernel runs on. ``cpu_baseline`` times
 the reference's own CPU convolve (compiled from its headers into
-oracle/_ref) on a 16-plane z-slab of the same workload with the dense
+oracle/_ref) on a 48-plane z-slab of the same workload with the dense
 15^3 outer-product kernel, which is what the reference runs for it.
 """
 from __future__ import annotations
@@ -34,7 +35,10 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
 MFMA16_PEAK_TFLOPS = 2500.0
+LEGS = ("dense15_reference", "dense15_fp32", "dense15_split_fp16", "dense15_split_bf16", "u16_binomial3",
+        "dense3_fp32", "image_2d", "sequence", "rl_fft", "rl_split", "rl_error", "rl_batch", "learned")
 METRIC = "Mvoxel/s 3-D conv (%HBM roofline); RL time-to-equal-likelihood, learned vs plain"
 
 
@@ -79,8 +83,11 @@ def traffic_from_profiles(kernel_tag: str):
         return None
 
 
-def cpu_baseline(volume: np.ndarray, taps: np.ndarray, planes: int = 16):
-    """Reference CPU convolve on a z-slab, threads = the box's CPU share (<= 16)."""
+def cpu_baseline(volume: np.ndarray, taps: np.ndarray, planes: int = 48):
+    """Reference CPU convolve on a z-slab. Threads: min(16, os.cpu_count()). The
+    reference's default is std::thread::hardware_concurrency(), which on the GPU
+    host reports every core of the machine, while a one-GPU job there is given a
+    share of 16 CPUs, so 16 threads is what a user of that one GPU gets."""
     from oracle import oracle
     cores = min(16, os.cpu_count() or 1)
     t = taps.astype(np.float64)
@@ -110,7 +117,9 @@ def cpu_baseline(volume: np.ndarray, taps: np.ndarray, planes: int = 16):
             "sample": f"{planes}-plane z-slab ({slab.shape[2]}x{slab.shape[1]}x{planes}) of the 256^3 workload, "
                       f"dense 15^3 outer-product kernel (the reference has no separable path), zero border, "
                       f"cpu_policy{{threads={cores}, grain=4096}}, median of 3 after 1 warm-up",
-            "median_s": median}
+            "median_s": median,
+            "cores_note": "min(16, os.cpu_count()): one GPU's share of the host's CPUs; the reference's "
+                          "default std::thread::hardware_concurrency() would count the whole host"}
 
 
 def _lib_sync(stream) -> None:
@@ -202,6 +211,44 @@ def bench_rl_split(timer, stream, iterations: int = 2):
         diff = float((est - ref).abs().max()) / scale
         out[name] = {"ms_per_iteration": round(ms / iterations, 2), "max_rel_diff_vs_fft": float(f"{diff:.3g}")}
         log(f"[bench] RL direct {name}: {ms / iterations:.1f} ms/iteration, max diff vs FFT {diff:.2e}")
+    return out
+
+
+def _errors(est: np.ndarray, ref: np.ndarray) -> dict:
+    diff = est.astype(np.float64) - ref
+    return {"l2_rel": float(f"{np.linalg.norm(diff) / np.linalg.norm(ref):.3g}"),
+            "max_abs_rel": float(f"{np.abs(diff).max() / np.abs(ref).max():.3g}")}
+
+
+def bench_rl_fp64_error(stream):
+    """Error of the GPU RL estimates against the FP64 oracle (the checker: scipy
+    float64 FFT convolutions restating examples/rl_deconvolution/main.cu:51-81):
+    normwise ||x - x64|| / ||x64|| and max|x - x64| / max|x64|, for the FFT method and
+    the split-precision direct method. Full 256x256x128 volume with the 31x31x63 PSF
+    after 2 iterations, and a 64^3 volume of the same construction (40 blobs) after
+    200 iterations, where the FP32 rounding has accumulated."""
+    import torch
+    from oracle import oracle
+    from iane_amd import deconvolution, hip_policy, precision, synth
+    policy = hip_policy(stream)
+    out = {"oracle": "oracle.rl: float64 FFT convolutions (scipy), zero border"}
+    for label, shape, blobs, iterations in (("256x256x128_2_iterations", (128, 256, 256), 300, 2),
+                                            ("64x64x64_200_iterations", (64, 64, 64), 40, 200)):
+        _, psf, y = synth.rl_volume_data(shape=shape, blobs=blobs)
+        t0 = time.perf_counter()
+        ref, _ = oracle.rl(y, psf.astype(np.float64), iterations)
+        log(f"[bench] RL error {label}: FP64 oracle in {time.perf_counter() - t0:.1f} s")
+        dy, dpsf = torch.from_numpy(y).cuda(), torch.from_numpy(psf).cuda()
+        leg = {}
+        for name, method, prec in (("fft", deconvolution.FFT, precision.fp32),
+                                   ("direct_split_fp16", deconvolution.DIRECT, precision.split_fp16),
+                                   ("direct_split_bf16", deconvolution.DIRECT, precision.split_bf16)):
+            plan = deconvolution.rl_plan(y.shape, dpsf, method, prec, policy=policy)
+            est = torch.empty_like(dy)
+            plan.run(dy, est, iterations).wait()
+            leg[name] = _errors(est.cpu().numpy(), ref)
+            log(f"[bench] RL error {label} {name}: {leg[name]}")
+        out[label] = leg
     return out
 
 
@@ -307,6 +354,8 @@ def main() -> int:
     ap.add_argument("--quick", action="store_true", help="headline only")
     ap.add_argument("--direct-only", action="store_true",
                     help="headline plus the two streaming direct-convolution legs (u16 3^3, f32 dense 3^3)")
+    ap.add_argument("--legs", default=None, metavar="NAME[,NAME...]",
+                    help="run only these secondary legs (after the headline): " + ", ".join(LEGS))
     ap.add_argument("--no-tune", action="store_true", help="default launch shapes (no autotuner launches)")
     ap.add_argument("--launch-shape", default=None, metavar="WAVES,CHUNK",
                     help="fix the separable kernel's launch shape (sep3.waves, sep3.chunk) instead of searching: "
@@ -315,6 +364,17 @@ def main() -> int:
                     help="also copy the 256^3 volume once (device to device, 64 MiB read + 64 MiB written): "
                          "the reference dispatch for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
+    if args.quick:
+        legs = set()
+    elif args.legs is not None:
+        legs = set(args.legs.split(",")) - {""}
+        unknown = legs - set(LEGS)
+        if unknown:
+            ap.error(f"unknown legs {sorted(unknown)}; choose from {', '.join(LEGS)}")
+    elif args.direct_only:
+        legs = {"u16_binomial3", "dense3_fp32"}
+    else:
+        legs = set(LEGS)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(slab decomposition with halo exchange is designed, not built)", file=sys.stderr)
@@ -379,10 +439,13 @@ def main() -> int:
     value = voxels / (ms_per_step / 1e3) / 1e6
     bytes_per_launch = voxels * 8.0
     achieved = bytes_per_launch / (ms_per_step / 1e3) / 1e9
-    if autotune.get("sep3.engine") == 1:
+    engine = autotune.get("sep3.engine")
+    if engine == 1:
         kernel, tag = "sep3_fused_kernel<15>", "sep3_fused_256"
-    else:
+    elif engine == 0:
         kernel, tag = f"sep3_mfma_kernel<15, {3 if autotune.get('sep3.waves') == 3 else 2}>", "sep3_mfma_256"
+    else:
+        kernel, tag = f"sep3_direct_kernel<15, {2 if autotune.get('sep3.waves') == 2 else 1}>", "sep3_direct_256"
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(tag),
                 "kernel": kernel, "launch_shape": tuned, "bytes_per_launch": bytes_per_launch,
@@ -394,12 +457,13 @@ def main() -> int:
         _lib.call("iane_hip_memcpy_async", out.data_ptr(), vol.data_ptr(), voxels * 4, 3, stream)
         _lib.call("iane_hip_stream_sync", stream)
         extra["copy_calibration_bytes"] = {"read": voxels * 4, "written": voxels * 4}
-    if not args.quick:
-        # dense random 15^3 kernel, reference (FP64, bit-exact) arithmetic: compute bound
-        dense = torch.from_numpy(data["dense"]).cuda()
-        dense_legs = () if args.direct_only else (
+    dense_legs = [(name, prec) for name, prec in (
             ("dense15_reference", precision.reference), ("dense15_fp32", precision.fp32),
             ("dense15_split_fp16", precision.split_fp16), ("dense15_split_bf16", precision.split_bf16))
+            if name in legs]
+    if dense_legs:
+        # dense random 15^3 kernel in each arithmetic: compute bound
+        dense = torch.from_numpy(data["dense"]).cuda()
         for name, prec in dense_legs:
             o = convolution_options(border_mode.zero, prec)
             fn = lambda: iane_amd.convolve(vol, dense, out, o, policy)  # noqa: E731
@@ -413,10 +477,13 @@ def main() -> int:
             if prec in (precision.split_fp16, precision.split_bf16):
                 # algorithmic 2*N*K flops against the dense FP16/BF16 MFMA peak
                 entry["frac_of_mfma_peak"] = round(tflops / MFMA16_PEAK_TFLOPS, 4)
+            elif prec == precision.fp32:
+                entry["frac_of_fp32_vector_peak"] = round(tflops / FP32_PEAK_TFLOPS, 3)
             else:
                 entry["frac_of_fp64_vector_peak"] = round(tflops / FP64_PEAK_TFLOPS, 3)
             extra[name] = entry
             log(f"[bench] {name}: {t:.2f} ms")
+    if "u16_binomial3" in legs:
         # u16 volume, 3^3 binomial int kernel, int32 result (bit-exact)
         v16 = torch.from_numpy(data["volume_u16"].view(np.int16)).cuda().view(torch.uint16)
         binom = torch.from_numpy(data["binomial"]).cuda()
@@ -433,7 +500,9 @@ def main() -> int:
                                   "kernel": "conv_stream_int_kernel<unsigned short, int, 3, 3>",
                                   "traffic": traffic_from_profiles("conv_stream_u16_256")}
         log(f"[bench] u16 3^3: {t * 1e3:.1f} us")
+    if "dense3_fp32" in legs:
         # f32 volume, dense random 3^3 kernel, FP32 arithmetic (z-streaming direct kernel)
+        k = 20
         k3 = torch.from_numpy(data["dense3"]).cuda()
         o = convolution_options(border_mode.zero, precision.fp32)
         fn = lambda: iane_amd.convolve(vol, k3, out, o, policy)  # noqa: E731
@@ -447,12 +516,19 @@ def main() -> int:
                                 "traffic": traffic_from_profiles("conv_stream_dense3_256")}
         log(f"[bench] f32 dense 3^3: {t * 1e3:.1f} us")
 
-    if not args.quick and not args.direct_only:
+    if "image_2d" in legs:
         extra["image_2d_1024"] = bench_image_2d(timer, stream)
+    if "sequence" in legs:
         extra["sequence_256"] = bench_sequence(timer, stream, vol)
+    if "rl_fft" in legs:
         extra["rl_fft_256x256x128_psf31x31x63"] = bench_rl(timer, stream)
+    if "rl_split" in legs:
         extra["rl_direct_split_256x256x128_psf31x31x63"] = bench_rl_split(timer, stream)
+    if "rl_error" in legs:
+        extra["rl_error_vs_fp64_oracle"] = bench_rl_fp64_error(stream)
+    if "rl_batch" in legs:
         extra["rl_batch_64_volumes"] = bench_rl_batch(timer, stream)
+    if "learned" in legs:
         extra["rl_learned_vs_plain_256x256x128"] = bench_learned(stream)
 
     base = None

@@ -59,3 +59,19 @@ def test_batched_time_to_likelihood_runs(net):
     print(out)
     assert out["k_max"] >= 0 and out["plain_ms"] > 0 and out["learned_ms"] > 0
     assert out["speedup"] > 1.0
+    # the timed run stops at the first check where every volume has reached its target
+    assert out["k_max"] % out["check_every"] == 0
+    assert max(out["k_per_volume"]) <= out["k_max"] < max(out["k_per_volume"]) + out["check_every"]
+
+
+def test_timed_gain_at_256x256x128(net):
+    """Four 256x256x128 volumes: the learned path, timed with its stopping rule
+    (likelihood read back every 2 iterations and compared with the plain-RL
+    target), takes at most a quarter of the GPU time of plain 200-iteration RL."""
+    import torch
+    psf = torch.from_numpy(learned.psf_kernel()).cuda()
+    ys = list(learned.observed_volumes(4, (128, 256, 256), seed=3))
+    out = learned.batch_time_to_likelihood(net, ys, [psf] * 4, iterations=200, max_iterations=120, repeats=2)
+    print(out)
+    assert 0 < out["k_max"] <= 50
+    assert out["speedup"] >= 4.0
