@@ -497,11 +497,11 @@ def main() -> int:
         extra["u16_binomial3"] = {"ms": round(t, 4), "Mvoxel_s": round(voxels / t / 1e3, 1),
                                   "GB_s": round(voxels * 6 / t / 1e6, 1),
                                   "frac_hbm": round(voxels * 6 / t / 1e6 / HBM_PEAK_GBS, 3),
-                                  "kernel": "conv_stream_int_kernel<unsigned short, int, 3, 3>",
-                                  "traffic": traffic_from_profiles("conv_stream_u16_256")}
+                                  "kernel": "conv_column_kernel<unsigned short, int, true, 4, 2>",
+                                  "traffic": traffic_from_profiles("conv_column_u16_256")}
         log(f"[bench] u16 3^3: {t * 1e3:.1f} us")
     if "dense3_fp32" in legs:
-        # f32 volume, dense random 3^3 kernel, FP32 arithmetic (z-streaming direct kernel)
+        # f32 volume, dense random 3^3 kernel, FP32 arithmetic (register column kernel)
         k = 20
         k3 = torch.from_numpy(data["dense3"]).cuda()
         o = convolution_options(border_mode.zero, precision.fp32)
@@ -512,8 +512,8 @@ def main() -> int:
         extra["dense3_fp32"] = {"ms": round(t, 4), "Mvoxel_s": round(voxels / t / 1e3, 1),
                                 "GB_s": round(voxels * 8 / t / 1e6, 1),
                                 "frac_hbm": round(voxels * 8 / t / 1e6 / HBM_PEAK_GBS, 3),
-                                "kernel": "conv_stream_kernel<float, float, 3, 3>",
-                                "traffic": traffic_from_profiles("conv_stream_dense3_256")}
+                                "kernel": "conv_column_kernel<float, float, false, 4, 2>",
+                                "traffic": traffic_from_profiles("conv_column_dense3_256")}
         log(f"[bench] f32 dense 3^3: {t * 1e3:.1f} us")
 
     if "image_2d" in legs:

@@ -22,7 +22,11 @@ different orders, so it changes bits.
 Direct-convolution switches, read at launch and not searched (every setting
 gives the same bits; tests compare them):
 
-    conv.stream         1 | 0              z-streaming kernel for small 3-D kernels (0: tiled kernel)
+    conv.stream         1 | 0              z-marching kernels for small 3-D kernels (0: tiled kernel)
+    conv.column         1 | 0              register column kernel for 3^3 kernels on 4-aligned rows
+                                           (0: the LDS streaming kernel)
+    conv.column_rows    4 | 2              output rows per lane of the column kernel
+    conv.column_chunk   0 (auto) | >= 1    output planes per column workgroup
     conv.stream_chunk   0 (auto) | 1 .. 64 output planes per streaming workgroup
     conv.stream_unroll  1 | 0              fully unrolled FP32 3^3 taps (0: row loop)
 

@@ -20,9 +20,13 @@
 // registers, one barrier per plane), so the input is read from memory once per
 // chunk instead of once per kernel plane. The ring has kz + 2 slots: the plane
 // stored at step z + 1 never overwrites a plane that step z still reads, so
-// the barrier after each store is the only one. Each output keeps the same FP64 (or
-// FP32) fma chain in the reference's tap order, so both kernels produce
-// identical bits.
+// the barrier after each store is the only one. Each output keeps the same FP64
+// fma chain in the reference's tap order (FP32 chains: kernel planes from the
+// back, see plane_tap), so both kernels produce identical bits.
+//
+// Column kernel (3 x 3 x 3 kernels on x-contiguous images): no LDS and no
+// barriers; input planes pass through registers and each plane adds its taps
+// to the three output planes it reaches (described at conv_column_kernel).
 #include "common.hpp"
 
 #include <atomic>
@@ -64,6 +68,17 @@ __device__ __forceinline__ bool border_coord(int& aCoord, int aExtent, int aBord
 	return true;
 }
 
+// Kernel plane of the step-th term of an output's fma chain. FP64 chains take
+// kernel planes in the reference's storage order (qz ascending: input planes
+// from the highest z down). FP32 chains take them from the back, i.e. input
+// planes in ascending z, the order in which a kernel marching up z receives
+// them; rows and x taps keep the storage order in both.
+template<class Acc>
+__device__ __forceinline__ int plane_tap(int aStep, int aKz)
+{
+	return std::is_same_v<Acc, float> ? aKz - 1 - aStep : aStep;
+}
+
 template<class Acc, int KX>
 __global__ __launch_bounds__(kThreads) void conv_tiled_kernel(conv_args a)
 {
@@ -100,7 +115,8 @@ __global__ __launch_bounds__(kThreads) void conv_tiled_kernel(conv_args a)
 	}
 
 	for (int qw = 0; qw < kw; ++qw) {
-		for (int qz = 0; qz < kz; ++qz) {
+		for (int step = 0; step < kz; ++step) {
+			const int qz = plane_tap<Acc>(step, kz);
 			__syncthreads();
 			// Coefficients of kernel plane (qz, qw), converted like the reference does.
 			for (int i = tid; i < kx * ky; i += kThreads) {
@@ -218,7 +234,8 @@ __global__ __launch_bounds__(kThreads) void conv_simple_kernel(conv_args a, long
 	}
 	Acc sum = Acc(0);
 	for (int qw = 0; qw < a.ker.extent[3]; ++qw) {
-		for (int qz = 0; qz < a.ker.extent[2]; ++qz) {
+		for (int step = 0; step < a.ker.extent[2]; ++step) {
+			const int qz = plane_tap<Acc>(step, a.ker.extent[2]);
 			for (int qy = 0; qy < a.ker.extent[1]; ++qy) {
 				for (int qx = 0; qx < a.ker.extent[0]; ++qx) {
 					const int q[4] = {qx, qy, qz, qw};
@@ -564,8 +581,9 @@ struct stream_coefficients {
 };
 
 // Accumulate the taps of the output plane whose newest input plane is in ring
-// slot aTop: aMac(acc, sample, coefficient) over kernel planes qz, rows qy (the
-// tiled kernel's order: rows from the back) and x taps from the back. UNROLL
+// slot aTop: aMac(acc, sample, coefficient) over kernel planes in plane_tap
+// order, rows qy (the tiled kernel's order: rows from the back) and x taps from
+// the back. UNROLL
 // false keeps runtime loops over the LDS coefficients even when KYZ > 0.
 template<int KX, int KYZ, bool UNROLL = true, class T, class Acc, class Mac>
 __device__ __forceinline__ void stream_taps(const stream_tile& t, const T* aRing,
@@ -586,7 +604,8 @@ __device__ __forceinline__ void stream_taps(const stream_tile& t, const T* aRing
 	};
 	if constexpr (KYZ > 0 && UNROLL) {
 #pragma unroll
-		for (int qz = 0; qz < KYZ; ++qz) {
+		for (int step = 0; step < KYZ; ++step) {
+			const int qz = plane_tap<Acc>(step, KYZ);
 			const T* slot = aRing + t.slot_before(aTop, qz) * t.plane_elems;
 #pragma unroll
 			for (int qy = 0; qy < KYZ; ++qy) {
@@ -594,7 +613,8 @@ __device__ __forceinline__ void stream_taps(const stream_tile& t, const T* aRing
 			}
 		}
 	} else {
-		for (int qz = 0; qz < aKz; ++qz) {
+		for (int step = 0; step < aKz; ++step) {
+			const int qz = plane_tap<Acc>(step, aKz);
 			const T* slot = aRing + t.slot_before(aTop, qz) * t.plane_elems;
 			for (int qy = 0; qy < aKy; ++qy) {
 				row_taps(slot, aKy - 1 - qy, aCoef.lds + (qz * aKy + qy) * KX);
@@ -678,6 +698,446 @@ __global__ __launch_bounds__(kThreads) void conv_stream_int_kernel(conv_args a, 
 	t.run<kStreamDepth, S>(a, ring, spare, compute);
 }
 
+// ---------------------------------------------------------------------------
+// Column kernel: 3 x 3 x 3 kernels on images whose rows are x-contiguous and
+// aligned for 4-sample vector accesses (x extent a multiple of 4). A workgroup
+// of four waves owns a 64 x 16R (x, y) tile of one z chunk; lane (tx, ty) of
+// wave w owns outputs x0 + 4 tx .. + 3 of the R rows starting at
+// y0 + R (4 w + ty). Input planes pass through registers, loaded P planes
+// ahead: R + 2 rows of one 4-sample vector per lane. The x neighbours (x - 1,
+// x + 4) come from the adjacent lanes of the lane's 16-lane DPP row; the lanes
+// at the tile's x ends load the samples beyond it. Each plane adds its taps to
+// the three output planes it reaches (a ring of three register accumulator
+// sets) and an output plane is stored after its last input plane, so there is
+// no LDS and no barrier. FP32 sums keep the FP32 chain of the other direct
+// kernels (plane_tap order); integer sums are exact in int32 when the kernel's
+// bound allows (otherwise each output is the FP64 chain in the reference's
+// order, read straight from memory).
+constexpr int kColumnThreads = 256;
+
+using column_v4 = unsigned __attribute__((ext_vector_type(4)));
+using column_v2 = unsigned __attribute__((ext_vector_type(2)));
+
+// One 4-sample vector of S as a single buffer access, kept as raw words until
+// it is used (unpacking at the load would wait for it there).
+template<class S>
+struct column_io;
+
+template<>
+struct column_io<float> {
+	using raw = column_v4;
+	__device__ static raw load(__amdgpu_buffer_rsrc_t aRes, int aOffset)
+	{
+		return __builtin_amdgcn_raw_buffer_load_b128(aRes, aOffset, 0, 0);
+	}
+	template<class T>
+	__device__ static void unpack(const raw& aRaw, T (&aOut)[4])
+	{
+		// Bit-cast the whole vector: this compiler reads a bit cast of one
+		// vector component (v.y) from the vector's first element.
+		using f4 = float __attribute__((ext_vector_type(4)));
+		const f4 f = __builtin_bit_cast(f4, aRaw);
+		aOut[0] = static_cast<T>(f.x);
+		aOut[1] = static_cast<T>(f.y);
+		aOut[2] = static_cast<T>(f.z);
+		aOut[3] = static_cast<T>(f.w);
+	}
+	__device__ static unsigned load_single(__amdgpu_buffer_rsrc_t aRes, int aOffset)
+	{
+		return __builtin_amdgcn_raw_buffer_load_b32(aRes, aOffset, 0, 0);
+	}
+	template<class T>
+	__device__ static T single(unsigned aRaw) { return static_cast<T>(__builtin_bit_cast(float, aRaw)); }
+};
+
+template<>
+struct column_io<uint16_t> {
+	using raw = column_v2;
+	__device__ static raw load(__amdgpu_buffer_rsrc_t aRes, int aOffset)
+	{
+		return __builtin_amdgcn_raw_buffer_load_b64(aRes, aOffset, 0, 0);
+	}
+	template<class T>
+	__device__ static void unpack(const raw& aRaw, T (&aOut)[4])
+	{
+		aOut[0] = static_cast<T>(static_cast<int>(aRaw.x & 0xffffu));
+		aOut[1] = static_cast<T>(static_cast<int>(aRaw.x >> 16));
+		aOut[2] = static_cast<T>(static_cast<int>(aRaw.y & 0xffffu));
+		aOut[3] = static_cast<T>(static_cast<int>(aRaw.y >> 16));
+	}
+	__device__ static unsigned load_single(__amdgpu_buffer_rsrc_t aRes, int aOffset)
+	{
+		return __builtin_amdgcn_raw_buffer_load_b16(aRes, aOffset, 0, 0);
+	}
+	template<class T>
+	__device__ static T single(unsigned aRaw) { return static_cast<T>(static_cast<int>(aRaw)); }
+};
+
+template<>
+struct column_io<uint8_t> {
+	using raw = unsigned;
+	__device__ static raw load(__amdgpu_buffer_rsrc_t aRes, int aOffset)
+	{
+		return __builtin_amdgcn_raw_buffer_load_b32(aRes, aOffset, 0, 0);
+	}
+	template<class T>
+	__device__ static void unpack(const raw& aRaw, T (&aOut)[4])
+	{
+#pragma unroll
+		for (int e = 0; e < 4; ++e) {
+			aOut[e] = static_cast<T>(static_cast<int>((aRaw >> (8 * e)) & 0xffu));
+		}
+	}
+	__device__ static unsigned load_single(__amdgpu_buffer_rsrc_t aRes, int aOffset)
+	{
+		return __builtin_amdgcn_raw_buffer_load_b8(aRes, aOffset, 0, 0);
+	}
+	template<class T>
+	__device__ static T single(unsigned aRaw) { return static_cast<T>(static_cast<int>(aRaw)); }
+};
+
+// Store four converted outputs of one row with one buffer access (two for f64).
+template<class D>
+__device__ __forceinline__ void column_store(__amdgpu_buffer_rsrc_t aRes, int aOffset, const D (&aOut)[4])
+{
+	if constexpr (sizeof(D) == 4) {
+		const column_v4 w = {__builtin_bit_cast(unsigned, aOut[0]), __builtin_bit_cast(unsigned, aOut[1]),
+			__builtin_bit_cast(unsigned, aOut[2]), __builtin_bit_cast(unsigned, aOut[3])};
+		__builtin_amdgcn_raw_buffer_store_b128(w, aRes, aOffset, 0, 0);
+	} else if constexpr (sizeof(D) == 2) {
+		const column_v2 w = {
+			static_cast<unsigned>(__builtin_bit_cast(uint16_t, aOut[0])) |
+				(static_cast<unsigned>(__builtin_bit_cast(uint16_t, aOut[1])) << 16),
+			static_cast<unsigned>(__builtin_bit_cast(uint16_t, aOut[2])) |
+				(static_cast<unsigned>(__builtin_bit_cast(uint16_t, aOut[3])) << 16)};
+		__builtin_amdgcn_raw_buffer_store_b64(w, aRes, aOffset, 0, 0);
+	} else if constexpr (sizeof(D) == 1) {
+		const unsigned w = static_cast<unsigned>(aOut[0]) | (static_cast<unsigned>(aOut[1]) << 8) |
+			(static_cast<unsigned>(aOut[2]) << 16) | (static_cast<unsigned>(aOut[3]) << 24);
+		__builtin_amdgcn_raw_buffer_store_b32(w, aRes, aOffset, 0, 0);
+	} else {
+		using v2d = double __attribute__((ext_vector_type(2)));
+		__builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(column_v4, v2d{aOut[0], aOut[1]}), aRes, aOffset,
+			0, 0);
+		__builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(column_v4, v2d{aOut[2], aOut[3]}), aRes,
+			aOffset + 16, 0, 0);
+	}
+}
+
+// v of the lane one to the left (row_shr:1) or right (row_shl:1) in the lane's
+// 16-lane row; the row's first (last) lane gets aEdge.
+template<class T>
+__device__ __forceinline__ T column_from_left(T aEdge, T aValue)
+{
+	return __builtin_bit_cast(T, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, aEdge),
+		__builtin_bit_cast(int, aValue), 0x111, 0xf, 0xf, false));
+}
+
+template<class T>
+__device__ __forceinline__ T column_from_right(T aEdge, T aValue)
+{
+	return __builtin_bit_cast(T, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, aEdge),
+		__builtin_bit_cast(int, aValue), 0x101, 0xf, 0xf, false));
+}
+
+// Where a column lane works: its tile, its x, its first output row and its z chunk.
+struct column_geom {
+	int x0, x, tx, yb, z0, z1;
+	int ext_x, ext_y, ext_z;
+	bool clamp;
+
+	template<int R>
+	__device__ static column_geom make(const conv_args& a, int aChunk)
+	{
+		const unsigned block = xcd_remap(blockIdx.x, gridDim.x);
+		const unsigned tile_x = block % a.tiles_x;
+		const unsigned rest = block / a.tiles_x;
+		const unsigned tile_y = rest % a.tiles_y;
+		const int chunk = static_cast<int>(rest / a.tiles_y);
+		const int lane = static_cast<int>(threadIdx.x) % 64;
+		const int group = static_cast<int>(threadIdx.x) / 64 * 4 + lane / 16;
+		column_geom g;
+		g.ext_x = a.src.extent[0];
+		g.ext_y = a.src.extent[1];
+		g.ext_z = a.src.extent[2];
+		g.tx = lane % 16;
+		g.x0 = static_cast<int>(tile_x) * 64;
+		g.x = g.x0 + 4 * g.tx;
+		g.yb = static_cast<int>(tile_y) * (16 * R) + group * R;
+		g.z0 = chunk * aChunk;
+		g.z1 = min(g.z0 + aChunk, g.ext_z);
+		g.clamp = a.border == IANE_BORDER_CLAMP;
+		return g;
+	}
+};
+
+// Coefficient qx + 3 qy + 9 qz of a 3^3 kernel, the same for every lane: read
+// into scalar registers.
+template<class T>
+__device__ __forceinline__ void column_coefficients(const conv_args& a, T (&aK)[27])
+{
+#pragma unroll
+	for (int i = 0; i < 27; ++i) {
+		const char* at = a.ker.ptr + (i % 3) * a.ker.stride[0] + (i / 3 % 3) * a.ker.stride[1] +
+			(i / 9) * a.ker.stride[2];
+		if constexpr (std::is_same_v<T, int>) {
+			aK[i] = __builtin_amdgcn_readfirstlane(load_as_int(at, a.ker.type));
+		} else {
+			aK[i] = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int,
+				load_as<float>(at, a.ker.type))));
+		}
+	}
+}
+
+// March the lane's column through the z chunk with T arithmetic: float fma
+// chains (FP32 sums, or exact integer sums below 2^24) or int32 24-bit
+// multiply-adds. EXACT: the sums are exact integers (converted like the
+// integer streaming kernel); otherwise FP32 sums converted like the others.
+template<class S, class D, class T, bool EXACT, int R, int P>
+__device__ __forceinline__ void column_march(const conv_args& a, const column_geom& g, const T (&k)[27])
+{
+	using io = column_io<S>;
+	using raw_t = typename io::raw;
+	constexpr int rows = R + 2;
+	constexpr int ss = sizeof(S);
+	constexpr int ds = sizeof(D);
+
+	// Byte offsets of the lane's vector and of its x-edge sample in each input
+	// row (kOob where the zero border makes the row zero or the lane has none).
+	int row_off[rows];
+	int edge_off[rows];
+	int edge_x = -1;
+	if (g.tx == 0 && g.x0 > 0) {
+		edge_x = g.x0 - 1;
+	} else if (g.tx == 15 && g.x0 + 64 < g.ext_x) {
+		edge_x = g.x0 + 64;
+	}
+#pragma unroll
+	for (int j = 0; j < rows; ++j) {
+		int yi = g.yb - 1 + j;
+		const bool in = border_coord(yi, g.ext_y, a.border);
+		const long long base = static_cast<long long>(yi) * a.src.stride[1];
+		row_off[j] = in && g.x < g.ext_x ? static_cast<int>(base + g.x * ss) : kOob;
+		edge_off[j] = in && edge_x >= 0 ? static_cast<int>(base + edge_x * ss) : kOob;
+	}
+
+	const auto load_plane = [&](int aPlane, bool aWanted, raw_t (&aVec)[rows], unsigned (&aEdge)[rows]) {
+		int zi = aPlane;
+		const bool in = aWanted && border_coord(zi, g.ext_z, a.border);
+		const auto plane = stream_resource(a.src.ptr + static_cast<long long>(in ? zi : 0) * a.src.stride[2],
+			in ? a.src_span : 0);
+#pragma unroll
+		for (int j = 0; j < rows; ++j) {
+			aVec[j] = io::load(plane, row_off[j]);
+			aEdge[j] = io::load_single(plane, edge_off[j]);
+		}
+	};
+
+	// Samples x - 1 .. x + 4 of one input row.
+	const auto window = [&](const raw_t& aVec, unsigned aEdge, T (&aOut)[6]) {
+		T s[4];
+		io::unpack(aVec, s);
+		const T edge = io::template single<T>(aEdge);
+		const T beyond_left = g.clamp ? s[0] : T(0);
+		const T beyond_right = g.clamp ? s[3] : T(0);
+		aOut[0] = column_from_left(g.x0 == 0 ? beyond_left : edge, s[3]);
+		const T right = column_from_right(g.x + 4 < g.ext_x ? edge : beyond_right, s[0]);
+		aOut[5] = g.x + 4 < g.ext_x ? right : beyond_right;
+		aOut[1] = s[0];
+		aOut[2] = s[1];
+		aOut[3] = s[2];
+		aOut[4] = s[3];
+	};
+
+	T acc[3][R][4];
+#pragma unroll
+	for (int set = 0; set < 3; ++set) {
+#pragma unroll
+		for (int r = 0; r < R; ++r) {
+#pragma unroll
+			for (int e = 0; e < 4; ++e) {
+				acc[set][r][e] = T(0);
+			}
+		}
+	}
+
+	// Add one input plane's taps of kernel plane qz to an accumulator set
+	// (rows and x taps from the back, as in every direct kernel).
+	const auto add_plane = [&](const T (&aWin)[rows][6], T (&aAcc)[R][4], int aQz) {
+#pragma unroll
+		for (int r = 0; r < R; ++r) {
+#pragma unroll
+			for (int e = 0; e < 4; ++e) {
+				T sum = aAcc[r][e];
+#pragma unroll
+				for (int qy = 0; qy < 3; ++qy) {
+#pragma unroll
+					for (int qx = 0; qx < 3; ++qx) {
+						const T sample = aWin[r + 2 - qy][e + 2 - qx];
+						const T kk = k[qx + 3 * qy + 9 * aQz];
+						if constexpr (std::is_same_v<T, int>) {
+							sum += __mul24(sample, kk);
+						} else {
+							sum = fma(sample, kk, sum);
+						}
+					}
+				}
+				aAcc[r][e] = sum;
+			}
+		}
+	};
+
+	const auto store_plane = [&](int aZ, T (&aAcc)[R][4]) {
+		const auto plane = stream_resource(a.dst.ptr + static_cast<long long>(aZ) * a.dst.stride[2], a.dst_span);
+#pragma unroll
+		for (int r = 0; r < R; ++r) {
+			const int y = g.yb + r;
+			if (g.x < g.ext_x && y < g.ext_y) {
+				D out[4];
+#pragma unroll
+				for (int e = 0; e < 4; ++e) {
+					if constexpr (EXACT) {
+						out[e] = convert_exact_int<D>(static_cast<int>(aAcc[r][e]));
+					} else {
+						out[e] = convert_out<D>(aAcc[r][e]);
+					}
+				}
+				column_store(plane, static_cast<int>(y * a.dst.stride[1] + g.x * ds), out);
+			}
+#pragma unroll
+			for (int e = 0; e < 4; ++e) {
+				aAcc[r][e] = T(0);
+			}
+		}
+	};
+
+	// Input planes first .. first + n - 1 = z0 - 1 .. z1. Plane first + i adds
+	// kernel plane 2 to output first + i + 1 (set i % 3), kernel plane 1 to
+	// output first + i (set (i + 2) % 3) and kernel plane 0 to output
+	// first + i - 1 (set (i + 1) % 3), which is then complete. The loop is
+	// unrolled by 6 (a multiple of the ring and of P) so sets and staging slots
+	// are registers; the skipped terms are those of outputs outside the chunk.
+	const int first = g.z0 - 1;
+	const int n = g.z1 - g.z0 + 2;
+	raw_t vec[P][rows];
+	unsigned edge[P][rows];
+#pragma unroll
+	for (int q = 0; q < P; ++q) {
+		load_plane(first + q, q < n, vec[q], edge[q]);
+	}
+	for (int i0 = 0; i0 < n; i0 += 6) {
+#pragma unroll
+		for (int ph = 0; ph < 6; ++ph) {
+			const int i = i0 + ph;
+			if (i < n) {
+				T win[rows][6];
+#pragma unroll
+				for (int j = 0; j < rows; ++j) {
+					window(vec[ph % P][j], edge[ph % P][j], win[j]);
+				}
+				load_plane(first + i + P, i + P < n, vec[ph % P], edge[ph % P]);
+				const int p = first + i;
+				if (p + 1 < g.z1) {
+					add_plane(win, acc[ph % 3], 2);
+				}
+				if (p >= g.z0 && p < g.z1) {
+					add_plane(win, acc[(ph + 2) % 3], 1);
+				}
+				if (p - 1 >= g.z0) {
+					add_plane(win, acc[(ph + 1) % 3], 0);
+					store_plane(p - 1, acc[(ph + 1) % 3]);
+				}
+			}
+		}
+	}
+}
+
+// Integer images whose sums fit neither FP32 nor int32 exactly (coefficients
+// of 2^23 and more, or huge sums): every output of the lane is the FP64 chain
+// in the reference's order, read straight from memory (compact loops).
+template<class S, class D, int R>
+__device__ void column_fp64_chain(const conv_args& a, const column_geom& g)
+{
+	constexpr int ds = sizeof(D);
+#pragma unroll 1
+	for (int z = g.z0; z < g.z1; ++z) {
+#pragma unroll 1
+		for (int r = 0; r < R; ++r) {
+			const int y = g.yb + r;
+			if (g.x >= g.ext_x || y >= g.ext_y) {
+				continue;
+			}
+			D out[4];
+#pragma unroll
+			for (int e = 0; e < 4; ++e) {
+				double sum = 0.0;
+#pragma unroll 1
+				for (int qz = 0; qz < 3; ++qz) {
+#pragma unroll 1
+					for (int qy = 0; qy < 3; ++qy) {
+#pragma unroll 1
+						for (int qx = 0; qx < 3; ++qx) {
+							int xi = g.x + e + 1 - qx;
+							int yi = y + 1 - qy;
+							int zi = z + 1 - qz;
+							const bool inside = border_coord(xi, g.ext_x, a.border) &&
+								border_coord(yi, g.ext_y, a.border) && border_coord(zi, g.ext_z, a.border);
+							const double sample = inside ? static_cast<double>(*reinterpret_cast<const S*>(
+								a.src.ptr + xi * a.src.stride[0] + yi * a.src.stride[1] + zi * a.src.stride[2])) :
+								0.0;
+							const int kk = load_as_int(a.ker.ptr + qx * a.ker.stride[0] + qy * a.ker.stride[1] +
+								qz * a.ker.stride[2], a.ker.type);
+							sum = fma(sample, static_cast<double>(kk), sum);
+						}
+					}
+				}
+				out[e] = convert_out<D>(sum);
+			}
+			const auto plane = stream_resource(a.dst.ptr + static_cast<long long>(z) * a.dst.stride[2], a.dst_span);
+			column_store(plane, static_cast<int>(y * a.dst.stride[1] + g.x * ds), out);
+		}
+	}
+}
+
+template<class S, class D, bool INTEGER, int R, int P>
+__global__ __launch_bounds__(kColumnThreads) void conv_column_kernel(conv_args a, int aChunk)
+{
+	const column_geom g = column_geom::make<R>(a, aChunk);
+	if constexpr (!INTEGER) {
+		float k[27];
+		column_coefficients(a, k);
+		column_march<S, D, float, false, R, P>(a, g, k);
+	} else {
+		// Every partial sum is bounded by max source x sum |k|. Below 2^24 the
+		// FP32 fma chain is exact (integer products and sums), below 2^31 with
+		// |k| < 2^23 the 24-bit integer multiply-adds are; both give the
+		// integer the reference's FP64 chain represents.
+		int k[27];
+		column_coefficients(a, k);
+		long long total = 0;
+		long long largest = 0;
+#pragma unroll
+		for (int i = 0; i < 27; ++i) {
+			const long long m = k[i] < 0 ? -static_cast<long long>(k[i]) : static_cast<long long>(k[i]);
+			total += m;
+			largest = m > largest ? m : largest;
+		}
+		constexpr long long source_max = sizeof(S) == 1 ? 255 : 65535;
+		if (total * source_max < (1LL << 24)) {
+			float kf[27];
+#pragma unroll
+			for (int i = 0; i < 27; ++i) {
+				kf[i] = static_cast<float>(k[i]);
+			}
+			column_march<S, D, float, true, R, P>(a, g, kf);
+		} else if (largest < (1LL << 23) && total * source_max < (1LL << 31)) {
+			column_march<S, D, int, true, R, P>(a, g, k);
+		} else {
+			column_fp64_chain<S, D, R>(a, g);
+		}
+	}
+}
+
 template<bool INTEGER, class S, class D, int KX, int KYZ>
 void launch_stream_shape(const conv_args& a, int aChunk, unsigned aBlocks, size_t aLds, hipStream_t aStream)
 {
@@ -711,6 +1171,12 @@ void launch_stream_types(const conv_args& a, int aChunk, unsigned aBlocks, size_
 	case 5: launch_stream_kx<INTEGER, S, D, 5>(a, aChunk, aBlocks, aLds, aStream); break;
 	default: launch_stream_kx<INTEGER, S, D, 7>(a, aChunk, aBlocks, aLds, aStream); break;
 	}
+}
+
+template<class S, class D, bool INTEGER, int R>
+void launch_column_rows(const conv_args& a, int aChunk, unsigned aBlocks, hipStream_t aStream)
+{
+	conv_column_kernel<S, D, INTEGER, R, 2><<<aBlocks, kColumnThreads, 0, aStream>>>(a, aChunk);
 }
 
 // Compute units of the current device (cached per device).
@@ -836,6 +1302,100 @@ bool launch_stream(conv_args& a, hipStream_t aStream)
 	return true;
 }
 
+// Column launch for 3 x 3 x 3 kernels on rank-3 images with x-contiguous rows
+// aligned for 4-sample vectors (source and destination), an x extent that is a
+// multiple of 4, and the type pairs f32 -> f32 (FP32), or u8 / u16 with an
+// integer kernel into i32, f32 or their own type (exact); returns false when
+// the streaming or tiled kernel should run instead.
+template<class Acc>
+bool launch_column(conv_args& a, hipStream_t aStream)
+{
+	if (tune_get("conv.column", 1) == 0) {
+		return false;
+	}
+	const view4& s = a.src;
+	const view4& d = a.dst;
+	if (a.ker.extent[0] != 3 || a.ker.extent[1] != 3 || a.ker.extent[2] != 3 || a.ker.extent[3] != 1 ||
+		s.extent[3] != 1 || s.extent[0] % 4 != 0) {
+		return false;
+	}
+	const int st = s.type;
+	const int dt = d.type;
+	const bool int_source = st == IANE_U8 || st == IANE_U16;
+	const bool integer = int_source && sizeof(Acc) == 8 &&
+		(a.ker.type == IANE_U8 || a.ker.type == IANE_U16 || a.ker.type == IANE_I32) &&
+		(dt == st || dt == IANE_I32 || dt == IANE_F32);
+	const bool fp32 = sizeof(Acc) == 4 && st == IANE_F32 && dt == IANE_F32;
+	if (!integer && !fp32) {
+		return false;
+	}
+	const long long ss = dtype_size(st);
+	const long long ds = dtype_size(dt);
+	const auto aligned = [](const view4& v, long long aVector) {
+		const auto address = reinterpret_cast<uintptr_t>(v.ptr);
+		return address % aVector == 0 && v.stride[1] % aVector == 0 && v.stride[2] % aVector == 0;
+	};
+	if (s.stride[0] != ss || d.stride[0] != ds || !aligned(s, 4 * ss) || !aligned(d, 4 * ds)) {
+		return false;
+	}
+	a.src_span = plane_span(s);
+	a.dst_span = plane_span(d);
+	if (a.src_span < 0 || a.dst_span < 0) {
+		return false;
+	}
+	const int r = tune_get("conv.column_rows", 4) == 2 ? 2 : 4;
+	a.tiles_x = ceil_div(s.extent[0], 64);
+	a.tiles_y = ceil_div(s.extent[1], 16 * r);
+	// Up to four workgroups per compute unit in one round, each marching an
+	// equal z chunk of at least 8 planes: the chunk's two halo planes are read
+	// again by its neighbour (from HBM: 256^3 f32 measured 1.25 x the input at
+	// 8 planes), while longer chunks leave compute units idle (8 to 16 planes
+	// measured fastest on 256^3).
+	const long long xy = static_cast<long long>(a.tiles_x) * a.tiles_y;
+	const long long want = 4LL * compute_units();
+	const long long chunks = std::max<long long>(1, ceil_div(want, xy));
+	int chunk = static_cast<int>(std::max<long long>(8, ceil_div(s.extent[2], chunks)));
+	const int forced = tune_get("conv.column_chunk", 0);
+	if (forced > 0) {
+		chunk = forced;
+	}
+	const long long blocks = xy * ceil_div(s.extent[2], chunk);
+	if (blocks > 0x7fffffffLL || blocks == 0) {
+		return false;
+	}
+	const unsigned nb = static_cast<unsigned>(blocks);
+	const auto go = [&](auto aSource, auto aDest, auto aInteger) {
+		using S = decltype(aSource);
+		using D = decltype(aDest);
+		constexpr bool I = decltype(aInteger)::value;
+		if (r == 4) {
+			launch_column_rows<S, D, I, 4>(a, chunk, nb, aStream);
+		} else {
+			launch_column_rows<S, D, I, 2>(a, chunk, nb, aStream);
+		}
+	};
+	if (fp32) {
+		go(float{}, float{}, std::false_type{});
+		return true;
+	}
+	const auto with_dst = [&](auto aSource) {
+		using S = decltype(aSource);
+		if (dt == IANE_I32) {
+			go(S{}, int32_t{}, std::true_type{});
+		} else if (dt == IANE_F32) {
+			go(S{}, float{}, std::true_type{});
+		} else {
+			go(S{}, S{}, std::true_type{});
+		}
+	};
+	if (st == IANE_U8) {
+		with_dst(uint8_t{});
+	} else {
+		with_dst(uint16_t{});
+	}
+	return true;
+}
+
 template<class Acc, int KX>
 void launch_tiled(const conv_args& a, size_t aLds, unsigned aBlocks, hipStream_t aStream)
 {
@@ -850,7 +1410,8 @@ void launch_conv(conv_args a, hipStream_t aStream, bool aForceSimple)
 	const size_t lds = sizeof(Acc) * (static_cast<size_t>(kTileY + ky - 1) * (kTileX + kx - 1) +
 		static_cast<size_t>(kx) * ky);
 	const long long count = element_count(a.dst);
-	if (!aForceSimple && tune_get("conv.stream", 1) != 0 && launch_stream<Acc>(a, aStream)) {
+	if (!aForceSimple && tune_get("conv.stream", 1) != 0 &&
+		(launch_column<Acc>(a, aStream) || launch_stream<Acc>(a, aStream))) {
 		return;
 	}
 	if (aForceSimple || lds > kMaxLds) {

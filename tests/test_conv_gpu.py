@@ -356,6 +356,79 @@ def test_full_size_separable_vs_fp64_oracle():
     assert err <= TOL_FP32
 
 
+@pytest.mark.parametrize("shape", [(20, 37, 68), (5, 33, 128), (70, 18, 64), (1, 5, 4), (13, 100, 196)])
+@pytest.mark.parametrize("border", [cases.CLAMP, cases.ZERO])
+def test_column_kernel_equals_tiled(shape, border):
+    """The column kernel (3x3x3 kernels, rows aligned for 4-sample vectors, x extent
+    a multiple of 4) against the streaming and the tiled kernel: identical bits for
+    FP32 sums and for integer images with integer kernels, whether the int32 sum is
+    exact in FP32 (binomial-like kernels), in int32 only, or coefficients up to 2^21
+    force the FP64 chain; both rows-per-lane settings and z chunks of 1, 5 and the
+    default (ragged last chunks included)."""
+    from iane_amd import autotune
+    rng = np.random.default_rng(sum(shape) + border)
+    for sdt, kdt, ddt, prec in [(np.float32, np.float32, np.float32, precision.fp32),
+                                (np.uint16, np.int32, np.int32, precision.reference),
+                                (np.uint16, np.uint8, np.uint16, precision.reference),
+                                (np.uint8, np.uint8, np.int32, precision.reference),
+                                (np.uint8, np.int32, np.uint8, precision.reference),
+                                (np.uint8, np.uint16, np.float32, precision.reference),
+                                (np.uint16, "mid", np.int32, precision.reference),
+                                (np.uint16, "big", np.int32, precision.reference)]:
+        if kdt in ("mid", "big"):
+            src, _ = _case(rng, shape, (3, 3, 3), sdt, np.int32)
+            bound = 1000 if kdt == "mid" else 1 << 21
+            ker = rng.integers(-bound, bound, (3, 3, 3)).astype(np.int32)
+        else:
+            src, ker = _case(rng, shape, (3, 3, 3), sdt, kdt)
+        outs = []
+        for config in ({"conv.stream": 0}, {"conv.column": 0}, {}, {"conv.column_rows": 2},
+                       {"conv.column_chunk": 1}, {"conv.column_chunk": 5}):
+            with autotune.applied(config):
+                outs.append(gpu_convolve(src, ker, ddt, border, prec))
+        for out in outs[1:]:
+            same_bits(out, outs[0])
+        if prec == precision.reference:
+            same_bits(outs[0], oracle.convolve(src, ker, ddt, border))
+
+
+def test_column_kernel_views():
+    """Views the column kernel must decline or handle: a source slice whose rows
+    are not 16-byte aligned, a strided x axis, an x extent not a multiple of 4, a
+    strided destination, and a negative z stride give the tiled kernel's bits."""
+    from iane_amd import _lib, autotune
+    rng = np.random.default_rng(5)
+    base = (rng.random((12, 40, 136)) * 10).astype(np.float32)
+    ker = (rng.random((3, 3, 3)) - 0.3).astype(np.float32)
+    dbase, dker = dev(base), dev(ker)
+    opts = convolution_options(border_mode.zero, precision.fp32)
+    for h, d in ((base[1:11, 2:38, 1:129], dbase[1:11, 2:38, 1:129]), (base[:, :, ::2], dbase[:, :, ::2]),
+                 (base[:, :, 4:130], dbase[:, :, 4:130]), (base[:, :, 0:126], dbase[:, :, 0:126]),
+                 (base[:, 3:, 8:], dbase[:, 3:, 8:])):
+        outs = []
+        for config in ({"conv.stream": 0}, {}):
+            out_base = empty((h.shape[0], h.shape[1], h.shape[2] * 2), np.float32)
+            for out in (out_base[:, :, ::2], out_base[:, :, :h.shape[2]]):
+                with autotune.applied(config):
+                    iane_amd.convolve(d, dker, out, opts).wait()
+                outs.append(host(out))
+        for out in outs[1:]:
+            same_bits(out, outs[0])
+    desc = iane_amd.view_desc(dbase)
+    desc.ptr = dbase.data_ptr() + (base.shape[0] - 1) * desc.stride_bytes[2]
+    desc.stride_bytes[2] = -desc.stride_bytes[2]
+    outs = []
+    for config in ({"conv.stream": 0}, {}):
+        out = empty(base.shape, np.float32)
+        with autotune.applied(config):
+            _lib.call("iane_hip_convolve", desc, iane_amd.view_desc(dker), iane_amd.view_desc(out), cases.ZERO,
+                      int(precision.fp32), 0, 0, 0)
+            iane_amd.hip_execution(0).wait()
+        outs.append(host(out))
+    same_bits(outs[0], outs[1])
+    same_bits(outs[1], gpu_convolve(base[::-1], ker, np.float32, cases.ZERO, precision.fp32))
+
+
 @pytest.mark.parametrize("shape,kshape", [((40, 37, 70), (3, 3, 3)), ((9, 20, 130), (5, 3, 7)), ((5, 33, 64), (7, 5, 1)),
                                           ((3, 17, 65), (5, 5, 5)), ((70, 16, 64), (3, 7, 3))])
 @pytest.mark.parametrize("border", [cases.CLAMP, cases.ZERO])
