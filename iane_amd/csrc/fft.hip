@@ -13,8 +13,9 @@
 #include "fft.hpp"
 
 #include <atomic>
-
+#include <complex>
 #include <map>
+#include <tuple>
 #include <memory>
 #include <mutex>
 #include <utility>
@@ -129,43 +130,168 @@ size_t max_lds_per_block()
 	return static_cast<size_t>(bytes);
 }
 
+namespace {
+
+// Forward DFT in double of a length whose prime factors are small (mixed-radix
+// decimation in time; naive DFT for a prime length). Host only: the Bluestein
+// filter spectrum.
+void host_dft(std::vector<std::complex<double>>& aX)
+{
+	const size_t n = aX.size();
+	if (n <= 1) {
+		return;
+	}
+	size_t p = 2;
+	while (n % p != 0) {
+		++p;
+	}
+	const double pi = 3.14159265358979323846;
+	std::vector<std::complex<double>> out(n);
+	if (p == n) {
+		for (size_t k = 0; k < n; ++k) {
+			std::complex<double> sum = 0.0;
+			for (size_t j = 0; j < n; ++j) {
+				sum += aX[j] * std::polar(1.0, -2.0 * pi * static_cast<double>((j * k) % n) / static_cast<double>(n));
+			}
+			out[k] = sum;
+		}
+		aX.swap(out);
+		return;
+	}
+	const size_t q = n / p;
+	std::vector<std::vector<std::complex<double>>> sub(p, std::vector<std::complex<double>>(q));
+	for (size_t r = 0; r < p; ++r) {
+		for (size_t j = 0; j < q; ++j) {
+			sub[r][j] = aX[r + p * j];
+		}
+		host_dft(sub[r]);
+	}
+	for (size_t k = 0; k < n; ++k) {
+		std::complex<double> sum = 0.0;
+		for (size_t r = 0; r < p; ++r) {
+			sum += sub[r][k % q] * std::polar(1.0, -2.0 * pi * static_cast<double>((r * k) % n) / static_cast<double>(n));
+		}
+		out[k] = sum;
+	}
+	aX.swap(out);
+}
+
+struct bluestein_tables {
+	float2* chirp = nullptr;
+	float2* spectrum = nullptr;
+};
+
+std::mutex g_bluestein_mutex;
+std::map<std::tuple<int, int, int>, bluestein_tables> g_bluestein;
+
+// Chirp exp(-i pi j^2 / n) (j < n; the angle reduced exactly with j^2 mod 2n)
+// and the spectrum of the length-aBig filter b[j] = b[aBig - j] = conj(chirp[j])
+// (j < n, zero between), divided by aBig. Cached per (device, n, aBig).
+bluestein_tables bluestein(int aN, int aBig)
+{
+	int device = 0;
+	check_hip(hipGetDevice(&device), "Query HIP device");
+	std::lock_guard lock(g_bluestein_mutex);
+	auto& slot = g_bluestein[{device, aN, aBig}];
+	if (!slot.chirp) {
+		const double pi = 3.14159265358979323846;
+		std::vector<std::complex<double>> chirp(aN);
+		for (long long j = 0; j < aN; ++j) {
+			const long long r = (j * j) % (2LL * aN);
+			chirp[j] = std::polar(1.0, -pi * static_cast<double>(r) / static_cast<double>(aN));
+		}
+		std::vector<std::complex<double>> filter(aBig, 0.0);
+		for (int j = 0; j < aN; ++j) {
+			filter[j] = std::conj(chirp[j]);
+			if (j > 0) {
+				filter[aBig - j] = std::conj(chirp[j]);
+			}
+		}
+		host_dft(filter);
+		std::vector<float2> c(aN), f(aBig);
+		for (int j = 0; j < aN; ++j) {
+			c[j] = make_float2(static_cast<float>(chirp[j].real()), static_cast<float>(chirp[j].imag()));
+		}
+		for (int k = 0; k < aBig; ++k) {
+			const std::complex<double> v = filter[k] / static_cast<double>(aBig);
+			f[k] = make_float2(static_cast<float>(v.real()), static_cast<float>(v.imag()));
+		}
+		bluestein_tables t;
+		check_hip(hipMalloc(&t.chirp, sizeof(float2) * aN), "Allocate Bluestein chirp");
+		check_hip(hipMalloc(&t.spectrum, sizeof(float2) * aBig), "Allocate Bluestein spectrum");
+		check_hip(hipMemcpy(t.chirp, c.data(), sizeof(float2) * aN, hipMemcpyHostToDevice), "Upload Bluestein chirp");
+		check_hip(hipMemcpy(t.spectrum, f.data(), sizeof(float2) * aBig, hipMemcpyHostToDevice),
+			"Upload Bluestein spectrum");
+		slot = t;
+	}
+	return slot;
+}
+
+// Complex elements of LDS one workgroup may use (1 KiB kept for the kernels'
+// static LDS: batched descriptors, reduction slots).
+long long lds_elems_limit()
+{
+	return (static_cast<long long>(max_lds_per_block()) - 1024) / static_cast<long long>(sizeof(float2));
+}
+
+} // namespace
+
+bool fft_line_fits(int aM)
+{
+	fft_radix_plan plan;
+	return factor_radices(aM, plan) && 2LL * (aM + 1) <= lds_elems_limit();
+}
+
 // Fill the radix plan, tables, LDS pitch and bundle of a pass for a transform
-// of logical length n along lines of `lines` x `outer`.
+// of logical length n along lines of `lines` x `outer`. A complex line length m
+// whose radices are supported runs its own stages; otherwise Bluestein's form
+// with the smallest transform length M >= 2m - 1 of radices 2..7. Both hold a
+// line (M + 1 points for Bluestein) twice in LDS: lines longer than that are
+// refused here (the plan splits such axes into shorter passes first).
 void setup_pass(fft_pass_args& a, int aMode, int aN, int aLines, int aOuter)
 {
 	a.mode = aMode;
 	a.n = aN;
 	const bool half = (aMode == PASS_R2C || aMode == PASS_C2R) && (aN % 2 == 0);
 	const int m = half ? aN / 2 : aN;
-	if (!factor_radices(m, a.plan)) {
-		fail(IANE_ERR_UNSUPPORTED, "FFT length " + std::to_string(aN) + " has a prime factor above 64");
+	a.line_m = m;
+	a.chirp = nullptr;
+	a.bspec = nullptr;
+	const long long limit = lds_elems_limit();
+	if (!factor_radices(m, a.plan) || 2LL * (m + 1) > limit) {
+		if (aMode == PASS_C2C_MULT) {
+			fail(IANE_ERR_UNSUPPORTED, "FFT length " + std::to_string(aN) + " needs radices 2..64 in a spectrum pass");
+		}
+		const int big = next_fft_length(2 * m - 1);
+		if (!factor_radices(big, a.plan) || 2LL * (big + 1) > limit) {
+			fail(IANE_ERR_UNSUPPORTED, "FFT line length " + std::to_string(aN) + " exceeds the LDS line limit");
+		}
+		const bluestein_tables t = bluestein(m, big);
+		a.chirp = t.chirp;
+		a.bspec = t.spectrum;
 	}
-	a.pitch = m + 1;
-	if (a.pitch > kFftLdsElems) {
-		fail(IANE_ERR_UNSUPPORTED, "FFT line length " + std::to_string(aN) + " exceeds the LDS line limit");
-	}
-	a.tw_m = twiddle_table(m);
+	const int stages = a.plan.m;
+	a.pitch = stages + 1;
+	a.tw_m = twiddle_table(stages);
 	a.tw_n = half ? twiddle_table(aN) : nullptr;
 	a.lines = aLines;
 	a.outer = aOuter;
+	// the twiddle tables follow the two line buffers in LDS when one line of
+	// each buffer and the tables fit; longer lines read them from memory
+	const int tables = pass_table_elems(stages, m, half);
+	a.tables_in_lds = 2LL * a.pitch + tables <= limit ? 1 : 0;
+	const int staged = a.tables_in_lds ? tables : 0;
 	// Lines per workgroup: as many as fit an LDS budget of "fft.lds_elems" complex
 	// elements per buffer (autotuned; default kFftLdsElems). A smaller budget means
 	// less LDS per workgroup and more resident workgroups per CU. Every line is
 	// transformed the same way whatever the bundle, so results do not depend on it.
-	const int budget = std::clamp(tune_get("fft.lds_elems", kFftLdsElems), a.pitch, kFftLdsElems);
+	const int budget = std::max(a.pitch, std::min(tune_get("fft.lds_elems", kFftLdsElems), kFftLdsElems));
 	// the twiddle tables come out of the same budget of 2 * budget elements, so
 	// the workgroups resident per CU do not change with them
-	const int tables = pass_table_elems(m, half);
-	a.bundle = std::max(1, std::min(aLines, (2 * budget - tables) / (2 * a.pitch)));
-	// the tables follow the two line buffers in LDS: fewer lines if the
-	// workgroup would exceed the device's LDS (1 KiB kept for the kernels'
-	// static LDS: batched descriptors, reduction slots)
-	const long long limit = (static_cast<long long>(max_lds_per_block()) - 1024) / static_cast<long long>(sizeof(float2));
-	while (a.bundle > 1 && 2LL * a.bundle * a.pitch + tables > limit) {
+	a.bundle = std::max(1, std::min(aLines, (2 * budget - staged) / (2 * a.pitch)));
+	// fewer lines if the workgroup would exceed the device's LDS
+	while (a.bundle > 1 && 2LL * a.bundle * a.pitch + staged > limit) {
 		--a.bundle;
-	}
-	if (2LL * a.bundle * a.pitch + tables > limit) {
-		fail(IANE_ERR_UNSUPPORTED, "FFT line length " + std::to_string(aN) + " exceeds the LDS line limit");
 	}
 	a.lds_elems = a.bundle * a.pitch;
 	a.groups_per_outer = ceil_div(aLines, a.bundle);

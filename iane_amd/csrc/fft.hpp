@@ -11,6 +11,13 @@
 // Real transforms along axis 0 use the half-length complex trick for even n
 // (z[j] = x[2j] + i x[2j+1], one m = n/2 FFT and an O(n) post/pre-process)
 // and a full-length complex FFT for odd n.
+//
+// A line length m with a prime factor above 64 runs Bluestein's chirp-z form in
+// the same LDS buffers: with w[j] = exp(sign i pi j^2 / m),
+//   X[k] = w[k] sum_j (x[j] w[j]) conj(w[k - j]),
+// a linear convolution evaluated with a length-M transform (M >= 2m - 1, radices
+// 2..7): chirp and zero-pad, forward FFT, multiply by the spectrum of the
+// wrapped conj(w) (precomputed in double, divided by M), inverse FFT, chirp.
 #pragma once
 
 #include "common.hpp"
@@ -315,7 +322,11 @@ struct fft_pass_args {
 	int pitch;        // LDS pitch per line (complex elements)
 	int lds_elems;    // complex elements per LDS buffer (bundle * pitch); two buffers
 	unsigned groups_per_outer;
-	fft_radix_plan plan; // the complex transform run in LDS (length m)
+	fft_radix_plan plan; // the complex transform run in LDS (length m; M for Bluestein)
+	int line_m;          // complex length of the line transform
+	int tables_in_lds;   // 1: twiddle tables staged in LDS after the line buffers; 0: read from memory
+	const float2* chirp; // Bluestein (else null): exp(-i pi j^2 / line_m), j < line_m
+	const float2* bspec; // Bluestein: spectrum of the wrapped conj(chirp), length M, divided by M
 	const float2* tw_m;
 	const float2* tw_n;  // W_n table for the real pre/post processing (even n)
 	const float2* mult;  // PASS_C2C_MULT: spectrum multiplier, read with gmult
@@ -546,16 +557,20 @@ __device__ inline void store_real_lines(float* out, const line_geom& g, int aOut
 		});
 }
 
-// Twiddle entries a pass keeps in LDS: W_m (m entries) and, for a half-length
-// real transform, W_n^k for k = 0 .. m (the only ones its pre/post step reads).
-__host__ __device__ inline int pass_table_elems(int aM, bool aHalf) { return aM + (aHalf ? aM + 1 : 0); }
+// Twiddle entries a pass keeps in LDS: W of the transform length (aStages
+// entries) and, for a half-length real transform of line length aM, W_n^k for
+// k = 0 .. aM (the only ones its pre/post step reads).
+__host__ __device__ inline int pass_table_elems(int aStages, int aM, bool aHalf)
+{
+	return aStages + (aHalf ? aM + 1 : 0);
+}
 
 // Dynamic LDS of a launch running pass `a`: two buffers of a.lds_elems, then
-// the twiddle tables.
+// the twiddle tables when they are staged.
 inline size_t pass_lds_bytes(const fft_pass_args& a)
 {
-	return sizeof(float2) *
-		(2 * static_cast<size_t>(a.lds_elems) + static_cast<size_t>(pass_table_elems(a.plan.m, a.tw_n != nullptr)));
+	const size_t tables = a.tables_in_lds ? pass_table_elems(a.plan.m, a.line_m, a.tw_n != nullptr) : 0;
+	return sizeof(float2) * (2 * static_cast<size_t>(a.lds_elems) + tables);
 }
 
 // The twiddle tables are copied into LDS after the two line buffers (W_m, then
@@ -578,7 +593,7 @@ __device__ __forceinline__ twiddle_fetch fetch_twiddles(const fft_pass_args& a)
 	// unconditional loads (no branch before their use): without a W_n table the
 	// second pair re-reads W_m and is never stored
 	const float2* tn = a.tw_n ? a.tw_n : a.tw_m;
-	const int last_n = a.tw_n ? m : m - 1;
+	const int last_n = a.tw_n ? a.line_m : m - 1;
 #pragma unroll
 	for (int u = 0; u < 2; ++u) {
 		const int i = static_cast<int>(threadIdx.x + u * blockDim.x);
@@ -602,14 +617,77 @@ __device__ __forceinline__ void store_table(const float2 (&aFetched)[2], const f
 	}
 }
 
-// aLds: the table area (W_m at aLds, W_n at aLds + m).
+// aLds: the table area (W_m at aLds, W_n at aLds + m). Nothing is staged when
+// the tables stay in memory (lines too long for both in LDS).
 __device__ __forceinline__ void store_twiddles(const twiddle_fetch& aFetched, const fft_pass_args& a, float2* aLds)
 {
+	if (!a.tables_in_lds) {
+		return;
+	}
 	const int m = a.plan.m;
 	store_table(aFetched.m, a.tw_m, m, aLds);
 	if (a.tw_n) {
-		store_table(aFetched.n, a.tw_n, m + 1, aLds + m);
+		store_table(aFetched.n, a.tw_n, a.line_m + 1, aLds + m);
 	}
+}
+
+// Where the stages and the real pre/post steps read the twiddles: the staged
+// copies after the line buffers, or the tables in memory.
+__device__ __forceinline__ void pass_twiddles(const fft_pass_args& a, const float2* aLds, const float2*& aTwM,
+	const float2*& aTwN)
+{
+	aTwM = a.tables_in_lds ? aLds : a.tw_m;
+	aTwN = a.tables_in_lds ? aLds + a.plan.m : a.tw_n;
+}
+
+// The complex transform (direction `sign`) of the `count` lines in buffer x
+// (scratch y): the radix stages of a.plan, or Bluestein's form when a.chirp is
+// set. Returns the buffer holding the result; ends with a barrier.
+__device__ __forceinline__ float2* run_line(const fft_pass_args& a, float2* x, float2* y, int count, const float2* tw,
+	float sign)
+{
+	if (a.chirp == nullptr) {
+		return sign < 0.0f ? run_stages(a.plan, x, y, count, a.pitch, tw, -1.0f)
+		                   : run_stages(a.plan, x, y, count, a.pitch, tw, 1.0f);
+	}
+	const int n = a.line_m;
+	const int big = a.plan.m;
+	const bool inverse = sign > 0.0f;
+	const fast_div by_big(big), by_n(n);
+	// x[j] w[j] for j < n, zero up to M (w = chirp, conjugated for the inverse)
+	for (int t = threadIdx.x; t < count * big; t += blockDim.x) {
+		const int w = by_big.div(t);
+		const int j = t - w * big;
+		float2* at = x + w * a.pitch + j;
+		if (j < n) {
+			const float2 c = a.chirp[j];
+			*at = cmul(*at, inverse ? cconj(c) : c);
+		} else {
+			*at = make_float2(0.0f, 0.0f);
+		}
+	}
+	__syncthreads();
+	float2* r = run_stages(a.plan, x, y, count, a.pitch, tw, -1.0f);
+	// times the filter spectrum: B for the forward direction; for the inverse the
+	// filter is the conjugate sequence, whose spectrum is conj(B[-k])
+	for (int t = threadIdx.x; t < count * big; t += blockDim.x) {
+		const int w = by_big.div(t);
+		const int k = t - w * big;
+		const float2 h = inverse ? cconj(a.bspec[k == 0 ? 0 : big - k]) : a.bspec[k];
+		float2* at = r + w * a.pitch + k;
+		*at = cmul(*at, h);
+	}
+	__syncthreads();
+	float2* q = run_stages(a.plan, r, r == x ? y : x, count, a.pitch, tw, 1.0f);
+	for (int t = threadIdx.x; t < count * n; t += blockDim.x) {
+		const int w = by_n.div(t);
+		const int k = t - w * n;
+		const float2 c = a.chirp[k];
+		float2* at = q + w * a.pitch + k;
+		*at = cmul(*at, inverse ? cconj(c) : c);
+	}
+	__syncthreads();
+	return q;
 }
 
 // One workgroup's share of a line pass: the lines of group `aBlock` of `a`.
@@ -619,11 +697,12 @@ __device__ __forceinline__ void fft_pass_body(const fft_pass_args& a, unsigned a
 {
 	float2* A = lds;
 	float2* B = lds + a.lds_elems;
-	const int m = a.plan.m;
+	const int m = a.line_m;
 	const twiddle_fetch fetched = fetch_twiddles(a);
 	float2* const tw_lds = lds + 2 * a.lds_elems;
-	const float2* tw_m = tw_lds;
-	const float2* tw_n = tw_lds + m; // read only by the half-length real modes, which stage it
+	const float2* tw_m;
+	const float2* tw_n; // read only by the half-length real modes
+	pass_twiddles(a, tw_lds, tw_m, tw_n);
 	int o, l0, count;
 	decode_group(a, aBlock, o, l0, count);
 	const int n = a.n;
@@ -631,9 +710,7 @@ __device__ __forceinline__ void fft_pass_body(const fft_pass_args& a, unsigned a
 		load_complex_lines(static_cast<const float2*>(a.in), a.gin, o, l0, count, a.load_n, m, A, a.pitch);
 		store_twiddles(fetched, a, tw_lds);
 		__syncthreads();
-		// a constant sign in each copy folds the conjugations and (sign i) rotations
-		const float2* r = a.sign < 0.0f ? run_stages(a.plan, A, B, count, a.pitch, tw_m, -1.0f)
-		                                : run_stages(a.plan, A, B, count, a.pitch, tw_m, 1.0f);
+		const float2* r = run_line(a, A, B, count, tw_m, a.sign);
 		store_complex_lines(static_cast<float2*>(a.out), a.gout, o, l0, count, a.store_n, r, a.pitch);
 	} else if constexpr (MODE == PASS_C2C_MULT) {
 		load_complex_lines(static_cast<const float2*>(a.in), a.gin, o, l0, count, a.load_n, m, A, a.pitch);
@@ -671,7 +748,7 @@ __device__ __forceinline__ void fft_pass_body(const fft_pass_args& a, unsigned a
 		load_real_lines(static_cast<const float*>(a.in), a.gin, o, l0, count, n, a.load_n, A, a.pitch);
 		store_twiddles(fetched, a, tw_lds);
 		__syncthreads();
-		float2* r = run_stages(a.plan, A, B, count, a.pitch, tw_m, -1.0f);
+		float2* r = run_line(a, A, B, count, tw_m, -1.0f);
 		if ((n & 1) == 0) {
 			float2* x = r == A ? B : A;
 			r2c_post(r, x, count, n, a.pitch, tw_n);
@@ -686,7 +763,7 @@ __device__ __forceinline__ void fft_pass_body(const fft_pass_args& a, unsigned a
 			__syncthreads();
 			c2r_pre(A, B, count, n, a.pitch, tw_n);
 			__syncthreads();
-			const float2* r = run_stages(a.plan, B, A, count, a.pitch, tw_m, 1.0f);
+			const float2* r = run_line(a, B, A, count, tw_m, 1.0f);
 			store_real_lines(static_cast<float*>(a.out), a.gout, o, l0, count, n, a.store_n, r, a.pitch);
 		} else {
 			load_complex_lines(static_cast<const float2*>(a.in), a.gin, o, l0, count, a.load_n, m, A, a.pitch);
@@ -694,7 +771,7 @@ __device__ __forceinline__ void fft_pass_body(const fft_pass_args& a, unsigned a
 			__syncthreads();
 			c2r_extend_odd(A, count, n, a.pitch);
 			__syncthreads();
-			const float2* r = run_stages(a.plan, A, B, count, a.pitch, tw_m, 1.0f);
+			const float2* r = run_line(a, A, B, count, tw_m, 1.0f);
 			store_real_lines(static_cast<float*>(a.out), a.gout, o, l0, count, n, a.store_n, r, a.pitch);
 		}
 	}

@@ -215,8 +215,9 @@ __device__ __forceinline__ void rl_row_body(const rl_row_args& a, unsigned aBloc
 	float2* A = lds;
 	float2* B = lds + f.lds_elems;
 	const twiddle_fetch fetched = fetch_twiddles(f);
-	const float2* tw_m = lds + 2 * f.lds_elems;
-	const float2* tw_n = tw_m + f.plan.m; // W_n^0 .. W_n^m (the rows are half-length real transforms)
+	const float2* tw_m;
+	const float2* tw_n; // W_n^0 .. W_n^m (the rows are half-length real transforms)
+	pass_twiddles(f, lds + 2 * f.lds_elems, tw_m, tw_n);
 	int o, l0, count;
 	decode_group(f, aBlock, o, l0, count);
 	const int n = f.n;

@@ -189,6 +189,74 @@ def test_r2c_c2r_nd(shape):
         del full
 
 
+# ------------------------------------- Bluestein and long lines -----
+def _dft_direct(x: np.ndarray, sign: int = -1) -> np.ndarray:
+    """O(n^2) float64 DFT of one line, in blocks of 256 bins, with the angle of
+    each term reduced exactly (j k mod n) before scaling."""
+    x = np.asarray(x, dtype=np.complex128)
+    n = x.size
+    j = np.arange(n, dtype=np.int64)
+    out = np.empty(n, dtype=np.complex128)
+    for k0 in range(0, n, 256):
+        k = np.arange(k0, min(n, k0 + 256), dtype=np.int64)
+        phase = (np.outer(k, j) % n).astype(np.float64) * (sign * 2.0 * np.pi / n)
+        out[k0:k0 + k.size] = np.exp(1j * phase) @ x
+    return out
+
+
+@pytest.mark.parametrize("n", [67, 127, 1009, 2053, 4099, 4096, 8192, 10000])
+def test_c2c_prime_and_long_lengths_vs_direct_dft(n):
+    """Lengths with a prime factor above 64 run Bluestein's chirp-z form (67, 127,
+    1009, 2053, 4099 are prime); lines past 4 095 points use the whole LDS with the
+    twiddles read from memory (4096, 8192, 10000). Forward and inverse against the
+    direct float64 DFT: max|gpu - dft| <= 1e-5 max|dft|."""
+    rng = np.random.default_rng(n)
+    x = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    ctx = execution_context()
+    for alg, sign in ((fft_forward, -1), (fft_inverse, 1)):
+        src, out = dev(x), cempty((n,))
+        op = ctx.prepare(alg, src, out)
+        ctx.run(op, src, out).wait()
+        err = rel(host(out), _dft_direct(x, sign))
+        print(f"n={n} sign={sign}: {err:.2e}")
+        assert err <= TOL_FFT
+
+
+@pytest.mark.parametrize("shape", [(1009,), (2018,), (8192,), (8190,), (5, 4099), (131, 20), (3, 67, 10), (12, 2, 1031)])
+def test_prime_and_long_axes_nd(shape):
+    """Bluestein and long lines on every axis and through the real transforms: an
+    odd prime real length (1009), an even one whose half length is prime (2018),
+    long even lines (8192, and 8190 = 2 x 4095 with half length 4095), and prime
+    extents on strided axes; C2C and R2C/C2R against the float64 FFT (itself
+    pinned by the brute-force DFT), round trips included."""
+    rng = np.random.default_rng(sum(shape))
+    ctx = execution_context()
+    x = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+    src, out, back = dev(x), cempty(shape), cempty(shape)
+    ctx.run(ctx.prepare(fft_forward, src, out), src, out).wait()
+    assert rel(host(out), oracle.fft_forward(x)) <= TOL_FFT
+    ctx.run(ctx.prepare(fft_inverse, out, back), out, back).wait()
+    assert rel(host(back), x * np.prod(shape)) <= TOL_FFT
+    r = rng.standard_normal(shape).astype(np.float32)
+    spec_shape = rfft_extent(shape)
+    rsrc, spec, rback = dev(r), cempty(spec_shape), rempty(shape)
+    ctx.run(ctx.prepare(rfft, rsrc, spec), rsrc, spec).wait()
+    assert rel(host(spec), oracle.rfft(r)) <= TOL_FFT
+    ctx.run(ctx.prepare(irfft, spec, rback), spec, rback).wait()
+    assert rel(host(rback), r.astype(np.float64) * np.prod(shape)) <= TOL_FFT
+
+
+def test_line_too_long_for_lds_is_refused():
+    """A line whose transform (or Bluestein length) cannot hold two copies in LDS
+    is refused with the unsupported status, not run."""
+    from iane_amd import _lib
+    ctx = execution_context()
+    x = dev(np.zeros(40009, np.complex64))  # prime: Bluestein length > 80 000
+    out = cempty((40009,))
+    with pytest.raises(_lib.unsupported):
+        ctx.prepare(fft_forward, x, out)
+
+
 def test_fft_workspace_reported_for_c2r():
     ctx = execution_context()
     spec, back = cempty((6, 9)), rempty((6, 16))
