@@ -25,7 +25,7 @@ def build(force: bool = False) -> dict:
                                                     for s in [source, *headers])
         if stale:
             cmd = ["g++", "-std=c++20", "-O2", "-Wall", "-Wextra", "-pthread", f"-I{ROOT / 'include'}", str(source),
-                   "-o", str(binary), f"-L{ROOT / 'iane_amd' / 'lib'}", "-liane_hip", "-lz",
+                   "-o", str(binary), f"-L{ROOT / 'iane_amd' / 'lib'}", "-liane_hip",
                    "-Wl,-rpath,$ORIGIN/../../iane_amd/lib"]
             subprocess.run(cmd, check=True)
         out[name] = binary
