@@ -281,10 +281,12 @@ inline void check_workspace(const void* aWorkspace, size_t aBytes, size_t aNeede
 
 // Split-precision MFMA convolution (conv_mfma.hip). The kernel part (scale and
 // Toeplitz table, split_kernel_bytes) can be built once per kernel; the source
-// part (scale and padded hi/lo planes, split_source_bytes) is scratch per call.
+// part (scale and padded part planes, split_source_bytes) is scratch per call.
+// Sizes depend on the precision (two 16-bit parts per operand for FP16, three
+// for BF16).
 bool split_applicable(const view4& aSrc, const view4& aKer, const view4& aDst);
-size_t split_kernel_bytes(const view4& aKer);
-size_t split_source_bytes(const view4& aSrc, const view4& aKer);
+size_t split_kernel_bytes(const view4& aKer, int aPrecision);
+size_t split_source_bytes(const view4& aSrc, const view4& aKer, int aPrecision);
 void split_prepare_kernel(const view4& aKer, int aPrecision, void* aKernelPart, hipStream_t aStream);
 void split_convolve_prepared(const view4& aSrc, const view4& aKer, const view4& aDst, int aBorder, int aPrecision,
 	const void* aKernelPart, void* aSourcePart, hipStream_t aStream);

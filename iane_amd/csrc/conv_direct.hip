@@ -1521,7 +1521,7 @@ size_t convolve_workspace(const convolve_call& c)
 	if (!c.split || element_count(c.dst) == 0) {
 		return 0;
 	}
-	return split_kernel_bytes(c.ker) + split_source_bytes(c.src, c.ker);
+	return split_kernel_bytes(c.ker, c.precision) + split_source_bytes(c.src, c.ker, c.precision);
 }
 
 } // namespace
@@ -1551,7 +1551,7 @@ extern "C" int iane_hip_convolve(const iane_view_desc* aSrc, const iane_view_des
 			char* kernel_part = static_cast<char*>(aWorkspace);
 			split_prepare_kernel(c.ker, c.precision, kernel_part, stream);
 			split_convolve_prepared(c.src, c.ker, c.dst, aBorder, c.precision, kernel_part,
-				kernel_part + split_kernel_bytes(c.ker), stream);
+				kernel_part + split_kernel_bytes(c.ker, c.precision), stream);
 			return;
 		}
 		convolve_direct(c.src, c.ker, c.dst, aBorder, c.precision, stream, false);

@@ -2,12 +2,14 @@
 //
 // The reference sums taps in FP64 (include/iane/algorithms/convolution.hpp:36-49);
 // this path computes the same sum on the matrix cores with FP32 accumulation.
-// Every FP32 operand v is split as v ~ hi + lo * 2^-L (hi, lo 16-bit), and a
-// product is formed from three MFMAs: hi*hi into one accumulator and
-// hi*lo + lo*hi into a second one that is scaled by 2^-L at the end (the lo*lo
-// term is below FP32 resolution for FP16). FP16 keeps 11+11 significant bits
-// per operand (power-of-two scales keep the values inside FP16 range); BF16
-// keeps 8+8 bits with the FP32 exponent range.
+// Every FP32 operand v is split into 16-bit parts. FP16: v ~ hi + lo * 2^-L,
+// 11+11 significant bits (power-of-two scales keep the values inside FP16
+// range), and a product is formed from three MFMAs: hi*hi into one accumulator
+// and hi*lo + lo*hi into a second one scaled by 2^-L at the end (lo*lo is below
+// FP32 resolution). BF16 (FP32 exponent range, 8 significant bits per part):
+// v ~ hi + mid + lo, 24 bits, and six MFMAs: hi*hi into the first accumulator,
+// hi*mid + mid*hi + hi*lo + lo*hi + mid*mid (the terms down to 2^-16 of the
+// leading one) into the second; mid*lo and smaller are below FP32 resolution.
 //
 // GEMM form. Along x the convolution is a Toeplitz product: for a window of
 // 16 outputs starting at x0, out[x0 + m] = sum_k A[m][k] * in[x0 - hx + k] with
@@ -56,6 +58,7 @@ struct split16;
 template<>
 struct split16<_Float16> {
 	using vec = f16x8;
+	static constexpr int parts = 2;
 	static constexpr int lo_shift = kLoShiftFp16;
 	__device__ static _Float16 to(float v) { return static_cast<_Float16>(v); }
 	__device__ static float from(_Float16 v) { return static_cast<float>(v); }
@@ -65,6 +68,7 @@ struct split16<_Float16> {
 template<>
 struct split16<__bf16> {
 	using vec = b16x8;
+	static constexpr int parts = 3;
 	static constexpr int lo_shift = 0;
 	__device__ static __bf16 to(float v) { return static_cast<__bf16>(v); }
 	__device__ static float from(__bf16 v) { return static_cast<float>(v); }
@@ -128,15 +132,32 @@ __global__ void scale_from_partials_kernel(const float* partial, int count, floa
 	}
 }
 
+// Write the 16-bit parts of v (scaled by 2^L after the first for FP16) to
+// aOut[0], aOut[aStride], ...
+template<class T16>
+__device__ __forceinline__ void store_parts(float v, T16* aOut, long long aStride)
+{
+	using S = split16<T16>;
+	const float lo_scale = static_cast<float>(1 << S::lo_shift);
+	const T16 hi = S::to(v);
+	aOut[0] = hi;
+	const float rest = (v - S::from(hi)) * lo_scale;
+	if constexpr (S::parts == 2) {
+		aOut[aStride] = S::to(rest);
+	} else {
+		const T16 mid = S::to(rest);
+		aOut[aStride] = mid;
+		aOut[2 * aStride] = S::to(rest - S::from(mid));
+	}
+}
+
 // ------------------------------------------------------------ padded split --
-// hi/lo planes of the padded source: padded (xp, yp, zp) holds the sample at
+// Part planes of the padded source: padded (xp, yp, zp) holds the sample at
 // (xp - hx, yp - hy, zp - hz) under the border rule.
 template<class T16>
 __global__ void split_pad_kernel(view4 src, int border, split_geom g, const float* src_scale, T16* out)
 {
-	using S = split16<T16>;
 	const float sd = *src_scale;
-	const float lo_scale = static_cast<float>(1 << S::lo_shift);
 	const long long total = g.part;
 	for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < total;
 		 i += static_cast<long long>(gridDim.x) * blockDim.x) {
@@ -152,22 +173,19 @@ __global__ void split_pad_kernel(view4 src, int border, split_geom g, const floa
 			z = min(max(z, 0), g.nz - 1);
 			v = load_as_float(src.ptr + x * src.stride[0] + y * src.stride[1] + z * src.stride[2], src.type) * sd;
 		}
-		const T16 hi = S::to(v);
-		out[i] = hi;
-		out[i + total] = S::to((v - S::from(hi)) * lo_scale);
+		store_parts(v, out + i, total);
 	}
 }
 
 // ------------------------------------------------------------------ A table --
-// table[((qz * qy_padded + qy) * steps + s) * 2 + part][lane][j]: the A operand
-// (hi or lo part) of the Toeplitz matrix of kernel row (qz, qy), K-step s, in
-// the MFMA lane layout (lane l holds A[l % 16][32 s + 8 (l / 16) + j]).
+// table[((qz * qy_padded + qy) * steps + s) * parts + part][lane][j]: the A
+// operand (one part) of the Toeplitz matrix of kernel row (qz, qy), K-step s,
+// in the MFMA lane layout (lane l holds A[l % 16][32 s + 8 (l / 16) + j]).
 template<class T16>
 __global__ void toeplitz_table_kernel(view4 ker, int hx, int qy_padded, int steps, const float* ker_scale, T16* table)
 {
 	using S = split16<T16>;
 	const float sk = *ker_scale;
-	const float lo_scale = static_cast<float>(1 << S::lo_shift);
 	const int kx = ker.extent[0], ky = ker.extent[1], kz = ker.extent[2];
 	const long long entries = static_cast<long long>(kz) * qy_padded * steps * 64 * 8;
 	for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < entries;
@@ -186,10 +204,7 @@ __global__ void toeplitz_table_kernel(view4 ker, int hx, int qy_padded, int step
 			v = static_cast<float>(load_as_double(ker.ptr + qx * ker.stride[0] + qy * ker.stride[1] + qz * ker.stride[2],
 				ker.type)) * sk;
 		}
-		const T16 hi = S::to(v);
-		const long long base = (row * 2) * 64 * 8 + lane * 8 + j;
-		table[base] = hi;
-		table[base + 64 * 8] = S::to((v - S::from(hi)) * lo_scale);
+		store_parts(v, table + (row * S::parts) * 64 * 8 + lane * 8 + j, 64 * 8);
 	}
 }
 
@@ -201,6 +216,7 @@ void conv_split_kernel(const T16* __restrict__ padded, const T16* __restrict__ t
 {
 	using S = split16<T16>;
 	using vec = typename S::vec;
+	constexpr int P = S::parts;
 	const unsigned block = xcd_remap(blockIdx.x, gridDim.x);
 	// The 4 waves of a workgroup take 4 consecutive z planes of one (strip,
 	// y block), and consecutive workgroups continue along z: workgroups that
@@ -229,23 +245,24 @@ void conv_split_kernel(const T16* __restrict__ padded, const T16* __restrict__ t
 	}
 	const long long lane_x = x0 + 16 * n + 8 * grp;  // padded x of this lane's first B element
 	const vec* tab = reinterpret_cast<const vec*>(table);
-	const vec* hi_rows = reinterpret_cast<const vec*>(padded);
-	const vec* lo_rows = reinterpret_cast<const vec*>(padded + g.part);
+	const vec* rows = reinterpret_cast<const vec*>(padded);
+	const long long part_vecs = g.part / 8;
 	const int chunks = g.qy_padded / QC;
 
 	for (int qz = 0; qz < g.kz; ++qz) {
 		const long long plane = static_cast<long long>(z + 2 * g.hz - qz) * g.plane;
 		for (int c = 0; c < chunks; ++c) {
 			const int q0 = c * QC;
-			vec a_hi[QC][STEPS];
-			vec a_lo[QC][STEPS];
+			vec a[P][QC][STEPS];
 #pragma unroll
 			for (int j = 0; j < QC; ++j) {
 #pragma unroll
 				for (int s = 0; s < STEPS; ++s) {
 					const long long row = (static_cast<long long>(qz) * g.qy_padded + q0 + j) * STEPS + s;
-					a_hi[j][s] = tab[(row * 2) * 64 + lane];
-					a_lo[j][s] = tab[(row * 2 + 1) * 64 + lane];
+#pragma unroll
+					for (int p = 0; p < P; ++p) {
+						a[p][j][s] = tab[(row * P + p) * 64 + lane];
+					}
 				}
 			}
 			// input row u pairs with output row r = u - (QC - 1) + j through qy = q0 + j
@@ -254,12 +271,13 @@ void conv_split_kernel(const T16* __restrict__ padded, const T16* __restrict__ t
 			for (int u = 0; u < kRows + QC - 1; ++u) {
 				const int yp = min(max(row0 + u, 0), g.py - 1);  // rows < 0 only meet zero A (padded qy)
 				const long long base = (plane + static_cast<long long>(yp) * g.px + lane_x) / 8;
-				vec b_hi[STEPS];
-				vec b_lo[STEPS];
+				vec b[P][STEPS];
 #pragma unroll
 				for (int s = 0; s < STEPS; ++s) {
-					b_hi[s] = hi_rows[base + 4 * s];
-					b_lo[s] = lo_rows[base + 4 * s];
+#pragma unroll
+					for (int p = 0; p < P; ++p) {
+						b[p][s] = rows[base + 4 * s + p * part_vecs];
+					}
 				}
 #pragma unroll
 				for (int j = 0; j < QC; ++j) {
@@ -267,9 +285,14 @@ void conv_split_kernel(const T16* __restrict__ padded, const T16* __restrict__ t
 					if (r >= 0 && r < kRows) {
 #pragma unroll
 						for (int s = 0; s < STEPS; ++s) {
-							acc_hi[r] = S::mfma(a_hi[j][s], b_hi[s], acc_hi[r]);
-							acc_lo[r] = S::mfma(a_hi[j][s], b_lo[s], acc_lo[r]);
-							acc_lo[r] = S::mfma(a_lo[j][s], b_hi[s], acc_lo[r]);
+							acc_hi[r] = S::mfma(a[0][j][s], b[0][s], acc_hi[r]);
+							acc_lo[r] = S::mfma(a[0][j][s], b[1][s], acc_lo[r]);
+							acc_lo[r] = S::mfma(a[1][j][s], b[0][s], acc_lo[r]);
+							if constexpr (P == 3) {
+								acc_lo[r] = S::mfma(a[0][j][s], b[2][s], acc_lo[r]);
+								acc_lo[r] = S::mfma(a[2][j][s], b[0][s], acc_lo[r]);
+								acc_lo[r] = S::mfma(a[1][j][s], b[1][s], acc_lo[r]);
+							}
 						}
 					}
 				}
@@ -347,12 +370,15 @@ constexpr size_t kPartialBytes = sizeof(float) * kPartialBlocks;      // block m
 
 size_t round_up(size_t aBytes) { return (aBytes + 255) / 256 * 256; }
 
-// Halves of the Toeplitz table of one kernel (hi and lo parts, every K-step).
-long long table_halves(const view4& aKer, split_variant aV)
+// 16-bit parts per operand: hi + lo (FP16), hi + mid + lo (BF16).
+int split_parts(int aPrecision) { return aPrecision == IANE_PREC_SPLIT_BF16 ? 3 : 2; }
+
+// Halves of the Toeplitz table of one kernel (every part, every K-step).
+long long table_halves(const view4& aKer, split_variant aV, int aParts)
 {
 	const int qc = variant_qc(aV);
 	const long long qy_padded = (aKer.extent[1] + qc - 1) / qc * qc;
-	return static_cast<long long>(aKer.extent[2]) * qy_padded * variant_steps(aV) * 2 * 64 * 8;
+	return static_cast<long long>(aKer.extent[2]) * qy_padded * variant_steps(aV) * aParts * 64 * 8;
 }
 
 template<class T16>
@@ -363,7 +389,8 @@ void build_kernel_part(const view4& aKer, split_variant aV, int aUnit, float* aK
 	const int qy_padded = (aKer.extent[1] + qc - 1) / qc * qc;
 	absmax_partial_kernel<<<64, 256, 0, aStream>>>(aKer, aPartial);
 	scale_from_partials_kernel<<<1, 256, 0, aStream>>>(aPartial, 64, aKerScale, aUnit);
-	toeplitz_table_kernel<T16><<<ceil_div(table_halves(aKer, aV) / 2, 256), 256, 0, aStream>>>(aKer, aKer.extent[0] / 2,
+	toeplitz_table_kernel<T16><<<ceil_div(table_halves(aKer, aV, split16<T16>::parts) / split16<T16>::parts, 256), 256,
+		0, aStream>>>(aKer, aKer.extent[0] / 2,
 		qy_padded, variant_steps(aV), aKerScale, aTable);
 	check_hip(hipGetLastError(), "split convolution kernel table");
 }
@@ -409,16 +436,18 @@ bool split_applicable(const view4& aSrc, const view4& aKer, const view4& aDst)
 		aSrc.extent[0] < (1 << 24) && aSrc.extent[1] < (1 << 24) && pick_variant(aKer) != SPLIT_NONE;
 }
 
-size_t split_kernel_bytes(const view4& aKer)
+size_t split_kernel_bytes(const view4& aKer, int aPrecision)
 {
 	// a kernel scale, the block maxima of the kernel, and the table (2-byte halves)
-	return kScaleBytes + kPartialBytes + round_up(2 * static_cast<size_t>(table_halves(aKer, pick_variant(aKer))));
+	return kScaleBytes + kPartialBytes +
+		round_up(2 * static_cast<size_t>(table_halves(aKer, pick_variant(aKer), split_parts(aPrecision))));
 }
 
-size_t split_source_bytes(const view4& aSrc, const view4& aKer)
+size_t split_source_bytes(const view4& aSrc, const view4& aKer, int aPrecision)
 {
 	const split_geom g = make_geom(aSrc, aKer, pick_variant(aKer));
-	return kScaleBytes + kPartialBytes + round_up(2 * sizeof(uint16_t) * static_cast<size_t>(g.part));
+	return kScaleBytes + kPartialBytes +
+		round_up(split_parts(aPrecision) * sizeof(uint16_t) * static_cast<size_t>(g.part));
 }
 
 // Build the kernel-side data (scale and Toeplitz table) into `aKernelPart`
@@ -438,7 +467,7 @@ void split_prepare_kernel(const view4& aKer, int aPrecision, void* aKernelPart, 
 }
 
 // Convolve with a kernel prepared by split_prepare_kernel; `aSourcePart` is
-// split_source_bytes of scratch (the source's scale and padded hi/lo planes).
+// split_source_bytes of scratch (the source's scale and padded part planes).
 void split_convolve_prepared(const view4& aSrc, const view4& aKer, const view4& aDst, int aBorder, int aPrecision,
 	const void* aKernelPart, void* aSourcePart, hipStream_t aStream)
 {

@@ -706,12 +706,12 @@ std::unique_ptr<rl_plan_impl> make_rl_plan(int aRank, const int32_t* aExtent, co
 			split_applicable(image, kernel, image);
 		if (split) {
 			// the PSF's and its flip's scale and Toeplitz table, once per plan
-			const size_t kernel_bytes = split_kernel_bytes(kernel);
+			const size_t kernel_bytes = split_kernel_bytes(kernel, aPrecision);
 			check_hip(hipMalloc(&impl->split_psf, kernel_bytes), "Allocate split PSF table");
 			check_hip(hipMalloc(&impl->split_flipped, kernel_bytes), "Allocate split PSF table");
 			split_prepare_kernel(kernel, aPrecision, impl->split_psf, stream);
 			split_prepare_kernel(impl->kernel_view(impl->flipped), aPrecision, impl->split_flipped, stream);
-			impl->split_scratch = split_source_bytes(image, kernel);
+			impl->split_scratch = split_source_bytes(image, kernel, aPrecision);
 		}
 		// run workspace: predicted, correction, log-likelihood partials, split source part
 		const size_t images = (2 * impl->count * sizeof(float) + 255) / 256 * 256;

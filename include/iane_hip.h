@@ -54,9 +54,11 @@ enum { IANE_BORDER_CLAMP = 0, IANE_BORDER_ZERO = 1 };
  * REFERENCE: FP64 fma per tap in kernel storage order, exactly the sum the
  *   reference forms (include/iane/algorithms/convolution.hpp:36-49), then the
  *   reference's output conversion (:51-75). Bit-exact to the reference.
- * FP32: FP32 fma, same tap order; float outputs within ~1e-6 relative.
- * SPLIT_BF16 / SPLIT_FP16: MFMA with hi+lo split operands and FP32
- *   accumulation (large kernels); float destinations only.            */
+ * FP32: FP32 fma chains (kernel planes from the back, rows and x taps in
+ *   storage order); float outputs within ~1e-6 relative.
+ * SPLIT_BF16 / SPLIT_FP16: MFMA on operands split into 16-bit parts (BF16:
+ *   hi + mid + lo, FP16: hi + lo) with FP32 accumulation (large kernels);
+ *   float destinations only; within ~1e-6 relative like FP32.          */
 enum {
 	IANE_PREC_REFERENCE = 0,
 	IANE_PREC_FP32 = 1,

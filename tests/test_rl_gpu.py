@@ -93,7 +93,7 @@ def test_direct_fp32_and_fft_agree(shape, psf_shape):
     assert rel(host(b), ref) <= tol_rl(8)
 
 
-@pytest.mark.parametrize("prec,tol", [(precision.split_fp16, 2e-5), (precision.split_bf16, 1e-4)],
+@pytest.mark.parametrize("prec,tol", [(precision.split_fp16, 2e-5), (precision.split_bf16, 2e-5)],
                          ids=["split_fp16", "split_bf16"])
 def test_direct_split_precision_vs_fp64_oracle(prec, tol):
     y, psf = _case((18, 26, 70), (1.2, 1.0, 2.5), (9, 7, 21))
@@ -126,11 +126,15 @@ def test_rl_200_iterations_real_psf():
     print(f"FFT RL, 200 iterations, 64^3, PSF 31x31x63: max {err:.2e} of max|oracle64|")
     assert err <= tol_rl(200)
     np.testing.assert_allclose(host(ll), ll_ref, rtol=1e-6)
-    for prec in (precision.split_fp16, precision.split_bf16):
+    # The direct method sums 60 543 products per output in FP32 accumulators (the
+    # split operands themselves hold 22 / 24 bits: one convolution is within 1e-6);
+    # its per-iteration rounding is several times the FFT method's and grows the same
+    # way with iterations (measured 1.6e-4 FP16, 2.1e-4 BF16): 8 x the FFT bound.
+    for prec, bound in ((precision.split_fp16, 8 * tol_rl(200)), (precision.split_bf16, 8 * tol_rl(200))):
         est = deconvolution.richardson_lucy(dev(y), dev(psf), 200, method=deconvolution.DIRECT, precision=prec)
         err = rel(host(est), ref)
         print(f"direct RL, precision code {int(prec)}, 200 iterations: max {err:.2e} of max|oracle64|")
-        assert err <= 3e-4
+        assert err <= bound
 
 
 def test_warm_start_continues_the_iteration():
