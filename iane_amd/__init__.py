@@ -168,9 +168,16 @@ def convolve(src, kernel, dst, options: convolution_options | None = None, polic
     if isinstance(src, texture_image_view):
         src = src.texels()  # texture source: its texels, border rule at the view boundary
     s, k, d = view_desc(src), view_desc(kernel), view_desc(dst)
-    ws = _workspace(convolve_workspace_bytes(src, kernel, dst, options), workspace)
+    border, prec = int(options.border), int(options.precision)
+    needed = 0
+    if prec in (precision.split_bf16, precision.split_fp16):  # the only path that takes scratch
+        import ctypes
+        query = ctypes.c_size_t()
+        _lib.call("iane_hip_convolve_workspace_bytes", s, k, d, border, prec, ctypes.byref(query))
+        needed = query.value
+    ws = _workspace(needed, workspace)
     ptr, nbytes = (0, 0) if isinstance(ws, int) else (ws.data_ptr(), ws.numel() * ws.element_size())
-    _lib.call("iane_hip_convolve", s, k, d, int(options.border), int(options.precision), ptr, nbytes, stream)
+    _lib.call("iane_hip_convolve", s, k, d, border, prec, ptr, nbytes, stream)
     return hip_execution(stream)
 
 

@@ -100,11 +100,11 @@ int next_fft_length(int aN)
 
 namespace {
 
-template<int MODE>
+template<int MODE, bool GENERAL>
 __global__ __launch_bounds__(kFftThreads) void fft_pass_kernel(fft_pass_args a)
 {
 	extern __shared__ float2 lds[];
-	fft_pass_body<MODE>(a, xcd_remap(blockIdx.x, gridDim.x), lds);
+	fft_pass_body<MODE, GENERAL>(a, xcd_remap(blockIdx.x, gridDim.x), lds);
 }
 
 } // namespace
@@ -277,9 +277,10 @@ void setup_pass(fft_pass_args& a, int aMode, int aN, int aLines, int aOuter)
 	a.lines = aLines;
 	a.outer = aOuter;
 	// the twiddle tables follow the two line buffers in LDS when one line of
-	// each buffer and the tables fit; longer lines read them from memory
+	// each buffer and the tables fit; longer lines and Bluestein's form run the
+	// general pass kernel, which reads them from memory
 	const int tables = pass_table_elems(stages, m, half);
-	a.tables_in_lds = 2LL * a.pitch + tables <= limit ? 1 : 0;
+	a.tables_in_lds = a.chirp == nullptr && 2LL * a.pitch + tables <= limit ? 1 : 0;
 	const int staged = a.tables_in_lds ? tables : 0;
 	// Lines per workgroup: as many as fit an LDS budget of "fft.lds_elems" complex
 	// elements per buffer (autotuned; default kFftLdsElems). A smaller budget means
@@ -308,11 +309,20 @@ void launch_pass(const fft_pass_args& a, hipStream_t aStream)
 	}
 	const unsigned nb = static_cast<unsigned>(blocks);
 	const size_t lds = pass_lds_bytes(a);
-	switch (a.mode) {
-	case PASS_C2C: fft_pass_kernel<PASS_C2C><<<nb, kFftThreads, lds, aStream>>>(a); break;
-	case PASS_R2C: fft_pass_kernel<PASS_R2C><<<nb, kFftThreads, lds, aStream>>>(a); break;
-	case PASS_C2R: fft_pass_kernel<PASS_C2R><<<nb, kFftThreads, lds, aStream>>>(a); break;
-	default: fft_pass_kernel<PASS_C2C_MULT><<<nb, kFftThreads, lds, aStream>>>(a); break;
+	if (!a.tables_in_lds) {
+		switch (a.mode) {
+		case PASS_C2C: fft_pass_kernel<PASS_C2C, true><<<nb, kFftThreads, lds, aStream>>>(a); break;
+		case PASS_R2C: fft_pass_kernel<PASS_R2C, true><<<nb, kFftThreads, lds, aStream>>>(a); break;
+		case PASS_C2R: fft_pass_kernel<PASS_C2R, true><<<nb, kFftThreads, lds, aStream>>>(a); break;
+		default: fail(IANE_ERR_UNSUPPORTED, "FFT spectrum pass line too long for LDS");
+		}
+	} else {
+		switch (a.mode) {
+		case PASS_C2C: fft_pass_kernel<PASS_C2C, false><<<nb, kFftThreads, lds, aStream>>>(a); break;
+		case PASS_R2C: fft_pass_kernel<PASS_R2C, false><<<nb, kFftThreads, lds, aStream>>>(a); break;
+		case PASS_C2R: fft_pass_kernel<PASS_C2R, false><<<nb, kFftThreads, lds, aStream>>>(a); break;
+		default: fft_pass_kernel<PASS_C2C_MULT, false><<<nb, kFftThreads, lds, aStream>>>(a); break;
+		}
 	}
 	check_hip(hipGetLastError(), "FFT pass launch");
 }

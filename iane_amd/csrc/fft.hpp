@@ -617,27 +617,15 @@ __device__ __forceinline__ void store_table(const float2 (&aFetched)[2], const f
 	}
 }
 
-// aLds: the table area (W_m at aLds, W_n at aLds + m). Nothing is staged when
-// the tables stay in memory (lines too long for both in LDS).
+// aLds: the table area (W_m at aLds, W_n at aLds + m). Only passes whose
+// tables are staged (tables_in_lds) call it.
 __device__ __forceinline__ void store_twiddles(const twiddle_fetch& aFetched, const fft_pass_args& a, float2* aLds)
 {
-	if (!a.tables_in_lds) {
-		return;
-	}
 	const int m = a.plan.m;
 	store_table(aFetched.m, a.tw_m, m, aLds);
 	if (a.tw_n) {
 		store_table(aFetched.n, a.tw_n, a.line_m + 1, aLds + m);
 	}
-}
-
-// Where the stages and the real pre/post steps read the twiddles: the staged
-// copies after the line buffers, or the tables in memory.
-__device__ __forceinline__ void pass_twiddles(const fft_pass_args& a, const float2* aLds, const float2*& aTwM,
-	const float2*& aTwN)
-{
-	aTwM = a.tables_in_lds ? aLds : a.tw_m;
-	aTwN = a.tables_in_lds ? aLds + a.plan.m : a.tw_n;
 }
 
 // The complex transform (direction `sign`) of the `count` lines in buffer x
@@ -691,8 +679,11 @@ __device__ __forceinline__ float2* run_line(const fft_pass_args& a, float2* x, f
 }
 
 // One workgroup's share of a line pass: the lines of group `aBlock` of `a`.
-// `lds` holds two buffers of a.lds_elems complex elements and the W_m table.
-template<int MODE>
+// `lds` holds two buffers of a.lds_elems complex elements and, unless GENERAL,
+// the twiddle tables. GENERAL passes (a.tables_in_lds == 0: lines too long for
+// the tables in LDS, or Bluestein's form) read the tables from memory; the
+// others read the staged copies, whose LDS address space the compiler sees.
+template<int MODE, bool GENERAL = false>
 __device__ __forceinline__ void fft_pass_body(const fft_pass_args& a, unsigned aBlock, float2* lds)
 {
 	float2* A = lds;
@@ -700,21 +691,35 @@ __device__ __forceinline__ void fft_pass_body(const fft_pass_args& a, unsigned a
 	const int m = a.line_m;
 	const twiddle_fetch fetched = fetch_twiddles(a);
 	float2* const tw_lds = lds + 2 * a.lds_elems;
-	const float2* tw_m;
-	const float2* tw_n; // read only by the half-length real modes
-	pass_twiddles(a, tw_lds, tw_m, tw_n);
+	const float2* tw_m = GENERAL ? a.tw_m : tw_lds;
+	const float2* tw_n = GENERAL ? a.tw_n : tw_lds + a.plan.m; // read only by the half-length real modes
 	int o, l0, count;
 	decode_group(a, aBlock, o, l0, count);
 	const int n = a.n;
+	const auto stage_tables = [&] {
+		if constexpr (!GENERAL) {
+			store_twiddles(fetched, a, tw_lds);
+		}
+	};
+	// the line transform of the bundle: Bluestein-capable only in GENERAL passes;
+	// a constant sign in each copy folds the conjugations and (sign i) rotations
+	const auto line = [&](float2* x, float2* y, float sign) -> float2* {
+		if constexpr (GENERAL) {
+			return run_line(a, x, y, count, tw_m, sign);
+		} else {
+			return sign < 0.0f ? run_stages(a.plan, x, y, count, a.pitch, tw_m, -1.0f)
+			                   : run_stages(a.plan, x, y, count, a.pitch, tw_m, 1.0f);
+		}
+	};
 	if constexpr (MODE == PASS_C2C) {
 		load_complex_lines(static_cast<const float2*>(a.in), a.gin, o, l0, count, a.load_n, m, A, a.pitch);
-		store_twiddles(fetched, a, tw_lds);
+		stage_tables();
 		__syncthreads();
-		const float2* r = run_line(a, A, B, count, tw_m, a.sign);
+		const float2* r = line(A, B, a.sign);
 		store_complex_lines(static_cast<float2*>(a.out), a.gout, o, l0, count, a.store_n, r, a.pitch);
 	} else if constexpr (MODE == PASS_C2C_MULT) {
 		load_complex_lines(static_cast<const float2*>(a.in), a.gin, o, l0, count, a.load_n, m, A, a.pitch);
-		store_twiddles(fetched, a, tw_lds);
+		stage_tables();
 		__syncthreads();
 		float2* r = run_stages(a.plan, A, B, count, a.pitch, tw_m, -1.0f);
 		const long long hbase = o * a.gmult.outer + l0 * a.gmult.line;
@@ -746,9 +751,9 @@ __device__ __forceinline__ void fft_pass_body(const fft_pass_args& a, unsigned a
 		store_complex_lines(static_cast<float2*>(a.out), a.gout, o, l0, count, a.store_n, q, a.pitch);
 	} else if constexpr (MODE == PASS_R2C) {
 		load_real_lines(static_cast<const float*>(a.in), a.gin, o, l0, count, n, a.load_n, A, a.pitch);
-		store_twiddles(fetched, a, tw_lds);
+		stage_tables();
 		__syncthreads();
-		float2* r = run_line(a, A, B, count, tw_m, -1.0f);
+		float2* r = line(A, B, -1.0f);
 		if ((n & 1) == 0) {
 			float2* x = r == A ? B : A;
 			r2c_post(r, x, count, n, a.pitch, tw_n);
@@ -759,19 +764,19 @@ __device__ __forceinline__ void fft_pass_body(const fft_pass_args& a, unsigned a
 	} else { // PASS_C2R
 		if ((n & 1) == 0) {
 			load_complex_lines(static_cast<const float2*>(a.in), a.gin, o, l0, count, a.load_n, m + 1, A, a.pitch);
-			store_twiddles(fetched, a, tw_lds);
+			stage_tables();
 			__syncthreads();
 			c2r_pre(A, B, count, n, a.pitch, tw_n);
 			__syncthreads();
-			const float2* r = run_line(a, B, A, count, tw_m, 1.0f);
+			const float2* r = line(B, A, 1.0f);
 			store_real_lines(static_cast<float*>(a.out), a.gout, o, l0, count, n, a.store_n, r, a.pitch);
 		} else {
 			load_complex_lines(static_cast<const float2*>(a.in), a.gin, o, l0, count, a.load_n, m, A, a.pitch);
-			store_twiddles(fetched, a, tw_lds);
+			stage_tables();
 			__syncthreads();
 			c2r_extend_odd(A, count, n, a.pitch);
 			__syncthreads();
-			const float2* r = run_line(a, A, B, count, tw_m, 1.0f);
+			const float2* r = line(A, B, 1.0f);
 			store_real_lines(static_cast<float*>(a.out), a.gout, o, l0, count, n, a.store_n, r, a.pitch);
 		}
 	}

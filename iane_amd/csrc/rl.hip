@@ -215,9 +215,8 @@ __device__ __forceinline__ void rl_row_body(const rl_row_args& a, unsigned aBloc
 	float2* A = lds;
 	float2* B = lds + f.lds_elems;
 	const twiddle_fetch fetched = fetch_twiddles(f);
-	const float2* tw_m;
-	const float2* tw_n; // W_n^0 .. W_n^m (the rows are half-length real transforms)
-	pass_twiddles(f, lds + 2 * f.lds_elems, tw_m, tw_n);
+	const float2* tw_m = lds + 2 * f.lds_elems; // staged tables (RL plans check tables_in_lds)
+	const float2* tw_n = tw_m + f.plan.m;       // W_n^0 .. W_n^m (the rows are half-length real transforms)
 	int o, l0, count;
 	decode_group(f, aBlock, o, l0, count);
 	const int n = f.n;
@@ -634,6 +633,14 @@ void rl_plan_impl::build_fft(hipStream_t stream)
 	check_hip(hipFreeAsync(wrapped, stream), "Free PSF buffer");
 	forward_mid = mid(0);
 	adjoint_mid = mid(1);
+	// the row kernels and the batched passes read the twiddles from their LDS copies
+	bool staged = r2c.tables_in_lds && c2r.tables_in_lds;
+	for (const fft_pass_args& a : forward_mid) {
+		staged = staged && a.tables_in_lds;
+	}
+	if (!staged) {
+		fail(IANE_ERR_UNSUPPORTED, "RL padded FFT line exceeds the LDS line limit");
+	}
 	// row-kernel template: the spectrum rows of the image region
 	row_tpl.f = c2r;
 	row_tpl.f.gout = {1, row_tpl.f.gin.line, row_tpl.f.gin.outer};
