@@ -55,7 +55,21 @@ bool factor_radices(int aM, fft_radix_plan& aPlan)
 			if (!push(r)) return false;
 		}
 	}
-	return rest == 1;
+	if (rest != 1) {
+		return false;
+	}
+	int ns = 1;
+	for (int s = 0; s < aPlan.stages; ++s) {
+		const int r = aPlan.radix[s];
+		fft_stage& st = aPlan.stage[s];
+		st.nb = aM / r;
+		st.ns = ns;
+		st.step = aM / (ns * r);
+		st.c_nb = fast_div(st.nb).c;
+		st.c_ns = fast_div(ns).c;
+		ns *= r;
+	}
+	return true;
 }
 
 namespace {
@@ -83,6 +97,33 @@ const float2* twiddle_table(int aM)
 	return slot;
 }
 
+int fixed_plan_id(const fft_radix_plan& aPlan)
+{
+	if (tune_get("fft.fixed", 1) == 0) {
+		return 0;
+	}
+	const auto is = [&](std::initializer_list<int> aRadices) {
+		if (static_cast<int>(aRadices.size()) != aPlan.stages) {
+			return false;
+		}
+		int s = 0;
+		for (int r : aRadices) {
+			if (aPlan.radix[s++] != r) {
+				return false;
+			}
+		}
+		return true;
+	};
+	static const std::initializer_list<int> table[kFixedPlans] = {
+		{4, 5, 7}, {8, 5, 7}, {8, 4, 5}, {8, 8, 2}, {8, 8, 4}, {8, 8, 8}, {8, 8, 8, 2}};
+	for (int id = 1; id <= kFixedPlans; ++id) {
+		if (is(table[id - 1])) {
+			return id;
+		}
+	}
+	return 0;
+}
+
 int next_fft_length(int aN)
 {
 	for (int n = aN + (aN & 1); ; n += 2) {
@@ -100,11 +141,19 @@ int next_fft_length(int aN)
 
 namespace {
 
-template<int MODE, bool GENERAL>
+template<int MODE, bool GENERAL, int FIXED = 0>
 __global__ __launch_bounds__(kFftThreads) void fft_pass_kernel(fft_pass_args a)
 {
 	extern __shared__ float2 lds[];
-	fft_pass_body<MODE, GENERAL>(a, xcd_remap(blockIdx.x, gridDim.x), lds);
+	fft_pass_body<MODE, GENERAL, FIXED>(a, xcd_remap(blockIdx.x, gridDim.x), lds);
+}
+
+template<int MODE>
+void launch_staged(const fft_pass_args& a, unsigned aBlocks, size_t aLds, hipStream_t aStream)
+{
+	with_fixed_plan(a.fixed, [&](auto aId) {
+		fft_pass_kernel<MODE, false, decltype(aId)::value><<<aBlocks, kFftThreads, aLds, aStream>>>(a);
+	});
 }
 
 } // namespace
@@ -281,6 +330,7 @@ void setup_pass(fft_pass_args& a, int aMode, int aN, int aLines, int aOuter)
 	// general pass kernel, which reads them from memory
 	const int tables = pass_table_elems(stages, m, half);
 	a.tables_in_lds = a.chirp == nullptr && 2LL * a.pitch + tables <= limit ? 1 : 0;
+	a.fixed = a.tables_in_lds ? fixed_plan_id(a.plan) : 0;
 	const int staged = a.tables_in_lds ? tables : 0;
 	// Lines per workgroup: as many as fit an LDS budget of "fft.lds_elems" complex
 	// elements per buffer (autotuned; default kFftLdsElems). A smaller budget means
@@ -318,10 +368,10 @@ void launch_pass(const fft_pass_args& a, hipStream_t aStream)
 		}
 	} else {
 		switch (a.mode) {
-		case PASS_C2C: fft_pass_kernel<PASS_C2C, false><<<nb, kFftThreads, lds, aStream>>>(a); break;
-		case PASS_R2C: fft_pass_kernel<PASS_R2C, false><<<nb, kFftThreads, lds, aStream>>>(a); break;
-		case PASS_C2R: fft_pass_kernel<PASS_C2R, false><<<nb, kFftThreads, lds, aStream>>>(a); break;
-		default: fft_pass_kernel<PASS_C2C_MULT, false><<<nb, kFftThreads, lds, aStream>>>(a); break;
+		case PASS_C2C: launch_staged<PASS_C2C>(a, nb, lds, aStream); break;
+		case PASS_R2C: launch_staged<PASS_R2C>(a, nb, lds, aStream); break;
+		case PASS_C2R: launch_staged<PASS_C2R>(a, nb, lds, aStream); break;
+		default: launch_staged<PASS_C2C_MULT>(a, nb, lds, aStream); break;
 		}
 	}
 	check_hip(hipGetLastError(), "FFT pass launch");

@@ -22,16 +22,30 @@
 
 #include "common.hpp"
 
+#include <type_traits>
+
 namespace iane_hip {
 
 constexpr int kFftThreads = 256;
 constexpr int kFftLdsElems = 4096;   // complex elements per LDS buffer (32 KiB; two buffers)
-constexpr int kFftMaxStages = 20;
+constexpr int kFftMaxStages = 16;   // radix >= 2 and lines <= 10240 points: at most 14 stages
+
+// One Stockham stage with its constants worked out on the host: nb = m / R
+// butterflies per line, ns = product of the earlier radices, step = m / (ns R)
+// (twiddle stride), and the fast_div multipliers of nb and ns.
+struct fft_stage {
+	int nb;
+	int ns;
+	int step;
+	unsigned c_nb;
+	unsigned c_ns;
+};
 
 struct fft_radix_plan {
 	int m;                       // complex line length
 	int stages;
 	int radix[kFftMaxStages];
+	fft_stage stage[kFftMaxStages];
 };
 
 // Division of n < 2^16 by a wave-uniform d in [1, 2^16): q = hi32(n * c) with
@@ -41,8 +55,10 @@ struct fft_radix_plan {
 struct fast_div {
 	unsigned d;
 	unsigned c;
-	__device__ __forceinline__ explicit fast_div(int aD)
+	__host__ __device__ __forceinline__ explicit fast_div(int aD)
 		: d(static_cast<unsigned>(aD)), c(aD > 1 ? 0xFFFFFFFFu / static_cast<unsigned>(aD) + 1u : 0u) {}
+	// a divisor whose multiplier the host already worked out
+	__device__ __forceinline__ fast_div(int aD, unsigned aC) : d(static_cast<unsigned>(aD)), c(aC) {}
 	__device__ __forceinline__ int div(int aN) const
 	{
 		return d == 1 ? aN : static_cast<int>(__umulhi(static_cast<unsigned>(aN), c));
@@ -180,13 +196,14 @@ __device__ __forceinline__ void dft(float2 (&v)[R], float sign, const float2* tw
 
 // One Stockham stage of radix R over `bundle` lines (LDS line pitch `pitch`).
 template<int R>
-__device__ __forceinline__ void stockham_stage(const float2* src, float2* dst, int m, int Ns, int bundle, int pitch,
-	const float2* tw, float sign)
+__device__ __forceinline__ void stockham_stage(const float2* src, float2* dst, const fft_stage& st, int m, int bundle,
+	int pitch, const float2* tw, float sign)
 {
-	const int nb = m / R;
+	const int nb = st.nb;
+	const int Ns = st.ns;
 	const int total = bundle * nb;
-	const int step = m / (Ns * R);
-	const fast_div by_nb(nb), by_ns(Ns);
+	const int step = st.step;
+	const fast_div by_nb(nb, st.c_nb), by_ns(Ns, st.c_ns);
 	for (int t = threadIdx.x; t < total; t += blockDim.x) {
 		const int w = by_nb.div(t);
 		const int j = t - w * nb;
@@ -261,28 +278,138 @@ __device__ inline void stockham_stage_generic(const float2* src, float2* dst, in
 __device__ __forceinline__ float2* run_stages(const fft_radix_plan& plan, float2* a, float2* b, int bundle, int pitch,
 	const float2* tw, float sign)
 {
-	int Ns = 1;
 	for (int s = 0; s < plan.stages; ++s) {
 		const int R = plan.radix[s];
+		const fft_stage& st = plan.stage[s];
 		switch (R) {
-		case 2: stockham_stage<2>(a, b, plan.m, Ns, bundle, pitch, tw, sign); break;
-		case 3: stockham_stage<3>(a, b, plan.m, Ns, bundle, pitch, tw, sign); break;
-		case 4: stockham_stage<4>(a, b, plan.m, Ns, bundle, pitch, tw, sign); break;
-		case 5: stockham_stage<5>(a, b, plan.m, Ns, bundle, pitch, tw, sign); break;
-		case 7: stockham_stage<7>(a, b, plan.m, Ns, bundle, pitch, tw, sign); break;
-		case 8: stockham_stage<8>(a, b, plan.m, Ns, bundle, pitch, tw, sign); break;
-		case 11: stockham_stage<11>(a, b, plan.m, Ns, bundle, pitch, tw, sign); break;
-		case 13: stockham_stage<13>(a, b, plan.m, Ns, bundle, pitch, tw, sign); break;
-		default: stockham_stage_generic(a, b, plan.m, R, Ns, bundle, pitch, tw, sign); break;
+		case 2: stockham_stage<2>(a, b, st, plan.m, bundle, pitch, tw, sign); break;
+		case 3: stockham_stage<3>(a, b, st, plan.m, bundle, pitch, tw, sign); break;
+		case 4: stockham_stage<4>(a, b, st, plan.m, bundle, pitch, tw, sign); break;
+		case 5: stockham_stage<5>(a, b, st, plan.m, bundle, pitch, tw, sign); break;
+		case 7: stockham_stage<7>(a, b, st, plan.m, bundle, pitch, tw, sign); break;
+		case 8: stockham_stage<8>(a, b, st, plan.m, bundle, pitch, tw, sign); break;
+		case 11: stockham_stage<11>(a, b, st, plan.m, bundle, pitch, tw, sign); break;
+		case 13: stockham_stage<13>(a, b, st, plan.m, bundle, pitch, tw, sign); break;
+		default: stockham_stage_generic(a, b, plan.m, R, st.ns, bundle, pitch, tw, sign); break;
 		}
 		__syncthreads();
 		float2* t = a;
 		a = b;
 		b = t;
-		Ns *= R;
 	}
 	return a;
 }
+
+// ---------------------------------------------------------------------------
+// Plans fixed at compile time. A line length whose radix sequence is in the
+// table below also runs as a chain of stages with the length, the radices and
+// the strides as template arguments: the same operations in the same order as
+// run_stages (so the same bits), without the per-stage division constants,
+// stride products and scalar spills of the run-time loop.
+template<int M, int NS, int R>
+__device__ __forceinline__ void stockham_stage_fixed(const float2* src, float2* dst, int bundle, int pitch,
+	const float2* tw, float sign)
+{
+	constexpr int nb = M / R;
+	constexpr int step = M / (NS * R);
+	const unsigned total = static_cast<unsigned>(bundle * nb);
+	for (unsigned t = threadIdx.x; t < total; t += blockDim.x) {
+		const unsigned w = t / nb;
+		const int j = static_cast<int>(t - w * nb);
+		const float2* s = src + w * pitch;
+		float2 v[R];
+#pragma unroll
+		for (int r = 0; r < R; ++r) {
+			v[r] = s[j + r * nb];
+		}
+		const int k = static_cast<int>(static_cast<unsigned>(j) % NS);
+		if constexpr (NS > 1) {
+#pragma unroll
+			for (int r = 1; r < R; ++r) {
+				float2 wv = tw[k * r * step];
+				if (sign > 0) {
+					wv = cconj(wv);
+				}
+				v[r] = cmul(v[r], wv);
+			}
+		}
+		dft<R>(v, sign, tw, M);
+		const int out = (j - k) * R + k;
+		float2* d = dst + w * pitch;
+#pragma unroll
+		for (int r = 0; r < R; ++r) {
+			d[out + r * NS] = v[r];
+		}
+	}
+}
+
+template<int M, int NS, int R, int... REST>
+__device__ __forceinline__ float2* fixed_stages(float2* a, float2* b, int bundle, int pitch, const float2* tw,
+	float sign)
+{
+	stockham_stage_fixed<M, NS, R>(a, b, bundle, pitch, tw, sign);
+	__syncthreads();
+	if constexpr (sizeof...(REST) == 0) {
+		return b;
+	} else {
+		return fixed_stages<M, NS * R, REST...>(b, a, bundle, pitch, tw, sign);
+	}
+}
+
+// The table: id -> line length and radix sequence (the order factor_radices
+// produces). id 0 is the run-time plan. The lengths are those of the RL padded
+// domain of 256 x 256 x 128 volumes with PSFs up to 31 x 31 x 63 (rows 140,
+// y 280, z 160) and powers of two from 128 to 1024.
+template<int M, int... R>
+struct fixed_sequence {
+	static constexpr int m = M;
+	__device__ static float2* run(float2* a, float2* b, int n, int p, const float2* tw, float s)
+	{
+		return fixed_stages<M, 1, R...>(a, b, n, p, tw, s);
+	}
+};
+template<int ID>
+struct fixed_plan;
+template<> struct fixed_plan<1> : fixed_sequence<140, 4, 5, 7> {};
+template<> struct fixed_plan<2> : fixed_sequence<280, 8, 5, 7> {};
+template<> struct fixed_plan<3> : fixed_sequence<160, 8, 4, 5> {};
+template<> struct fixed_plan<4> : fixed_sequence<128, 8, 8, 2> {};
+template<> struct fixed_plan<5> : fixed_sequence<256, 8, 8, 4> {};
+template<> struct fixed_plan<6> : fixed_sequence<512, 8, 8, 8> {};
+template<> struct fixed_plan<7> : fixed_sequence<1024, 8, 8, 8, 2> {};
+constexpr int kFixedPlans = 7;
+
+// Call aFn(std::integral_constant<int, id>) for the plan id (0 .. kFixedPlans):
+// the launch sites instantiate one kernel per id.
+template<class Fn>
+inline void with_fixed_plan(int aId, Fn aFn)
+{
+	switch (aId) {
+	case 1: aFn(std::integral_constant<int, 1>{}); break;
+	case 2: aFn(std::integral_constant<int, 2>{}); break;
+	case 3: aFn(std::integral_constant<int, 3>{}); break;
+	case 4: aFn(std::integral_constant<int, 4>{}); break;
+	case 5: aFn(std::integral_constant<int, 5>{}); break;
+	case 6: aFn(std::integral_constant<int, 6>{}); break;
+	case 7: aFn(std::integral_constant<int, 7>{}); break;
+	default: aFn(std::integral_constant<int, 0>{}); break;
+	}
+}
+
+// run_stages, or the fixed plan FIXED (its sequence must be the plan's).
+template<int FIXED>
+__device__ __forceinline__ float2* run_plan(const fft_radix_plan& plan, float2* a, float2* b, int bundle, int pitch,
+	const float2* tw, float sign)
+{
+	if constexpr (FIXED == 0) {
+		return run_stages(plan, a, b, bundle, pitch, tw, sign);
+	} else {
+		return fixed_plan<FIXED>::run(a, b, bundle, pitch, tw, sign);
+	}
+}
+
+// Host: the fixed plan id of a radix sequence (0: none).
+int fixed_plan_id(const fft_radix_plan& aPlan);
 
 // Host: factor m into supported radices (largest first; any leftover factor
 // becomes one generic stage, limited to 64).
@@ -324,6 +451,7 @@ struct fft_pass_args {
 	unsigned groups_per_outer;
 	fft_radix_plan plan; // the complex transform run in LDS (length m; M for Bluestein)
 	int line_m;          // complex length of the line transform
+	int fixed;           // compile-time plan id of a.plan (0: run-time stages)
 	int tables_in_lds;   // 1: twiddle tables staged in LDS after the line buffers; 0: read from memory
 	const float2* chirp; // Bluestein (else null): exp(-i pi j^2 / line_m), j < line_m
 	const float2* bspec; // Bluestein: spectrum of the wrapped conj(chirp), length M, divided by M
@@ -683,7 +811,7 @@ __device__ __forceinline__ float2* run_line(const fft_pass_args& a, float2* x, f
 // the twiddle tables. GENERAL passes (a.tables_in_lds == 0: lines too long for
 // the tables in LDS, or Bluestein's form) read the tables from memory; the
 // others read the staged copies, whose LDS address space the compiler sees.
-template<int MODE, bool GENERAL = false>
+template<int MODE, bool GENERAL = false, int FIXED = 0>
 __device__ __forceinline__ void fft_pass_body(const fft_pass_args& a, unsigned aBlock, float2* lds)
 {
 	float2* A = lds;
@@ -707,8 +835,8 @@ __device__ __forceinline__ void fft_pass_body(const fft_pass_args& a, unsigned a
 		if constexpr (GENERAL) {
 			return run_line(a, x, y, count, tw_m, sign);
 		} else {
-			return sign < 0.0f ? run_stages(a.plan, x, y, count, a.pitch, tw_m, -1.0f)
-			                   : run_stages(a.plan, x, y, count, a.pitch, tw_m, 1.0f);
+			return sign < 0.0f ? run_plan<FIXED>(a.plan, x, y, count, a.pitch, tw_m, -1.0f)
+			                   : run_plan<FIXED>(a.plan, x, y, count, a.pitch, tw_m, 1.0f);
 		}
 	};
 	if constexpr (MODE == PASS_C2C) {
@@ -721,7 +849,7 @@ __device__ __forceinline__ void fft_pass_body(const fft_pass_args& a, unsigned a
 		load_complex_lines(static_cast<const float2*>(a.in), a.gin, o, l0, count, a.load_n, m, A, a.pitch);
 		stage_tables();
 		__syncthreads();
-		float2* r = run_stages(a.plan, A, B, count, a.pitch, tw_m, -1.0f);
+		float2* r = run_plan<FIXED>(a.plan, A, B, count, a.pitch, tw_m, -1.0f);
 		const long long hbase = o * a.gmult.outer + l0 * a.gmult.line;
 		const fast_div by_m(m), by_count(count);
 		// (line w, bin k) of element t
@@ -747,7 +875,7 @@ __device__ __forceinline__ void fft_pass_body(const fft_pass_args& a, unsigned a
 				r[f.slot] = cscale(cmul(r[f.slot], h), a.mult_scale);
 			});
 		__syncthreads();
-		const float2* q = run_stages(a.plan, r, r == A ? B : A, count, a.pitch, tw_m, 1.0f);
+		const float2* q = run_plan<FIXED>(a.plan, r, r == A ? B : A, count, a.pitch, tw_m, 1.0f);
 		store_complex_lines(static_cast<float2*>(a.out), a.gout, o, l0, count, a.store_n, q, a.pitch);
 	} else if constexpr (MODE == PASS_R2C) {
 		load_real_lines(static_cast<const float*>(a.in), a.gin, o, l0, count, n, a.load_n, A, a.pitch);

@@ -207,7 +207,7 @@ struct rl_row_args {
 // the result back into the same spectrum rows.
 // One workgroup's rows (group `aBlock` of `a`); the log-likelihood partial of
 // the group goes to a.partials[aSlot].
-template<int EPI>
+template<int EPI, int FIXED = 0>
 __device__ __forceinline__ void rl_row_body(const rl_row_args& a, unsigned aBlock, unsigned aSlot, float2* lds,
 	ll_sum* shared)
 {
@@ -226,7 +226,7 @@ __device__ __forceinline__ void rl_row_body(const rl_row_args& a, unsigned aBloc
 	__syncthreads();
 	c2r_pre(A, B, count, n, f.pitch, tw_n);
 	__syncthreads();
-	float2* r = run_stages(f.plan, B, A, count, f.pitch, tw_m, 1.0f);
+	float2* r = run_plan<FIXED>(f.plan, B, A, count, f.pitch, tw_m, 1.0f);
 	// pointwise step on real samples 2j, 2j+1 of each row
 	ll_sum ll{0.0f, 0.0f};
 	const long long base = o * a.real_outer + l0 * a.real_line;
@@ -290,21 +290,21 @@ __device__ __forceinline__ void rl_row_body(const rl_row_args& a, unsigned aBloc
 		return;
 	}
 	__syncthreads();
-	float2* z = run_stages(f.plan, r, r == A ? B : A, count, f.pitch, tw_m, -1.0f);
+	float2* z = run_plan<FIXED>(f.plan, r, r == A ? B : A, count, f.pitch, tw_m, -1.0f);
 	float2* xb = z == A ? B : A;
 	r2c_post(z, xb, count, n, f.pitch, tw_n);
 	__syncthreads();
 	store_complex_lines(static_cast<float2*>(f.out), f.gout, o, l0, count, f.store_n, xb, f.pitch);
 }
 
-template<int EPI>
+template<int EPI, int FIXED>
 __global__ __launch_bounds__(kFftThreads) void rl_row_kernel(rl_row_args a)
 {
 	extern __shared__ float2 lds[];
 	__shared__ ll_sum shared[kFftThreads / 64];
 	// the partial of line group g goes to slot g (the batched kernel does the same)
 	const unsigned group = xcd_remap(blockIdx.x, gridDim.x);
-	rl_row_body<EPI>(a, group, group, lds, shared);
+	rl_row_body<EPI, FIXED>(a, group, group, lds, shared);
 }
 
 // ---------------------------------------------------- batched (many volumes) --
@@ -336,7 +336,8 @@ __device__ __forceinline__ void copy_descriptor(T& aDst, const T& aSrc)
 	}
 	__syncthreads();
 }
-template<int MODE>
+// FIXED: the compile-time plan every volume of the launch shares (0: run-time stages).
+template<int MODE, int FIXED = 0>
 __global__ __launch_bounds__(kFftThreads) void fft_pass_batched_kernel(
 	const fft_pass_args* __restrict__ args, const unsigned* __restrict__ offsets, int count)
 {
@@ -345,10 +346,10 @@ __global__ __launch_bounds__(kFftThreads) void fft_pass_batched_kernel(
 	const unsigned b = xcd_remap(blockIdx.x, gridDim.x);
 	const int v = __builtin_amdgcn_readfirstlane(volume_of(offsets, count, b));
 	copy_descriptor(desc, args[v]);
-	fft_pass_body<MODE>(desc, b - offsets[v], lds);
+	fft_pass_body<MODE, false, FIXED>(desc, b - offsets[v], lds);
 }
 
-template<int EPI>
+template<int EPI, int FIXED = 0>
 __global__ __launch_bounds__(kFftThreads) void rl_row_batched_kernel(
 	const rl_row_args* __restrict__ args, const unsigned* __restrict__ offsets, int count)
 {
@@ -358,7 +359,7 @@ __global__ __launch_bounds__(kFftThreads) void rl_row_batched_kernel(
 	const unsigned b = xcd_remap(blockIdx.x, gridDim.x);
 	const int v = __builtin_amdgcn_readfirstlane(volume_of(offsets, count, b));
 	copy_descriptor(desc, args[v]);
-	rl_row_body<EPI>(desc, b - offsets[v], b, lds, shared);
+	rl_row_body<EPI, FIXED>(desc, b - offsets[v], b, lds, shared);
 }
 
 // log-likelihood of volume v (partials offsets[v] .. offsets[v+1]-1) into out[v * stride]
@@ -772,7 +773,11 @@ unsigned row_blocks(const fft_pass_args& a)
 template<int EPI>
 void launch_row(const rl_row_args& a, hipStream_t s)
 {
-	rl_row_kernel<EPI><<<row_blocks(a.f), kFftThreads, pass_lds_bytes(a.f), s>>>(a);
+	const unsigned blocks = row_blocks(a.f);
+	const size_t lds = pass_lds_bytes(a.f);
+	with_fixed_plan(a.f.fixed, [&](auto aId) {
+		rl_row_kernel<EPI, decltype(aId)::value><<<blocks, kFftThreads, lds, s>>>(a);
+	});
 	check_hip(hipGetLastError(), "RL row launch");
 }
 
@@ -1114,6 +1119,7 @@ struct batched_launch {
 	unsigned* d_offsets = nullptr;
 	unsigned blocks = 0;
 	size_t lds_bytes = 0;  // dynamic LDS of the launch: the largest of the volumes' passes
+	int fixed = 0;         // the compile-time plan of every volume's pass, if they share one
 
 	batched_launch() = default;
 	batched_launch(const batched_launch&) = delete;
@@ -1128,8 +1134,10 @@ struct batched_launch {
 	void finalize(const fft_pass_args& (*aPass)(const Args&))
 	{
 		offsets.assign(host.size() + 1, 0);
+		fixed = host.empty() ? 0 : aPass(host[0]).fixed;
 		for (size_t v = 0; v < host.size(); ++v) {
 			const fft_pass_args& f = aPass(host[v]);
+			fixed = f.fixed == fixed ? fixed : 0;
 			offsets[v + 1] = offsets[v] + static_cast<unsigned>(static_cast<long long>(f.groups_per_outer) * f.outer);
 			lds_bytes = std::max(lds_bytes, pass_lds_bytes(f));
 		}
@@ -1150,21 +1158,34 @@ struct batched_launch {
 const fft_pass_args& pass_of(const fft_pass_args& a) { return a; }
 const fft_pass_args& pass_of_row(const rl_row_args& a) { return a.f; }
 
+template<int MODE>
+void launch_batched_fixed(const batched_launch<fft_pass_args>& b, hipStream_t s)
+{
+	const int n = static_cast<int>(b.host.size());
+	with_fixed_plan(b.fixed, [&](auto aId) {
+		fft_pass_batched_kernel<MODE, decltype(aId)::value><<<b.blocks, kFftThreads, b.lds_bytes, s>>>(b.d_args,
+			b.d_offsets, n);
+	});
+}
+
 void launch_batched(const batched_launch<fft_pass_args>& b, int aMode, hipStream_t s)
 {
 	const int n = static_cast<int>(b.host.size());
 	switch (aMode) {
-	case PASS_C2C: fft_pass_batched_kernel<PASS_C2C><<<b.blocks, kFftThreads, b.lds_bytes, s>>>(b.d_args, b.d_offsets, n); break;
+	case PASS_C2C: launch_batched_fixed<PASS_C2C>(b, s); break;
 	case PASS_R2C: fft_pass_batched_kernel<PASS_R2C><<<b.blocks, kFftThreads, b.lds_bytes, s>>>(b.d_args, b.d_offsets, n); break;
 	case PASS_C2R: fft_pass_batched_kernel<PASS_C2R><<<b.blocks, kFftThreads, b.lds_bytes, s>>>(b.d_args, b.d_offsets, n); break;
-	default: fft_pass_batched_kernel<PASS_C2C_MULT><<<b.blocks, kFftThreads, b.lds_bytes, s>>>(b.d_args, b.d_offsets, n); break;
+	default: launch_batched_fixed<PASS_C2C_MULT>(b, s); break;
 	}
 }
 
 template<int EPI>
 void launch_row_batched(const batched_launch<rl_row_args>& b, int aCount, hipStream_t s)
 {
-	rl_row_batched_kernel<EPI><<<b.blocks, kFftThreads, b.lds_bytes, s>>>(b.d_args, b.d_offsets, aCount);
+	with_fixed_plan(b.fixed, [&](auto aId) {
+		rl_row_batched_kernel<EPI, decltype(aId)::value><<<b.blocks, kFftThreads, b.lds_bytes, s>>>(b.d_args,
+			b.d_offsets, aCount);
+	});
 }
 
 } // namespace

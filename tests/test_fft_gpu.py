@@ -246,6 +246,31 @@ def test_prime_and_long_axes_nd(shape):
     assert rel(host(rback), r.astype(np.float64) * np.prod(shape)) <= TOL_FFT
 
 
+@pytest.mark.parametrize("shape", [(3, 128), (4, 256), (2, 512), (1024,), (12, 280), (5, 160, 140), (20, 512)])
+def test_compile_time_plans_give_the_run_time_bits(shape):
+    """Line lengths with a compile-time plan (140, 160, 280, 128 .. 1024) run the
+    same operations in the same order as the run-time stages: identical bits for
+    C2C both ways and for R2C/C2R ("fft.fixed" switches the table off)."""
+    from iane_amd import autotune
+    rng = np.random.default_rng(sum(shape))
+    x = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+    r = rng.standard_normal(shape).astype(np.float32)
+    outs = []
+    for fixed in (0, 1):
+        with autotune.applied({"fft.fixed": fixed}):
+            ctx = execution_context()
+            src, out, back = dev(x), cempty(shape), cempty(shape)
+            ctx.run(ctx.prepare(fft_forward, src, out), src, out)
+            ctx.run(ctx.prepare(fft_inverse, out, back), out, back)
+            rsrc, spec, rback = dev(r), cempty(rfft_extent(shape)), rempty(shape)
+            ctx.run(ctx.prepare(rfft, rsrc, spec), rsrc, spec)
+            ctx.run(ctx.prepare(irfft, spec, rback), spec, rback).wait()
+            outs.append([host(t) for t in (out, back, spec, rback)])
+    for a, b in zip(*outs):
+        np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
+    assert rel(outs[1][0], oracle.fft_forward(x)) <= TOL_FFT
+
+
 def test_line_too_long_for_lds_is_refused():
     """A line whose transform (or Bluestein length) cannot hold two copies in LDS
     is refused with the unsupported status, not run."""

@@ -137,6 +137,26 @@ def test_rl_200_iterations_real_psf():
         assert err <= bound
 
 
+def test_compile_time_fft_plans_keep_the_rl_bits():
+    """The 256 x 256 x 128 RL domain (rows 140, y 280, z 160) runs compile-time FFT
+    plans; with them switched off ("fft.fixed" = 0) the estimate and the
+    log-likelihoods are bit-identical (a 256 x 256 x 122 volume: the same padded
+    domain 280 x 280 x 160 with the 31 x 31 x 63 PSF)."""
+    import torch
+    from iane_amd import autotune
+    y, psf = _c3_like((122, 256, 256))
+    outs = []
+    for fixed in (0, 1):
+        with autotune.applied({"fft.fixed": fixed}):
+            plan = deconvolution.rl_plan(y.shape, dev(psf), deconvolution.FFT)
+            est = torch.empty_like(dev(y))
+            ll = torch.empty(3, dtype=torch.float64, device="cuda")
+            plan.run(dev(y), est, 3, ll_history=ll).wait()
+            outs.append((host(est), host(ll)))
+    np.testing.assert_array_equal(outs[0][0].view(np.uint32), outs[1][0].view(np.uint32))
+    np.testing.assert_array_equal(outs[0][1], outs[1][1])
+
+
 def test_warm_start_continues_the_iteration():
     import torch
     y, psf = _case((24, 28), (1.3, 1.1), (7, 7))
