@@ -27,6 +27,8 @@ struct pw_input {
 	int type;
 	int packed;          // element i sits at ptr + i * size
 	int size;
+	int rank;
+	int extent[3];       // read by point samples (any extent)
 	long long stride[4];
 };
 
@@ -55,6 +57,28 @@ __device__ __forceinline__ float pw_load(const pw_input& aIn, long long aIndex, 
 		at = aIn.ptr + offset;
 	}
 	return load_as_float(at, aIn.type);
+}
+
+// Point sample at view-local coordinates (s0, s1, s2) of a rank <= 3 input:
+// clamp to the edge texel centres, read texel floor(coordinate).
+__device__ __forceinline__ float pw_sample(const pw_input& aIn, float aS0, float aS1, float aS2)
+{
+	const float s[3] = {aS0, aS1, aS2};
+	long long offset = 0;
+#pragma unroll
+	for (int k = 0; k < 3; ++k) {
+		if (k < aIn.rank) {
+			const int extent = aIn.extent[k];
+			const float c = s[k];
+			if (extent <= 0 || !isfinite(c)) {
+				return 0.0f;
+			}
+			const float last = static_cast<float>(extent) - 0.5f;
+			const float clamped = c < 0.5f ? 0.5f : (c > last ? last : c);
+			offset += static_cast<long long>(floorf(clamped)) * aIn.stride[k];
+		}
+	}
+	return load_as_float(aIn.ptr + offset, aIn.type);
 }
 
 // C++ assignment of a float to the destination element; out-of-range and NaN
@@ -117,7 +141,13 @@ __global__ __launch_bounds__(kPwThreads) void pointwise_kernel(pw_args a)
 			case IANE_PW_EQ: r = x == y ? 1.0f : 0.0f; break;
 			case IANE_PW_NE: r = x != y ? 1.0f : 0.0f; break;
 			case IANE_PW_SELECT: r = x != 0.0f ? y : regs[op.c][t]; break;
-			default: r = fmaf(x, y, regs[op.c][t]); break;  // IANE_PW_FMA
+			case IANE_PW_FMA: r = fmaf(x, y, regs[op.c][t]); break;
+			case IANE_PW_COORD:
+				r = static_cast<float>(op.a == 0 ? coord[0] : op.a == 1 ? coord[1] : op.a == 2 ? coord[2] : coord[3]);
+				break;
+			default: // IANE_PW_SAMPLE: input a at (r[b], r[c], r[value])
+				r = pw_sample(a.inputs[op.a], y, regs[op.c][t], regs[static_cast<int>(op.value)][t]);
+				break;
 			}
 			regs[op.dst][t] = r;
 		}
@@ -209,23 +239,42 @@ extern "C" int iane_hip_pointwise(const iane_pw_program* aProgram, const iane_vi
 		a.dst = dst;
 		a.dst_packed = is_packed(dst) ? 1 : 0;
 		a.count = element_count(dst);
+		// inputs read at the output position must match its extents; inputs
+		// that are only point-sampled may have any extent (rank 1..3)
+		bool loaded[IANE_PW_MAX_INPUTS] = {};
+		bool sampled[IANE_PW_MAX_INPUTS] = {};
+		for (int o = 0; o < p.n_ops; ++o) {
+			const iane_pw_op& op = p.ops[o];
+			if ((op.code == IANE_PW_INPUT || op.code == IANE_PW_SAMPLE) && op.a >= 0 && op.a < p.n_inputs) {
+				(op.code == IANE_PW_INPUT ? loaded : sampled)[op.a] = true;
+			}
+		}
 		for (int i = 0; i < p.n_inputs; ++i) {
 			const auto in = make_view4(&p.inputs[i], "pointwise input");
 			if (in.type == IANE_C64) {
 				fail(IANE_ERR_UNSUPPORTED, "pointwise expressions read u8, u16, i32, f32 or f64 elements");
 			}
-			if (in.rank != dst.rank) {
-				invalid("Zip operands require matching nonnegative dimensions");
-			}
-			for (int k = 0; k < 4; ++k) {
-				if (in.extent[k] != dst.extent[k]) {
-					invalid("Source and destination require matching nonnegative dimensions");
+			if (loaded[i] || !sampled[i]) {
+				if (in.rank != dst.rank) {
+					invalid("Zip operands require matching nonnegative dimensions");
 				}
+				for (int k = 0; k < 4; ++k) {
+					if (in.extent[k] != dst.extent[k]) {
+						invalid("Source and destination require matching nonnegative dimensions");
+					}
+				}
+			}
+			if (sampled[i] && in.rank > 3) {
+				fail(IANE_ERR_UNSUPPORTED, "point samples read views of rank 1..3");
 			}
 			a.inputs[i].ptr = in.ptr;
 			a.inputs[i].type = in.type;
 			a.inputs[i].size = dtype_size(in.type);
-			a.inputs[i].packed = is_packed(in) && a.dst_packed ? 1 : 0;
+			a.inputs[i].packed = is_packed(in) && a.dst_packed && loaded[i] ? 1 : 0;
+			a.inputs[i].rank = in.rank;
+			for (int k = 0; k < 3; ++k) {
+				a.inputs[i].extent[k] = in.extent[k];
+			}
 			for (int k = 0; k < 4; ++k) {
 				a.inputs[i].stride[k] = in.stride[k];
 			}
@@ -239,12 +288,31 @@ extern "C" int iane_hip_pointwise(const iane_pw_program* aProgram, const iane_vi
 		};
 		for (int o = 0; o < p.n_ops; ++o) {
 			const iane_pw_op& op = p.ops[o];
-			if (op.code < IANE_PW_INPUT || op.code > IANE_PW_FMA) {
+			if (op.code < IANE_PW_INPUT || op.code > IANE_PW_SAMPLE) {
 				invalid("unknown pointwise operation");
 			}
 			if (op.code == IANE_PW_INPUT) {
 				if (op.a < 0 || op.a >= p.n_inputs) {
 					invalid("pointwise program reads a missing input");
+				}
+			} else if (op.code == IANE_PW_COORD) {
+				if (op.a < 0 || op.a >= 4) {
+					invalid("pointwise coordinate axis out of range");
+				}
+			} else if (op.code == IANE_PW_SAMPLE) {
+				if (op.a < 0 || op.a >= p.n_inputs) {
+					invalid("pointwise program reads a missing input");
+				}
+				const int rank = p.inputs[op.a].rank;
+				reg(op.b);
+				if (rank >= 2) {
+					reg(op.c);
+				}
+				if (rank >= 3) {
+					if (op.value != static_cast<float>(static_cast<int>(op.value))) {
+						invalid("pointwise program reads a register no earlier operation wrote");
+					}
+					reg(static_cast<int>(op.value));
 				}
 			} else if (op.code != IANE_PW_CONST) {
 				reg(op.a);
@@ -262,13 +330,21 @@ extern "C" int iane_hip_pointwise(const iane_pw_program* aProgram, const iane_vi
 			a.ops[o] = op;
 			// unused operand fields still index the register file: keep them in range
 			a.ops[o].a = (op.code == IANE_PW_INPUT || op.code == IANE_PW_CONST) ? op.dst : op.a;
-			const bool binary = op.code >= IANE_PW_ADD && (op.code < IANE_PW_NEG || op.code >= IANE_PW_POW);
+			const bool binary = op.code >= IANE_PW_ADD && op.code <= IANE_PW_FMA &&
+				(op.code < IANE_PW_NEG || op.code >= IANE_PW_POW);
 			a.ops[o].b = binary ? op.b : a.ops[o].a;
 			a.ops[o].c = (op.code == IANE_PW_SELECT || op.code == IANE_PW_FMA) ? op.c : a.ops[o].a;
-			if (op.code == IANE_PW_INPUT) {
-				a.ops[o].a = op.a;  // the input index, not a register
+			if (op.code == IANE_PW_INPUT || op.code == IANE_PW_COORD) {
+				a.ops[o].a = op.a;  // the input index or axis, not a register
 				a.ops[o].b = op.dst;
 				a.ops[o].c = op.dst;
+			}
+			if (op.code == IANE_PW_SAMPLE) {
+				const int rank = p.inputs[op.a].rank;
+				a.ops[o].a = op.a;  // the input index; coordinates in b, c, value
+				a.ops[o].b = op.b;
+				a.ops[o].c = rank >= 2 ? op.c : op.b;
+				a.ops[o].value = static_cast<float>(rank >= 3 ? static_cast<int>(op.value) : op.b);
 			}
 			defined[op.dst] = true;
 		}

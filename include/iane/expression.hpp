@@ -320,6 +320,44 @@ public:
 		return e;
 	}
 
+	/// The coordinate along aAxis of each element (as float): the position
+	/// for_each_position hands its function, for device expressions.
+	static device_expression coordinate(extents_t aSize, int aAxis)
+	{
+		if (aAxis < 0 || aAxis >= N) {
+			throw std::invalid_argument("Coordinate axis out of range");
+		}
+		device_expression e(aSize);
+		pointwise::program_builder program(e.mOps);
+		e.mResult = program.push(IANE_PW_COORD, aAxis);
+		return e;
+	}
+
+	/// Point sample of a rank-R device view (a texture view's texels) at the
+	/// view-local coordinates the expressions give, one per axis of the view;
+	/// the sampled view may have any extent.
+	template<int R>
+	static device_expression sample(const iane_view_desc& aSource, const device_expression* const (&aCoords)[R])
+	{
+		static_assert(R >= 1 && R <= 3, "point samples read views of rank 1..3");
+		device_expression e(aCoords[0]->dimensions());
+		int regs[3] = {0, 0, 0};
+		for (int i = 0; i < R; ++i) {
+			const auto size = aCoords[i]->dimensions();
+			for (int k = 0; k < N; ++k) {
+				if (math::detail::axis(size, k) != math::detail::axis(e.mSize, k)) {
+					throw std::invalid_argument("Zip operands require matching nonnegative dimensions");
+				}
+			}
+			regs[i] = e.append(*aCoords[i]);
+		}
+		const int input = e.add_input(aSource);
+		pointwise::program_builder program(e.mOps);
+		e.mResult = program.push(IANE_PW_SAMPLE, input, regs[0], R >= 2 ? regs[1] : regs[0],
+			static_cast<float>(R >= 3 ? regs[2] : regs[0]));
+		return e;
+	}
+
 	/// aFunction(operands[0], operands[1], ...) per pixel.
 	template<pointwise::function F>
 	static device_expression apply(extents_t aSize, const F& aFunction, std::initializer_list<const device_expression*> aOperands)
@@ -364,34 +402,42 @@ public:
 	}
 
 private:
-	// Copy another expression's inputs and operations in (inputs shared when
-	// they describe the same view); returns its result register here.
+	// The index of an input view (shared when an earlier input describes the
+	// same view).
+	int add_input(const iane_view_desc& aView)
+	{
+		for (std::size_t i = 0; i < mInputs.size(); ++i) {
+			if (std::memcmp(&mInputs[i], &aView, sizeof(aView)) == 0) {
+				return static_cast<int>(i);
+			}
+		}
+		if (mInputs.size() >= IANE_PW_MAX_INPUTS) {
+			throw std::length_error("Pointwise expression exceeds 8 input views");
+		}
+		mInputs.push_back(aView);
+		return static_cast<int>(mInputs.size()) - 1;
+	}
+
+	// Copy another expression's inputs and operations in; returns its result
+	// register here.
 	int append(const device_expression& aOther)
 	{
 		std::vector<int> input_map;
 		for (const auto& in : aOther.mInputs) {
-			int found = -1;
-			for (std::size_t i = 0; i < mInputs.size(); ++i) {
-				if (std::memcmp(&mInputs[i], &in, sizeof(in)) == 0) {
-					found = static_cast<int>(i);
-				}
-			}
-			if (found < 0) {
-				if (mInputs.size() >= IANE_PW_MAX_INPUTS) {
-					throw std::length_error("Pointwise expression exceeds 8 input views");
-				}
-				found = static_cast<int>(mInputs.size());
-				mInputs.push_back(in);
-			}
-			input_map.push_back(found);
+			input_map.push_back(add_input(in));
 		}
 		const int base = static_cast<int>(mOps.size());
 		pointwise::program_builder program(mOps);
 		for (const auto& op : aOther.mOps) {
-			if (op.code == IANE_PW_INPUT) {
-				program.push(IANE_PW_INPUT, input_map[op.a]);
-			} else {
-				program.push(op.code, op.a + base, op.b + base, op.c + base, op.value);
+			switch (op.code) {
+			case IANE_PW_INPUT: program.push(IANE_PW_INPUT, input_map[op.a]); break;
+			case IANE_PW_COORD: program.push(IANE_PW_COORD, op.a); break;
+			case IANE_PW_CONST: program.push(IANE_PW_CONST, 0, 0, 0, op.value); break;
+			case IANE_PW_SAMPLE:
+				program.push(IANE_PW_SAMPLE, input_map[op.a], op.b + base, op.c + base,
+					static_cast<float>(static_cast<int>(op.value) + base));
+				break;
+			default: program.push(op.code, op.a + base, op.b + base, op.c + base, op.value); break;
 			}
 		}
 		return aOther.mResult + base;
@@ -481,6 +527,39 @@ auto generate_view(const TDomain& aDomain, TFunction aFunction)
 	static_assert(concepts::HostView<TDomain>, "generate_view over device views needs a device compiler: use constant_view, "
 		"map_view, zip_view or the view algebra with iane::pointwise functions");
 	return generated_view<TDomain::dimension, TFunction>(aDomain.dimensions(), std::move(aFunction));
+}
+
+/**
+ * The coordinate along aAxis at every position of the domain, as float (no
+ * reference counterpart: the device form of the position for_each_position
+ * passes to its function, for expressions such as select(coordinate == 7, ...)).
+ */
+template<class TDomain>
+requires(concepts::DeviceView<TDomain>)
+auto coordinate_view(const TDomain& aDomain, int aAxis)
+{
+	return device_expression<TDomain::dimension>::coordinate(aDomain.dimensions(), aAxis);
+}
+
+/**
+ * texture_image_view::sample (view/texture_image_view.hpp:56-70) for device
+ * expressions: the point sample of aTexture at view-local coordinates measured
+ * from the outer corner, one coordinate expression (or device view) per texture
+ * axis, evaluated over their common domain. Per axis the coordinate is clamped
+ * to the edge texel centres [0.5, extent - 0.5] and texel floor(coordinate) is
+ * read; an empty view or a nonfinite coordinate gives 0.
+ */
+template<class T, int R, class... TCoord>
+requires(sizeof...(TCoord) == R && (concepts::DeviceView<TCoord> && ...))
+auto sample(const texture_image_view<T, R>& aTexture, const TCoord&... aCoords)
+{
+	static_assert(detail::abi_element<T>, "point samples read u8, u16, i32, f32 or f64 texels");
+	const auto coords = std::make_tuple(detail::as_expression(aCoords)...);
+	return std::apply([&](const auto&... aExpr) {
+		using expression_t = std::remove_cvref_t<decltype(std::get<0>(coords))>;
+		const expression_t* pointers[R] = {&aExpr...};
+		return expression_t::template sample<R>(detail::describe(aTexture), pointers);
+	}, coords);
 }
 
 /// constant_view (procedural.hpp:108-117): the same value at every coordinate.

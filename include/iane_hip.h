@@ -164,7 +164,15 @@ enum {
 	IANE_PW_LT = 20, IANE_PW_LE = 21, IANE_PW_GT = 22, IANE_PW_GE = 23, IANE_PW_EQ = 24,
 	IANE_PW_NE = 25,    /* 1.0f if the comparison holds, else 0.0f */
 	IANE_PW_SELECT = 26, /* r[a] != 0 ? r[b] : r[c] */
-	IANE_PW_FMA = 27     /* fmaf(r[a], r[b], r[c]) */
+	IANE_PW_FMA = 27,    /* fmaf(r[a], r[b], r[c]) */
+	IANE_PW_COORD = 28,  /* r[dst] = the output element's coordinate along axis a */
+	/* r[dst] = point sample of input a (rank 1..3, any extent) at view-local
+	 * coordinates r[b] (axis 0), r[c] (axis 1), r[(int)value] (axis 2), measured
+	 * from the outer corner: per axis the coordinate is clamped to the edge texel
+	 * centres [0.5, extent - 0.5] and the texel floor(coordinate) is read; an
+	 * empty axis or a nonfinite coordinate gives 0. The point sampling of
+	 * texture_image_view::sample, include/iane/view/texture_image_view.hpp:56-70. */
+	IANE_PW_SAMPLE = 29
 };
 typedef struct iane_pw_op {
 	int32_t code;

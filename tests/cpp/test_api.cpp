@@ -919,6 +919,76 @@ int rl_golden(const std::string& aDir, bool aDevice)
 // comparisons), the procedural copy and view algebra of copy_cases.hpp on
 // device views, device for_each on the stream it was given
 // (test_for_each_cuda.cu:19-36), and errors.
+// texture_image_view::sample on device views, as device expressions.
+// test_convolution_cuda.cu:170-191: point samples of a cropped 1-D texture at
+// fractional coordinates (clamped to the crop's edge texel centres), an empty
+// view giving 0; then a 2-D crop sampled on a coordinate grid, a NaN coordinate
+// giving 0, against the same rule evaluated on the host.
+void texture_sample()
+{
+	using namespace iane::pointwise;
+	{
+		iane::host_image<int, 1> source(4);
+		const int values[]{99, 10, 20, 99};
+		std::copy_n(values, 4, source.pointer());
+		iane::texture_image<int, 1> texture(4);
+		iane::copy(source.const_view(), texture).wait();
+		const auto crop = texture.const_view().subview(1, 3).subview(0, 2);
+		const auto empty = crop.subview(0, 0);
+		const float coordinates[]{-1.0f, 0.0f, 0.5f, 0.75f, 1.0f, 1.5f, 2.0f, 0.0f};
+		iane::host_image<float, 1> coords(8);
+		std::copy_n(coordinates, 8, coords.pointer());
+		iane::device_image<float, 1> dcoords(8);
+		iane::copy(coords.const_view(), dcoords.view()).wait();
+		iane::device_image<int, 1> device_output(8);
+		const auto index = iane::coordinate_view(device_output.view(), 0);
+		const auto from_crop = iane::sample(crop, dcoords.const_view());
+		const auto from_empty = iane::sample(empty, dcoords.const_view());
+		const auto picked = iane::device_expression<1>::apply(8, select(_1 == 7.0f, _2, _3),
+			{&index, &from_empty, &from_crop});
+		iane::copy(picked, device_output.view()).wait();
+		iane::host_image<int, 1> output(8);
+		iane::copy(device_output.const_view(), output.view()).wait();
+		const int expected[]{10, 10, 10, 10, 20, 20, 20, 0};
+		CHECK(std::equal(expected, expected + 8, output.pointer()));
+	}
+	{
+		using v2 = iane::math::vec2i;
+		iane::host_image<float, 2> source(v2{5, 4});
+		for (int y = 0; y < 4; ++y) {
+			for (int x = 0; x < 5; ++x) {
+				source.view()[v2{x, y}] = static_cast<float>(10 * y + x);
+			}
+		}
+		iane::texture_image<float, 2> texture(v2{5, 4});
+		iane::copy(source.const_view(), texture).wait();
+		const auto crop = texture.const_view().subview(v2{1, 1}, v2{3, 2});
+		const v2 grid{7, 5};
+		iane::device_image<float, 2> out(grid);
+		const auto gx = iane::coordinate_view(out.view(), 0);
+		const auto gy = iane::coordinate_view(out.view(), 1);
+		const auto cx = iane::device_expression<2>::apply(grid, select(_1 == 6.0f, value(NAN), _1 * 0.7f - 0.4f), {&gx});
+		const auto cy = iane::device_expression<2>::apply(grid, _1 * 0.9f + 0.2f, {&gy});
+		iane::copy(iane::sample(crop, cx, cy), out.view()).wait();
+		iane::host_image<float, 2> got(grid);
+		iane::copy(out.const_view(), got.view()).wait();
+		const auto point = [](float c, int extent) {
+			const float last = static_cast<float>(extent) - 0.5f;
+			return static_cast<int>(std::floor(c < 0.5f ? 0.5f : (c > last ? last : c)));
+		};
+		bool same = true;
+		for (int y = 0; y < grid[1]; ++y) {
+			for (int x = 0; x < grid[0]; ++x) {
+				const float sx = static_cast<float>(x) * 0.7f - 0.4f;
+				const float sy = static_cast<float>(y) * 0.9f + 0.2f;
+				const float expect = x == 6 ? 0.0f : source.const_view()[v2{1 + point(sx, 3), 1 + point(sy, 2)}];
+				same = same && got.const_view()[v2{x, y}] == expect;
+			}
+		}
+		CHECK(same);
+	}
+}
+
 void device_expressions()
 {
 	using v2 = iane::math::vec2i;
@@ -1057,6 +1127,7 @@ int main(int argc, char** argv)
 		texture_uploads();
 		texture_convolution_matches_cpu();
 		device_expressions();
+		texture_sample();
 		sequence_case<true>();
 	}
 	std::printf("%s: %d checks, %d failures\n", device ? "device" : "host", g_checks, g_failures);
