@@ -291,6 +291,19 @@ bool fft_line_fits(int aM)
 	return factor_radices(aM, plan) && 2LL * (aM + 1) <= lds_elems_limit();
 }
 
+namespace {
+
+// 2: the line's own stages fit in LDS; 1: Bluestein's form fits; 0: neither.
+int line_feasibility(int aM)
+{
+	if (fft_line_fits(aM)) {
+		return 2;
+	}
+	return aM > 1 && fft_line_fits(next_fft_length(2 * aM - 1)) ? 1 : 0;
+}
+
+} // namespace
+
 // Fill the radix plan, tables, LDS pitch and bundle of a pass for a transform
 // of logical length n along lines of `lines` x `outer`. A complex line length m
 // whose radices are supported runs its own stages; otherwise Bluestein's form
@@ -403,7 +416,7 @@ void axis_lines(const int* e, int rank, int axis, line_geom& g, int& lines, int&
 
 using namespace iane_hip;
 
-enum { BUF_IN = 0, BUF_OUT = 1, BUF_WS = 2 };
+enum { BUF_IN = 0, BUF_OUT = 1, BUF_WS = 2, BUF_WS2 = 3 };
 
 struct iane_fft_plan {
 	int rank = 0;
@@ -412,9 +425,106 @@ struct iane_fft_plan {
 	size_t in_bytes = 0;
 	size_t out_bytes = 0;
 	size_t workspace = 0;
+	size_t ws2_offset = 0;                    // second workspace region (four-step passes of a C2R plan)
 	std::vector<fft_pass_args> passes;
 	std::vector<std::pair<int, int>> buffers; // (in, out) per pass
 };
+
+namespace {
+
+// The complex passes along `axis` of an array with extents e[] (packed spectrum
+// layout) from buffer aSrc to buffer aDst. A line the LDS holds (directly or in
+// Bluestein's form) is one pass. A longer line n = n1 n2 (both factors such
+// lines, preferring radix factors and balanced sizes) is split in four steps:
+// with j = j2 + n2 j1 and k = k1 + n1 k2,
+//   pass A: length-n1 transforms over j1 (points n2 apart) in the layout of
+//           the source, each result times W_n^(j2 k1), into aScratch;
+//   pass B: length-n2 transforms over j2, stored to X[k1 + n1 k2] in aDst.
+// Returns false when no factorisation fits.
+bool add_c2c_axis(iane_fft_plan& aPlan, const int* e, int aRank, int aAxis, float aSign, int aSrc, int aDst,
+	int aScratch)
+{
+	line_geom g;
+	int lines, outer;
+	axis_lines(e, aRank, aAxis, g, lines, outer);
+	const int n = e[aAxis];
+	if (line_feasibility(n) > 0) {
+		fft_pass_args a{};
+		setup_pass(a, PASS_C2C, n, lines, outer);
+		a.sign = aSign;
+		a.load_n = a.store_n = n;
+		a.gin = a.gout = g;
+		aPlan.passes.push_back(a);
+		aPlan.buffers.push_back({aSrc, aDst});
+		return true;
+	}
+	int best_n1 = 0;
+	int best_score = -1;
+	for (int d = 2; static_cast<long long>(d) * d <= n; ++d) {
+		if (n % d != 0) {
+			continue;
+		}
+		for (const int n1 : {d, n / d}) {
+			const int f1 = line_feasibility(n1);
+			const int f2 = line_feasibility(n / n1);
+			if (f1 == 0 || f2 == 0) {
+				continue;
+			}
+			const int score = (f1 + f2) * (1 << 24) + std::min(n1, n / n1);
+			if (score > best_score) {
+				best_score = score;
+				best_n1 = n1;
+			}
+		}
+	}
+	if (best_n1 == 0) {
+		return false;
+	}
+	const int n1 = best_n1;
+	const int n2 = n / n1;
+	long long inner = 1;
+	for (int i = 0; i < aAxis; ++i) {
+		inner *= e[i];
+	}
+	long long above = 1;
+	for (int i = aAxis + 1; i < aRank; ++i) {
+		above *= e[i];
+	}
+	const long long stride = inner * n;  // between the lines' outer groups
+	if (inner * n2 > 0x7fffffffLL || above * n1 > 0x7fffffffLL) {
+		return false;
+	}
+	fft_pass_args pa{};
+	setup_pass(pa, PASS_C2C, n1, static_cast<int>(inner * n2), static_cast<int>(above));
+	pa.sign = aSign;
+	pa.load_n = pa.store_n = n1;
+	pa.gin = pa.gout = {inner * n2, 1, stride, 0, 0};
+	pa.post_n = n;
+	pa.post_div = static_cast<int>(inner);
+	pa.tw_post = twiddle_table(n);
+	pa.tables_in_lds = 0;  // the general pass kernel (the twiddle step is there)
+	pa.fixed = 0;
+	fft_pass_args pb{};
+	setup_pass(pb, PASS_C2C, n2, static_cast<int>(inner), static_cast<int>(above * n1));
+	pb.sign = aSign;
+	pb.load_n = pb.store_n = n2;
+	pb.gin = {inner, 1, inner * n2, 0, 0};
+	pb.gout = {inner * n1, 1, inner, stride, n1};
+	pb.tables_in_lds = 0;
+	pb.fixed = 0;
+	aPlan.passes.push_back(pa);
+	aPlan.buffers.push_back({aSrc, aScratch});
+	aPlan.passes.push_back(pb);
+	aPlan.buffers.push_back({aScratch, aDst});
+	return true;
+}
+
+void refuse_long_line(int aN)
+{
+	fail(IANE_ERR_UNSUPPORTED, "FFT line length " + std::to_string(aN) + " exceeds the LDS line limit");
+}
+
+} // namespace
 
 extern "C" {
 
@@ -454,26 +564,29 @@ int iane_hip_fft_plan_create(iane_fft_plan** aPlan, int aRank, const int32_t* aE
 		}
 		const size_t real_bytes = static_cast<size_t>(count) * sizeof(float);
 		const size_t spec_bytes = static_cast<size_t>(spec_count) * sizeof(float2);
+		bool four_step = false;
+		const auto axis_pass = [&](int aAxis, float aSign, int aSrc, int aDst, int aScratch) {
+			const size_t before = plan->passes.size();
+			if (!add_c2c_axis(*plan, spec, aRank, aAxis, aSign, aSrc, aDst, aScratch)) {
+				refuse_long_line(spec[aAxis]);
+			}
+			four_step = four_step || plan->passes.size() - before > 1;
+		};
 		if (aKind == IANE_FFT_C2C_FORWARD || aKind == IANE_FFT_C2C_INVERSE) {
 			const float sign = aKind == IANE_FFT_C2C_FORWARD ? -1.0f : 1.0f;
 			for (int axis = 0; axis < aRank; ++axis) {
-				fft_pass_args a{};
-				line_geom g;
-				int lines, outer;
-				axis_lines(spec, aRank, axis, g, lines, outer);
-				setup_pass(a, PASS_C2C, spec[axis], lines, outer);
-				a.sign = sign;
-				a.load_n = a.store_n = spec[axis];
-				a.gin = a.gout = g;
-				plan->passes.push_back(a);
-				plan->buffers.push_back({axis == 0 ? BUF_IN : BUF_OUT, BUF_OUT});
+				axis_pass(axis, sign, axis == 0 ? BUF_IN : BUF_OUT, BUF_OUT, BUF_WS);
 			}
 			plan->in_bytes = plan->out_bytes = spec_bytes;
+			plan->workspace = four_step ? spec_bytes : 0;
 		} else if (aKind == IANE_FFT_R2C) {
 			fft_pass_args a{};
 			line_geom g;
 			int lines, outer;
 			axis_lines(plan->extent, aRank, 0, g, lines, outer);
+			if (line_feasibility(plan->extent[0] % 2 == 0 ? plan->extent[0] / 2 : plan->extent[0]) == 0) {
+				refuse_long_line(plan->extent[0]);  // real lines are not split (yet)
+			}
 			setup_pass(a, PASS_R2C, plan->extent[0], lines, outer);
 			a.load_n = plan->extent[0];
 			a.store_n = spec[0];
@@ -482,36 +595,24 @@ int iane_hip_fft_plan_create(iane_fft_plan** aPlan, int aRank, const int32_t* aE
 			plan->passes.push_back(a);
 			plan->buffers.push_back({BUF_IN, BUF_OUT});
 			for (int axis = 1; axis < aRank; ++axis) {
-				fft_pass_args c{};
-				axis_lines(spec, aRank, axis, g, lines, outer);
-				setup_pass(c, PASS_C2C, spec[axis], lines, outer);
-				c.sign = -1.0f;
-				c.load_n = c.store_n = spec[axis];
-				c.gin = c.gout = g;
-				plan->passes.push_back(c);
-				plan->buffers.push_back({BUF_OUT, BUF_OUT});
+				axis_pass(axis, -1.0f, BUF_OUT, BUF_OUT, BUF_WS);
 			}
 			plan->in_bytes = real_bytes;
 			plan->out_bytes = spec_bytes;
+			plan->workspace = four_step ? spec_bytes : 0;
 		} else { // C2R
 			int src = BUF_IN;
 			for (int axis = aRank - 1; axis >= 1; --axis) {
-				fft_pass_args c{};
-				line_geom g;
-				int lines, outer;
-				axis_lines(spec, aRank, axis, g, lines, outer);
-				setup_pass(c, PASS_C2C, spec[axis], lines, outer);
-				c.sign = 1.0f;
-				c.load_n = c.store_n = spec[axis];
-				c.gin = c.gout = g;
-				plan->passes.push_back(c);
-				plan->buffers.push_back({src, BUF_WS});
+				axis_pass(axis, 1.0f, src, BUF_WS, BUF_WS2);
 				src = BUF_WS;
 			}
 			fft_pass_args a{};
 			line_geom g;
 			int lines, outer;
 			axis_lines(plan->extent, aRank, 0, g, lines, outer);
+			if (line_feasibility(plan->extent[0] % 2 == 0 ? plan->extent[0] / 2 : plan->extent[0]) == 0) {
+				refuse_long_line(plan->extent[0]);  // real lines are not split (yet)
+			}
 			setup_pass(a, PASS_C2R, plan->extent[0], lines, outer);
 			a.load_n = spec[0];
 			a.store_n = plan->extent[0];
@@ -520,6 +621,10 @@ int iane_hip_fft_plan_create(iane_fft_plan** aPlan, int aRank, const int32_t* aE
 			plan->passes.push_back(a);
 			plan->buffers.push_back({src, BUF_OUT});
 			plan->workspace = aRank > 1 ? spec_bytes : 0;
+			if (four_step) {
+				plan->ws2_offset = (spec_bytes + 255) / 256 * 256;
+				plan->workspace = plan->ws2_offset + spec_bytes;
+			}
 			plan->in_bytes = spec_bytes;
 			plan->out_bytes = real_bytes;
 		}
@@ -565,7 +670,8 @@ int iane_hip_fft_exec(iane_fft_plan* aPlan, const void* aIn, void* aOut, void* a
 			}
 		}
 		auto stream = static_cast<hipStream_t>(aStream);
-		void* buffers[3] = {const_cast<void*>(aIn), aOut, aWorkspace};
+		void* buffers[4] = {const_cast<void*>(aIn), aOut, aWorkspace,
+			aWorkspace ? static_cast<char*>(aWorkspace) + aPlan->ws2_offset : nullptr};
 		for (size_t i = 0; i < aPlan->passes.size(); ++i) {
 			fft_pass_args a = aPlan->passes[i];
 			a.in = buffers[aPlan->buffers[i].first];

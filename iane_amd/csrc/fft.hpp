@@ -429,7 +429,17 @@ struct line_geom {
 	long long elem;
 	long long line;
 	long long outer;
+	// a second outer level (0: none): outer group o sits at
+	// (o % outer_split) * outer + (o / outer_split) * outer_hi
+	long long outer_hi;
+	int outer_split;
 };
+
+__host__ __device__ __forceinline__ long long outer_offset(const line_geom& g, int aOuter)
+{
+	return g.outer_split == 0 ? aOuter * g.outer :
+		(aOuter % g.outer_split) * g.outer + (aOuter / g.outer_split) * g.outer_hi;
+}
 
 enum { PASS_C2C = 0, PASS_R2C = 1, PASS_C2R = 2, PASS_C2C_MULT = 3 };
 
@@ -452,6 +462,9 @@ struct fft_pass_args {
 	fft_radix_plan plan; // the complex transform run in LDS (length m; M for Bluestein)
 	int line_m;          // complex length of the line transform
 	int fixed;           // compile-time plan id of a.plan (0: run-time stages)
+	int post_n;          // four-step first pass (else 0): point k of line l times W_post_n^((l / post_div) k)
+	int post_div;
+	const float2* tw_post; // W_post_n table
 	int tables_in_lds;   // 1: twiddle tables staged in LDS after the line buffers; 0: read from memory
 	const float2* chirp; // Bluestein (else null): exp(-i pi j^2 / line_m), j < line_m
 	const float2* bspec; // Bluestein: spectrum of the wrapped conj(chirp), length M, divided by M
@@ -507,7 +520,7 @@ __device__ __forceinline__ void batched_loop(int total, Fetch aFetch, Use aUse)
 __device__ inline void load_complex_lines(const float2* in, const line_geom& g, int aOuter, int aLine0, int count,
 	int load_n, int width, float2* buf, int pitch)
 {
-	const long long base = aOuter * g.outer + aLine0 * g.line;
+	const long long base = outer_offset(g, aOuter) + aLine0 * g.line;
 	const int total = count * width;
 	const int last = load_n > 0 ? load_n - 1 : 0;
 	const float2 zero = make_float2(0.0f, 0.0f);
@@ -542,7 +555,7 @@ __device__ inline void load_complex_lines(const float2* in, const line_geom& g, 
 __device__ inline void store_complex_lines(float2* out, const line_geom& g, int aOuter, int aLine0, int count,
 	int store_n, const float2* buf, int pitch)
 {
-	const long long base = aOuter * g.outer + aLine0 * g.line;
+	const long long base = outer_offset(g, aOuter) + aLine0 * g.line;
 	const int total = count * store_n;
 	// the fetch (an LDS read) also returns the destination index
 	struct fetched {
@@ -575,7 +588,7 @@ __device__ inline void store_complex_lines(float2* out, const line_geom& g, int 
 __device__ inline void load_real_lines(const float* in, const line_geom& g, int aOuter, int aLine0, int count,
 	int n, int load_n, float2* buf, int pitch)
 {
-	const long long base = aOuter * g.outer + aLine0 * g.line;
+	const long long base = outer_offset(g, aOuter) + aLine0 * g.line;
 	const bool even = (n & 1) == 0;
 	const int width = even ? n / 2 : n;
 	const int total = count * width;
@@ -660,7 +673,7 @@ __device__ inline void c2r_extend_odd(float2* x, int count, int n, int pitch)
 __device__ inline void store_real_lines(float* out, const line_geom& g, int aOuter, int aLine0, int count, int n,
 	int store_n, const float2* buf, int pitch)
 {
-	const long long base = aOuter * g.outer + aLine0 * g.line;
+	const long long base = outer_offset(g, aOuter) + aLine0 * g.line;
 	const bool even = (n & 1) == 0;
 	const int total = count * store_n;
 	const fast_div by_store(store_n), by_count(count);
@@ -843,7 +856,23 @@ __device__ __forceinline__ void fft_pass_body(const fft_pass_args& a, unsigned a
 		load_complex_lines(static_cast<const float2*>(a.in), a.gin, o, l0, count, a.load_n, m, A, a.pitch);
 		stage_tables();
 		__syncthreads();
-		const float2* r = line(A, B, a.sign);
+		float2* r = line(A, B, a.sign);
+		if constexpr (GENERAL) {
+			if (a.post_n) {
+				// four-step: the twiddle between the two line passes
+				for (int t = threadIdx.x; t < count * m; t += blockDim.x) {
+					const int w = t / m;
+					const int k = t - w * m;
+					const int j2 = (l0 + w) / a.post_div;
+					float2 tw = a.tw_post[static_cast<long long>(j2) * k];
+					if (a.sign > 0.0f) {
+						tw = cconj(tw);
+					}
+					r[w * a.pitch + k] = cmul(r[w * a.pitch + k], tw);
+				}
+				__syncthreads();
+			}
+		}
 		store_complex_lines(static_cast<float2*>(a.out), a.gout, o, l0, count, a.store_n, r, a.pitch);
 	} else if constexpr (MODE == PASS_C2C_MULT) {
 		load_complex_lines(static_cast<const float2*>(a.in), a.gin, o, l0, count, a.load_n, m, A, a.pitch);

@@ -271,15 +271,68 @@ def test_compile_time_plans_give_the_run_time_bits(shape):
     assert rel(outs[1][0], oracle.fft_forward(x)) <= TOL_FFT
 
 
+@pytest.mark.parametrize("n", [16384, 20000, 10006, 65536])
+def test_c2c_four_step_lines(n):
+    """Lines longer than two copies fit in LDS run as two passes (four-step:
+    n = n1 n2, a twiddle between them, the second pass storing the transposed
+    order through the plan's workspace): 16384 = 128 x 128, 20000, 10006 =
+    2 x 5003 (the prime factor in Bluestein's form), 65536 = 256 x 256. Forward
+    and inverse against the direct float64 DFT (numpy's FFT for 65536)."""
+    rng = np.random.default_rng(n)
+    x = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    ctx = execution_context()
+    for alg, sign in ((fft_forward, -1), (fft_inverse, 1)):
+        src, out = dev(x), cempty((n,))
+        op = ctx.prepare(alg, src, out)
+        assert op.state.scratch_bytes() == n * 8
+        ctx.run(op, src, out).wait()
+        if n <= 20000:
+            ref = _dft_direct(x, sign)
+        else:
+            ref = np.fft.fft(x.astype(np.complex128)) if sign < 0 else np.fft.ifft(x.astype(np.complex128)) * n
+        err = rel(host(out), ref)
+        print(f"n={n} sign={sign}: {err:.2e}")
+        assert err <= TOL_FFT
+
+
+@pytest.mark.parametrize("shape", [(12000, 3), (2, 10006, 5), (13000, 4), (3, 16384)])
+def test_four_step_axes_nd(shape):
+    """Long strided axes (12000 and 10006 along axis 1, 13000 along the last
+    numpy axis of a real transform's spectrum) and a long axis 0 (16384): C2C
+    both ways and, where axis 0 is short, R2C/C2R (the inverse's second
+    workspace region), against the float64 FFT, round trips included."""
+    rng = np.random.default_rng(sum(shape))
+    ctx = execution_context()
+    x = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+    src, out, back = dev(x), cempty(shape), cempty(shape)
+    ctx.run(ctx.prepare(fft_forward, src, out), src, out).wait()
+    assert rel(host(out), oracle.fft_forward(x)) <= TOL_FFT
+    ctx.run(ctx.prepare(fft_inverse, out, back), out, back).wait()
+    assert rel(host(back), x * np.prod(shape)) <= TOL_FFT
+    if shape[-1] <= 20000:
+        r = rng.standard_normal(shape).astype(np.float32)
+        spec_shape = rfft_extent(shape)
+        rsrc, spec, rback = dev(r), cempty(spec_shape), rempty(shape)
+        ctx.run(ctx.prepare(rfft, rsrc, spec), rsrc, spec).wait()
+        assert rel(host(spec), oracle.rfft(r)) <= TOL_FFT
+        ctx.run(ctx.prepare(irfft, spec, rback), spec, rback).wait()
+        assert rel(host(rback), r.astype(np.float64) * np.prod(shape)) <= TOL_FFT
+
+
 def test_line_too_long_for_lds_is_refused():
-    """A line whose transform (or Bluestein length) cannot hold two copies in LDS
-    is refused with the unsupported status, not run."""
+    """A line with no factorisation into LDS-sized transforms (a prime above the
+    Bluestein limit) is refused with the unsupported status, not run; so is a
+    real line too long for one pass (real lines are not split)."""
     from iane_amd import _lib
     ctx = execution_context()
     x = dev(np.zeros(40009, np.complex64))  # prime: Bluestein length > 80 000
     out = cempty((40009,))
     with pytest.raises(_lib.unsupported):
         ctx.prepare(fft_forward, x, out)
+    r = dev(np.zeros(50000, np.float32))
+    spec = cempty(rfft_extent((50000,)))
+    with pytest.raises(_lib.unsupported):
+        ctx.prepare(rfft, r, spec)
 
 
 def test_fft_workspace_reported_for_c2r():
