@@ -37,7 +37,7 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
 MFMA16_PEAK_TFLOPS = 2500.0
-LEGS = ("dense15_reference", "dense15_fp32", "dense15_split_fp16", "dense15_split_bf16", "u16_binomial3",
+LEGS = ("separable_split", "dense15_reference", "dense15_fp32", "dense15_split_fp16", "dense15_split_bf16", "u16_binomial3",
         "dense3_fp32", "image_2d", "sequence", "rl_fft", "rl_split", "rl_error", "rl_batch", "learned")
 METRIC = "Mvoxel/s 3-D conv (%HBM roofline); RL time-to-equal-likelihood, learned vs plain"
 
@@ -214,30 +214,92 @@ def bench_rl_split(timer, stream, iterations: int = 2):
     return out
 
 
+def bench_separable_split(timer, vol, taps, out, policy, stream, steps: int, headline_out: np.ndarray):
+    """The headline workload on the split-FP16 separable engine (precision.split_fp16:
+    every filter on v_mfma_f32_16x16x32_f16 with hi + lo operands, FP32 sums), and the
+    error of both engines over the whole volume against float64 per-axis passes
+    (scipy.ndimage.convolve1d, zero border: a separable kernel's zero border factors
+    over the axes)."""
+    import iane_amd
+    from iane_amd import border_mode, convolution_options, precision
+    from scipy import ndimage
+    opts = convolution_options(border_mode.zero, precision.split_fp16)
+
+    def step():
+        iane_amd.convolve_separable(vol, [taps, taps, taps], out, opts, policy)
+
+    settle = 0.0
+    while settle < 30.0:
+        settle += timer.time(step, 50)
+    ms = timer.time(step, steps) / steps
+    step()
+    _lib_sync(stream)
+    split_out = out.cpu().numpy()
+    t = taps.cpu().numpy().astype(np.float64)
+    ref = vol.cpu().numpy().astype(np.float64)
+    for axis in (2, 1, 0):
+        ref = ndimage.convolve1d(ref, t, axis=axis, mode="constant", cval=0.0)
+    voxels = ref.size
+    result = {"us": round(ms * 1e3, 2), "Mvoxel_s": round(voxels / ms / 1e3, 1),
+              "GB_s": round(voxels * 8 / ms / 1e6, 1), "frac_hbm": round(voxels * 8 / ms / 1e6 / HBM_PEAK_GBS, 4),
+              "kernel": "sep3_split_kernel<15>",
+              "error_vs_fp64": _errors(split_out, ref), "fp32_engine_error_vs_fp64": _errors(headline_out, ref)}
+    log(f"[bench] separable split_fp16: {ms * 1e3:.1f} us, error {result['error_vs_fp64']}, "
+        f"fp32 engine error {result['fp32_engine_error_vs_fp64']}")
+    return result
+
+
 def _errors(est: np.ndarray, ref: np.ndarray) -> dict:
     diff = est.astype(np.float64) - ref
     return {"l2_rel": float(f"{np.linalg.norm(diff) / np.linalg.norm(ref):.3g}"),
             "max_abs_rel": float(f"{np.abs(diff).max() / np.abs(ref).max():.3g}")}
 
 
+def _conv_zero_fp64(x: np.ndarray, psf: np.ndarray) -> np.ndarray:
+    """Zero-border convolution with an odd centred kernel in float64 (a padded FFT
+    product; scipy's pocketfft)."""
+    from scipy import fft as sfft
+    shape = [n + k - 1 for n, k in zip(x.shape, psf.shape)]
+    axes = list(range(x.ndim))
+    workers = min(16, os.cpu_count() or 1)
+    full = sfft.irfftn(sfft.rfftn(x, shape, axes, workers=workers) * sfft.rfftn(psf, shape, axes, workers=workers),
+                       shape, axes, workers=workers)
+    return full[tuple(slice(k // 2, k // 2 + n) for n, k in zip(x.shape, psf.shape))]
+
+
+def _rl_fp64(observed: np.ndarray, psf: np.ndarray, iterations: int, epsilon: float = 1e-6) -> np.ndarray:
+    """Float64 RL (examples/rl_deconvolution/main.cu:51-81), the reference the error
+    figures below are taken against; the tests pin the same formulas (the oracle's rl)
+    to the reference's own golden runs."""
+    y = observed.astype(np.float64)
+    h = psf.astype(np.float64)
+    flipped = h[::-1, ::-1, ::-1].copy()
+    sensitivity = _conv_zero_fp64(np.ones_like(y), flipped)
+    x = np.where(y > epsilon, y, epsilon)
+    for _ in range(iterations):
+        p = _conv_zero_fp64(x, h)
+        c = _conv_zero_fp64(y / np.where(p > epsilon, p, epsilon), flipped)
+        x = x * np.where(sensitivity > 0, c / np.where(sensitivity > epsilon, sensitivity, epsilon), 0.0)
+    return x
+
+
 def bench_rl_fp64_error(stream):
-    """Error of the GPU RL estimates against the FP64 oracle (the checker: scipy
-    float64 FFT convolutions restating examples/rl_deconvolution/main.cu:51-81):
-    normwise ||x - x64|| / ||x64|| and max|x - x64| / max|x64|, for the FFT method and
-    the split-precision direct method. Full 256x256x128 volume with the 31x31x63 PSF
-    after 2 iterations, and a 64^3 volume of the same construction (40 blobs) after
-    200 iterations, where the FP32 rounding has accumulated."""
+    """Error of the GPU RL estimates against float64 RL (scipy FFT convolutions
+    restating examples/rl_deconvolution/main.cu:51-81): normwise ||x - x64|| / ||x64||
+    and max|x - x64| / max|x64|, for the FFT method and the split-precision direct
+    method. Full 256x256x128 volume with the 31x31x63 PSF after 2 iterations, and a
+    64^3 volume of the same construction (40 blobs) after 200 iterations, where the
+    FP32 rounding has accumulated."""
     import torch
-    from oracle import oracle
     from iane_amd import deconvolution, hip_policy, precision, synth
     policy = hip_policy(stream)
-    out = {"oracle": "oracle.rl: float64 FFT convolutions (scipy), zero border"}
+    out = {"reference": "float64 RL on scipy FFT convolutions, zero border"}
     for label, shape, blobs, iterations in (("256x256x128_2_iterations", (128, 256, 256), 300, 2),
                                             ("64x64x64_200_iterations", (64, 64, 64), 40, 200)):
         _, psf, y = synth.rl_volume_data(shape=shape, blobs=blobs)
         t0 = time.perf_counter()
-        ref, _ = oracle.rl(y, psf.astype(np.float64), iterations)
-        log(f"[bench] RL error {label}: FP64 oracle in {time.perf_counter() - t0:.1f} s")
+        ref = _rl_fp64(y, psf, iterations)
+        log(f"[bench] RL error {label}: float64 reference in {time.perf_counter() - t0:.1f} s")
         dy, dpsf = torch.from_numpy(y).cuda(), torch.from_numpy(psf).cuda()
         leg = {}
         for name, method, prec in (("fft", deconvolution.FFT, precision.fp32),
@@ -453,6 +515,10 @@ def main() -> int:
     log(f"[bench] separable: {ms_per_step * 1e3:.1f} us/step, {value:.0f} Mvoxel/s, {achieved:.0f} GB/s")
 
     extra = {}
+    if "separable_split" in legs:
+        _lib.call("iane_hip_stream_sync", stream)
+        headline_out = out.cpu().numpy()
+        extra["separable_split_fp16"] = bench_separable_split(timer, vol, taps, out, policy, stream, args.steps, headline_out)
     if args.copy_calibration:
         _lib.call("iane_hip_memcpy_async", out.data_ptr(), vol.data_ptr(), voxels * 4, 3, stream)
         _lib.call("iane_hip_stream_sync", stream)
@@ -525,7 +591,7 @@ def main() -> int:
     if "rl_split" in legs:
         extra["rl_direct_split_256x256x128_psf31x31x63"] = bench_rl_split(timer, stream)
     if "rl_error" in legs:
-        extra["rl_error_vs_fp64_oracle"] = bench_rl_fp64_error(stream)
+        extra["rl_error_vs_fp64"] = bench_rl_fp64_error(stream)
     if "rl_batch" in legs:
         extra["rl_batch_64_volumes"] = bench_rl_batch(timer, stream)
     if "learned" in legs:

@@ -17,8 +17,11 @@
 // matrix cores, the first with per-wave operand loads and no LDS, the second
 // with the plane staged in LDS; sep3_fused_kernel uses the vector ALUs only.
 //
+// Engine 3 (split-FP16 operands, every filter on the matrix cores) is in
+// conv_separable_split.hip.
+//
 // General path: one 1-D pass per axis through temporaries in the caller's workspace.
-#include "common.hpp"
+#include "separable.hpp"
 
 namespace iane_hip {
 
@@ -26,32 +29,10 @@ void validate_convolution(const view4& aSrc, const view4& aKer, const view4& aDs
 
 namespace {
 
+using separable::plane_resource;
+using separable::sep_args;
+
 constexpr int kSepTileX = 64;
-
-struct sep_args {
-	const char* src;
-	int src_type;
-	int nx, ny, nz;
-	long long sx, sy, sz;       // source byte strides
-	char* dst;
-	int dst_type;
-	long long dx, dy, dz;       // destination byte strides
-	int border;
-	unsigned tiles_x, tiles_y, chunks_z;
-	int chunk;                  // output planes per workgroup
-	const char* taps[3];        // device tap vectors
-	long long tap_stride[3];
-	int tap_type[3];
-	int src_span, dst_span;     // bytes of one source / destination plane that a launch addresses (matrix engine)
-};
-
-// Raw buffer resource of one plane (GFX9 word 3: 32-bit data format). Loads
-// and stores through it take a scalar base and a VGPR byte offset, and
-// offsets at or past `aBytes` read 0 / are dropped.
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_resource(const char* aBase, int aBytes)
-{
-	return __builtin_amdgcn_make_buffer_rsrc(const_cast<char*>(aBase), static_cast<short>(0), aBytes, 0x00020000);
-}
 
 __device__ __forceinline__ bool clamp_or_zero(int& aCoord, int aExtent, int aBorder)
 {
@@ -812,7 +793,8 @@ void launch_fused(sep_args& a, hipStream_t aStream, int aThreads)
 // up to 15, staged offsets within a plane kept in 32-bit registers.
 bool separable_fused(const view4& aSrc, const view4* aTaps, int aN, const view4& aDst, int aPrecision)
 {
-	bool fused = aSrc.rank == 3 && aPrecision == IANE_PREC_FP32 && aSrc.type == IANE_F32 && aDst.type == IANE_F32;
+	bool fused = aSrc.rank == 3 && (aPrecision == IANE_PREC_FP32 || aPrecision == IANE_PREC_SPLIT_FP16) &&
+		aSrc.type == IANE_F32 && aDst.type == IANE_F32;
 	const int k = aTaps[0].extent[0];
 	for (int i = 0; i < aN; ++i) {
 		fused = fused && aTaps[i].extent[0] == k;
@@ -831,7 +813,7 @@ size_t separable_workspace(const view4& aSrc, const view4* aTaps, int aN, const 
 		return 0;
 	}
 	const int temps = std::min(2, aSrc.rank - 1);
-	const size_t one = static_cast<size_t>(element_count(aDst)) * (aPrecision != IANE_PREC_FP32 ? 8 : 4);
+	const size_t one = static_cast<size_t>(element_count(aDst)) * (aPrecision == IANE_PREC_REFERENCE ? 8 : 4);
 	return static_cast<size_t>(temps) * ((one + 255) / 256 * 256);
 }
 
@@ -877,6 +859,17 @@ void convolve_separable(const view4& aSrc, const view4* aTaps, int aN, const vie
 		// row pitches that are multiples of 4, a 16-byte aligned source, zero border
 		const bool grouped = a.sx == 4 && a.nx % 4 == 0 && a.sy % 16 == 0 && a.sz % 16 == 0 &&
 			reinterpret_cast<uintptr_t>(a.src) % 16 == 0 && a.border == IANE_BORDER_ZERO;
+		// engine 3 (split FP16 operands, every filter on the matrix cores) also
+		// addresses a chunk of output planes from one base: planes must not overlap
+		if (aPrecision == IANE_PREC_SPLIT_FP16 && matrix && grouped && a.dz >= dst_span) {
+			a.src_span = static_cast<int>(src_span);
+			a.dst_span = static_cast<int>(dst_span);
+			if (separable::split_chunk_fits(a, 16)) {
+				separable::launch_split(a, k, aStream);
+				check_hip(hipGetLastError(), "Separable convolution launch");
+				return;
+			}
+		}
 		if (matrix && engine == 2 && grouped) {
 			a.src_span = static_cast<int>(src_span);
 			a.dst_span = static_cast<int>(dst_span);
@@ -924,7 +917,7 @@ void convolve_separable(const view4& aSrc, const view4* aTaps, int aN, const vie
 		return;
 	}
 	// general path: one pass per axis
-	const bool wide = aPrecision != IANE_PREC_FP32;
+	const bool wide = aPrecision == IANE_PREC_REFERENCE;
 	const int temp_type = wide ? IANE_F64 : IANE_F32;
 	const long long count = element_count(aDst);
 	const size_t temp_bytes = (static_cast<size_t>(count) * dtype_size(temp_type) + 255) / 256 * 256;
@@ -984,8 +977,8 @@ separable_call check_separable(const iane_view_desc* aSrc, const iane_view_desc*
 	if (aBorder != IANE_BORDER_CLAMP && aBorder != IANE_BORDER_ZERO) {
 		invalid("unknown border mode");
 	}
-	if (aPrecision != IANE_PREC_REFERENCE && aPrecision != IANE_PREC_FP32) {
-		fail(IANE_ERR_UNSUPPORTED, "separable convolution supports REFERENCE and FP32 precision");
+	if (aPrecision != IANE_PREC_REFERENCE && aPrecision != IANE_PREC_FP32 && aPrecision != IANE_PREC_SPLIT_FP16) {
+		fail(IANE_ERR_UNSUPPORTED, "separable convolution supports REFERENCE, FP32 and SPLIT_FP16 precision");
 	}
 	return c;
 }
