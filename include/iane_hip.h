@@ -130,6 +130,11 @@ int iane_hip_convolve(const iane_view_desc* src, const iane_view_desc* kernel,
 /* Separable convolution: the kernel is the outer product taps[0] (axis 0)
  * x taps[1] x ... (each a rank-1 F32/F64 view of odd extent). Equal, up to
  * rounding, to iane_hip_convolve with the dense outer-product kernel.
+ * Precision REFERENCE (FP64 per-axis passes), FP32, or SPLIT_FP16 (rank-3 F32
+ * images, zero border, x extent a multiple of 4: every filter on FP16 matrix
+ * instructions with hi + lo operands and FP32 sums, error relative to the
+ * largest sample of a 32 x 32 plane region and a K-plane band; other shapes
+ * fall back to the FP32 engines). SPLIT_BF16 is refused (UNSUPPORTED).
  * No reference counterpart: the reference always runs the dense kernel. */
 int iane_hip_convolve_separable_workspace_bytes(const iane_view_desc* src, const iane_view_desc* taps,
 	int n_taps, const iane_view_desc* dst, int border, int precision, size_t* bytes);

@@ -56,3 +56,39 @@ def test_invalid_descriptor_reports_invalid_argument():
     assert b"odd" in lib.iane_hip_last_error()
     with pytest.raises(_lib.invalid_argument):
         _lib.check(status)
+
+
+def test_separable_precision_contract_without_gpu():
+    """The separable entry point takes REFERENCE, FP32 and SPLIT_FP16 (the split
+    engine's fused path needs no workspace) and refuses SPLIT_BF16; the query runs
+    on the host only."""
+    import ctypes
+    from iane_amd import _lib
+    lib = _lib.load()
+
+    def desc(extents, dtype=_lib.F32):
+        d = _lib.iane_view_desc()
+        d.ptr = 256  # never dereferenced by the query
+        d.rank = len(extents)
+        d.dtype = dtype
+        stride = 4
+        for i, e in enumerate(extents):
+            d.extent[i] = e
+            d.stride_bytes[i] = stride
+            stride *= e
+        return d
+
+    src = desc((32, 20, 12))
+    taps = (_lib.iane_view_desc * 3)(desc((5,)), desc((5,)), desc((5,)))
+    size = ctypes.c_size_t(77)
+    for prec, fused in ((0, False), (1, True), (3, True)):
+        status = lib.iane_hip_convolve_separable_workspace_bytes(ctypes.byref(src), taps, 3, ctypes.byref(src), 1,
+                                                                 prec, ctypes.byref(size))
+        assert status == _lib.OK, lib.iane_hip_last_error()
+        # FP32 / SPLIT_FP16 on rank-3 F32 with equal odd taps <= 15: the single-kernel
+        # path, no workspace; REFERENCE: FP64 per-axis passes through two temporaries
+        assert (size.value == 0) == fused
+    status = lib.iane_hip_convolve_separable_workspace_bytes(ctypes.byref(src), taps, 3, ctypes.byref(src), 1, 2,
+                                                             ctypes.byref(size))
+    assert status == _lib.ERR_UNSUPPORTED
+    assert b"SPLIT_FP16" in lib.iane_hip_last_error()
