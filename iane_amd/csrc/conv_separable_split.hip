@@ -14,9 +14,10 @@ constexpr int kSxTileX = 64;  // a workgroup's 4 waves side by side in x
 constexpr int kSxTileY = 16;
 typedef float sx_f32x4 __attribute__((ext_vector_type(4)));
 
-// Split-FP16 matrix engine (engine 3, precision split_fp16): all three filters
-// are Toeplitz products on v_mfma_f32_16x16x32_f16, whose rate is 16 times
-// that of the FP32 matrix instruction. Every operand v is carried as two FP16
+// Split-FP16 matrix engine (engine 3, precision split_fp16): the x and y
+// filters (and, in z mode 0, the z filter) are Toeplitz products on
+// v_mfma_f32_16x16x32_f16, whose rate is 16 times that of the FP32 matrix
+// instruction. Every operand v is carried as two FP16
 // parts after an exact power-of-two scale S: hi = f16(v S) rounded toward
 // zero, lo = f16(v S - hi) rounded to nearest (21-22 significant bits), and a
 // product is three MFMAs (lo*hi, hi*lo, hi*hi) into one FP32 accumulator.
@@ -39,12 +40,15 @@ typedef float sx_f32x4 __attribute__((ext_vector_type(4)));
 //      two row tiles cover the 16 + K - 1 rows the y filter needs;
 //   y  A = taps in the K order the x accumulators leave the rows in
 //      (lane group g holds rows 4g..4g+3 of both row tiles), B = x outputs;
-//   z  the y outputs (lane (g, n): rows 4g..4g+3 at column n) go, split, into
-//      a per-wave LDS ring of 32 planes, column-major (16 y + x, 32 FP16 slots
-//      and 4 pad halves per column: 8-byte reads of 8 slots hit distinct
-//      banks); every 16 planes a window of 32 ring planes gives 16 output
-//      planes: A = compensated taps, B = 8 consecutive ring slots of a column
-//      per lane group, one MFMA triple per tile row.
+//   z  mode 1 (default): the y outputs (lane (g, n): rows 4g..4g+3 at column
+//      n), rescaled to their true values, enter a register ring of 16 planes
+//      and the z taps run as FP32 FMAs; no LDS, so two waves per SIMD fit.
+//      Mode 0: the y outputs go, split, into a per-wave LDS ring of 32 planes,
+//      column-major (16 y + x, 32 FP16 slots and 4 pad halves per column:
+//      8-byte reads of 8 slots hit distinct banks); every 16 planes a window
+//      of 32 ring planes gives 16 output planes: A = compensated taps, B = 8
+//      consecutive ring slots of a column per lane group, one MFMA triple per
+//      tile row (148 KB of LDS per workgroup: one wave per SIMD).
 // No barriers: each wave's LDS region is its own. Zero border only: loads of
 // 16-byte groups wholly outside the image read 0 through the buffer
 // resource's range check.
@@ -124,6 +128,16 @@ __device__ __forceinline__ int sx_select_below(int aValue, int aLimit, int aIfBe
 	return r;
 }
 
+// The same for an unsigned comparison (a negative value as unsigned is past
+// any limit below 2^31).
+__device__ __forceinline__ int sx_select_below_u(unsigned aValue, unsigned aLimit, int aIfBelow, int aOtherwise)
+{
+	int r;
+	asm("s_cmp_lt_u32 %1, %2\n\ts_cselect_b32 %0, %3, %4"
+	    : "=s"(r) : "s"(aValue), "s"(aLimit), "s"(aIfBelow), "s"(aOtherwise) : "scc");
+	return r;
+}
+
 // Uniform integer -> scalar register.
 __device__ __forceinline__ int sx_uniform(int aValue)
 {
@@ -138,12 +152,16 @@ __device__ __forceinline__ int sx_exponent(float aValue)
 	return aValue == 0.0f ? 0 : e;
 }
 
-template<int K>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void sep3_split_kernel(sep_args a)
+// Z: where the z filter runs. 0: split FP16 MFMA windows over the per-wave LDS
+// ring (above). 1: a register ring of 16 y-filtered planes rescaled to their
+// true values, z taps as FP32 FMAs on the vector ALUs; no LDS, so W waves per
+// SIMD can share a CU.
+template<int K, int Z, int W>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))) void sep3_split_kernel(sep_args a)
 {
 	constexpr int C = K / 2;
 	constexpr int P = 4;  // planes of loads in flight (divides the 16-plane block)
-	__shared__ unsigned lds[4 * kSxWaveDwords];
+	__shared__ unsigned lds[Z == 0 ? 4 * kSxWaveDwords : 1];
 
 	const unsigned block = xcd_remap(blockIdx.x, gridDim.x);
 	const unsigned tile_x = block % a.tiles_x;
@@ -357,6 +375,55 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 	sx_f32x4 dy = y_stage(dx);
 	x_stage(1, 1, dx);
 	sx_f32x4 dy_even = zero4;
+	// register ring (Z == 1): plane p's y output at its true scale in slot p mod 16
+	float ring[16][4];
+	float kz[K];
+	unsigned ring_lane[4];
+	if constexpr (Z == 1) {
+#pragma unroll
+		for (int j = 0; j < 16; ++j) {
+#pragma unroll
+			for (int jj = 0; jj < 4; ++jj) {
+				ring[j][jj] = 0.0f;
+			}
+		}
+#pragma unroll
+		for (int t = 0; t < K; ++t) {
+			kz[t] = __int_as_float(sx_uniform(__float_as_int(load_as_float(a.taps[2] + t * a.tap_stride[2], a.tap_type[2]))));
+		}
+		// this lane's outputs: rows y0 + 4g + jj at column x0 + n
+#pragma unroll
+		for (int jj = 0; jj < 4; ++jj) {
+			const bool ok = x0 + n < a.nx && y0 + 4 * g + jj < a.ny;
+			ring_lane[jj] = ok ? static_cast<unsigned>((x0 + n) * a.dx + (4 * g + jj) * a.dy) : static_cast<unsigned>(dst_bytes);
+		}
+	}
+	// y output of plane aPlane -> true scale (2^(e - 141 + s_x - b_y)), into the
+	// ring; the z filter of output plane aPlane - 2C (newest plane with tap 0)
+	auto ring_stage = [&](int aPlane, int aSlot, const sx_f32x4& aDy) {
+		const int e = __builtin_amdgcn_readlane(sigv, aPlane & 63);
+		const int shift = e - 141 + s[0] - b[1];
+#pragma unroll
+		for (int jj = 0; jj < 4; ++jj) {
+			ring[aSlot][jj] = ldexpf(aDy[jj], shift);
+		}
+		float zout[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+		for (int t = K - 1; t >= 0; --t) {
+#pragma unroll
+			for (int jj = 0; jj < 4; ++jj) {
+				zout[jj] = fmaf(ring[(aSlot - t + 16) % 16][jj], kz[t], zout[jj]);
+			}
+		}
+		const int o = aPlane - 2 * C;  // output plane of the chunk
+		const int soff = sx_select_below_u(static_cast<unsigned>(o), static_cast<unsigned>(zend - z0),
+			o * static_cast<int>(a.dz) + y0 * static_cast<int>(a.dy), static_cast<int>(dst_bytes));
+#pragma unroll
+		for (int jj = 0; jj < 4; ++jj) {
+			__builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(zout[jj]), out_res, static_cast<int>(ring_lane[jj]),
+				soff, 0);
+		}
+	};
 	for (int k = 0; k <= windows; ++k) {
 #pragma unroll
 		for (int q = 0; q < 16; ++q) {
@@ -364,17 +431,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 			sx_f32x4 dx_next[2];
 			x_stage(p + 2, (q + 2) % P, dx_next);
 			const sx_f32x4 dy_next = y_stage(dx);
-			if (q & 1) {
-				z_stage(p, dy_even, dy);
+			if constexpr (Z == 0) {
+				if (q & 1) {
+					z_stage(p, dy_even, dy);
+				} else {
+					dy_even = dy;
+				}
 			} else {
-				dy_even = dy;
+				ring_stage(p, q, dy);
 			}
 			dy = dy_next;
 			dx[0] = dx_next[0];
 			dx[1] = dx_next[1];
 		}
-		if (k > 0) {
-			z_window(k - 1);
+		if constexpr (Z == 0) {
+			if (k > 0) {
+				z_window(k - 1);
+			}
 		}
 	}
 }
@@ -385,10 +458,14 @@ void launch_split_k(sep_args& a, hipStream_t aStream)
 	a.tiles_x = ceil_div(a.nx, kSxTileX);
 	a.tiles_y = ceil_div(a.ny, kSxTileY);
 	const long long tiles = static_cast<long long>(a.tiles_x) * a.tiles_y;
+	// z on the register ring by default (53 vs 57 us on the 15^3 headline: the
+	// LDS windows' store bursts and single wave per SIMD cost more than the z FMAs)
+	const int zmode = tune_get("sep3.split_z", 1) == 0 ? 0 : 1;
+	const int waves = zmode == 1 && tune_get("sep3.waves", 1) == 2 ? 2 : 1;
 	int chunk = tune_get("sep3.chunk", 0);
 	if (chunk <= 0) {
-		// one workgroup per CU (the LDS ring takes 148 KB), long chunks
-		const long long chunks = std::max<long long>(1, (256 + tiles - 1) / tiles);
+		// one workgroup per CU and wave slot (the LDS ring takes 148 KB): long chunks
+		const long long chunks = std::max<long long>(1, (256 * waves + tiles - 1) / tiles);
 		chunk = static_cast<int>(ceil_div(a.nz, chunks));
 	}
 	chunk = std::max(16, (chunk + 15) / 16 * 16);  // whole 16-plane windows
@@ -398,7 +475,13 @@ void launch_split_k(sep_args& a, hipStream_t aStream)
 	a.chunk = chunk;
 	a.chunks_z = ceil_div(a.nz, chunk);
 	const unsigned nb = static_cast<unsigned>(tiles * a.chunks_z);
-	sep3_split_kernel<K><<<nb, 256, 0, aStream>>>(a);
+	if (zmode == 0) {
+		sep3_split_kernel<K, 0, 1><<<nb, 256, 0, aStream>>>(a);
+	} else if (waves == 2) {
+		sep3_split_kernel<K, 1, 2><<<nb, 256, 0, aStream>>>(a);
+	} else {
+		sep3_split_kernel<K, 1, 1><<<nb, 256, 0, aStream>>>(a);
+	}
 }
 
 } // namespace

@@ -8,7 +8,9 @@ FP64 oracle (a separable kernel with the zero border factors over axes, so the
 oracle is the dense outer-product kernel or three 1-D passes):
     max|gpu - oracle64| <= TOL * max|oracle64|,   TOL = 1e-5.
 The scale handling is exercised with planes of very different magnitude,
-very small and very large images, tiny and huge taps, and all-zero inputs.
+very small and very large images, tiny and huge taps, and all-zero inputs;
+every case runs with both z filters (register ring, default; split MFMA
+windows over an LDS ring).
 """
 from __future__ import annotations
 
@@ -23,6 +25,14 @@ from tests import cases
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-5
+
+
+@pytest.fixture(autouse=True, params=[1, 0], ids=["ring_z", "mfma_z"])
+def z_mode(request):
+    """Every case runs with the z filter on the register ring (default) and on
+    split MFMA windows over the LDS ring."""
+    with autotune.applied({"sep3.split_z": request.param}):
+        yield request.param
 
 
 def dev(a: np.ndarray):
@@ -172,3 +182,19 @@ def test_split_full_size_headline_vs_fp64_oracle():
     l2 = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
     print(f"split separable 15^3 at 256^3 vs FP64 oracle: max {err:.2e}, normwise {l2:.2e}")
     assert err <= TOL
+
+
+@pytest.mark.parametrize("waves,chunk", [(1, 0), (2, 0), (2, 32), (1, 48)])
+@pytest.mark.parametrize("k", [3, 9, 15])
+def test_split_register_ring_z(k, waves, chunk):
+    """The same engine with the z filter on a register ring of y-filtered planes
+    (FP32 vector FMAs, no LDS; one or two waves per SIMD), ragged tiles and planes
+    of different magnitude."""
+    rng = np.random.default_rng(200 + k)
+    expo = rng.integers(-20, 21, 41)
+    src = ((rng.random((41, 37, 68)) - 0.4) * np.exp2(expo)[:, None, None]).astype(np.float32)
+    taps = [rng.standard_normal(k).astype(np.float32) for _ in range(3)]
+    with autotune.applied({"sep3.split_z": 1, "sep3.waves": waves, "sep3.chunk": chunk}):
+        got = host(run_split(src, taps))
+    assert np.isfinite(got).all()
+    assert rel_err(got, separable_ref(src, taps)) <= TOL
