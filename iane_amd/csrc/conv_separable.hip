@@ -5,8 +5,9 @@
 // axis independently and the zero rule zeroes a sample when any axis is out of
 // range, and both factor over axes.
 //
-// Fast path (rank 3, FP32 accumulation, equal odd tap lengths K <= 15): one
-// fused kernel. A workgroup owns a 64 x 16 (x, y) output tile and streams a
+// Fast path (rank 2 or 3, F32 images, FP32 or SPLIT_FP16 arithmetic, odd tap
+// lengths up to 15, each centred in the window K of the longest; a rank-2
+// image is one plane with the unit kernel along z): one fused kernel. A workgroup owns a 64 x 16 (x, y) output tile and streams a
 // chunk of z planes. For every input plane it stages the plane region with a
 // K-1 halo in x and y into LDS, filters it along x and y, and pushes the
 // filtered plane into a per-thread register ring of K planes from which the z
@@ -79,9 +80,9 @@ __global__ __launch_bounds__(kSepThreads) void sep3_fused_kernel(sep_args a)
 	float kx[K], ky[K], kz[K];
 #pragma unroll
 	for (int i = 0; i < K; ++i) {
-		kx[i] = uniform_float(load_as_float(a.taps[0] + i * a.tap_stride[0], a.tap_type[0]));
-		ky[i] = uniform_float(load_as_float(a.taps[1] + i * a.tap_stride[1], a.tap_type[1]));
-		kz[i] = uniform_float(load_as_float(a.taps[2] + i * a.tap_stride[2], a.tap_type[2]));
+		kx[i] = uniform_float(separable::sep_tap<K>(a, 0, i));
+		ky[i] = uniform_float(separable::sep_tap<K>(a, 1, i));
+		kz[i] = uniform_float(separable::sep_tap<K>(a, 2, i));
 	}
 
 	// Staging offsets of this thread's elements (same for every plane).
@@ -322,10 +323,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))) voi
 
 	// MFMA operands built from the taps (per lane, constant over the run)
 	float bx[8], ay[8], kz[K];
-	const auto tap = [&](int aAxis, int aIndex) {
-		return (aIndex >= 0 && aIndex < K) ? load_as_float(a.taps[aAxis] + aIndex * a.tap_stride[aAxis], a.tap_type[aAxis])
-		                                   : 0.0f;
-	};
+	const auto tap = [&](int aAxis, int aIndex) { return separable::sep_tap<K>(a, aAxis, aIndex); };
 #pragma unroll
 	for (int s = 0; s < 8; ++s) {
 		bx[s] = tap(0, n + 2 * C - (4 * s + g));
@@ -333,7 +331,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))) voi
 	}
 #pragma unroll
 	for (int i = 0; i < K; ++i) {
-		kz[i] = uniform_float(load_as_float(a.taps[2] + i * a.tap_stride[2], a.tap_type[2]));
+		kz[i] = uniform_float(separable::sep_tap<K>(a, 2, i));
 	}
 
 	// The plane loop below has no branch but its exit: every condition is an
@@ -510,11 +508,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))) voi
 // chains interleave: iteration p runs the x filter of plane p (two row tiles)
 // and the y filter of plane p - 1 (two partial sums), while the vector ALUs run
 // the z filter that completes output plane p - 2 - (K - 1) from the ring.
-template<int K, int W>
+template<int K, int W, int KZ>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))) void sep3_direct_kernel(sep_args a)
 {
-	constexpr int C = K / 2;
-	constexpr int R = K + 1;                                    // ring slots = iterations per unrolled group
+	constexpr int C = K / 2;     // x and y taps (window of K)
+	constexpr int CZ = KZ / 2;   // z taps: K, or 1 for a single plane
+	constexpr int R = KZ + 1;                                    // ring slots = iterations per unrolled group
 	constexpr int P = (R % 4 == 0) ? 4 : ((R % 3 == 0) ? 3 : 2);  // planes of loads in flight (P divides R)
 	const unsigned block = xcd_remap(blockIdx.x, gridDim.x);
 	const unsigned tile_x = block % a.tiles_x;
@@ -530,21 +529,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))) voi
 	const int z0 = static_cast<int>(chunk) * a.chunk;
 	const int zend = min(z0 + a.chunk, a.nz);
 
-	const auto tap = [&](int aAxis, int aIndex) {
-		return (aIndex >= 0 && aIndex < K) ? load_as_float(a.taps[aAxis] + aIndex * a.tap_stride[aAxis], a.tap_type[aAxis])
-		                                   : 0.0f;
-	};
+	const auto tap = [&](int aAxis, int aIndex) { return separable::sep_tap<K>(a, aAxis, aIndex); };
 	// window position of K-step s for lane group g (columns for x, rows for y)
 	const auto window = [&](int aStep) { return 16 * (aStep / 4) + 4 * g + aStep % 4; };
-	float bx[8], ay[8], kz[K];
+	float bx[8], ay[8], kz[KZ];
 #pragma unroll
 	for (int s = 0; s < 8; ++s) {
 		bx[s] = tap(0, n + 8 + C - window(s));
 		ay[s] = tap(1, n + 8 + C - window(s));
 	}
 #pragma unroll
-	for (int i = 0; i < K; ++i) {
-		kz[i] = uniform_float(load_as_float(a.taps[2] + i * a.tap_stride[2], a.tap_type[2]));
+	for (int i = 0; i < KZ; ++i) {
+		kz[i] = uniform_float(separable::sep_tap<KZ>(a, 2, i));
 	}
 	// the lane's four 16-byte groups per plane: row tile mt, half h (columns 16 h + 4 g .. + 3)
 	unsigned offset[2][2];
@@ -567,10 +563,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))) voi
 		                                               : static_cast<unsigned>(a.dst_span);
 	}
 
-	const int planes = (zend - z0) + K - 1;  // input planes z0 - C .. zend - 1 + C
+	const int planes = (zend - z0) + KZ - 1;  // input planes z0 - CZ .. zend - 1 + CZ
 	mx_f32x4 pre[P][2][2];
 	auto fetch = [&](int aPlane, int aSlot) {
-		const int zi = z0 - C + aPlane;
+		const int zi = z0 - CZ + aPlane;
 		const bool in = aPlane < planes && zi >= 0 && zi < a.nz;
 		const auto plane = plane_resource(a.src + (in ? zi : 0) * a.sz, in ? a.src_span : 0);
 #pragma unroll
@@ -583,7 +579,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))) voi
 		}
 	};
 	auto z_store = [&](int aPlane, const float* aOut) {
-		const int z = z0 + aPlane - (K - 1);
+		const int z = z0 + aPlane - (KZ - 1);
 		const bool ok = z >= z0 && z < zend;
 		const auto plane = plane_resource(a.dst + (ok ? z : 0) * a.dz, ok ? a.dst_span : 0);
 #pragma unroll
@@ -592,7 +588,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))) voi
 				static_cast<int>(store_offset[r]), 0, 0);
 		}
 	};
-	float ring[R][4];  // y-filtered planes by plane index mod R (K of them feed an output)
+	float ring[R][4];  // y-filtered planes by plane index mod R (KZ of them feed an output)
 #pragma unroll
 	for (int j = 0; j < R; ++j) {
 #pragma unroll
@@ -639,7 +635,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))) voi
 					u1 = mfma_f32_16x16x4(ay[s], t_prev[1][s - 4], u1);
 				}
 #pragma unroll
-				for (int q = 2 * s; q < 2 * s + 2 && q < K; ++q) {
+				for (int q = 2 * s; q < 2 * s + 2 && q < KZ; ++q) {
 					const int ring_slot = ((zr - q) % R + R) % R;
 #pragma unroll
 					for (int r = 0; r < 4; ++r) {
@@ -742,16 +738,16 @@ void launch_mfma(sep_args& a, hipStream_t aStream)
 	}
 }
 
-template<int K>
+template<int K, int KZ>
 void launch_direct(sep_args& a, hipStream_t aStream)
 {
 	unsigned nb = plan_tiles(a, kMxTileX, kMxTileY);
 	if (tune_get("sep3.chunk", 0) <= 0) {
-		// Long z chunks cut the halo (each chunk filters K - 1 extra planes in x
+		// Long z chunks cut the halo (each chunk filters KZ - 1 extra planes in x
 		// and y); one workgroup per CU (256 of them, one wave per SIMD) is
-		// enough for this kernel. The chunk is a multiple of K + 1, so the
-		// unrolled groups of K + 1 iterations have no idle iterations.
-		constexpr int R = K + 1;
+		// enough for this kernel. The chunk is a multiple of KZ + 1, so the
+		// unrolled groups of KZ + 1 iterations have no idle iterations.
+		constexpr int R = KZ + 1;
 		const long long tiles = static_cast<long long>(a.tiles_x) * a.tiles_y;
 		const long long chunks = std::max<long long>(1, (256 + tiles - 1) / tiles);
 		int chunk = static_cast<int>(ceil_div(a.nz, chunks));
@@ -761,9 +757,20 @@ void launch_direct(sep_args& a, hipStream_t aStream)
 		nb = static_cast<unsigned>(tiles * a.chunks_z);
 	}
 	if (tune_get("sep3.waves", 1) == 2) {
-		sep3_direct_kernel<K, 2><<<nb, 256, 0, aStream>>>(a);
+		sep3_direct_kernel<K, 2, KZ><<<nb, 256, 0, aStream>>>(a);
 	} else {
-		sep3_direct_kernel<K, 1><<<nb, 256, 0, aStream>>>(a);
+		sep3_direct_kernel<K, 1, KZ><<<nb, 256, 0, aStream>>>(a);
+	}
+}
+
+// A single-plane image (rank 2) runs with a one-tap z filter.
+template<int K>
+void launch_direct_rank(sep_args& a, bool aPlanar, hipStream_t aStream)
+{
+	if (aPlanar) {
+		launch_direct<K, 1>(a, aStream);
+	} else {
+		launch_direct<K, K>(a, aStream);
 	}
 }
 
@@ -789,17 +796,23 @@ void launch_fused(sep_args& a, hipStream_t aStream, int aThreads)
 
 } // namespace
 
-// The one-kernel path: rank 3, FP32 images and arithmetic, equal tap lengths
-// up to 15, staged offsets within a plane kept in 32-bit registers.
+// The single-kernel path's tap window: the longest tap vector (all are odd).
+int separable_window(const view4* aTaps, int aN)
+{
+	int k = 1;
+	for (int i = 0; i < aN; ++i) {
+		k = std::max(k, aTaps[i].extent[0]);
+	}
+	return k;
+}
+
+// The one-kernel path: rank 2 or 3, F32 images, FP32 or SPLIT_FP16 arithmetic,
+// tap lengths up to 15, staged offsets within a plane kept in 32-bit registers.
 bool separable_fused(const view4& aSrc, const view4* aTaps, int aN, const view4& aDst, int aPrecision)
 {
-	bool fused = aSrc.rank == 3 && (aPrecision == IANE_PREC_FP32 || aPrecision == IANE_PREC_SPLIT_FP16) &&
-		aSrc.type == IANE_F32 && aDst.type == IANE_F32;
-	const int k = aTaps[0].extent[0];
-	for (int i = 0; i < aN; ++i) {
-		fused = fused && aTaps[i].extent[0] == k;
-	}
-	fused = fused && k <= 15;
+	const bool fused = (aSrc.rank == 3 || aSrc.rank == 2) &&
+		(aPrecision == IANE_PREC_FP32 || aPrecision == IANE_PREC_SPLIT_FP16) && aSrc.type == IANE_F32 &&
+		aDst.type == IANE_F32 && separable_window(aTaps, aN) <= 15;
 	return fused && (static_cast<long long>(aSrc.extent[0]) + 64) * std::llabs(aSrc.stride[0]) +
 		(static_cast<long long>(aSrc.extent[1]) + 64) * std::llabs(aSrc.stride[1]) < (1LL << 31);
 }
@@ -821,14 +834,19 @@ void convolve_separable(const view4& aSrc, const view4* aTaps, int aN, const vie
 	int aPrecision, void* aWorkspace, hipStream_t aStream, bool aAllowFused)
 {
 	const int rank = aSrc.rank;
-	const int k = aTaps[0].extent[0];
+	const int k = separable_window(aTaps, aN);
 	const bool fused = aAllowFused && separable_fused(aSrc, aTaps, aN, aDst, aPrecision);
 	if (fused) {
+		// taps of unequal (odd) lengths are centred in the window of the longest;
+		// a rank-2 image is one plane with the unit kernel along z
+		const bool planar = rank == 2;
 		sep_args a{};
 		for (int axis = 0; axis < 3; ++axis) {
-			a.taps[axis] = aTaps[axis].ptr;
-			a.tap_stride[axis] = aTaps[axis].stride[0];
-			a.tap_type[axis] = aTaps[axis].type;
+			const bool has = axis < aN;
+			a.taps[axis] = has ? aTaps[axis].ptr : nullptr;
+			a.tap_stride[axis] = has ? aTaps[axis].stride[0] : 0;
+			a.tap_type[axis] = has ? aTaps[axis].type : IANE_F32;
+			a.tap_len[axis] = has ? aTaps[axis].extent[0] : 1;
 		}
 		a.src = aSrc.ptr;
 		a.src_type = aSrc.type;
@@ -844,6 +862,13 @@ void convolve_separable(const view4& aSrc, const view4* aTaps, int aN, const vie
 		a.dy = aDst.stride[1];
 		a.dz = aDst.stride[2];
 		a.border = aBorder;
+		if (planar) {
+			// one plane: give the absent z axis a stride past the plane so plane
+			// arithmetic stays in range (only plane 0 is ever addressed)
+			a.sz = (static_cast<long long>(a.nx) - 1) * a.sx + (static_cast<long long>(a.ny) - 1) * a.sy + 4;
+			a.sz = (a.sz + 15) / 16 * 16;
+			a.dz = (static_cast<long long>(a.nx) - 1) * a.dx + (static_cast<long long>(a.ny) - 1) * a.dy + 4;
+		}
 		// Engines (all FP32 products with FP32 sums, in different orders):
 		//   2 (default where it applies): barrier-free, x and y filters on FP32
 		//     MFMA with operands loaded per wave, long z chunks;
@@ -861,7 +886,7 @@ void convolve_separable(const view4& aSrc, const view4* aTaps, int aN, const vie
 			reinterpret_cast<uintptr_t>(a.src) % 16 == 0 && a.border == IANE_BORDER_ZERO;
 		// engine 3 (split FP16 operands, every filter on the matrix cores) also
 		// addresses a chunk of output planes from one base: planes must not overlap
-		if (aPrecision == IANE_PREC_SPLIT_FP16 && matrix && grouped && a.dz >= dst_span) {
+		if (aPrecision == IANE_PREC_SPLIT_FP16 && !planar && matrix && grouped && a.dz >= dst_span) {
 			a.src_span = static_cast<int>(src_span);
 			a.dst_span = static_cast<int>(dst_span);
 			if (separable::split_chunk_fits(a, 16)) {
@@ -874,14 +899,14 @@ void convolve_separable(const view4& aSrc, const view4* aTaps, int aN, const vie
 			a.src_span = static_cast<int>(src_span);
 			a.dst_span = static_cast<int>(dst_span);
 			switch (k) {
-			case 1: launch_direct<1>(a, aStream); break;
-			case 3: launch_direct<3>(a, aStream); break;
-			case 5: launch_direct<5>(a, aStream); break;
-			case 7: launch_direct<7>(a, aStream); break;
-			case 9: launch_direct<9>(a, aStream); break;
-			case 11: launch_direct<11>(a, aStream); break;
-			case 13: launch_direct<13>(a, aStream); break;
-			default: launch_direct<15>(a, aStream); break;
+			case 1: launch_direct_rank<1>(a, planar, aStream); break;
+			case 3: launch_direct_rank<3>(a, planar, aStream); break;
+			case 5: launch_direct_rank<5>(a, planar, aStream); break;
+			case 7: launch_direct_rank<7>(a, planar, aStream); break;
+			case 9: launch_direct_rank<9>(a, planar, aStream); break;
+			case 11: launch_direct_rank<11>(a, planar, aStream); break;
+			case 13: launch_direct_rank<13>(a, planar, aStream); break;
+			default: launch_direct_rank<15>(a, planar, aStream); break;
 			}
 			check_hip(hipGetLastError(), "Separable convolution launch");
 			return;

@@ -188,18 +188,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))) voi
 	for (int axis = 0; axis < 3; ++axis) {
 		float wmax = 0.0f, wsum = 0.0f;
 		for (int t = 0; t < K; ++t) {
-			const float w = fabsf(load_as_float(a.taps[axis] + t * a.tap_stride[axis], a.tap_type[axis]));
+			const float w = fabsf(sep_tap<K>(a, axis, t));
 			wmax = fmaxf(wmax, w);
 			wsum += w;
 		}
 		b[axis] = sx_uniform(wmax == 0.0f ? 0 : 15 - sx_exponent(wmax));
 		s[axis] = sx_uniform(sx_exponent(wsum));
 	}
-	const auto tap = [&](int aAxis, int aIndex) {
-		return (aIndex >= 0 && aIndex < K)
-			? ldexpf(load_as_float(a.taps[aAxis] + aIndex * a.tap_stride[aAxis], a.tap_type[aAxis]), b[aAxis])
-			: 0.0f;
-	};
+	const auto tap = [&](int aAxis, int aIndex) { return ldexpf(sep_tap<K>(a, aAxis, aIndex), b[aAxis]); };
 	const float c1 = __int_as_float(sx_uniform(__float_as_int(ldexpf(1.0f, -b[0] - s[0]))));
 	const float c2 = __int_as_float(sx_uniform(__float_as_int(ldexpf(1.0f, -b[1] - s[1]))));
 	const int rescale = b[2] - s[0] - s[1] + 141;  // output = D * 2^(E - rescale)
@@ -389,7 +385,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))) voi
 		}
 #pragma unroll
 		for (int t = 0; t < K; ++t) {
-			kz[t] = __int_as_float(sx_uniform(__float_as_int(load_as_float(a.taps[2] + t * a.tap_stride[2], a.tap_type[2]))));
+			kz[t] = __int_as_float(sx_uniform(__float_as_int(sep_tap<K>(a, 2, t))));
 		}
 		// this lane's outputs: rows y0 + 4g + jj at column x0 + n
 #pragma unroll

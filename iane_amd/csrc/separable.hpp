@@ -20,9 +20,10 @@ struct sep_args {
 	int border;
 	unsigned tiles_x, tiles_y, chunks_z;
 	int chunk;                  // output planes per workgroup
-	const char* taps[3];        // device tap vectors
+	const char* taps[3];        // device tap vectors (null: the unit kernel, a 2-D image's z axis)
 	long long tap_stride[3];
 	int tap_type[3];
+	int tap_len[3];             // odd tap counts, each at most the engine's compile-time length
 	int src_span, dst_span;     // bytes of one source / destination plane that a launch addresses (matrix engine)
 };
 
@@ -32,6 +33,25 @@ struct sep_args {
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_resource(const char* aBase, int aBytes)
 {
 	return __builtin_amdgcn_make_buffer_rsrc(const_cast<char*>(aBase), static_cast<short>(0), aBytes, 0x00020000);
+}
+
+// Tap aIndex of axis aAxis in an engine's window of KA taps (KA odd, at least
+// the axis' odd tap count): the axis' taps centred in the window, zero around
+// them (a shorter kernel is the longer one with zero outer taps, so every axis
+// can use the engine's one compile-time length); a null tap pointer is the unit
+// kernel (1 at the centre).
+template<int KA>
+__device__ __forceinline__ float sep_tap(const sep_args& a, int aAxis, int aIndex)
+{
+	if (aIndex < 0 || aIndex >= KA) {
+		return 0.0f;
+	}
+	if (a.taps[aAxis] == nullptr) {
+		return aIndex == KA / 2 ? 1.0f : 0.0f;
+	}
+	const int i = aIndex - (KA - a.tap_len[aAxis]) / 2;
+	return (i >= 0 && i < a.tap_len[aAxis]) ? load_as_float(a.taps[aAxis] + i * a.tap_stride[aAxis], a.tap_type[aAxis])
+	                                        : 0.0f;
 }
 
 // Split-FP16 engine (conv_separable_split.hip): output chunks addressed with

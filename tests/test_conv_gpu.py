@@ -461,3 +461,42 @@ def test_streaming_kernel_equals_tiled(shape, kshape, border):
         if prec == precision.reference:
             same_bits(outs[0], oracle.convolve(src, ker, ddt, border))
 
+
+
+@pytest.mark.parametrize("lengths", [(3, 7, 15), (15, 5, 1), (1, 13, 9), (11, 11, 3)])
+@pytest.mark.parametrize("border", [cases.CLAMP, cases.ZERO])
+@pytest.mark.parametrize("engine", [2, 0, 1], ids=["direct", "staged", "vector"])
+def test_separable_unequal_tap_lengths(lengths, border, engine):
+    """Tap vectors of different odd lengths (x, y, z) run on the single-kernel
+    engines, each centred in the window of the longest (anisotropic Gaussians)."""
+    from iane_amd import autotune
+    rng = np.random.default_rng(sum(lengths))
+    src = (rng.random((23, 37, 68)) - 0.2).astype(np.float32)
+    taps = [rng.standard_normal(k).astype(np.float32) for k in lengths]
+    t64 = [t.astype(np.float64) for t in taps]
+    dense = np.multiply.outer(np.multiply.outer(t64[2], t64[1]), t64[0])
+    ref64 = oracle.convolve(src, dense, np.float64, border, threads=8)
+    out = empty(src.shape, np.float32)
+    with autotune.applied({"sep3.engine": engine}):
+        iane_amd.convolve_separable(dev(src), [dev(t) for t in taps], out,
+                                    convolution_options(border_mode(border), precision.fp32)).wait()
+    assert rel_err(host(out), ref64) <= TOL_FP32
+
+
+@pytest.mark.parametrize("shape", [(1024, 1024), (45, 72), (300, 20), (7, 4), (64, 130)])
+@pytest.mark.parametrize("lengths", [(7, 7), (15, 3), (1, 9)])
+@pytest.mark.parametrize("border", [cases.CLAMP, cases.ZERO])
+def test_separable_rank2(shape, lengths, border):
+    """2-D images on the single-kernel engines (one plane, a one-tap z filter)."""
+    rng = np.random.default_rng(shape[0] + 7 * lengths[0])
+    src = rng.random(shape).astype(np.float32)
+    taps = [rng.random(k).astype(np.float32) for k in lengths]
+    dense = np.multiply.outer(taps[1].astype(np.float64), taps[0].astype(np.float64))
+    ref64 = oracle.convolve(src, dense, np.float64, border, threads=8)
+    for engine in (2, 0, 1):
+        from iane_amd import autotune
+        out = empty(src.shape, np.float32)
+        with autotune.applied({"sep3.engine": engine}):
+            iane_amd.convolve_separable(dev(src), [dev(t) for t in taps], out,
+                                        convolution_options(border_mode(border), precision.fp32)).wait()
+        assert rel_err(host(out), ref64) <= TOL_FP32, engine

@@ -198,3 +198,23 @@ def test_split_register_ring_z(k, waves, chunk):
         got = host(run_split(src, taps))
     assert np.isfinite(got).all()
     assert rel_err(got, separable_ref(src, taps)) <= TOL
+
+
+@pytest.mark.parametrize("lengths", [(3, 7, 15), (15, 5, 1), (9, 1, 13)])
+def test_split_unequal_tap_lengths(lengths):
+    rng = np.random.default_rng(sum(lengths) + 1)
+    src = (rng.random((40, 33, 64)) - 0.3).astype(np.float32)
+    taps = [rng.standard_normal(k).astype(np.float32) for k in lengths]
+    got = host(run_split(src, taps))
+    assert rel_err(got, separable_ref(src, taps)) <= TOL
+
+
+def test_split_rank2_runs_the_fp32_engines():
+    rng = np.random.default_rng(9)
+    src = rng.random((50, 64)).astype(np.float32)
+    taps = [rng.random(7).astype(np.float32), rng.random(5).astype(np.float32)]
+    out = dev(np.full(src.shape, 77, np.float32))
+    iane_amd.convolve_separable(dev(src), [dev(t) for t in taps], out,
+                                convolution_options(border_mode.zero, precision.split_fp16)).wait()
+    dense = np.multiply.outer(taps[1].astype(np.float64), taps[0].astype(np.float64))
+    assert rel_err(host(out), oracle.convolve(src, dense, np.float64, oracle.ZERO)) <= TOL
