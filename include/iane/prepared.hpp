@@ -8,15 +8,18 @@
 // submission.
 // fft_forward / fft_inverse / rfft / irfft / rfft_extent / fft_complex mirror
 // include/iane/algorithms/fft/{fft.hpp:196-231, rfft.hpp:107-161, layout.hpp:15-21,
-// complex.hpp:6-9} for device views of rank 1..3, on the Stockham FFT of
-// libiane_hip.so. (The reference's host FFT is FFTW; host views are not
-// provided here.)
+// complex.hpp:6-9} for views of rank 1..3: device views on the Stockham FFT of
+// libiane_hip.so, host views on the header's own FFT (include/iane/host_fft.hpp,
+// in the role of the reference's FFTW plans).
 #pragma once
 
 #include <iane/core.hpp>
+#include <iane/host_fft.hpp>
 
+#include <array>
 #include <atomic>
 #include <cstring>
+#include <memory>
 #include <optional>
 
 namespace iane {
@@ -235,7 +238,7 @@ struct fft_plan_deleter {
 };
 
 template<class V>
-std::size_t check_packed_fft_view(const V& aView)
+std::size_t check_packed_fft_view(const V& aView, std::size_t aAlignment = alignof(fft_complex))
 {
 	using T = typename V::value_type;
 	const auto [count, strides] = resource::detail::packed_layout<T, V::dimension>(aView.dimensions());
@@ -246,11 +249,131 @@ std::size_t check_packed_fft_view(const V& aView)
 		}
 	}
 	const auto address = reinterpret_cast<std::uintptr_t>(aView.pointer());
-	if (address == 0 || address % alignof(fft_complex) != 0) {
+	if (address == 0 || address % aAlignment != 0) {
 		throw std::invalid_argument("FFT requires aligned input and output buffers");
 	}
 	return count * sizeof(T);
 }
+
+// The checks of fft.hpp:29-66 and rfft.hpp:29-45 for both backends: element
+// types, positive extents, the packed half spectrum of real transforms, packed
+// aligned views (host real buffers need float alignment, as for FFTW), and
+// disjoint buffers. Returns the real extents (1 past the rank).
+template<int Kind, class TSrc, class TDst>
+std::array<int32_t, 3> check_fft_views(const TSrc& aSource, const TDst& aDestination, std::size_t aRealAlignment)
+{
+	constexpr int N = TSrc::dimension;
+	using in_t = typename TSrc::value_type;
+	using out_t = typename TDst::value_type;
+	if constexpr (Kind == IANE_FFT_R2C) {
+		static_assert(std::same_as<std::remove_const_t<in_t>, float> && std::same_as<out_t, fft_complex>);
+	} else if constexpr (Kind == IANE_FFT_C2R) {
+		static_assert(std::same_as<std::remove_const_t<in_t>, fft_complex> && std::same_as<out_t, float>);
+	} else {
+		static_assert(std::same_as<std::remove_const_t<in_t>, fft_complex> && std::same_as<out_t, fft_complex>);
+	}
+	const auto real = Kind == IANE_FFT_C2R ? aDestination.dimensions() : aSource.dimensions();
+	const auto other = Kind == IANE_FFT_C2R ? aSource.dimensions() : aDestination.dimensions();
+	std::array<int32_t, 3> extent{1, 1, 1};
+	for (int i = 0; i < N; ++i) {
+		extent[i] = math::detail::axis(real, i);
+		if (extent[i] <= 0) {
+			throw std::invalid_argument(Kind >= IANE_FFT_R2C ? "Real FFT requires positive dimensions"
+			                                                 : "FFT requires positive dimensions");
+		}
+		const int expect = (Kind >= IANE_FFT_R2C && i == 0) ? extent[0] / 2 + 1 : extent[i];
+		if (math::detail::axis(other, i) != expect) {
+			throw std::invalid_argument(Kind >= IANE_FFT_R2C
+				? "Real FFT requires matching real and packed spectrum extents"
+				: "Source and destination require matching nonnegative dimensions");
+		}
+	}
+	const auto in_bytes = check_packed_fft_view(aSource, Kind == IANE_FFT_R2C ? aRealAlignment : alignof(fft_complex));
+	const auto out_bytes = check_packed_fft_view(aDestination,
+		Kind == IANE_FFT_C2R ? aRealAlignment : alignof(fft_complex));
+	const auto src = reinterpret_cast<std::uintptr_t>(aSource.pointer());
+	const auto dst = reinterpret_cast<std::uintptr_t>(aDestination.pointer());
+	if (src <= dst ? dst - src < in_bytes : src - dst < out_bytes) {
+		throw std::invalid_argument("FFT requires disjoint input and output buffers");
+	}
+	return extent;
+}
+
+// A prepared host FFT: the line transforms of each axis, reusable on new packed
+// buffers of the prepared shape. C2R works on a private copy of its input
+// spectrum in the context's scratch (rfft.hpp:70-82), so the caller's spectrum
+// stays as it was.
+template<int Kind, int N>
+struct host_fft_state {
+	std::array<int32_t, 3> extent;  // real extents
+	std::array<std::shared_ptr<const host_fft_line>, 3> lines;
+	std::size_t copy_bytes = 0;
+
+	std::size_t scratch_bytes() const { return copy_bytes; }
+
+	template<class TSrc, class TDst>
+	cpu_execution run(TSrc aSource, TDst aDestination, scratch_space<cpu_policy>& aScratch, cpu_policy) const
+	{
+		if (check_fft_views<Kind>(aSource, aDestination, alignof(float)) != extent) {
+			throw std::invalid_argument("FFT views must match the prepared shape");
+		}
+		constexpr bool real = Kind >= IANE_FFT_R2C;
+		const int n0 = extent[0];
+		int work[3] = {real ? n0 / 2 + 1 : n0, extent[1], extent[2]};
+		const std::size_t rows = static_cast<std::size_t>(work[1]) * work[2];
+		const std::size_t count = static_cast<std::size_t>(work[0]) * rows;
+		std::vector<host_complex> data(count);
+		if constexpr (Kind == IANE_FFT_C2R) {
+			auto* copy = static_cast<fft_complex*>(aScratch.bytes(copy_bytes, alignof(fft_complex)));
+			std::memcpy(copy, aSource.pointer(), copy_bytes);
+			for (std::size_t i = 0; i < count; ++i) {
+				data[i] = host_complex(copy[i].real, copy[i].imag);
+			}
+			for (int axis = N - 1; axis >= 1; --axis) {
+				host_fft_axis(data, work, N, axis, *lines[axis]);
+			}
+			// axis 0: the full Hermitian line of each half spectrum, inverse, real part
+			std::vector<host_complex> full(static_cast<std::size_t>(n0)), out(static_cast<std::size_t>(n0));
+			float* dst = aDestination.pointer();
+			for (std::size_t r = 0; r < rows; ++r) {
+				const host_complex* half = data.data() + r * work[0];
+				for (int k = 0; k < n0; ++k) {
+					full[k] = k < work[0] ? half[k] : std::conj(half[n0 - k]);
+				}
+				lines[0]->transform(full.data(), 1, out.data());
+				for (int t = 0; t < n0; ++t) {
+					dst[r * n0 + t] = static_cast<float>(out[t].real());
+				}
+			}
+		} else {
+			if constexpr (Kind == IANE_FFT_R2C) {
+				const float* src = aSource.pointer();
+				std::vector<host_complex> in(static_cast<std::size_t>(n0)), out(static_cast<std::size_t>(n0));
+				for (std::size_t r = 0; r < rows; ++r) {
+					for (int t = 0; t < n0; ++t) {
+						in[t] = host_complex(src[r * n0 + t], 0.0);
+					}
+					lines[0]->transform(in.data(), 1, out.data());
+					std::copy(out.begin(), out.begin() + work[0], data.begin() + static_cast<std::ptrdiff_t>(r * work[0]));
+				}
+			} else {
+				const fft_complex* src = aSource.pointer();
+				for (std::size_t i = 0; i < count; ++i) {
+					data[i] = host_complex(src[i].real, src[i].imag);
+				}
+				host_fft_axis(data, work, N, 0, *lines[0]);
+			}
+			for (int axis = 1; axis < N; ++axis) {
+				host_fft_axis(data, work, N, axis, *lines[axis]);
+			}
+			fft_complex* dst = aDestination.pointer();
+			for (std::size_t i = 0; i < count; ++i) {
+				dst[i] = fft_complex{static_cast<float>(data[i].real()), static_cast<float>(data[i].imag())};
+			}
+		}
+		return {};
+	}
+};
 
 // A prepared FFT: a plan for one shape, reusable on new packed buffers (fft.hpp:174-193).
 template<int Kind, int N>
@@ -285,42 +408,30 @@ struct fft_algorithm {
 	auto prepare(TSrc aSource, TDst aDestination, hip_policy) const
 	{
 		constexpr int N = TSrc::dimension;
-		using in_t = typename TSrc::value_type;
-		using out_t = typename TDst::value_type;
-		if constexpr (Kind == IANE_FFT_R2C) {
-			static_assert(std::same_as<in_t, float> && std::same_as<out_t, fft_complex>);
-		} else if constexpr (Kind == IANE_FFT_C2R) {
-			static_assert(std::same_as<in_t, fft_complex> && std::same_as<out_t, float>);
-		} else {
-			static_assert(std::same_as<in_t, fft_complex> && std::same_as<out_t, fft_complex>);
-		}
+		auto extent = check_fft_views<Kind>(aSource, aDestination, alignof(fft_complex));
 		const auto real = Kind == IANE_FFT_C2R ? aDestination.dimensions() : aSource.dimensions();
-		const auto other = Kind == IANE_FFT_C2R ? aSource.dimensions() : aDestination.dimensions();
-		int32_t extent[3] = {1, 1, 1};
-		for (int i = 0; i < N; ++i) {
-			extent[i] = math::detail::axis(real, i);
-			if (extent[i] <= 0) {
-				throw std::invalid_argument(Kind >= IANE_FFT_R2C ? "Real FFT requires positive dimensions"
-				                                                 : "FFT requires positive dimensions");
-			}
-			const int expect = (Kind >= IANE_FFT_R2C && i == 0) ? extent[0] / 2 + 1 : extent[i];
-			if (math::detail::axis(other, i) != expect) {
-				throw std::invalid_argument(Kind >= IANE_FFT_R2C
-					? "Real FFT requires matching real and packed spectrum extents"
-					: "Source and destination require matching nonnegative dimensions");
-			}
-		}
-		const auto in_bytes = check_packed_fft_view(aSource);
-		const auto out_bytes = check_packed_fft_view(aDestination);
-		const auto src = reinterpret_cast<std::uintptr_t>(aSource.pointer());
-		const auto dst = reinterpret_cast<std::uintptr_t>(aDestination.pointer());
-		if (src <= dst ? dst - src < in_bytes : src - dst < out_bytes) {
-			throw std::invalid_argument("FFT requires disjoint input and output buffers");
-		}
 		iane_fft_plan* raw = nullptr;
-		check_status(iane_hip_fft_plan_create(&raw, N, extent, Kind));
+		check_status(iane_hip_fft_plan_create(&raw, N, extent.data(), Kind));
 		fft_state<Kind, N> state{real, std::unique_ptr<iane_fft_plan, fft_plan_deleter>(raw), 0};
 		check_status(iane_hip_fft_plan_workspace_bytes(raw, &state.workspace));
+		return state;
+	}
+
+	template<class TSrc, class TDst>
+	requires(concepts::HostView<TSrc> && concepts::HostView<TDst> && concepts::MemoryView<TSrc> &&
+		concepts::MemoryView<TDst> && TSrc::dimension == TDst::dimension && TSrc::dimension <= 3)
+	auto prepare(TSrc aSource, TDst aDestination, cpu_policy) const
+	{
+		constexpr int N = TSrc::dimension;
+		host_fft_state<Kind, N> state{check_fft_views<Kind>(aSource, aDestination, alignof(float)), {}, 0};
+		constexpr bool inverse = Kind == IANE_FFT_C2C_INVERSE || Kind == IANE_FFT_C2R;
+		for (int axis = 0; axis < N; ++axis) {
+			state.lines[axis] = std::make_shared<const host_fft_line>(state.extent[axis], inverse);
+		}
+		if constexpr (Kind == IANE_FFT_C2R) {
+			state.copy_bytes = static_cast<std::size_t>(state.extent[0] / 2 + 1) * state.extent[1] * state.extent[2] *
+				sizeof(fft_complex);
+		}
 		return state;
 	}
 };

@@ -315,6 +315,180 @@ void real_fft_known_answers(iane::math::vec2i aSize)
 	}
 }
 
+// The same FFT cases on host views (the reference's FFTW path): prepared_cases.hpp:73-123
+// known answers with a round trip, test_prepared.cpp:41-61 (a plan reused on new buffers,
+// aliasing rejected at run time, no scratch), and real_fft_cases.hpp:12-77.
+void host_fft_known_answers()
+{
+	using coord_t = iane::math::vec2i;
+	const coord_t size{4, 2};
+	iane::host_image<iane::fft_complex, 2> input(size), output(size), round(size);
+	for (int i = 0; i < 8; ++i) {
+		input.pointer()[i] = {0.0f, 0.0f};
+	}
+	input.pointer()[1] = {1.0f, 0.0f};
+	auto context = iane::make_execution_context();
+	auto forward = context.prepare(iane::fft_forward, input.const_view(), output.view());
+	auto inverse = context.prepare(iane::fft_inverse, output.const_view(), round.view());
+	context.run(forward, input.const_view(), output.view()).wait();
+	context.run(inverse, output.const_view(), round.view()).wait();
+	CHECK(context.scratch_capacity() == 0);
+	const float real[] = {1, 0, -1, 0};
+	const float imag[] = {0, -1, 0, 1};
+	for (int i = 0; i < 8; ++i) {
+		CHECK(std::abs(output.pointer()[i].real - real[i % 4]) < 1e-5f);
+		CHECK(std::abs(output.pointer()[i].imag - imag[i % 4]) < 1e-5f);
+		CHECK(std::abs(round.pointer()[i].real - (i == 1 ? 8.0f : 0.0f)) < 1e-5f);
+		CHECK(std::abs(round.pointer()[i].imag) < 1e-5f);
+	}
+	// a plan prepared on prototypes runs on new buffers; aliasing is refused at run time
+	iane::host_image<iane::fft_complex, 2> prototype_in(size), prototype_out(size), twos(size), spectrum(size);
+	for (int i = 0; i < 8; ++i) {
+		twos.pointer()[i] = {2.0f, 0.0f};
+	}
+	auto plan = context.prepare(iane::fft_forward, prototype_in.const_view(), prototype_out.view());
+	check_throws<std::invalid_argument>([&] { context.run(plan, twos.const_view(), twos.view()); }, "aliased host FFT");
+	context.run(plan, twos.const_view(), spectrum.view()).wait();
+	CHECK(spectrum.pointer()[0].real == 16.0f);
+	for (int i = 1; i < 8; ++i) {
+		CHECK(spectrum.pointer()[i].real == 0.0f && spectrum.pointer()[i].imag == 0.0f);
+	}
+	CHECK(context.scratch_capacity() == 0);
+	iane::host_image<iane::fft_complex, 2> other(coord_t{4, 3});
+	check_throws<std::invalid_argument>([&] { context.run(plan, other.const_view(), other.view()); }, "shape");
+}
+
+void host_real_fft_known_answers(iane::math::vec2i aSize)
+{
+	using coord_t = iane::math::vec2i;
+	const auto packed_size = iane::rfft_extent(aSize);
+	const int count = aSize[0] * aSize[1];
+	iane::host_image<float, 2> input(aSize), output(aSize);
+	iane::host_image<iane::fft_complex, 2> packed(packed_size), before(packed_size);
+	for (int i = 0; i < count; ++i) {
+		input.pointer()[i] = static_cast<float>((i * 7) % 11 - 4);
+	}
+	auto exec = iane::make_execution_context();
+	auto forward = exec.prepare(iane::rfft, input.const_view(), packed.view());
+	auto inverse = exec.prepare(iane::irfft, packed.const_view(), output.view());
+	CHECK(exec.scratch_capacity() == static_cast<std::size_t>(packed_size[0] * packed_size[1]) * sizeof(iane::fft_complex));
+	exec.run(forward, input.const_view(), packed.view()).wait();
+	std::memcpy(before.pointer(), packed.pointer(), sizeof(iane::fft_complex) * packed_size[0] * packed_size[1]);
+	exec.run(inverse, packed.const_view(), output.view()).wait();
+	for (int i = 0; i < count; ++i) {
+		CHECK(std::abs(output.pointer()[i] - count * static_cast<float>((i * 7) % 11 - 4)) < 2e-4f);
+		CHECK(input.pointer()[i] == static_cast<float>((i * 7) % 11 - 4));
+	}
+	for (int ky = 0; ky < aSize[1]; ++ky) {
+		for (int kx = 0; kx < packed_size[0]; ++kx) {
+			double re = 0, im = 0;
+			for (int y = 0; y < aSize[1]; ++y) {
+				for (int x = 0; x < aSize[0]; ++x) {
+					const double phase = -2 * std::numbers::pi *
+						(static_cast<double>(kx * x) / aSize[0] + static_cast<double>(ky * y) / aSize[1]);
+					const double v = input.const_view()[coord_t{x, y}];
+					re += v * std::cos(phase);
+					im += v * std::sin(phase);
+				}
+			}
+			const auto saved = before.const_view()[coord_t{kx, ky}];
+			const auto kept = packed.const_view()[coord_t{kx, ky}];
+			CHECK(std::abs(saved.real - re) < 3e-5);
+			CHECK(std::abs(saved.imag - im) < 3e-5);
+			CHECK(std::memcmp(&saved, &kept, sizeof(saved)) == 0);
+		}
+	}
+}
+
+// Host FFT lines of every kind of length (powers of two, smooth, primes up to 64
+// on the generic radix, larger primes and lengths with a large prime factor on
+// Bluestein's form) and 2-D / 3-D shapes against the direct float64 DFT: max error
+// within 2e-6 of max|X|, and forward then inverse returns N x the input.
+void host_fft_against_dft()
+{
+	const auto check_shape = [](const int* aExtent, int aRank) {
+		const int n0 = aExtent[0], n1 = aRank > 1 ? aExtent[1] : 1, n2 = aRank > 2 ? aExtent[2] : 1;
+		const int count = n0 * n1 * n2;
+		std::vector<iane::fft_complex> x(count), X(count), back(count);
+		for (int i = 0; i < count; ++i) {
+			x[i] = {static_cast<float>(std::sin(0.37 * i + 0.1) + 0.25), static_cast<float>(std::cos(1.7 * i) * 0.5)};
+		}
+		auto ctx = iane::make_execution_context();
+		double err = 0, peak = 0, round = 0;
+		const auto dft = [&](int k0, int k1, int k2) {
+			std::complex<double> sum = 0;
+			for (int t2 = 0; t2 < n2; ++t2) {
+				for (int t1 = 0; t1 < n1; ++t1) {
+					for (int t0 = 0; t0 < n0; ++t0) {
+						const double phase = -2 * std::numbers::pi * (static_cast<double>(k0) * t0 / n0 +
+							static_cast<double>(k1) * t1 / n1 + static_cast<double>(k2) * t2 / n2);
+						const auto& v = x[(t2 * n1 + t1) * n0 + t0];
+						sum += std::complex<double>(v.real, v.imag) * std::polar(1.0, phase);
+					}
+				}
+			}
+			return sum;
+		};
+		if (aRank == 1) {
+			auto src = iane::host::make_const_view<iane::fft_complex, 1>(x.data(), n0);
+			auto dst = iane::host::make_view<iane::fft_complex, 1>(X.data(), n0);
+			auto inv = iane::host::make_view<iane::fft_complex, 1>(back.data(), n0);
+			auto f = ctx.prepare(iane::fft_forward, src, dst);
+			auto b = ctx.prepare(iane::fft_inverse, iane::host::make_const_view<iane::fft_complex, 1>(X.data(), n0), inv);
+			ctx.run(f, src, dst).wait();
+			ctx.run(b, iane::host::make_const_view<iane::fft_complex, 1>(X.data(), n0), inv).wait();
+		} else if (aRank == 2) {
+			const iane::math::vec2i s{n0, n1};
+			auto src = iane::host::make_const_view<iane::fft_complex, 2>(x.data(), s);
+			auto dst = iane::host::make_view<iane::fft_complex, 2>(X.data(), s);
+			auto inv = iane::host::make_view<iane::fft_complex, 2>(back.data(), s);
+			auto f = ctx.prepare(iane::fft_forward, src, dst);
+			auto b = ctx.prepare(iane::fft_inverse, iane::host::make_const_view<iane::fft_complex, 2>(X.data(), s), inv);
+			ctx.run(f, src, dst).wait();
+			ctx.run(b, iane::host::make_const_view<iane::fft_complex, 2>(X.data(), s), inv).wait();
+		} else {
+			const iane::math::vec3i s{n0, n1, n2};
+			auto src = iane::host::make_const_view<iane::fft_complex, 3>(x.data(), s);
+			auto dst = iane::host::make_view<iane::fft_complex, 3>(X.data(), s);
+			auto inv = iane::host::make_view<iane::fft_complex, 3>(back.data(), s);
+			auto f = ctx.prepare(iane::fft_forward, src, dst);
+			auto b = ctx.prepare(iane::fft_inverse, iane::host::make_const_view<iane::fft_complex, 3>(X.data(), s), inv);
+			ctx.run(f, src, dst).wait();
+			ctx.run(b, iane::host::make_const_view<iane::fft_complex, 3>(X.data(), s), inv).wait();
+		}
+		for (int k2 = 0; k2 < n2; ++k2) {
+			for (int k1 = 0; k1 < n1; ++k1) {
+				for (int k0 = 0; k0 < n0; ++k0) {
+					const auto want = dft(k0, k1, k2);
+					const auto& got = X[(k2 * n1 + k1) * n0 + k0];
+					err = std::max(err, std::abs(std::complex<double>(got.real, got.imag) - want));
+					peak = std::max(peak, std::abs(want));
+				}
+			}
+		}
+		for (int i = 0; i < count; ++i) {
+			round = std::max(round, std::abs(std::complex<double>(back[i].real, back[i].imag) -
+				std::complex<double>(x[i].real, x[i].imag) * static_cast<double>(count)));
+		}
+		CHECK(err <= 2e-6 * peak);
+		CHECK(round <= 4e-6 * peak);
+		if (err > 2e-6 * peak || round > 4e-6 * peak) {
+			std::printf("host FFT %dx%dx%d: error %.3g, round trip %.3g, peak %.3g\n", n0, n1, n2, err, round, peak);
+		}
+	};
+	for (int n : {1, 2, 3, 4, 5, 7, 8, 12, 17, 31, 61, 64, 67, 97, 128, 210, 256, 331, 1009, 2 * 67, 4 * 101}) {
+		check_shape(&n, 1);
+	}
+	const int shape2[][2] = {{6, 67}, {16, 9}, {1, 5}, {67, 3}};
+	for (const auto& s : shape2) {
+		check_shape(s, 2);
+	}
+	const int shape3[][3] = {{5, 4, 3}, {8, 1, 7}, {3, 71, 2}};
+	for (const auto& s : shape3) {
+		check_shape(s, 3);
+	}
+}
+
 // main.cu:51-81 on the host with the header's host convolve (the reference's
 // CPU semantics) and on the device with the DIRECT method in reference arithmetic.
 void rl_device_equals_host_loop()
@@ -1108,6 +1282,11 @@ int main(int argc, char** argv)
 		crop_flip(false);
 		policy_validation();
 		sequence_case<false>();
+		host_fft_known_answers();
+		host_real_fft_known_answers({4, 3});
+		host_real_fft_known_answers({5, 3});
+		host_real_fft_known_answers({6, 4});
+		host_fft_against_dft();
 	} else {
 		int n = 0;
 		if (iane_hip_device_count(&n) != IANE_OK || n == 0) {
