@@ -52,6 +52,7 @@ struct conv_args {
 	unsigned tiles_y;
 	int src_span; // bytes of one (x, y) plane (streaming kernels)
 	int dst_span;
+	int column_dot2; // column kernel: u16 images with small nonnegative taps on v_dot2_u32_u16
 };
 
 // Sample the source at an integer coordinate with the border rule; returns
@@ -1052,6 +1053,175 @@ __device__ __forceinline__ void column_march(const conv_args& a, const column_ge
 	}
 }
 
+// u16 images with coefficients in [0, 2^16) whose sums stay below 2^31: the
+// samples stay packed two per dword and every tap pair is one v_dot2_u32_u16
+// into an exact u32 sum. With kernel row (qy, qz) packed as B10 = {k1, k0},
+// B21 = {k2, k1}, B2h = {0, k2}, B0l = {k0, 0} (low, high half; k_q the tap
+// qx = q, which meets sample x + e + 1 - q), the lane's pairs d0 = {s0, s1} and
+// d1 = {s2, s3}, the left lane's d1 (high half s-1) and the right lane's d0
+// (low half s4) give the four outputs of a row in eight instructions:
+//   e = 0: d0.B10 + L.B2h    e = 1: d0.B21 + d1.B0l
+//   e = 2: d1.B10 + d0.B2h   e = 3: d1.B21 + R.B0l
+// Loads, borders, lane shifts and the three-plane accumulator ring are those
+// of column_march.
+typedef unsigned short column_u16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ unsigned column_dot2(unsigned aPair, unsigned aCoeff, unsigned aSum)
+{
+	return __builtin_amdgcn_udot2(__builtin_bit_cast(column_u16x2, aPair), __builtin_bit_cast(column_u16x2, aCoeff),
+		aSum, false);
+}
+
+template<class D, int R, int P>
+__device__ __forceinline__ void column_march_dot2(const conv_args& a, const column_geom& g, const int (&k)[27])
+{
+	using io = column_io<uint16_t>;
+	constexpr int rows = R + 2;
+	constexpr int ss = 2;
+	constexpr int ds = sizeof(D);
+
+	unsigned b10[9], b21[9], b2h[9], b0l[9];  // kernel row qy + 3 qz
+#pragma unroll
+	for (int row = 0; row < 9; ++row) {
+		const unsigned k0 = static_cast<unsigned>(k[3 * row]);
+		const unsigned k1 = static_cast<unsigned>(k[3 * row + 1]);
+		const unsigned k2 = static_cast<unsigned>(k[3 * row + 2]);
+		b10[row] = k1 | (k0 << 16);
+		b21[row] = k2 | (k1 << 16);
+		b2h[row] = k2 << 16;
+		b0l[row] = k0;
+	}
+
+	int row_off[rows];
+	int edge_off[rows];
+	int edge_x = -1;
+	if (g.tx == 0 && g.x0 > 0) {
+		edge_x = g.x0 - 1;
+	} else if (g.tx == 15 && g.x0 + 64 < g.ext_x) {
+		edge_x = g.x0 + 64;
+	}
+#pragma unroll
+	for (int j = 0; j < rows; ++j) {
+		int yi = g.yb - 1 + j;
+		const bool in = border_coord(yi, g.ext_y, a.border);
+		const long long base = static_cast<long long>(yi) * a.src.stride[1];
+		row_off[j] = in && g.x < g.ext_x ? static_cast<int>(base + g.x * ss) : kOob;
+		edge_off[j] = in && edge_x >= 0 ? static_cast<int>(base + edge_x * ss) : kOob;
+	}
+
+	const auto load_plane = [&](int aPlane, bool aWanted, column_v2 (&aVec)[rows], unsigned (&aEdge)[rows]) {
+		int zi = aPlane;
+		const bool in = aWanted && border_coord(zi, g.ext_z, a.border);
+		const auto plane = stream_resource(a.src.ptr + static_cast<long long>(in ? zi : 0) * a.src.stride[2],
+			in ? a.src_span : 0);
+#pragma unroll
+		for (int j = 0; j < rows; ++j) {
+			aVec[j] = io::load(plane, row_off[j]);
+			aEdge[j] = io::load_single(plane, edge_off[j]);
+		}
+	};
+
+	// The lane's pairs of one input row and its neighbours' (left: high half
+	// s-1; right: low half s4), the zero or clamp rule applied at the image ends.
+	struct pairs {
+		unsigned d0, d1, left, right;
+	};
+	const auto window = [&](const column_v2& aVec, unsigned aEdge) {
+		pairs w;
+		w.d0 = aVec.x;
+		w.d1 = aVec.y;
+		const unsigned beyond_left = g.clamp ? (aVec.x & 0xffffu) : 0u;
+		const unsigned beyond_right = g.clamp ? (aVec.y >> 16) : 0u;
+		w.left = column_from_left((g.x0 == 0 ? beyond_left : aEdge) << 16, aVec.y);
+		const unsigned right = column_from_right(g.x + 4 < g.ext_x ? aEdge : beyond_right, aVec.x);
+		w.right = g.x + 4 < g.ext_x ? right : beyond_right;
+		return w;
+	};
+
+	unsigned acc[3][R][4];
+#pragma unroll
+	for (int set = 0; set < 3; ++set) {
+#pragma unroll
+		for (int r = 0; r < R; ++r) {
+#pragma unroll
+			for (int e = 0; e < 4; ++e) {
+				acc[set][r][e] = 0u;
+			}
+		}
+	}
+
+	const auto add_plane = [&](const pairs (&aWin)[rows], unsigned (&aAcc)[R][4], int aQz) {
+#pragma unroll
+		for (int r = 0; r < R; ++r) {
+#pragma unroll
+			for (int qy = 0; qy < 3; ++qy) {
+				const pairs& w = aWin[r + 2 - qy];
+				const int row = qy + 3 * aQz;
+				aAcc[r][0] = column_dot2(w.left, b2h[row], column_dot2(w.d0, b10[row], aAcc[r][0]));
+				aAcc[r][1] = column_dot2(w.d1, b0l[row], column_dot2(w.d0, b21[row], aAcc[r][1]));
+				aAcc[r][2] = column_dot2(w.d0, b2h[row], column_dot2(w.d1, b10[row], aAcc[r][2]));
+				aAcc[r][3] = column_dot2(w.right, b0l[row], column_dot2(w.d1, b21[row], aAcc[r][3]));
+			}
+		}
+	};
+
+	const auto store_plane = [&](int aZ, unsigned (&aAcc)[R][4]) {
+		const auto plane = stream_resource(a.dst.ptr + static_cast<long long>(aZ) * a.dst.stride[2], a.dst_span);
+#pragma unroll
+		for (int r = 0; r < R; ++r) {
+			const int y = g.yb + r;
+			if (g.x < g.ext_x && y < g.ext_y) {
+				D out[4];
+#pragma unroll
+				for (int e = 0; e < 4; ++e) {
+					out[e] = convert_exact_int<D>(static_cast<int>(aAcc[r][e]));
+				}
+				column_store(plane, static_cast<int>(y * a.dst.stride[1] + g.x * ds), out);
+			}
+#pragma unroll
+			for (int e = 0; e < 4; ++e) {
+				aAcc[r][e] = 0u;
+			}
+		}
+	};
+
+	// the march of column_march: plane first + i adds kernel plane 2 to output
+	// first + i + 1, plane 1 to output first + i, plane 0 to output first + i - 1
+	const int first = g.z0 - 1;
+	const int n = g.z1 - g.z0 + 2;
+	column_v2 vec[P][rows];
+	unsigned edge[P][rows];
+#pragma unroll
+	for (int q = 0; q < P; ++q) {
+		load_plane(first + q, q < n, vec[q], edge[q]);
+	}
+	for (int i0 = 0; i0 < n; i0 += 6) {
+#pragma unroll
+		for (int ph = 0; ph < 6; ++ph) {
+			const int i = i0 + ph;
+			if (i < n) {
+				pairs win[rows];
+#pragma unroll
+				for (int j = 0; j < rows; ++j) {
+					win[j] = window(vec[ph % P][j], edge[ph % P][j]);
+				}
+				load_plane(first + i + P, i + P < n, vec[ph % P], edge[ph % P]);
+				const int p = first + i;
+				if (p + 1 < g.z1) {
+					add_plane(win, acc[ph % 3], 2);
+				}
+				if (p >= g.z0 && p < g.z1) {
+					add_plane(win, acc[(ph + 2) % 3], 1);
+				}
+				if (p - 1 >= g.z0) {
+					add_plane(win, acc[(ph + 1) % 3], 0);
+					store_plane(p - 1, acc[(ph + 1) % 3]);
+				}
+			}
+		}
+	}
+}
+
 // Integer images whose sums fit neither FP32 nor int32 exactly (coefficients
 // of 2^23 and more, or huge sums): every output of the lane is the FP64 chain
 // in the reference's order, read straight from memory (compact loops).
@@ -1123,7 +1293,17 @@ __global__ __launch_bounds__(kColumnThreads) void conv_column_kernel(conv_args a
 			largest = m > largest ? m : largest;
 		}
 		constexpr long long source_max = sizeof(S) == 1 ? 255 : 65535;
-		if (total * source_max < (1LL << 24)) {
+		bool small_nonnegative = true;
+#pragma unroll
+		for (int i = 0; i < 27; ++i) {
+			small_nonnegative = small_nonnegative && k[i] >= 0 && k[i] < (1 << 16);
+		}
+		if (std::is_same_v<S, uint16_t> && small_nonnegative && total * source_max < (1LL << 31) &&
+			a.column_dot2) {
+			if constexpr (std::is_same_v<S, uint16_t>) {
+				column_march_dot2<D, R, P>(a, g, k);
+			}
+		} else if (total * source_max < (1LL << 24)) {
 			float kf[27];
 #pragma unroll
 			for (int i = 0; i < 27; ++i) {
@@ -1343,6 +1523,7 @@ bool launch_column(conv_args& a, hipStream_t aStream)
 	if (a.src_span < 0 || a.dst_span < 0) {
 		return false;
 	}
+	a.column_dot2 = tune_get("conv.column_dot2", 1) != 0;
 	const int r = tune_get("conv.column_rows", 4) == 2 ? 2 : 4;
 	a.tiles_x = ceil_div(s.extent[0], 64);
 	a.tiles_y = ceil_div(s.extent[1], 16 * r);

@@ -500,3 +500,35 @@ def test_separable_rank2(shape, lengths, border):
             iane_amd.convolve_separable(dev(src), [dev(t) for t in taps], out,
                                         convolution_options(border_mode(border), precision.fp32)).wait()
         assert rel_err(host(out), ref64) <= TOL_FP32, engine
+
+
+@pytest.mark.parametrize("shape", [(20, 37, 68), (5, 33, 128), (70, 18, 64), (1, 5, 4), (13, 100, 196), (9, 64, 256)])
+@pytest.mark.parametrize("border", [cases.CLAMP, cases.ZERO])
+@pytest.mark.parametrize("taps", ["binomial", "random", "large", "sparse"])
+def test_column_u16_dot2_path(shape, border, taps):
+    """u16 images with 3^3 kernels of nonnegative coefficients below 2^16 (sums below
+    2^31) run on packed v_dot2_u32_u16 pairs: the same integers as the exact FP32 /
+    int32 column paths and as the oracle (bit-exact, i32 and u16 destinations)."""
+    from iane_amd import autotune
+    rng = np.random.default_rng(sum(shape) + len(taps))
+    src = rng.integers(0, 65536, shape).astype(np.uint16)
+    if taps == "binomial":
+        b = np.array([1, 2, 1])
+        ker = np.multiply.outer(np.multiply.outer(b, b), b).astype(np.int32)
+    elif taps == "random":
+        ker = rng.integers(0, 300, (3, 3, 3)).astype(np.int32)
+    elif taps == "large":
+        ker = rng.integers(1000, 1200, (3, 3, 3)).astype(np.int32)  # 27 * 1199 * 65535 < 2^31
+    else:
+        ker = np.zeros((3, 3, 3), np.int32)
+        ker[0, 1, 2] = 65535 // 64
+        ker[2, 0, 0] = 7
+    ref = oracle.convolve(src, ker, np.int32, border, threads=8)
+    with autotune.applied({"conv.column": 1, "conv.column_dot2": 1}):
+        got = gpu_convolve(src, ker, np.int32, border)
+    same_bits(got, ref)
+    with autotune.applied({"conv.column": 1, "conv.column_dot2": 0}):
+        same_bits(gpu_convolve(src, ker, np.int32, border), ref)
+    if taps == "binomial":
+        with autotune.applied({"conv.column": 1}):
+            same_bits(gpu_convolve(src, ker, np.uint16, border), oracle.convolve(src, ker, np.uint16, border, threads=8))
