@@ -35,10 +35,14 @@ FLAGS = [
 #   conv_separable: the iterative ILP scheduling strategy (FP32 engine 57.6 ->
 #     50.4 us: it interleaves the x, y and z work of neighbouring planes);
 #   conv_separable_split: matrix-instruction results in VGPRs (every filter's
-#     accumulators feed the next filter's operand split directly).
+#     accumulators feed the next filter's operand split directly);
+#   conv_mfma: a higher full-unroll limit, so the split kernel's rows of one
+#     pass (up to 30 rows of 45 matrix instructions) unroll and its operand
+#     and row buffers stay in registers.
 EXTRA_FLAGS = {
     "conv_separable": ["-mllvm", "-amdgpu-sched-strategy=iterative-ilp"],
     "conv_separable_split": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
+    "conv_mfma": ["-mllvm", "-pragma-unroll-threshold=200000"],
 }
 
 

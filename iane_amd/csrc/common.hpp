@@ -262,6 +262,14 @@ __device__ __forceinline__ unsigned xcd_remap(unsigned aBlock, unsigned aCount)
 	return xcd * q + (xcd < r ? xcd : r) + index;
 }
 
+// Raw buffer resource of one plane (GFX9 word 3: 32-bit data format). Loads
+// and stores through it take a scalar base and a VGPR byte offset, and
+// offsets at or past `aBytes` read 0 / are dropped.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_resource(const char* aBase, int aBytes)
+{
+	return __builtin_amdgcn_make_buffer_rsrc(const_cast<char*>(aBase), static_cast<short>(0), aBytes, 0x00020000);
+}
+
 inline unsigned ceil_div(long long a, long long b) { return static_cast<unsigned>((a + b - 1) / b); }
 
 // A caller-supplied workspace: it must hold `aNeeded` bytes (the size the

@@ -209,7 +209,20 @@ __global__ void toeplitz_table_kernel(view4 ker, int hx, int qy_padded, int step
 }
 
 // ------------------------------------------------------------- main kernel --
-template<class T16, int QC, int STEPS>
+// A pass is one (qz, qy chunk): its QC Toeplitz operands (every part and
+// K-step) sit in registers while the U = kRows + QC - 1 input rows it needs
+// stream past them; input row u meets operand j for output row
+// r = u - (QC - 1) + j. Loads run ahead of the matrix work, with one wave per
+// SIMD and nothing else to hide their latency:
+//  - input rows D rows ahead through a ring of D row buffers; the ring runs
+//    on into the next pass, so its first rows arrive while this pass's last
+//    rows (which meet few operands) are multiplied;
+//  - operand j of the next pass is loaded right after its last use in this
+//    one (output row kRows - 1), QC rows before its first use.
+// Rows and operands come through buffer resources with the row offset in a
+// scalar register. The order of the products is the plain (pass, row, j)
+// order, so the sums do not depend on D.
+template<class T16, int QC, int STEPS, int D>
 __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(1, 1)))
 void conv_split_kernel(const T16* __restrict__ padded, const T16* __restrict__ table,
 	const float* __restrict__ src_scale, const float* __restrict__ ker_scale, view4 dst, split_geom g, unsigned tasks)
@@ -217,11 +230,14 @@ void conv_split_kernel(const T16* __restrict__ padded, const T16* __restrict__ t
 	using S = split16<T16>;
 	using vec = typename S::vec;
 	constexpr int P = S::parts;
+	constexpr int U = kRows + QC - 1;        // input rows per pass
+	constexpr int UP = (U + D - 1) / D * D;  // rows per pass including idle ring slots
 	const unsigned block = xcd_remap(blockIdx.x, gridDim.x);
 	// The 4 waves of a workgroup take 4 consecutive z planes of one (strip,
 	// y block), and consecutive workgroups continue along z: workgroups that
 	// run together on one XCD then read overlapping input planes from its L2.
-	const unsigned wave = threadIdx.x >> 6;
+	// wave-uniform (the buffer resources below are scalar)
+	const unsigned wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 	const unsigned zgroups = (static_cast<unsigned>(g.nz) + kWaves - 1) / kWaves;
 	const unsigned zgroup = block % zgroups;
 	const unsigned column = block / zgroups;
@@ -243,60 +259,96 @@ void conv_split_kernel(const T16* __restrict__ padded, const T16* __restrict__ t
 		acc_hi[r] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 		acc_lo[r] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 	}
-	const long long lane_x = x0 + 16 * n + 8 * grp;  // padded x of this lane's first B element
-	const vec* tab = reinterpret_cast<const vec*>(table);
-	const vec* rows = reinterpret_cast<const vec*>(padded);
-	const long long part_vecs = g.part / 8;
 	const int chunks = g.qy_padded / QC;
+	const int passes = g.kz * chunks;
+	const char* parts = reinterpret_cast<const char*>(padded);
+	const long long part_bytes = 2 * g.part;
+	const long long plane_bytes = 2 * g.plane;
+	const int row_bytes = 2 * g.px;
+	const int lane_bytes = 2 * (x0 + 16 * n + 8 * grp);  // this lane's first B element in a padded row
+	const auto tab = plane_resource(reinterpret_cast<const char*>(table),
+		static_cast<int>(static_cast<long long>(g.kz) * g.qy_padded * STEPS * P * 1024));
 
-	for (int qz = 0; qz < g.kz; ++qz) {
-		const long long plane = static_cast<long long>(z + 2 * g.hz - qz) * g.plane;
-		for (int c = 0; c < chunks; ++c) {
-			const int q0 = c * QC;
-			vec a[P][QC][STEPS];
+	vec a[P][QC][STEPS];
+	vec b[D][P][STEPS];
+	// operand j of pass t (the last pass again past the end)
+	const auto load_operands = [&](int aPass, int aJ) {
+		const int t = min(aPass, passes - 1);
+		const int qz = t / chunks;
+		const int qy = (t - qz * chunks) * QC + aJ;
+#pragma unroll
+		for (int s = 0; s < STEPS; ++s) {
+			const int row = (qz * g.qy_padded + qy) * STEPS + s;
+#pragma unroll
+			for (int p = 0; p < P; ++p) {
+				a[p][aJ][s] = __builtin_bit_cast(vec,
+					__builtin_amdgcn_raw_buffer_load_b128(tab, 16 * lane, (row * P + p) * 1024, 0));
+			}
+		}
+	};
+	// input row u of pass t into ring slot aSlot
+	const auto load_row = [&](int aPass, int aU, int aSlot) {
+		const int t = min(aPass, passes - 1);
+		const int qz = t / chunks;
+		const int q0 = (t - qz * chunks) * QC;
+		// rows < 0 only meet zero operands (padded qy); rows past U are idle slots
+		const int yp = min(max(y0 + 2 * g.hy - q0 - (QC - 1) + aU, 0), g.py - 1);
+		const char* plane = parts + static_cast<long long>(z + 2 * g.hz - qz) * plane_bytes;
+#pragma unroll
+		for (int p = 0; p < P; ++p) {
+			const auto rows = plane_resource(plane + p * part_bytes, static_cast<int>(plane_bytes));
+#pragma unroll
+			for (int s = 0; s < STEPS; ++s) {
+				b[aSlot][p][s] = __builtin_bit_cast(vec,
+					__builtin_amdgcn_raw_buffer_load_b128(rows, lane_bytes + 64 * s, yp * row_bytes, 0));
+			}
+		}
+	};
+#pragma unroll
+	for (int j = 0; j < QC; ++j) {
+		load_operands(0, j);
+	}
+#pragma unroll
+	for (int d = 0; d < D; ++d) {
+		load_row(0, d, d);
+	}
+	for (int t = 0; t < passes; ++t) {
+#pragma unroll
+		for (int u = 0; u < UP; ++u) {
+			const int slot = u % D;
 #pragma unroll
 			for (int j = 0; j < QC; ++j) {
+				const int r = u - (QC - 1) + j;
+				if (u < U && r >= 0 && r < kRows) {
 #pragma unroll
-				for (int s = 0; s < STEPS; ++s) {
-					const long long row = (static_cast<long long>(qz) * g.qy_padded + q0 + j) * STEPS + s;
-#pragma unroll
-					for (int p = 0; p < P; ++p) {
-						a[p][j][s] = tab[(row * P + p) * 64 + lane];
-					}
-				}
-			}
-			// input row u pairs with output row r = u - (QC - 1) + j through qy = q0 + j
-			const int row0 = y0 + 2 * g.hy - q0 - (QC - 1);
-#pragma unroll
-			for (int u = 0; u < kRows + QC - 1; ++u) {
-				const int yp = min(max(row0 + u, 0), g.py - 1);  // rows < 0 only meet zero A (padded qy)
-				const long long base = (plane + static_cast<long long>(yp) * g.px + lane_x) / 8;
-				vec b[P][STEPS];
-#pragma unroll
-				for (int s = 0; s < STEPS; ++s) {
-#pragma unroll
-					for (int p = 0; p < P; ++p) {
-						b[p][s] = rows[base + 4 * s + p * part_vecs];
-					}
-				}
-#pragma unroll
-				for (int j = 0; j < QC; ++j) {
-					const int r = u - (QC - 1) + j;
-					if (r >= 0 && r < kRows) {
-#pragma unroll
-						for (int s = 0; s < STEPS; ++s) {
-							acc_hi[r] = S::mfma(a[0][j][s], b[0][s], acc_hi[r]);
-							acc_lo[r] = S::mfma(a[0][j][s], b[1][s], acc_lo[r]);
-							acc_lo[r] = S::mfma(a[1][j][s], b[0][s], acc_lo[r]);
-							if constexpr (P == 3) {
-								acc_lo[r] = S::mfma(a[0][j][s], b[2][s], acc_lo[r]);
-								acc_lo[r] = S::mfma(a[2][j][s], b[0][s], acc_lo[r]);
-								acc_lo[r] = S::mfma(a[1][j][s], b[1][s], acc_lo[r]);
-							}
+					for (int s = 0; s < STEPS; ++s) {
+						acc_hi[r] = S::mfma(a[0][j][s], b[slot][0][s], acc_hi[r]);
+						acc_lo[r] = S::mfma(a[0][j][s], b[slot][1][s], acc_lo[r]);
+						acc_lo[r] = S::mfma(a[1][j][s], b[slot][0][s], acc_lo[r]);
+						if constexpr (P == 3) {
+							acc_lo[r] = S::mfma(a[0][j][s], b[slot][2][s], acc_lo[r]);
+							acc_lo[r] = S::mfma(a[2][j][s], b[slot][0][s], acc_lo[r]);
+							acc_lo[r] = S::mfma(a[1][j][s], b[slot][1][s], acc_lo[r]);
 						}
 					}
 				}
 			}
+			// the slot is consumed: row u + D, of this pass or the next
+			if (u + D < UP) {
+				load_row(t, u + D, slot);
+			} else {
+				load_row(t + 1, u + D - UP, slot);
+			}
+			// operands whose last use was this row: the next pass's
+#pragma unroll
+			for (int j = 0; j < QC; ++j) {
+				if (u == kRows + QC - 2 - j) {
+					load_operands(t + 1, j);
+				}
+			}
+			// keep each row's loads behind its products: hoisted above them
+			// they would need a second set of row registers
+			__builtin_amdgcn_sched_barrier(0);
 		}
 	}
 
@@ -406,8 +458,12 @@ void run_source_part(const view4& aSrc, const split_geom& aGeom, const view4& aD
 	scale_from_partials_kernel<<<1, 256, 0, aStream>>>(partial, kPartialBlocks, src_scale, aUnit);
 	split_pad_kernel<T16><<<ceil_div(aGeom.part, 256 * 8), 256, 0, aStream>>>(aSrc, aBorder, aGeom, src_scale, padded);
 	const unsigned tasks = static_cast<unsigned>(aGeom.strips * aGeom.yblocks) * ceil_div(aGeom.nz, kWaves);
-	conv_split_kernel<T16, QC, STEPS><<<tasks, 64 * kWaves, 0, aStream>>>(padded, aTable, src_scale, aKerScale, aDst,
-		aGeom, tasks);
+	// input rows in flight: 6 (a 15-row chunk's pass boundary then has about
+	// 400 cycles of matrix work between a row's load and its use); fewer for
+	// the three-part BF16 operands, within the register file
+	constexpr int D = split16<T16>::parts == 3 ? (STEPS == 2 ? 4 : 3) : 6;
+	conv_split_kernel<T16, QC, STEPS, D><<<tasks, 64 * kWaves, 0, aStream>>>(padded, aTable, src_scale, aKerScale,
+		aDst, aGeom, tasks);
 	check_hip(hipGetLastError(), "split convolution launch");
 }
 
@@ -432,8 +488,15 @@ void run_source_part(const view4& aSrc, const view4& aKer, const view4& aDst, in
 // to 49 wide along x; other shapes run the FP32 direct kernel instead.
 bool split_applicable(const view4& aSrc, const view4& aKer, const view4& aDst)
 {
-	return aSrc.extent[3] == 1 && aKer.extent[3] == 1 && (aDst.type == IANE_F32 || aDst.type == IANE_F64) &&
-		aSrc.extent[0] < (1 << 24) && aSrc.extent[1] < (1 << 24) && pick_variant(aKer) != SPLIT_NONE;
+	if (!(aSrc.extent[3] == 1 && aKer.extent[3] == 1 && (aDst.type == IANE_F32 || aDst.type == IANE_F64) &&
+		aSrc.extent[0] < (1 << 24) && aSrc.extent[1] < (1 << 24) && pick_variant(aKer) != SPLIT_NONE)) {
+		return false;
+	}
+	// the kernel addresses a padded plane and the operand table through 32-bit
+	// buffer offsets
+	const split_variant v = pick_variant(aKer);
+	const split_geom g = make_geom(aSrc, aKer, v);
+	return 2 * g.plane < (1LL << 31) && 2 * table_halves(aKer, v, 3) < (1LL << 31);
 }
 
 size_t split_kernel_bytes(const view4& aKer, int aPrecision)

@@ -27,13 +27,7 @@ struct sep_args {
 	int src_span, dst_span;     // bytes of one source / destination plane that a launch addresses (matrix engine)
 };
 
-// Raw buffer resource of one plane (GFX9 word 3: 32-bit data format). Loads
-// and stores through it take a scalar base and a VGPR byte offset, and
-// offsets at or past `aBytes` read 0 / are dropped.
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_resource(const char* aBase, int aBytes)
-{
-	return __builtin_amdgcn_make_buffer_rsrc(const_cast<char*>(aBase), static_cast<short>(0), aBytes, 0x00020000);
-}
+using iane_hip::plane_resource;
 
 // Tap aIndex of axis aAxis in an engine's window of KA taps (KA odd, at least
 // the axis' odd tap count): the axis' taps centred in the window, zero around
