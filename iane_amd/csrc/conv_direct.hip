@@ -1359,6 +1359,269 @@ void launch_column_rows(const conv_args& a, int aChunk, unsigned aBlocks, hipStr
 	conv_column_kernel<S, D, INTEGER, R, 2><<<aBlocks, kColumnThreads, 0, aStream>>>(a, aChunk);
 }
 
+// ---------------------------------------------------------------------------
+// Plane kernel: square K x K kernels with one plane (3 <= K <= 9) on images
+// whose rows are x-contiguous and aligned for 4-sample vectors. No LDS and no
+// barriers: a lane holds 4 x-outputs of R rows; it loads its 4 samples of the
+// R + K - 1 input rows with one vector access each and takes the K / 2 samples
+// on either side from the neighbouring lanes of its 16-lane DPP row (the lanes
+// at the ends of a row load the vector beyond the tile). Input rows are used
+// from the last down, so every output's fma chain runs kernel rows ascending
+// and x taps ascending (input rows and samples from the back): the order of
+// the tiled kernel, whose bits it reproduces: the reference's FP64 chain in
+// REFERENCE precision, FP32 sums in FP32, exact sums for integer images.
+enum plane_mode { PLANE_FP32, PLANE_REFERENCE, PLANE_INTEGER };
+
+// The K x K coefficients as T (converted like load_as_float / load_as_double:
+// one rounding from the stored type), with the element type dispatched once.
+template<class T, int K>
+__device__ __forceinline__ void plane_coefficients(const conv_args& a, T (&aK)[K][K])
+{
+	with_element_type(a.ker.type, [&](auto aType) {
+		using E = decltype(aType);
+#pragma unroll
+		for (int qy = 0; qy < K; ++qy) {
+#pragma unroll
+			for (int qx = 0; qx < K; ++qx) {
+				const E v = *reinterpret_cast<const E*>(a.ker.ptr + qx * a.ker.stride[0] + qy * a.ker.stride[1]);
+				if constexpr (std::is_same_v<T, int>) {
+					aK[qy][qx] = static_cast<int>(v);
+				} else {
+					aK[qy][qx] = static_cast<T>(v);
+				}
+			}
+		}
+	});
+}
+
+template<class S, class D, class T, bool EXACT, int K, int R>
+__device__ __forceinline__ void plane_rows(const conv_args& a, int aPlane, int aTileX, int aYb)
+{
+	using io = column_io<S>;
+	using raw_t = typename io::raw;
+	using U = std::conditional_t<std::is_same_v<S, float>, float, int>;  // a sample in 32 bits
+	constexpr int C = K / 2;
+	constexpr int rows = R + K - 1;
+	constexpr int ss = sizeof(S);
+	constexpr int ds = sizeof(D);
+	const int lane = static_cast<int>(threadIdx.x) % 64;
+	const int tx = lane % 16;
+	const int ext_x = a.src.extent[0];
+	const int ext_y = a.src.extent[1];
+	const int x0 = aTileX * 64;
+	const int x = x0 + 4 * tx;
+	const bool clamp = a.border == IANE_BORDER_CLAMP;
+
+	T k[K][K];
+	plane_coefficients<T, K>(a, k);
+	// the vector beyond the tile for the lanes at the ends of a row
+	int edge_x = -1;
+	if (tx == 0 && x0 > 0) {
+		edge_x = x0 - 4;
+	} else if (tx == 15 && x0 + 64 < ext_x) {
+		edge_x = x0 + 64;
+	}
+	const auto src = stream_resource(a.src.ptr + static_cast<long long>(aPlane) * a.src.stride[2], a.src_span);
+	raw_t vec[rows];
+	raw_t edge[rows];
+#pragma unroll
+	for (int j = 0; j < rows; ++j) {
+		int yi = aYb - C + j;
+		const bool in = border_coord(yi, ext_y, a.border);
+		const long long base = static_cast<long long>(yi) * a.src.stride[1];
+		vec[j] = io::load(src, in && x < ext_x ? static_cast<int>(base + x * ss) : kOob);
+		edge[j] = io::load(src, in && edge_x >= 0 ? static_cast<int>(base + edge_x * ss) : kOob);
+	}
+
+	T acc[R][4];
+#pragma unroll
+	for (int r = 0; r < R; ++r) {
+#pragma unroll
+		for (int e = 0; e < 4; ++e) {
+			acc[r][e] = T(0);
+		}
+	}
+#pragma unroll
+	for (int j = rows - 1; j >= 0; --j) {
+		// samples x - C .. x + 3 + C of input row j (the border rule at the image ends)
+		U mine[4];
+		U beyond[4];
+		io::unpack(vec[j], mine);
+		io::unpack(edge[j], beyond);
+		const U before = clamp ? mine[0] : U(0);
+		const U after = clamp ? mine[3] : U(0);
+		T w[4 + 2 * C];
+#pragma unroll
+		for (int c = 0; c < C; ++c) {
+			const int from = 4 - C + c;  // component of the left lane's vector
+			const U left = column_from_left(x0 == 0 ? before : beyond[from], mine[from]);
+			w[c] = static_cast<T>(left);
+			const U right = column_from_right(x + 4 < ext_x ? beyond[c] : after, mine[c]);
+			w[C + 4 + c] = static_cast<T>(x + 4 < ext_x ? right : after);
+		}
+#pragma unroll
+		for (int e = 0; e < 4; ++e) {
+			w[C + e] = static_cast<T>(mine[e]);
+		}
+		// outputs r meet row j through kernel row qy = r + K - 1 - j
+#pragma unroll
+		for (int r = 0; r < R; ++r) {
+			const int qy = r + K - 1 - j;
+			if (qy >= 0 && qy < K) {
+#pragma unroll
+				for (int e = 0; e < 4; ++e) {
+#pragma unroll
+					for (int qx = 0; qx < K; ++qx) {
+						if constexpr (std::is_same_v<T, int>) {
+							acc[r][e] += __mul24(w[e + K - 1 - qx], k[qy][qx]);
+						} else {
+							acc[r][e] = fma(w[e + K - 1 - qx], k[qy][qx], acc[r][e]);
+						}
+					}
+				}
+			}
+		}
+	}
+
+	const auto dst = stream_resource(a.dst.ptr + static_cast<long long>(aPlane) * a.dst.stride[2], a.dst_span);
+#pragma unroll
+	for (int r = 0; r < R; ++r) {
+		const int y = aYb + r;
+		if (x < ext_x && y < ext_y) {
+			D out[4];
+#pragma unroll
+			for (int e = 0; e < 4; ++e) {
+				if constexpr (EXACT) {
+					out[e] = convert_out<D>(static_cast<double>(acc[r][e]));
+				} else {
+					out[e] = convert_out<D>(acc[r][e]);
+				}
+			}
+			column_store(dst, static_cast<int>(y * a.dst.stride[1] + x * ds), out);
+		}
+	}
+}
+
+// Integer images whose sums fit neither FP32 nor int32 exactly: each of the
+// lane's outputs is the FP64 chain in the reference's order, read straight
+// from memory (compact loops).
+template<class S, class D, int K, int R>
+__device__ void plane_fp64_chain(const conv_args& a, int aPlane, int aTileX, int aYb)
+{
+	constexpr int C = K / 2;
+	constexpr int ds = sizeof(D);
+	const int x = aTileX * 64 + 4 * (static_cast<int>(threadIdx.x) % 16);
+	const char* src = a.src.ptr + static_cast<long long>(aPlane) * a.src.stride[2];
+	const auto dst = stream_resource(a.dst.ptr + static_cast<long long>(aPlane) * a.dst.stride[2], a.dst_span);
+#pragma unroll 1
+	for (int r = 0; r < R; ++r) {
+		const int y = aYb + r;
+		if (x >= a.src.extent[0] || y >= a.src.extent[1]) {
+			continue;
+		}
+		D out[4];
+#pragma unroll
+		for (int e = 0; e < 4; ++e) {
+			double sum = 0.0;
+#pragma unroll 1
+			for (int qy = 0; qy < K; ++qy) {
+#pragma unroll 1
+				for (int qx = 0; qx < K; ++qx) {
+					int xi = x + e + C - qx;
+					int yi = y + C - qy;
+					const bool inside = border_coord(xi, a.src.extent[0], a.border) &&
+						border_coord(yi, a.src.extent[1], a.border);
+					const double sample = inside ? static_cast<double>(*reinterpret_cast<const S*>(
+						src + xi * a.src.stride[0] + yi * a.src.stride[1])) : 0.0;
+					sum = fma(sample, static_cast<double>(load_as_int(a.ker.ptr + qx * a.ker.stride[0] +
+						qy * a.ker.stride[1], a.ker.type)), sum);
+				}
+			}
+			out[e] = convert_out<D>(sum);
+		}
+		column_store(dst, static_cast<int>(y * a.dst.stride[1] + x * ds), out);
+	}
+}
+
+template<class S, class D, int MODE, int K, int R>
+__global__ __launch_bounds__(kColumnThreads) void conv_plane_kernel(conv_args a)
+{
+	const unsigned block = xcd_remap(blockIdx.x, gridDim.x);
+	const int tile_x = static_cast<int>(block % a.tiles_x);
+	const unsigned rest = block / a.tiles_x;
+	const int tile_y = static_cast<int>(rest % a.tiles_y);
+	const int plane = static_cast<int>(rest / a.tiles_y);
+	const int group = static_cast<int>(threadIdx.x) / 64 * 4 + static_cast<int>(threadIdx.x) % 64 / 16;
+	const int yb = tile_y * (16 * R) + group * R;
+	if constexpr (MODE == PLANE_FP32) {
+		plane_rows<S, D, float, false, K, R>(a, plane, tile_x, yb);
+	} else if constexpr (MODE == PLANE_REFERENCE) {
+		plane_rows<S, D, double, false, K, R>(a, plane, tile_x, yb);
+	} else {
+		// Integer samples and coefficients: every partial sum is bounded by
+		// max source x sum |k|. Below 2^24 the FP32 chain is exact, below 2^31
+		// with |k| < 2^23 the 24-bit integer multiply-adds are; both give the
+		// integer of the reference's FP64 chain. Beyond, each output is that
+		// FP64 chain read straight from memory.
+		int k[K][K];
+		plane_coefficients<int, K>(a, k);
+		long long total = 0;
+		long long largest = 0;
+#pragma unroll
+		for (int qy = 0; qy < K; ++qy) {
+#pragma unroll
+			for (int qx = 0; qx < K; ++qx) {
+				const int kk = k[qy][qx];
+				const long long m = kk < 0 ? -static_cast<long long>(kk) : static_cast<long long>(kk);
+				total += m;
+				largest = m > largest ? m : largest;
+			}
+		}
+		constexpr long long source_max = sizeof(S) == 1 ? 255 : 65535;
+		if (total * source_max < (1LL << 24)) {
+			plane_rows<S, D, float, true, K, R>(a, plane, tile_x, yb);
+		} else if (largest < (1LL << 23) && total * source_max < (1LL << 31)) {
+			plane_rows<S, D, int, true, K, R>(a, plane, tile_x, yb);
+		} else {
+			plane_fp64_chain<S, D, K, R>(a, plane, tile_x, yb);
+		}
+	}
+}
+
+// Rows per lane the kernel instantiates: FP64 or integer sums of 7 x 7 kernels
+// keep at most 2 (4 would not fit the register file).
+template<int MODE, int K>
+constexpr int plane_max_rows() { return MODE != PLANE_FP32 && K >= 7 ? 2 : 4; }
+
+template<class S, class D, int MODE, int K>
+void launch_plane_rows(const conv_args& a, int aRows, unsigned aBlocks, hipStream_t aStream)
+{
+	if (aRows == 4) {
+		if constexpr (plane_max_rows<MODE, K>() >= 4) {
+			conv_plane_kernel<S, D, MODE, K, 4><<<aBlocks, kColumnThreads, 0, aStream>>>(a);
+		}
+	} else if (aRows == 2) {
+		conv_plane_kernel<S, D, MODE, K, 2><<<aBlocks, kColumnThreads, 0, aStream>>>(a);
+	} else {
+		conv_plane_kernel<S, D, MODE, K, 1><<<aBlocks, kColumnThreads, 0, aStream>>>(a);
+	}
+}
+
+template<class S, class D, int MODE>
+void launch_plane_k(const conv_args& a, int aRows, unsigned aBlocks, hipStream_t aStream)
+{
+	switch (a.ker.extent[0]) {
+	case 3: launch_plane_rows<S, D, MODE, 3>(a, aRows, aBlocks, aStream); break;
+	case 5: launch_plane_rows<S, D, MODE, 5>(a, aRows, aBlocks, aStream); break;
+	case 7: launch_plane_rows<S, D, MODE, 7>(a, aRows, aBlocks, aStream); break;
+	default:
+		if constexpr (MODE == PLANE_FP32) {
+			launch_plane_rows<S, D, MODE, 9>(a, aRows, aBlocks, aStream);
+		}
+		break;
+	}
+}
+
 // Compute units of the current device (cached per device).
 int compute_units()
 {
@@ -1577,6 +1840,90 @@ bool launch_column(conv_args& a, hipStream_t aStream)
 	return true;
 }
 
+// Plane launch for square single-plane kernels (K in 3..7, 9 with FP32 sums) on
+// images of one or more planes (rank <= 3) with x-contiguous rows aligned for
+// 4-sample vectors and an x extent that is a multiple of 4, for f32 -> f32
+// (FP32 or REFERENCE accumulation) and u8 / u16 with an integer kernel into
+// i32 or their own type; returns false when another kernel should run.
+template<class Acc>
+bool launch_plane(conv_args& a, hipStream_t aStream)
+{
+	if (tune_get("conv.plane", 1) == 0) {
+		return false;
+	}
+	const view4& s = a.src;
+	const view4& d = a.dst;
+	const int kx = a.ker.extent[0];
+	const bool fp32 = sizeof(Acc) == 4;
+	if (kx != a.ker.extent[1] || a.ker.extent[2] != 1 || a.ker.extent[3] != 1 || s.extent[3] != 1 ||
+		s.extent[0] % 4 != 0 || kx < 3 || kx > (fp32 ? 9 : 7)) {
+		return false;
+	}
+	const int st = s.type;
+	const int dt = d.type;
+	const bool integer = (st == IANE_U8 || st == IANE_U16) && sizeof(Acc) == 8 &&
+		(a.ker.type == IANE_U8 || a.ker.type == IANE_U16 || a.ker.type == IANE_I32) && (dt == st || dt == IANE_I32);
+	const bool real = st == IANE_F32 && dt == IANE_F32;
+	if (!integer && !real) {
+		return false;
+	}
+	const long long ss = dtype_size(st);
+	const long long ds = dtype_size(dt);
+	const auto aligned = [](const view4& v, long long aVector) {
+		const auto address = reinterpret_cast<uintptr_t>(v.ptr);
+		return address % aVector == 0 && v.stride[1] % aVector == 0 && v.stride[2] % aVector == 0;
+	};
+	if (s.stride[0] != ss || d.stride[0] != ds || !aligned(s, 4 * ss) || !aligned(d, 4 * ds)) {
+		return false;
+	}
+	a.src_span = plane_span(s);
+	a.dst_span = plane_span(d);
+	if (a.src_span < 0 || a.dst_span < 0) {
+		return false;
+	}
+	// Rows per lane: the most (fewer input rows loaded per output row) that
+	// still gives a workgroup per compute unit (1024^2 f32 7 x 7 in a graph:
+	// 6.0 us at 1 row, 5.2 us at 2 and 4).
+	a.tiles_x = ceil_div(s.extent[0], 64);
+	const long long planes = s.extent[2];
+	const auto blocks_for = [&](int aRows) {
+		return static_cast<long long>(a.tiles_x) * ceil_div(s.extent[1], 16 * aRows) * planes;
+	};
+	const int max_rows = fp32 || kx < 7 ? 4 : 2;
+	int rows = tune_get("conv.plane_rows", 0);
+	if (rows != 1 && rows != 2 && rows != 4) {
+		const long long want = compute_units();
+		rows = blocks_for(4) >= want ? 4 : (blocks_for(2) >= want ? 2 : 1);
+	}
+	rows = std::min(rows, max_rows);
+	a.tiles_y = ceil_div(s.extent[1], 16 * rows);
+	const long long blocks = blocks_for(rows);
+	if (blocks > 0x7fffffffLL || blocks == 0) {
+		return false;
+	}
+	const unsigned nb = static_cast<unsigned>(blocks);
+	if (real) {
+		if (fp32) {
+			launch_plane_k<float, float, PLANE_FP32>(a, rows, nb, aStream);
+		} else {
+			launch_plane_k<float, float, PLANE_REFERENCE>(a, rows, nb, aStream);
+		}
+	} else if (st == IANE_U8) {
+		if (dt == IANE_I32) {
+			launch_plane_k<uint8_t, int32_t, PLANE_INTEGER>(a, rows, nb, aStream);
+		} else {
+			launch_plane_k<uint8_t, uint8_t, PLANE_INTEGER>(a, rows, nb, aStream);
+		}
+	} else {
+		if (dt == IANE_I32) {
+			launch_plane_k<uint16_t, int32_t, PLANE_INTEGER>(a, rows, nb, aStream);
+		} else {
+			launch_plane_k<uint16_t, uint16_t, PLANE_INTEGER>(a, rows, nb, aStream);
+		}
+	}
+	return true;
+}
+
 template<class Acc, int KX>
 void launch_tiled(const conv_args& a, size_t aLds, unsigned aBlocks, hipStream_t aStream)
 {
@@ -1592,7 +1939,7 @@ void launch_conv(conv_args a, hipStream_t aStream, bool aForceSimple)
 		static_cast<size_t>(kx) * ky);
 	const long long count = element_count(a.dst);
 	if (!aForceSimple && tune_get("conv.stream", 1) != 0 &&
-		(launch_column<Acc>(a, aStream) || launch_stream<Acc>(a, aStream))) {
+		(launch_column<Acc>(a, aStream) || launch_plane<Acc>(a, aStream) || launch_stream<Acc>(a, aStream))) {
 		return;
 	}
 	if (aForceSimple || lds > kMaxLds) {

@@ -392,6 +392,74 @@ def test_column_kernel_equals_tiled(shape, border):
             same_bits(outs[0], oracle.convolve(src, ker, ddt, border))
 
 
+@pytest.mark.parametrize("shape", [(37, 68), (33, 128), (18, 64), (5, 4), (100, 196), (3, 21, 44), (300, 1024)])
+@pytest.mark.parametrize("border", [cases.CLAMP, cases.ZERO])
+@pytest.mark.parametrize("k", [3, 5, 7, 9])
+def test_plane_kernel_equals_tiled(shape, border, k):
+    """The plane kernel (square single-plane kernels on rows aligned for 4-sample
+    vectors, x extent a multiple of 4; 2-D images and stacks of planes) against the
+    tiled kernel: identical bits for FP32 sums, for the reference's FP64 chain, and
+    for integer images with integer kernels whether the sum is exact in FP32, in
+    int32 only, or coefficients up to 2^21 force the FP64 chain; 1, 2 and 4 rows per
+    lane. REFERENCE precision and integer images also equal the oracle bit for bit,
+    FP32 is within TOL_FP32 of the FP64 oracle."""
+    from iane_amd import autotune
+    rng = np.random.default_rng(sum(shape) + 10 * border + k)
+    kshape = (1,) * (len(shape) - 2) + (k, k)
+    combos = [(np.float32, np.float32, np.float32, precision.fp32),
+              (np.float32, np.float32, np.float32, precision.reference),
+              (np.float32, np.float64, np.float32, precision.reference),
+              (np.uint8, np.int32, np.int32, precision.reference),
+              (np.uint8, np.uint8, np.uint8, precision.reference),
+              (np.uint16, np.uint16, np.uint16, precision.reference),
+              (np.uint16, "mid", np.int32, precision.reference),
+              (np.uint8, "big", np.int32, precision.reference)]
+    for sdt, kdt, ddt, prec in combos:
+        if kdt in ("mid", "big"):
+            src, _ = _case(rng, shape, kshape, sdt, np.int32)
+            bound = 400 if kdt == "mid" else 1 << 21
+            ker = rng.integers(-bound, bound, kshape).astype(np.int32)
+        else:
+            src, ker = _case(rng, shape, kshape, sdt, kdt)
+        outs = []
+        for config in ({"conv.plane": 0}, {}, {"conv.plane_rows": 1}, {"conv.plane_rows": 2},
+                       {"conv.plane_rows": 4}):
+            with autotune.applied(config):
+                outs.append(gpu_convolve(src, ker, ddt, border, prec))
+        for out in outs[1:]:
+            same_bits(out, outs[0])
+        if prec == precision.reference:
+            same_bits(outs[0], oracle.convolve(src, ker, ddt, border))
+        else:
+            ref = oracle.convolve(src, ker, np.float64, border)
+            assert np.max(np.abs(outs[0] - ref)) <= TOL_FP32 * np.max(np.abs(ref))
+
+
+def test_plane_kernel_views():
+    """Views the plane kernel must decline or handle: rows that are not 16-byte
+    aligned, a strided x axis, an x extent not a multiple of 4, a strided
+    destination, row offsets into a larger image: the tiled kernel's bits."""
+    from iane_amd import autotune
+    rng = np.random.default_rng(6)
+    base = (rng.random((90, 136)) * 10).astype(np.float32)
+    ker = (rng.random((7, 7)) - 0.3).astype(np.float32)
+    dbase, dker = dev(base), dev(ker)
+    for prec in (precision.fp32, precision.reference):
+        opts = convolution_options(border_mode.clamp, prec)
+        for h, d in ((base[2:88, 1:129], dbase[2:88, 1:129]), (base[:, ::2], dbase[:, ::2]),
+                     (base[:, 4:130], dbase[:, 4:130]), (base[:, 0:126], dbase[:, 0:126]),
+                     (base[3:, 8:], dbase[3:, 8:])):
+            outs = []
+            for config in ({"conv.plane": 0}, {}):
+                out_base = empty((h.shape[0], h.shape[1] * 2), np.float32)
+                for out in (out_base[:, ::2], out_base[:, :h.shape[1]]):
+                    with autotune.applied(config):
+                        iane_amd.convolve(d, dker, out, opts).wait()
+                    outs.append(host(out))
+            for out in outs[1:]:
+                same_bits(out, outs[0])
+
+
 def test_column_kernel_views():
     """Views the column kernel must decline or handle: a source slice whose rows
     are not 16-byte aligned, a strided x axis, an x extent not a multiple of 4, a
