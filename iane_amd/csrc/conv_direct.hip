@@ -873,21 +873,30 @@ struct column_geom {
 };
 
 // Coefficient qx + 3 qy + 9 qz of a 3^3 kernel, the same for every lane: read
-// into scalar registers.
+// into scalar registers, converted like load_as_int / load_as_float, with one
+// element-type dispatch for all 27 (a switch per coefficient would put each
+// load in its own block and wait for it there).
 template<class T>
 __device__ __forceinline__ void column_coefficients(const conv_args& a, T (&aK)[27])
 {
+	with_element_type(a.ker.type, [&](auto aType) {
+		using E = decltype(aType);
+		E raw[27];
 #pragma unroll
-	for (int i = 0; i < 27; ++i) {
-		const char* at = a.ker.ptr + (i % 3) * a.ker.stride[0] + (i / 3 % 3) * a.ker.stride[1] +
-			(i / 9) * a.ker.stride[2];
-		if constexpr (std::is_same_v<T, int>) {
-			aK[i] = __builtin_amdgcn_readfirstlane(load_as_int(at, a.ker.type));
-		} else {
-			aK[i] = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int,
-				load_as<float>(at, a.ker.type))));
+		for (int i = 0; i < 27; ++i) {
+			raw[i] = *reinterpret_cast<const E*>(a.ker.ptr + (i % 3) * a.ker.stride[0] +
+				(i / 3 % 3) * a.ker.stride[1] + (i / 9) * a.ker.stride[2]);
 		}
-	}
+#pragma unroll
+		for (int i = 0; i < 27; ++i) {
+			if constexpr (std::is_same_v<T, int>) {
+				aK[i] = __builtin_amdgcn_readfirstlane(static_cast<int>(raw[i]));
+			} else {
+				aK[i] = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int,
+					static_cast<float>(raw[i]))));
+			}
+		}
+	});
 }
 
 // March the lane's column through the z chunk with T arithmetic: float fma

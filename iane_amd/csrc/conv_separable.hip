@@ -78,11 +78,21 @@ __global__ __launch_bounds__(kSepThreads) void sep3_fused_kernel(sep_args a)
 	const int tid = threadIdx.x;
 
 	float kx[K], ky[K], kz[K];
+	{
+		int at[K];
 #pragma unroll
-	for (int i = 0; i < K; ++i) {
-		kx[i] = uniform_float(separable::sep_tap<K>(a, 0, i));
-		ky[i] = uniform_float(separable::sep_tap<K>(a, 1, i));
-		kz[i] = uniform_float(separable::sep_tap<K>(a, 2, i));
+		for (int i = 0; i < K; ++i) {
+			at[i] = i;
+		}
+		separable::sep_taps<K>(a, 0, at, kx);
+		separable::sep_taps<K>(a, 1, at, ky);
+		separable::sep_taps<K>(a, 2, at, kz);
+#pragma unroll
+		for (int i = 0; i < K; ++i) {
+			kx[i] = uniform_float(kx[i]);
+			ky[i] = uniform_float(ky[i]);
+			kz[i] = uniform_float(kz[i]);
+		}
 	}
 
 	// Staging offsets of this thread's elements (same for every plane).
@@ -323,15 +333,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))) voi
 
 	// MFMA operands built from the taps (per lane, constant over the run)
 	float bx[8], ay[8], kz[K];
-	const auto tap = [&](int aAxis, int aIndex) { return separable::sep_tap<K>(a, aAxis, aIndex); };
+	{
+		int xat[8], yat[8], zat[K];
 #pragma unroll
-	for (int s = 0; s < 8; ++s) {
-		bx[s] = tap(0, n + 2 * C - (4 * s + g));
-		ay[s] = tap(1, n + 2 * C - (16 * (s / 4) + 4 * g + s % 4));
-	}
+		for (int s = 0; s < 8; ++s) {
+			xat[s] = n + 2 * C - (4 * s + g);
+			yat[s] = n + 2 * C - (16 * (s / 4) + 4 * g + s % 4);
+		}
 #pragma unroll
-	for (int i = 0; i < K; ++i) {
-		kz[i] = uniform_float(separable::sep_tap<K>(a, 2, i));
+		for (int i = 0; i < K; ++i) {
+			zat[i] = i;
+		}
+		separable::sep_taps<K>(a, 0, xat, bx);
+		separable::sep_taps<K>(a, 1, yat, ay);
+		separable::sep_taps<K>(a, 2, zat, kz);
+#pragma unroll
+		for (int i = 0; i < K; ++i) {
+			kz[i] = uniform_float(kz[i]);
+		}
 	}
 
 	// The plane loop below has no branch but its exit: every condition is an
@@ -529,18 +548,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))) voi
 	const int z0 = static_cast<int>(chunk) * a.chunk;
 	const int zend = min(z0 + a.chunk, a.nz);
 
-	const auto tap = [&](int aAxis, int aIndex) { return separable::sep_tap<K>(a, aAxis, aIndex); };
 	// window position of K-step s for lane group g (columns for x, rows for y)
 	const auto window = [&](int aStep) { return 16 * (aStep / 4) + 4 * g + aStep % 4; };
 	float bx[8], ay[8], kz[KZ];
+	int at[8];
 #pragma unroll
 	for (int s = 0; s < 8; ++s) {
-		bx[s] = tap(0, n + 8 + C - window(s));
-		ay[s] = tap(1, n + 8 + C - window(s));
+		at[s] = n + 8 + C - window(s);
 	}
+	separable::sep_taps<K>(a, 0, at, bx);
+	separable::sep_taps<K>(a, 1, at, ay);
+	int zat[KZ];
 #pragma unroll
 	for (int i = 0; i < KZ; ++i) {
-		kz[i] = uniform_float(separable::sep_tap<KZ>(a, 2, i));
+		zat[i] = i;
+	}
+	separable::sep_taps<KZ>(a, 2, zat, kz);
+#pragma unroll
+	for (int i = 0; i < KZ; ++i) {
+		kz[i] = uniform_float(kz[i]);
 	}
 	// the lane's four 16-byte groups per plane: row tile mt, half h (columns 16 h + 4 g .. + 3)
 	unsigned offset[2][2];

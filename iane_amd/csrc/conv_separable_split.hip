@@ -184,18 +184,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))) voi
 
 	// tap scales per axis: largest tap -> [2^14, 2^15), and 2^s > sum |taps|
 	int b[3], s[3];
+	int all[K];
+#pragma unroll
+	for (int t = 0; t < K; ++t) {
+		all[t] = t;
+	}
 #pragma unroll
 	for (int axis = 0; axis < 3; ++axis) {
+		float w[K];
+		sep_taps<K>(a, axis, all, w);
 		float wmax = 0.0f, wsum = 0.0f;
+#pragma unroll
 		for (int t = 0; t < K; ++t) {
-			const float w = fabsf(sep_tap<K>(a, axis, t));
-			wmax = fmaxf(wmax, w);
-			wsum += w;
+			wmax = fmaxf(wmax, fabsf(w[t]));
+			wsum += fabsf(w[t]);
 		}
 		b[axis] = sx_uniform(wmax == 0.0f ? 0 : 15 - sx_exponent(wmax));
 		s[axis] = sx_uniform(sx_exponent(wsum));
 	}
-	const auto tap = [&](int aAxis, int aIndex) { return ldexpf(sep_tap<K>(a, aAxis, aIndex), b[aAxis]); };
+	// the taps at N window positions of one axis, scaled by 2^b
+	const auto taps_at = [&](int aAxis, const auto& aAt, auto& aOut) {
+		sep_taps<K>(a, aAxis, aAt, aOut);
+#pragma unroll
+		for (auto& v : aOut) {
+			v = ldexpf(v, b[aAxis]);
+		}
+	};
 	const float c1 = __int_as_float(sx_uniform(__float_as_int(ldexpf(1.0f, -b[0] - s[0]))));
 	const float c2 = __int_as_float(sx_uniform(__float_as_int(ldexpf(1.0f, -b[1] - s[1]))));
 	const int rescale = b[2] - s[0] - s[1] + 141;  // output = D * 2^(E - rescale)
@@ -206,17 +220,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))) voi
 	float wz[8];  // z taps of this lane's A elements (row n, planes 8g + j of a window), scaled
 	{
 		float xv[8], yv[8];
+		int at[8], zat[8];
 #pragma unroll
 		for (int j = 0; j < 8; ++j) {
-			xv[j] = tap(0, n + 8 + C - window(j));
-			yv[j] = tap(1, n + 8 + C - window(j));
+			at[j] = n + 8 + C - window(j);
+			zat[j] = n - 8 * g - j + 2 * C;
 		}
+		taps_at(0, at, xv);
+		taps_at(1, at, yv);
+		taps_at(2, zat, wz);
 		sx_split8(xv, 1.0f, bxh, bxl);
 		sx_split8(yv, 1.0f, ayh, ayl);
-	}
-#pragma unroll
-	for (int j = 0; j < 8; ++j) {
-		wz[j] = tap(2, n - 8 * g - j + 2 * C);
 	}
 
 	// the lane's four 16-byte groups per plane: row tile mt, half h
@@ -383,9 +397,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))) voi
 				ring[j][jj] = 0.0f;
 			}
 		}
+		{
+			int all[K];
+#pragma unroll
+			for (int t = 0; t < K; ++t) {
+				all[t] = t;
+			}
+			sep_taps<K>(a, 2, all, kz);
+		}
 #pragma unroll
 		for (int t = 0; t < K; ++t) {
-			kz[t] = __int_as_float(sx_uniform(__float_as_int(sep_tap<K>(a, 2, t))));
+			kz[t] = __int_as_float(sx_uniform(__float_as_int(kz[t])));
 		}
 		// this lane's outputs: rows y0 + 4g + jj at column x0 + n
 #pragma unroll

@@ -29,24 +29,46 @@ struct sep_args {
 
 using iane_hip::plane_resource;
 
-// Tap aIndex of axis aAxis in an engine's window of KA taps (KA odd, at least
-// the axis' odd tap count): the axis' taps centred in the window, zero around
-// them (a shorter kernel is the longer one with zero outer taps, so every axis
-// can use the engine's one compile-time length); a null tap pointer is the unit
-// kernel (1 at the centre).
-template<int KA>
-__device__ __forceinline__ float sep_tap(const sep_args& a, int aAxis, int aIndex)
+// Taps at window positions aIndex[0..N) of axis aAxis in an engine's window of
+// KA taps (KA odd, at least the axis' odd tap count): the axis' taps centred in
+// the window, zero around them (a shorter kernel is the longer one with zero
+// outer taps, so every axis can use the engine's one compile-time length); a
+// null tap pointer is the unit kernel (1 at the centre). One element-type
+// dispatch per axis and no branch per tap: a position outside the axis' taps
+// reads its first tap and selects zero, so all the axis' loads are in flight
+// together (a switch around each load would put each in its own block and
+// wait for it there).
+template<int KA, int N>
+__device__ __forceinline__ void sep_taps(const sep_args& a, int aAxis, const int (&aIndex)[N], float (&aOut)[N])
 {
-	if (aIndex < 0 || aIndex >= KA) {
-		return 0.0f;
+	const char* taps = a.taps[aAxis];
+	if (taps == nullptr) {
+#pragma unroll
+		for (int i = 0; i < N; ++i) {
+			aOut[i] = aIndex[i] == KA / 2 ? 1.0f : 0.0f;
+		}
+		return;
 	}
-	if (a.taps[aAxis] == nullptr) {
-		return aIndex == KA / 2 ? 1.0f : 0.0f;
-	}
-	const int i = aIndex - (KA - a.tap_len[aAxis]) / 2;
-	return (i >= 0 && i < a.tap_len[aAxis]) ? load_as_float(a.taps[aAxis] + i * a.tap_stride[aAxis], a.tap_type[aAxis])
-	                                        : 0.0f;
+	const int len = a.tap_len[aAxis];
+	const int shift = (KA - len) / 2;
+	const long long stride = a.tap_stride[aAxis];
+	with_element_type(a.tap_type[aAxis], [&](auto aType) {
+		using E = decltype(aType);
+		E raw[N];
+		bool ok[N];
+#pragma unroll
+		for (int i = 0; i < N; ++i) {
+			const int t = aIndex[i] - shift;
+			ok[i] = aIndex[i] >= 0 && aIndex[i] < KA && t >= 0 && t < len;
+			raw[i] = *reinterpret_cast<const E*>(taps + (ok[i] ? t : 0) * stride);
+		}
+#pragma unroll
+		for (int i = 0; i < N; ++i) {
+			aOut[i] = ok[i] ? static_cast<float>(raw[i]) : 0.0f;
+		}
+	});
 }
+
 
 // Split-FP16 engine (conv_separable_split.hip): output chunks addressed with
 // 32-bit offsets need aChunk planes to fit; the launcher takes the common
