@@ -8,11 +8,13 @@
 // to the view's extent (clamp). In REFERENCE precision the sum is an FP64 fma
 // chain in exactly that order, so results are bit-identical to the reference.
 //
-// Tiled kernel: one workgroup (256 threads) computes a 64 x 32 output tile of
-// one (z, w) plane. For every kernel plane (qz, qw) it stages the input plane
-// region with its x/y halo (border applied at staging) into LDS as the
-// accumulator type, then each thread accumulates a 4 x 2 register block with a
-// sliding register window along x (kernel width is a template parameter).
+// Tiled kernel: one workgroup (256 threads) computes a 64 x 64 output tile of
+// one (z, w) plane (64 x 32 when that tile's LDS would not fit). For every
+// kernel plane (qz, qw) it stages the input plane region with its x/y halo
+// (border applied at staging) into LDS as the accumulator type, then each
+// thread accumulates a 4 x 4 (4 x 2) register block, reading each input row's
+// window along x once into registers and adding it to every output row it
+// reaches (kernel width is a template parameter).
 //
 // Streaming kernel (rank <= 3, small kernels with kz >= 2): a workgroup owns a
 // 64 x 16 (x, y) tile and walks a chunk of output planes; every input plane is
@@ -37,9 +39,7 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kThreadsX = 16;
 constexpr int kRx = 4;                 // outputs per thread along x
-constexpr int kRy = 2;                 // outputs per thread along y
 constexpr int kTileX = kThreadsX * kRx; // 64
-constexpr int kTileY = (kThreads / kThreadsX) * kRy; // 32
 constexpr size_t kMaxLds = 64 * 1024;
 
 struct conv_args {
@@ -80,16 +80,17 @@ __device__ __forceinline__ int plane_tap(int aStep, int aKz)
 	return std::is_same_v<Acc, float> ? aKz - 1 - aStep : aStep;
 }
 
-template<class Acc, int KX>
+template<class Acc, int KX, int RY>
 __global__ __launch_bounds__(kThreads) void conv_tiled_kernel(conv_args a)
 {
+	constexpr int kTy = (kThreads / kThreadsX) * RY;  // output rows per tile
 	extern __shared__ __align__(16) char smem[];
 	const int kx = KX > 0 ? KX : a.ker.extent[0];
 	const int ky = a.ker.extent[1];
 	const int kz = a.ker.extent[2];
 	const int kw = a.ker.extent[3];
 	const int cols = kTileX + kx - 1;
-	const int rows = kTileY + ky - 1;
+	const int rows = kTy + ky - 1;
 	Acc* tile = reinterpret_cast<Acc*>(smem);
 	Acc* coef = tile + rows * cols;
 
@@ -101,14 +102,14 @@ __global__ __launch_bounds__(kThreads) void conv_tiled_kernel(conv_args a)
 	const int pz = static_cast<int>(plane % static_cast<unsigned>(a.dst.extent[2]));
 	const int pw = static_cast<int>(plane / static_cast<unsigned>(a.dst.extent[2]));
 	const int x0 = static_cast<int>(tile_x) * kTileX;
-	const int y0 = static_cast<int>(tile_y) * kTileY;
+	const int y0 = static_cast<int>(tile_y) * kTy;
 	const int tid = threadIdx.x;
 	const int tx = tid % kThreadsX;
 	const int ty = tid / kThreadsX;
 
-	Acc acc[kRy][kRx];
+	Acc acc[RY][kRx];
 #pragma unroll
-	for (int ry = 0; ry < kRy; ++ry) {
+	for (int ry = 0; ry < RY; ++ry) {
 #pragma unroll
 		for (int rx = 0; rx < kRx; ++rx) {
 			acc[ry][rx] = Acc(0);
@@ -163,33 +164,43 @@ __global__ __launch_bounds__(kThreads) void conv_tiled_kernel(conv_args a)
 				}
 			});
 			__syncthreads();
-			for (int qy = 0; qy < ky; ++qy) {
-				const Acc* krow = coef + qy * kx;
-				if constexpr (KX > 0) {
-					constexpr int span = kRx + KX - 1;
+			if constexpr (KX > 0) {
+				// Tile row ty * RY + rho meets output row ry through kernel row
+				// qy = ry + ky - 1 - rho. Rows are taken from the last down, so
+				// each output's chain still runs qy ascending, x taps ascending
+				// (the reference's order), and every window is read from LDS once.
+				constexpr int span = kRx + KX - 1;
+				for (int rho = RY + ky - 2; rho >= 0; --rho) {
+					const Acc* row = tile + (ty * RY + rho) * cols + tx * kRx;
+					Acc window[span];
 #pragma unroll
-					for (int ry = 0; ry < kRy; ++ry) {
-						const Acc* row = tile + (ty * kRy + ry + (ky - 1 - qy)) * cols + tx * kRx;
-						Acc window[span];
+					for (int j = 0; j < span; ++j) {
+						window[j] = row[j];
+					}
 #pragma unroll
-						for (int j = 0; j < span; ++j) {
-							window[j] = row[j];
-						}
+					for (int ry = 0; ry < RY; ++ry) {
+						const int qy = ry + ky - 1 - rho;
+						if (qy >= 0 && qy < ky) {
+							const Acc* krow = coef + qy * kx;
 #pragma unroll
-						for (int qx = 0; qx < KX; ++qx) {
-							const Acc k = krow[qx];
+							for (int qx = 0; qx < KX; ++qx) {
+								const Acc k = krow[qx];
 #pragma unroll
-							for (int rx = 0; rx < kRx; ++rx) {
-								acc[ry][rx] = fma(window[rx + KX - 1 - qx], k, acc[ry][rx]);
+								for (int rx = 0; rx < kRx; ++rx) {
+									acc[ry][rx] = fma(window[rx + KX - 1 - qx], k, acc[ry][rx]);
+								}
 							}
 						}
 					}
-				} else {
+				}
+			} else {
+				for (int qy = 0; qy < ky; ++qy) {
+					const Acc* krow = coef + qy * kx;
 					for (int qx = 0; qx < kx; ++qx) {
 						const Acc k = krow[qx];
 #pragma unroll
-						for (int ry = 0; ry < kRy; ++ry) {
-							const Acc* row = tile + (ty * kRy + ry + (ky - 1 - qy)) * cols + tx * kRx + (kx - 1 - qx);
+						for (int ry = 0; ry < RY; ++ry) {
+							const Acc* row = tile + (ty * RY + ry + (ky - 1 - qy)) * cols + tx * kRx + (kx - 1 - qx);
 #pragma unroll
 							for (int rx = 0; rx < kRx; ++rx) {
 								acc[ry][rx] = fma(row[rx], k, acc[ry][rx]);
@@ -203,8 +214,8 @@ __global__ __launch_bounds__(kThreads) void conv_tiled_kernel(conv_args a)
 
 	char* out = a.dst.ptr + pz * a.dst.stride[2] + pw * a.dst.stride[3];
 #pragma unroll
-	for (int ry = 0; ry < kRy; ++ry) {
-		const int py = y0 + ty * kRy + ry;
+	for (int ry = 0; ry < RY; ++ry) {
+		const int py = y0 + ty * RY + ry;
 		if (py >= a.dst.extent[1]) {
 			continue;
 		}
@@ -1934,9 +1945,13 @@ bool launch_plane(conv_args& a, hipStream_t aStream)
 }
 
 template<class Acc, int KX>
-void launch_tiled(const conv_args& a, size_t aLds, unsigned aBlocks, hipStream_t aStream)
+void launch_tiled(const conv_args& a, int aRows, size_t aLds, unsigned aBlocks, hipStream_t aStream)
 {
-	conv_tiled_kernel<Acc, KX><<<aBlocks, kThreads, aLds, aStream>>>(a);
+	if (aRows == 4) {
+		conv_tiled_kernel<Acc, KX, 4><<<aBlocks, kThreads, aLds, aStream>>>(a);
+	} else {
+		conv_tiled_kernel<Acc, KX, 2><<<aBlocks, kThreads, aLds, aStream>>>(a);
+	}
 }
 
 template<class Acc>
@@ -1944,8 +1959,14 @@ void launch_conv(conv_args a, hipStream_t aStream, bool aForceSimple)
 {
 	const int kx = a.ker.extent[0];
 	const int ky = a.ker.extent[1];
-	const size_t lds = sizeof(Acc) * (static_cast<size_t>(kTileY + ky - 1) * (kTileX + kx - 1) +
-		static_cast<size_t>(kx) * ky);
+	// 4 output rows per thread (64-row tiles: each LDS window row feeds more
+	// outputs) when the tile fits, else 2 (32-row tiles)
+	const auto tile_lds = [&](int aRows) {
+		return sizeof(Acc) * (static_cast<size_t>((kThreads / kThreadsX) * aRows + ky - 1) * (kTileX + kx - 1) +
+			static_cast<size_t>(kx) * ky);
+	};
+	const int rows = tile_lds(4) <= kMaxLds ? 4 : 2;
+	const size_t lds = tile_lds(rows);
 	const long long count = element_count(a.dst);
 	if (!aForceSimple && tune_get("conv.stream", 1) != 0 &&
 		(launch_column<Acc>(a, aStream) || launch_plane<Acc>(a, aStream) || launch_stream<Acc>(a, aStream))) {
@@ -1956,22 +1977,22 @@ void launch_conv(conv_args a, hipStream_t aStream, bool aForceSimple)
 		return;
 	}
 	a.tiles_x = ceil_div(a.dst.extent[0], kTileX);
-	a.tiles_y = ceil_div(a.dst.extent[1], kTileY);
+	a.tiles_y = ceil_div(a.dst.extent[1], (kThreads / kThreadsX) * rows);
 	const long long blocks = static_cast<long long>(a.tiles_x) * a.tiles_y * a.dst.extent[2] * a.dst.extent[3];
 	if (blocks > 0x7fffffffLL) {
 		fail(IANE_ERR_UNSUPPORTED, "convolution exceeds the grid size limit");
 	}
 	const unsigned nb = static_cast<unsigned>(blocks);
 	switch (kx) {
-	case 1: launch_tiled<Acc, 1>(a, lds, nb, aStream); break;
-	case 3: launch_tiled<Acc, 3>(a, lds, nb, aStream); break;
-	case 5: launch_tiled<Acc, 5>(a, lds, nb, aStream); break;
-	case 7: launch_tiled<Acc, 7>(a, lds, nb, aStream); break;
-	case 9: launch_tiled<Acc, 9>(a, lds, nb, aStream); break;
-	case 11: launch_tiled<Acc, 11>(a, lds, nb, aStream); break;
-	case 13: launch_tiled<Acc, 13>(a, lds, nb, aStream); break;
-	case 15: launch_tiled<Acc, 15>(a, lds, nb, aStream); break;
-	default: launch_tiled<Acc, 0>(a, lds, nb, aStream); break;
+	case 1: launch_tiled<Acc, 1>(a, rows, lds, nb, aStream); break;
+	case 3: launch_tiled<Acc, 3>(a, rows, lds, nb, aStream); break;
+	case 5: launch_tiled<Acc, 5>(a, rows, lds, nb, aStream); break;
+	case 7: launch_tiled<Acc, 7>(a, rows, lds, nb, aStream); break;
+	case 9: launch_tiled<Acc, 9>(a, rows, lds, nb, aStream); break;
+	case 11: launch_tiled<Acc, 11>(a, rows, lds, nb, aStream); break;
+	case 13: launch_tiled<Acc, 13>(a, rows, lds, nb, aStream); break;
+	case 15: launch_tiled<Acc, 15>(a, rows, lds, nb, aStream); break;
+	default: launch_tiled<Acc, 0>(a, rows, lds, nb, aStream); break;
 	}
 }
 
