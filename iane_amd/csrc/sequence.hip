@@ -250,17 +250,43 @@ __device__ __forceinline__ Acc shfl_xor(Acc aValue, int aMask)
 // neutral element where that index is >= aCount. Every load is issued before
 // any is used: indices past the end are clamped to the last element (a valid
 // address) and replaced afterwards, so there is no branch between the loads.
+// Byte offsets within one load or store span (a tile or a short line) are
+// 24-bit multiplies when the step is below 2^22 bytes and every offset of the
+// span stays below 2^31 (every packed line and the axis-0 lines of ordinary
+// views), instead of 64-bit arithmetic per element.
+constexpr long long kSeqSmallStep = 1LL << 22;
+
+__host__ __device__ __forceinline__ bool small_span(long long aStep, long long aCount)
+{
+	const long long step = aStep < 0 ? -aStep : aStep;
+	return step < kSeqSmallStep && aCount < (1LL << 23) && step * aCount < (1LL << 31);
+}
+
 template<class Acc, int OP, int PER>
 __device__ __forceinline__ void load_line(const seq_map& m, long long aBase, long long aBegin, long long aCount,
 	long long aFirst, long long aStep, int aPer, Acc (&aOut)[PER])
 {
+	const char* span = m.src + src_offset(m, aBase, aBegin);
+	const bool small = small_span(m.src_step, aCount);
+	const int step = static_cast<int>(m.src_step);
 	with_element_type(m.src_type, [&](auto aType) {
 		using T = decltype(aType);
 		T raw[PER];
+		if (small) {
+			const int first = static_cast<int>(aFirst);
+			const int stride = static_cast<int>(aStep);
+			const int last = static_cast<int>(aCount - 1);
 #pragma unroll
-		for (int e = 0; e < PER; ++e) {
-			const long long j = min(aFirst + e * aStep, aCount - 1);
-			raw[e] = *reinterpret_cast<const T*>(m.src + src_offset(m, aBase, aBegin + j));
+			for (int e = 0; e < PER; ++e) {
+				const int j = min(first + e * stride, last);
+				raw[e] = *reinterpret_cast<const T*>(span + __mul24(j, step));
+			}
+		} else {
+#pragma unroll
+			for (int e = 0; e < PER; ++e) {
+				const long long j = min(aFirst + e * aStep, aCount - 1);
+				raw[e] = *reinterpret_cast<const T*>(span + j * m.src_step);
+			}
 		}
 #pragma unroll
 		for (int e = 0; e < PER; ++e) {
@@ -299,6 +325,63 @@ __global__ __launch_bounds__(kSeqThreads) void row_reduce_wave_kernel(seq_map m,
 	if (lane == 0) {
 		store_value(m.dst + line_base(m.dst_other_stride, m, line), apply<Acc, OP>(aInit, acc));
 	}
+}
+
+// Short contiguous lines (a step of one element, every line start 16-byte
+// aligned, the length a whole number of 16-byte vectors): a wave takes
+// 8 / VPL lines; each lane loads vectors lane + 64 i (i < VPL) of every one of
+// them before any is used (8 x 16 bytes in flight per lane instead of one
+// element per load), folds its vectors in order, then a butterfly per line;
+// lane 0 stores init (+) total.
+template<class Acc, int OP, int VPL>
+__global__ __launch_bounds__(kSeqThreads) void row_reduce_vec_kernel(seq_map m, Acc aInit)
+{
+	constexpr int kLines = 8 / VPL;
+	using vec_t = unsigned __attribute__((ext_vector_type(4)));
+	const int lane = threadIdx.x & 63;
+	const long long first = (static_cast<long long>(blockIdx.x) * (kSeqThreads / 64) + (threadIdx.x >> 6)) * kLines;
+	if (first >= m.lines) {
+		return; // uniform over the wave
+	}
+	with_element_type(m.src_type, [&](auto aType) {
+		using T = decltype(aType);
+		constexpr int V = 16 / static_cast<int>(sizeof(T));
+		struct parts {
+			T e[V];
+		};
+		const int vectors = static_cast<int>(m.length / V);
+		vec_t raw[kLines][VPL];
+#pragma unroll
+		for (int l = 0; l < kLines; ++l) {
+			const long long line = min(first + l, m.lines - 1);
+			const char* base = m.src + line_base(m.src_other_stride, m, line);
+#pragma unroll
+			for (int i = 0; i < VPL; ++i) {
+				const int v = min(lane + 64 * i, vectors - 1);
+				raw[l][i] = *reinterpret_cast<const vec_t*>(base + 16LL * v);
+			}
+		}
+#pragma unroll
+		for (int l = 0; l < kLines; ++l) {
+			Acc acc = identity<Acc, OP>();
+#pragma unroll
+			for (int i = 0; i < VPL; ++i) {
+				const parts p = __builtin_bit_cast(parts, raw[l][i]);
+				const bool valid = lane + 64 * i < vectors;
+#pragma unroll
+				for (int j = 0; j < V; ++j) {
+					acc = apply<Acc, OP>(acc, valid ? static_cast<Acc>(p.e[j]) : identity<Acc, OP>());
+				}
+			}
+#pragma unroll
+			for (int mask = 32; mask > 0; mask >>= 1) {
+				acc = apply<Acc, OP>(acc, shfl_xor(acc, mask));
+			}
+			if (lane == 0 && first + l < m.lines) {
+				store_value(m.dst + line_base(m.dst_other_stride, m, first + l), apply<Acc, OP>(aInit, acc));
+			}
+		}
+	});
 }
 
 // Totals of tiles of 256 * aPer elements (all tiles for a reduction; all but the last for a scan);
@@ -431,11 +514,23 @@ __global__ __launch_bounds__(kSeqThreads) void row_output_kernel(seq_map m, cons
 		}
 	}
 	__syncthreads();
+	char* span = m.dst + dst_offset(m, dst_base, begin);
+	if (small_span(m.dst_step, n)) {
+		const int step = static_cast<int>(m.dst_step);
 #pragma unroll
-	for (int e = 0; e < PER; ++e) {
-		const int j = threadIdx.x + e * kSeqThreads;
-		if (e < aPer && j < n) {
-			store_value(m.dst + dst_offset(m, dst_base, begin + j), stage[slot(j)]);
+		for (int e = 0; e < PER; ++e) {
+			const int j = threadIdx.x + e * kSeqThreads;
+			if (e < aPer && j < n) {
+				store_value(span + __mul24(j, step), stage[slot(j)]);
+			}
+		}
+	} else {
+#pragma unroll
+		for (int e = 0; e < PER; ++e) {
+			const int j = threadIdx.x + e * kSeqThreads;
+			if (e < aPer && j < n) {
+				store_value(span + j * m.dst_step, stage[slot(j)]);
+			}
 		}
 	}
 }
@@ -545,6 +640,24 @@ unsigned grid_of(long long aBlocks)
 	return static_cast<unsigned>(aBlocks);
 }
 
+// Vectors per lane of row_reduce_vec_kernel for this run's lines (1, 2 or 4),
+// or 0 when the lines are not contiguous, 16-byte aligned whole vectors of at
+// most 256 vectors.
+int vector_lines(const seq_map& m)
+{
+	const long long size = dtype_size(m.src_type);
+	if (m.src_step != size || (m.length * size) % 16 != 0 || reinterpret_cast<uintptr_t>(m.src) % 16 != 0) {
+		return 0;
+	}
+	for (int k = 0; k < m.other_rank; ++k) {
+		if (m.src_other_stride[k] % 16 != 0) {
+			return 0;
+		}
+	}
+	const long long vectors = m.length * size / 16;
+	return vectors <= 64 ? 1 : (vectors <= 128 ? 2 : (vectors <= 256 ? 4 : 0));
+}
+
 // Tile totals with as many loads per thread as the tile needs (4 or 16).
 template<class Acc, int OP>
 void launch_row_totals(const iane_sequence_plan& p, const seq_map& m, Acc* aPartials, long long aTasks, int aSkipLast,
@@ -577,6 +690,19 @@ void run_kind(const iane_sequence_plan& p, const seq_map& m, void* aPartials, hi
 	if (p.row) {
 		const long long tasks = m.lines * p.tiles;
 		if constexpr (KIND == IANE_SEQ_REDUCE) {
+			const int vpl = vector_lines(m);
+			if (p.wave_reduce && vpl > 0) {
+				const long long lines_per_block = 4LL * (8 / vpl);
+				const unsigned grid = grid_of((m.lines + lines_per_block - 1) / lines_per_block);
+				if (vpl == 1) {
+					row_reduce_vec_kernel<Acc, OP, 1><<<grid, kSeqThreads, 0, s>>>(m, init);
+				} else if (vpl == 2) {
+					row_reduce_vec_kernel<Acc, OP, 2><<<grid, kSeqThreads, 0, s>>>(m, init);
+				} else {
+					row_reduce_vec_kernel<Acc, OP, 4><<<grid, kSeqThreads, 0, s>>>(m, init);
+				}
+				return;
+			}
 			if (p.wave_reduce) {
 				const unsigned grid = grid_of((m.lines + 3) / 4);
 				if (m.length <= 64 * 4) {
