@@ -376,30 +376,59 @@ def bench_rl_batch(timer, stream, iterations: int = 100):
 def bench_sequence(timer, stream, vol):
     """Scan / reduce on the 256^3 f32 volume: a whole f64-accumulated sum (the
     log-likelihood reduction pattern; 4 B read per voxel), an f32 inclusive scan
-    in storage order (4 B read + 4 B written) and per-row sums along x."""
+    in storage order (4 B read + 4 B written) and per-row sums along x. `us` is
+    the host-paced time per call (one Python call each); `graph_us` the GPU time
+    per call of 20 calls captured in a HIP graph and replayed (each replay checked
+    equal to the eager result), which `frac_hbm` uses."""
     import torch
     import iane_amd
     policy = iane_amd.hip_policy(stream)
     ctx = iane_amd.make_execution_context(policy)
+    capture = torch.cuda.Stream()
+    capture_ctx = iane_amd.make_execution_context(iane_amd.hip_policy(capture.cuda_stream))
     voxels = vol.numel()
     total = torch.empty(1, dtype=torch.float64, device="cuda")
     scan = torch.empty_like(vol)
     rows = torch.empty(vol.shape[:-1], dtype=torch.float32, device="cuda")
     torch.cuda.synchronize()
-    legs = (("reduce_sum_f32_to_f64", iane_amd.reduce("sum", 0.0), total, 4.0),
-            ("inclusive_scan_sum_f32", iane_amd.inclusive_scan("sum", 0.0), scan, 8.0),
-            ("reduce_along_x_sum_f32", iane_amd.reduce_along(0, "sum", 0.0), rows, 4.0 + 4.0 / vol.shape[-1]))
+    legs = (("reduce_sum_f32_to_f64", lambda: iane_amd.reduce("sum", 0.0), total, 4.0),
+            ("inclusive_scan_sum_f32", lambda: iane_amd.inclusive_scan("sum", 0.0), scan, 8.0),
+            ("reduce_along_x_sum_f32", lambda: iane_amd.reduce_along(0, "sum", 0.0), rows, 4.0 + 4.0 / vol.shape[-1]))
     out = {}
-    for name, algorithm, dst, bytes_per_voxel in legs:
-        op = ctx.prepare(algorithm, vol, dst)
+    k = 20
+    for name, make, dst, bytes_per_voxel in legs:
+        op = ctx.prepare(make(), vol, dst)
         fn = lambda: ctx.run(op, vol, dst)  # noqa: E731
         fn()
         _lib_sync(stream)
-        k = 20
         t = timer.time(fn, k) / k
-        gbs = voxels * bytes_per_voxel / t / 1e6
-        out[name] = {"us": round(t * 1e3, 2), "GB_s": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 3)}
-        log(f"[bench] {name}: {t * 1e3:.1f} us, {gbs:.0f} GB/s")
+        eager = dst.clone()
+        gop = capture_ctx.prepare(make(), vol, dst)
+        with torch.cuda.stream(capture):
+            capture_ctx.run(gop, vol, dst)
+        capture.synchronize()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph, stream=capture):
+            for _ in range(k):
+                capture_ctx.run(gop, vol, dst)
+        for _ in range(2):
+            dst.zero_()
+            graph.replay()
+            torch.cuda.synchronize()
+            if not torch.equal(dst, eager):
+                raise RuntimeError(f"{name}: graph replay differs from the eager call")
+        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        with torch.cuda.stream(capture):
+            start.record()
+            graph.replay()
+            graph.replay()
+            stop.record()
+        stop.synchronize()
+        tg = start.elapsed_time(stop) / (2 * k)
+        gbs = voxels * bytes_per_voxel / tg / 1e6
+        out[name] = {"us": round(t * 1e3, 2), "graph_us": round(tg * 1e3, 2), "GB_s": round(gbs, 1),
+                     "frac_hbm": round(gbs / HBM_PEAK_GBS, 3)}
+        log(f"[bench] {name}: {t * 1e3:.1f} us per call, {tg * 1e3:.1f} us in a graph, {gbs:.0f} GB/s")
     return out
 
 
