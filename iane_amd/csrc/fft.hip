@@ -36,6 +36,18 @@ bool factor_radices(int aM, fft_radix_plan& aPlan)
 		rest /= r;
 		return true;
 	};
+	// odd radices first: the first stage writes its outputs R elements apart,
+	// which an odd R spreads over distinct LDS banks
+	for (int r : {3, 5, 7, 11, 13}) {
+		while (rest % r == 0) {
+			if (!push(r)) return false;
+		}
+	}
+	for (int r = 17; r <= 64; r += 2) {
+		while (rest % r == 0) {
+			if (!push(r)) return false;
+		}
+	}
 	while (rest % 8 == 0) {
 		if (!push(8)) return false;
 	}
@@ -44,16 +56,6 @@ bool factor_radices(int aM, fft_radix_plan& aPlan)
 	}
 	while (rest % 2 == 0) {
 		if (!push(2)) return false;
-	}
-	for (int r : {3, 5, 7, 11, 13}) {
-		while (rest % r == 0) {
-			if (!push(r)) return false;
-		}
-	}
-	for (int r = 17; rest > 1 && r <= 64; r += 2) {
-		while (rest % r == 0) {
-			if (!push(r)) return false;
-		}
 	}
 	if (rest != 1) {
 		return false;
@@ -115,7 +117,7 @@ int fixed_plan_id(const fft_radix_plan& aPlan)
 		return true;
 	};
 	static const std::initializer_list<int> table[kFixedPlans] = {
-		{4, 5, 7}, {8, 5, 7}, {8, 4, 5}, {8, 8, 2}, {8, 8, 4}, {8, 8, 8}, {8, 8, 8, 2}};
+		{5, 7, 4}, {5, 7, 8}, {5, 8, 4}, {8, 8, 2}, {8, 8, 4}, {8, 8, 8}, {8, 8, 8, 2}};
 	for (int id = 1; id <= kFixedPlans; ++id) {
 		if (is(table[id - 1])) {
 			return id;

@@ -370,9 +370,9 @@ struct fixed_sequence {
 };
 template<int ID>
 struct fixed_plan;
-template<> struct fixed_plan<1> : fixed_sequence<140, 4, 5, 7> {};
-template<> struct fixed_plan<2> : fixed_sequence<280, 8, 5, 7> {};
-template<> struct fixed_plan<3> : fixed_sequence<160, 8, 4, 5> {};
+template<> struct fixed_plan<1> : fixed_sequence<140, 5, 7, 4> {};
+template<> struct fixed_plan<2> : fixed_sequence<280, 5, 7, 8> {};
+template<> struct fixed_plan<3> : fixed_sequence<160, 5, 8, 4> {};
 template<> struct fixed_plan<4> : fixed_sequence<128, 8, 8, 2> {};
 template<> struct fixed_plan<5> : fixed_sequence<256, 8, 8, 4> {};
 template<> struct fixed_plan<6> : fixed_sequence<512, 8, 8, 8> {};
