@@ -99,18 +99,20 @@ __device__ __forceinline__ void store_outputs(void* out, int out_bf16, int relu,
 }
 
 // Layers with 16 input and at most 16 output channels (the full-resolution
-// layers of the network): the same implicit GEMM, with K ordered by x shift so
-// that an input row is loaded once per wave and reused for all three x taps.
-// K = 3 shifts x 5 row pairs x 32 (two (dz, dy) rows of 16 channels; the tenth
-// row is zero padding): chunk (dx, p) holds rows 2p (lane groups 0, 1) and
-// 2p + 1 (groups 2, 3), channels 8 (g % 2) .. + 7, all at x shift dx. A lane
-// loads voxel x0 + n of its row; lane 0 of each group of 16 also loads voxel
-// x0 - 1 and lane 15 voxel x0 + 16 (the "edge" value, zero elsewhere). The
-// x - 1 and x + 1 operands are the centre values moved one lane within each
-// group of 16 (DPP row shifts), the lane left without a source (0 or 15)
-// taking its edge value. All ten loads of a row are issued before its first
-// MFMA. The weights of all 15 chunks stay in registers, loaded once for
-// kFwd16Rows consecutive y rows that the wave computes in turn.
+// layers of the network): the same implicit GEMM, with K ordered so that a
+// wave slides along y and loads each input row once. K = 3 dy x 2 chunks x
+// 3 x shifts x 32: chunk A of kernel row dy holds the rows (dz = 0, dy) (lane
+// groups 0, 1) and (dz = 1, dy) (groups 2, 3), chunk B the row (dz = 2, dy)
+// (groups 0, 1; groups 2, 3 are zero padding), channels 8 (g % 2) .. + 7. An
+// input row y' (its three dz planes) fills one slot of a four-slot register
+// ring and serves output rows y' - 1, y', y' + 1 as dy = 2, 1, 0; each output
+// row loads one new row (two 16-byte loads and their edges) ahead of its 18
+// MFMAs instead of all nine (dz, dy) rows. A lane loads voxel x0 + n of its
+// row; lane 0 of each group of 16 also loads voxel x0 - 1 and lane 15 voxel
+// x0 + 16 (the "edge" value, zero elsewhere); the x - 1 and x + 1 operands are
+// the centre values moved one lane within each group of 16 (DPP row shifts),
+// the lane left without a source (0 or 15) taking its edge value. The weights
+// of all 18 chunks stay in registers for the wave's kFwd16Rows rows.
 __device__ __forceinline__ bf16x8 shift_lanes(bf16x8 aCentre, bf16x8 aEdge, int aDx)
 {
 	using i32x4 = __attribute__((__vector_size__(4 * sizeof(int)))) int;
@@ -155,18 +157,24 @@ __global__ __launch_bounds__(64 * kConvWaves) void conv3d_fwd16_kernel(const __b
 	for (int j = 0; j < 8; ++j) {
 		zero8[j] = static_cast<__bf16>(0.0f);
 	}
-	// A operands of all chunks: output channel n, row 2p + grp / 2, tap (row, dx)
-	bf16x8 wreg[3][5];
+	// the input plane of this lane's chunk A and chunk B rows (B: none for groups 2, 3)
+	const int dz_a = grp >> 1;
+	const bool has_b = grp < 2;
+	// A operands: output channel n, tap (dz, dy, dx) of the lane's chunk row
+	bf16x8 wreg[3][2][3];
 #pragma unroll
-	for (int d = 0; d < 3; ++d) {
+	for (int dy = 0; dy < 3; ++dy) {
 #pragma unroll
-		for (int p = 0; p < 5; ++p) {
-			const int r = 2 * p + (grp >> 1);
-			const bool ok = n < g.cout && r < 9;
-			const long long at = ok ? (static_cast<long long>(n) * 27 + r * 3 + d) * kCin + ch : 0;
-			wreg[d][p] = *reinterpret_cast<const bf16x8*>(w + at);
-			if (!ok) {
-				wreg[d][p] = zero8;
+		for (int c = 0; c < 2; ++c) {
+			const int dz = c == 0 ? dz_a : 2;
+			const bool ok = n < g.cout && (c == 0 || has_b);
+#pragma unroll
+			for (int d = 0; d < 3; ++d) {
+				const long long at = ok ? (static_cast<long long>(n) * 27 + dz * 9 + dy * 3 + d) * kCin + ch : 0;
+				wreg[dy][c][d] = *reinterpret_cast<const bf16x8*>(w + at);
+				if (!ok) {
+					wreg[dy][c][d] = zero8;
+				}
 			}
 		}
 	}
@@ -178,31 +186,48 @@ __global__ __launch_bounds__(64 * kConvWaves) void conv3d_fwd16_kernel(const __b
 	const int edge_x = n == 0 ? x0 - 1 : (n == 15 ? x0 + 16 : -1);
 	const bool edge_ok = edge_x >= 0 && edge_x < g.nx;
 	const f32x4 bv = bias4(bias, g.cout, grp * 4);
-	for (int y = y0; y < min(y0 + kFwd16Rows, g.ny); ++y) {
-		bf16x8 centre[5], edge[5];
+	// ring slot of input row y0 - 1 + i: i % 4; chunk c's centre and edge
+	bf16x8 centre[4][2], edge[4][2];
+	const auto load_row = [&](int aI) {
+		const int yi = y0 - 1 + aI;
 #pragma unroll
-		for (int p = 0; p < 5; ++p) {
-			const int r = 2 * p + (grp >> 1);
-			const int yi = y + (r % 3) - 1;
-			const int zi = z + (r / 3) - 1;
-			const bool row_ok = r < 9 && yi >= 0 && yi < g.ny && zi >= 0 && zi < g.nz;
+		for (int c = 0; c < 2; ++c) {
+			const int zi = z + (c == 0 ? dz_a : 2) - 1;
+			const bool row_ok = (c == 0 || has_b) && yi >= 0 && yi < g.ny && zi >= 0 && zi < g.nz;
 			// byte offsets into the volume's buffer resource; past its end reads 0
 			const int row = ((zi * g.ny + yi) * g.nx) * (kCin * 2) + ch * 2;
 			const int centre_at = row_ok && x < g.nx ? row + x * (kCin * 2) : kOobOffset;
 			const int edge_at = row_ok && edge_ok ? row + edge_x * (kCin * 2) : kOobOffset;
-			centre[p] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(volume, centre_at, 0, 0));
-			edge[p] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(volume, edge_at, 0, 0));
+			centre[aI % 4][c] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(volume, centre_at, 0, 0));
+			edge[aI % 4][c] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(volume, edge_at, 0, 0));
 		}
-		// keep the loads ahead of the MFMAs (each waits only for its own operands)
+	};
+	load_row(0);
+	load_row(1);
+	load_row(2);
+#pragma unroll
+	for (int j = 0; j < kFwd16Rows; ++j) {
+		const int y = y0 + j;
+		// the row two ahead into the free slot, then this row's products
+		// (each MFMA waits only for its own operands)
+		if (j + 3 < kFwd16Rows + 2) {
+			load_row(j + 3);
+		}
 		__builtin_amdgcn_sched_barrier(0);
 		f32x4 acc = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-		for (int p = 0; p < 5; ++p) {
-			acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wreg[0][p], shift_lanes(centre[p], edge[p], -1), acc, 0, 0, 0);
-			acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wreg[1][p], centre[p], acc, 0, 0, 0);
-			acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wreg[2][p], shift_lanes(centre[p], edge[p], 1), acc, 0, 0, 0);
+		for (int dy = 0; dy < 3; ++dy) {
+			const int slot = (j + dy) % 4;
+#pragma unroll
+			for (int c = 0; c < 2; ++c) {
+				acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wreg[dy][c][0], shift_lanes(centre[slot][c], edge[slot][c], -1),
+					acc, 0, 0, 0);
+				acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wreg[dy][c][1], centre[slot][c], acc, 0, 0, 0);
+				acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wreg[dy][c][2], shift_lanes(centre[slot][c], edge[slot][c], 1),
+					acc, 0, 0, 0);
+			}
 		}
-		if (x < g.nx) {
+		if (x < g.nx && y < g.ny) {
 			const long long voxel = (static_cast<long long>(z) * g.ny + y) * g.nx + x;
 			store_outputs(out, out_bf16, relu, bv, voxel, g.cout, grp * 4, acc);
 		}
