@@ -435,6 +435,22 @@ def test_plane_kernel_equals_tiled(shape, border, k):
             assert np.max(np.abs(outs[0] - ref)) <= TOL_FP32 * np.max(np.abs(ref))
 
 
+@pytest.mark.parametrize("shape,kshape", [((70, 90), (31, 31)), ((9, 40, 52), (3, 21, 27)), ((45, 77), (17, 23))])
+@pytest.mark.parametrize("border", [cases.CLAMP, cases.ZERO])
+def test_tiled_kernel_tile_heights(shape, kshape, border):
+    """Kernels large enough that the FP64 64 x 64 tile's LDS would not fit run the
+    tiled kernel with 64 x 32 tiles (and FP32 still with 64 x 64): REFERENCE
+    precision bit for bit against the oracle, FP32 within TOL_FP32."""
+    rng = np.random.default_rng(sum(shape) + sum(kshape) + border)
+    src = (rng.random(shape) * 10 - 2).astype(np.float32)
+    ker = (rng.random(kshape) - 0.4).astype(np.float32)
+    same_bits(gpu_convolve(src, ker, np.float32, border, precision.reference),
+              oracle.convolve(src, ker, np.float32, border))
+    ref = oracle.convolve(src, ker, np.float64, border)
+    got = gpu_convolve(src, ker, np.float32, border, precision.fp32)
+    assert np.max(np.abs(got - ref)) <= TOL_FP32 * np.max(np.abs(ref))
+
+
 def test_plane_kernel_views():
     """Views the plane kernel must decline or handle: rows that are not 16-byte
     aligned, a strided x axis, an x extent not a multiple of 4, a strided
