@@ -28,6 +28,7 @@ gives the same bits; tests compare them):
     conv.column_rows    4 | 2              output rows per lane of the column kernel
     conv.column_chunk   0 (auto) | >= 1    output planes per column workgroup
     conv.column_dot2    1 | 0              u16 column sums on packed v_dot2_u32_u16 pairs
+    conv.column_alternate 1 | 0            exact integer column sums: odd z chunks march down
     conv.plane          1 | 0              register plane kernel for square single-plane kernels
                                            (3 .. 9 taps) on 4-aligned rows (0: the tiled kernel)
     conv.plane_rows     0 (auto) | 1, 2, 4 output rows per lane of the plane kernel

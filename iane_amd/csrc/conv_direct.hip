@@ -53,6 +53,7 @@ struct conv_args {
 	int src_span; // bytes of one (x, y) plane (streaming kernels)
 	int dst_span;
 	int column_dot2; // column kernel: u16 images with small nonnegative taps on v_dot2_u32_u16
+	int column_alternate; // column kernel, exact integer sums: odd z chunks march down
 };
 
 // Sample the source at an integer coordinate with the border rule; returns
@@ -856,6 +857,8 @@ __device__ __forceinline__ T column_from_right(T aEdge, T aValue)
 struct column_geom {
 	int x0, x, tx, yb, z0, z1;
 	int ext_x, ext_y, ext_z;
+	int chunk;
+	int step;  // +1: the chunk is marched up in z, -1: down
 	bool clamp;
 
 	template<int R>
@@ -878,10 +881,17 @@ struct column_geom {
 		g.yb = static_cast<int>(tile_y) * (16 * R) + group * R;
 		g.z0 = chunk * aChunk;
 		g.z1 = min(g.z0 + aChunk, g.ext_z);
+		g.chunk = chunk;
+		g.step = 1;
 		g.clamp = a.border == IANE_BORDER_CLAMP;
 		return g;
 	}
 };
+
+__device__ __forceinline__ bool column_in_chunk(const column_geom& g, int aZ)
+{
+	return aZ >= g.z0 && aZ < g.z1;
+}
 
 // Coefficient qx + 3 qy + 9 qz of a 3^3 kernel, the same for every lane: read
 // into scalar registers, converted like load_as_int / load_as_float, with one
@@ -1032,19 +1042,22 @@ __device__ __forceinline__ void column_march(const conv_args& a, const column_ge
 		}
 	};
 
-	// Input planes first .. first + n - 1 = z0 - 1 .. z1. Plane first + i adds
-	// kernel plane 2 to output first + i + 1 (set i % 3), kernel plane 1 to
-	// output first + i (set (i + 2) % 3) and kernel plane 0 to output
-	// first + i - 1 (set (i + 1) % 3), which is then complete. The loop is
+	// Input planes first, first + step, .. (n of them: z0 - 1 .. z1 marching up,
+	// z1 .. z0 - 1 marching down). Plane p = first + step i adds kernel plane 2
+	// to output p + step (set i % 3), kernel plane 1 to output p (set
+	// (i + 2) % 3) and kernel plane 0 to output p - step (set (i + 1) % 3),
+	// which is then complete; a downward march takes the kernel with its z
+	// planes swapped (only exact integer sums, whose order is free). The loop is
 	// unrolled by 6 (a multiple of the ring and of P) so sets and staging slots
 	// are registers; the skipped terms are those of outputs outside the chunk.
-	const int first = g.z0 - 1;
+	const int step = g.step;
+	const int first = step > 0 ? g.z0 - 1 : g.z1;
 	const int n = g.z1 - g.z0 + 2;
 	raw_t vec[P][rows];
 	unsigned edge[P][rows];
 #pragma unroll
 	for (int q = 0; q < P; ++q) {
-		load_plane(first + q, q < n, vec[q], edge[q]);
+		load_plane(first + step * q, q < n, vec[q], edge[q]);
 	}
 	for (int i0 = 0; i0 < n; i0 += 6) {
 #pragma unroll
@@ -1056,17 +1069,17 @@ __device__ __forceinline__ void column_march(const conv_args& a, const column_ge
 				for (int j = 0; j < rows; ++j) {
 					window(vec[ph % P][j], edge[ph % P][j], win[j]);
 				}
-				load_plane(first + i + P, i + P < n, vec[ph % P], edge[ph % P]);
-				const int p = first + i;
-				if (p + 1 < g.z1) {
+				const int p = first + step * i;
+				load_plane(p + step * P, i + P < n, vec[ph % P], edge[ph % P]);
+				if (column_in_chunk(g, p + step)) {
 					add_plane(win, acc[ph % 3], 2);
 				}
-				if (p >= g.z0 && p < g.z1) {
+				if (column_in_chunk(g, p)) {
 					add_plane(win, acc[(ph + 2) % 3], 1);
 				}
-				if (p - 1 >= g.z0) {
+				if (column_in_chunk(g, p - step)) {
 					add_plane(win, acc[(ph + 1) % 3], 0);
-					store_plane(p - 1, acc[(ph + 1) % 3]);
+					store_plane(p - step, acc[(ph + 1) % 3]);
 				}
 			}
 		}
@@ -1205,15 +1218,16 @@ __device__ __forceinline__ void column_march_dot2(const conv_args& a, const colu
 		}
 	};
 
-	// the march of column_march: plane first + i adds kernel plane 2 to output
-	// first + i + 1, plane 1 to output first + i, plane 0 to output first + i - 1
-	const int first = g.z0 - 1;
+	// the march of column_march: plane p = first + step i adds kernel plane 2
+	// to output p + step, plane 1 to output p, plane 0 to output p - step
+	const int step = g.step;
+	const int first = step > 0 ? g.z0 - 1 : g.z1;
 	const int n = g.z1 - g.z0 + 2;
 	column_v2 vec[P][rows];
 	unsigned edge[P][rows];
 #pragma unroll
 	for (int q = 0; q < P; ++q) {
-		load_plane(first + q, q < n, vec[q], edge[q]);
+		load_plane(first + step * q, q < n, vec[q], edge[q]);
 	}
 	for (int i0 = 0; i0 < n; i0 += 6) {
 #pragma unroll
@@ -1225,17 +1239,17 @@ __device__ __forceinline__ void column_march_dot2(const conv_args& a, const colu
 				for (int j = 0; j < rows; ++j) {
 					win[j] = window(vec[ph % P][j], edge[ph % P][j]);
 				}
-				load_plane(first + i + P, i + P < n, vec[ph % P], edge[ph % P]);
-				const int p = first + i;
-				if (p + 1 < g.z1) {
+				const int p = first + step * i;
+				load_plane(p + step * P, i + P < n, vec[ph % P], edge[ph % P]);
+				if (column_in_chunk(g, p + step)) {
 					add_plane(win, acc[ph % 3], 2);
 				}
-				if (p >= g.z0 && p < g.z1) {
+				if (column_in_chunk(g, p)) {
 					add_plane(win, acc[(ph + 2) % 3], 1);
 				}
-				if (p - 1 >= g.z0) {
+				if (column_in_chunk(g, p - step)) {
 					add_plane(win, acc[(ph + 1) % 3], 0);
-					store_plane(p - 1, acc[(ph + 1) % 3]);
+					store_plane(p - step, acc[(ph + 1) % 3]);
 				}
 			}
 		}
@@ -1304,6 +1318,24 @@ __global__ __launch_bounds__(kColumnThreads) void conv_column_kernel(conv_args a
 		// integer the reference's FP64 chain represents.
 		int k[27];
 		column_coefficients(a, k);
+		// Exact sums do not depend on the order of their terms, so odd chunks
+		// march down (the kernel's z planes swapped): two neighbouring chunks
+		// then read their two shared planes at the same time, at both ends,
+		// where the second read finds them in L2 (marching the same way, one
+		// reads them a whole chunk after the other).
+		column_geom gz = g;
+		if (a.column_alternate && (g.chunk & 1)) {
+			gz.step = -1;
+			int flipped[27];
+#pragma unroll
+			for (int i = 0; i < 27; ++i) {
+				flipped[i] = k[i % 9 + 9 * (2 - i / 9)];
+			}
+#pragma unroll
+			for (int i = 0; i < 27; ++i) {
+				k[i] = flipped[i];
+			}
+		}
 		long long total = 0;
 		long long largest = 0;
 #pragma unroll
@@ -1321,7 +1353,7 @@ __global__ __launch_bounds__(kColumnThreads) void conv_column_kernel(conv_args a
 		if (std::is_same_v<S, uint16_t> && small_nonnegative && total * source_max < (1LL << 31) &&
 			a.column_dot2) {
 			if constexpr (std::is_same_v<S, uint16_t>) {
-				column_march_dot2<D, R, P>(a, g, k);
+				column_march_dot2<D, R, P>(a, gz, k);
 			}
 		} else if (total * source_max < (1LL << 24)) {
 			float kf[27];
@@ -1329,9 +1361,9 @@ __global__ __launch_bounds__(kColumnThreads) void conv_column_kernel(conv_args a
 			for (int i = 0; i < 27; ++i) {
 				kf[i] = static_cast<float>(k[i]);
 			}
-			column_march<S, D, float, true, R, P>(a, g, kf);
+			column_march<S, D, float, true, R, P>(a, gz, kf);
 		} else if (largest < (1LL << 23) && total * source_max < (1LL << 31)) {
-			column_march<S, D, int, true, R, P>(a, g, k);
+			column_march<S, D, int, true, R, P>(a, gz, k);
 		} else {
 			column_fp64_chain<S, D, R>(a, g);
 		}
@@ -1807,6 +1839,7 @@ bool launch_column(conv_args& a, hipStream_t aStream)
 		return false;
 	}
 	a.column_dot2 = tune_get("conv.column_dot2", 1) != 0;
+	a.column_alternate = tune_get("conv.column_alternate", 1) != 0;
 	const int r = tune_get("conv.column_rows", 4) == 2 ? 2 : 4;
 	a.tiles_x = ceil_div(s.extent[0], 64);
 	a.tiles_y = ceil_div(s.extent[1], 16 * r);

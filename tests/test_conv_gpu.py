@@ -364,7 +364,8 @@ def test_column_kernel_equals_tiled(shape, border):
     FP32 sums and for integer images with integer kernels, whether the int32 sum is
     exact in FP32 (binomial-like kernels), in int32 only, or coefficients up to 2^21
     force the FP64 chain; both rows-per-lane settings and z chunks of 1, 5 and the
-    default (ragged last chunks included)."""
+    default (ragged last chunks included); integer sums with odd chunks marching
+    down (the default) and with every chunk marching up."""
     from iane_amd import autotune
     rng = np.random.default_rng(sum(shape) + border)
     for sdt, kdt, ddt, prec in [(np.float32, np.float32, np.float32, precision.fp32),
@@ -383,7 +384,8 @@ def test_column_kernel_equals_tiled(shape, border):
             src, ker = _case(rng, shape, (3, 3, 3), sdt, kdt)
         outs = []
         for config in ({"conv.stream": 0}, {"conv.column": 0}, {}, {"conv.column_rows": 2},
-                       {"conv.column_chunk": 1}, {"conv.column_chunk": 5}):
+                       {"conv.column_chunk": 1}, {"conv.column_chunk": 5},
+                       {"conv.column_chunk": 5, "conv.column_alternate": 0}):
             with autotune.applied(config):
                 outs.append(gpu_convolve(src, ker, ddt, border, prec))
         for out in outs[1:]:
@@ -612,6 +614,10 @@ def test_column_u16_dot2_path(shape, border, taps):
         got = gpu_convolve(src, ker, np.int32, border)
     same_bits(got, ref)
     with autotune.applied({"conv.column": 1, "conv.column_dot2": 0}):
+        same_bits(gpu_convolve(src, ker, np.int32, border), ref)
+    with autotune.applied({"conv.column": 1, "conv.column_chunk": 3, "conv.column_alternate": 0}):
+        same_bits(gpu_convolve(src, ker, np.int32, border), ref)
+    with autotune.applied({"conv.column": 1, "conv.column_chunk": 3}):
         same_bits(gpu_convolve(src, ker, np.int32, border), ref)
     if taps == "binomial":
         with autotune.applied({"conv.column": 1}):
