@@ -68,11 +68,15 @@ struct seq_map {
 	long long dst_other_stride[3];
 	long long src_step, dst_step; // byte stride along the line
 	long long lines, length;
+	bool nested;                  // other axes nest like one axis (stride k+1 = stride k x extent k) in both views
 };
 
 // Line index -> byte offset of the line's first element (lines < 2^31).
 __device__ __forceinline__ long long line_base(const long long* aStrides, const seq_map& m, long long aLine)
 {
+	if (m.nested) {
+		return aLine * aStrides[0];
+	}
 	unsigned rest = static_cast<unsigned>(aLine);
 	long long offset = 0;
 	for (int k = 0; k < m.other_rank; ++k) {
@@ -416,8 +420,9 @@ __global__ __launch_bounds__(kSeqThreads) void row_totals_kernel(seq_map m, Acc*
 }
 
 // Carries of one line: slot k <- init (+) totals of tiles 0..k-1 (scans), or the
-// line's result (reduction, written to the destination). Each thread takes a
-// segment of kSeqPer consecutive totals, loaded together.
+// line's result (reduction, written to the destination). Each thread takes
+// kSeqPer totals, loaded together: a consecutive segment (scans) or every
+// 256th (reductions).
 template<class Acc, int OP, int KIND>
 __global__ __launch_bounds__(kSeqThreads) void row_carries_kernel(seq_map m, Acc* aPartials, long long aTiles, Acc aInit)
 {
@@ -427,18 +432,23 @@ __global__ __launch_bounds__(kSeqThreads) void row_carries_kernel(seq_map m, Acc
 	const long long read = KIND == IANE_SEQ_REDUCE ? aTiles : aTiles - 1;
 	Acc carry = aInit;
 	for (long long start = 0; start < aTiles; start += kSeqTile) {
+		// a scan needs each thread's totals consecutive; a reduction takes them
+		// strided by the workgroup, so that each load of a wave is one coalesced run
 		const long long first = start + static_cast<long long>(threadIdx.x) * kSeqPer;
+		const auto index = [&](int aE) {
+			return KIND == IANE_SEQ_REDUCE ? start + threadIdx.x + static_cast<long long>(aE) * kSeqThreads : first + aE;
+		};
 		Acc v[kSeqPer];
 #pragma unroll
 		for (int e = 0; e < kSeqPer; ++e) {
-			v[e] = p[min(first + e, read - 1)];
+			v[e] = p[min(index(e), read - 1)];
 		}
 		Acc run = identity<Acc, OP>();
 		Acc before_local[kSeqPer];
 #pragma unroll
 		for (int e = 0; e < kSeqPer; ++e) {
 			before_local[e] = run;
-			run = apply<Acc, OP>(run, first + e < read ? v[e] : identity<Acc, OP>());
+			run = apply<Acc, OP>(run, index(e) < read ? v[e] : identity<Acc, OP>());
 		}
 		Acc total;
 		const Acc before = block_exclusive_scan<Acc, OP>(run, total, shared);
@@ -985,6 +995,16 @@ extern "C" int iane_hip_sequence_run(const iane_sequence_plan* aPlan, const iane
 				++k;
 			}
 			m.other_rank = k;
+			// lines at line x stride: the kernels skip the per-line index division
+			const auto nests = [&](const long long* aStride) {
+				for (int j = 1; j < m.other_rank; ++j) {
+					if (aStride[j] != aStride[j - 1] * m.other_extent[j - 1]) {
+						return false;
+					}
+				}
+				return true;
+			};
+			m.nested = nests(m.src_other_stride) && nests(m.dst_other_stride);
 		}
 		void* partials = aPlan->partials_bytes ? ws : nullptr;
 		switch (aPlan->acc_type) {
