@@ -72,6 +72,16 @@ class Timer:
         return ms.value
 
 
+def settle_clocks(timer, fn, ms: float = 20.0, max_calls: int = 2000):
+    """Run `fn` untimed until it has kept the GPU busy for `ms` milliseconds (at
+    most `max_calls` calls): clocks take tens of milliseconds of sustained work
+    to settle, and a short leg timed straight after a cold start reads them low."""
+    busy, calls = 0.0, 0
+    while busy < ms and calls < max_calls:
+        busy += timer.time(fn, 50)
+        calls += 50
+
+
 def traffic_from_profiles(kernel_tag: str):
     """HBM bytes per launch from a committed rocprofv3 PMC summary, if present."""
     path = ROOT / "profiles" / f"{kernel_tag}_traffic.json"
@@ -614,6 +624,7 @@ def main() -> int:
         fn = lambda: iane_amd.convolve(v16, binom, o32, o, policy)  # noqa: E731
         fn()
         _lib.call("iane_hip_stream_sync", stream)
+        settle_clocks(timer, fn)
         k = 20
         t = timer.time(fn, k) / k
         extra["u16_binomial3"] = {"ms": round(t, 4), "Mvoxel_s": round(voxels / t / 1e3, 1),
@@ -630,6 +641,7 @@ def main() -> int:
         fn = lambda: iane_amd.convolve(vol, k3, out, o, policy)  # noqa: E731
         fn()
         _lib.call("iane_hip_stream_sync", stream)
+        settle_clocks(timer, fn)
         t = timer.time(fn, k) / k
         extra["dense3_fp32"] = {"ms": round(t, 4), "Mvoxel_s": round(voxels / t / 1e3, 1),
                                 "GB_s": round(voxels * 8 / t / 1e6, 1),
