@@ -419,6 +419,57 @@ __global__ __launch_bounds__(kSeqThreads) void row_totals_kernel(seq_map m, Acc*
 	}
 }
 
+// row_totals_kernel for full-size tiles (kSeqPer elements per thread) of
+// contiguous lines of whole 16-byte vectors: thread t folds vectors t, t + 256,
+// .. of the tile (each wave load one coalesced 1 KiB run), all loads in flight
+// before the first is folded.
+template<class Acc, int OP>
+__global__ __launch_bounds__(kSeqThreads) void row_totals_vec_kernel(seq_map m, Acc* aPartials, long long aTiles,
+	int aSkipLast)
+{
+	using vec_t = unsigned __attribute__((ext_vector_type(4)));
+	__shared__ Acc shared[4];
+	const long long task = blockIdx.x;
+	const long long line = task / aTiles;
+	const long long t = task - line * aTiles;
+	if (aSkipLast && t == aTiles - 1) {
+		return; // uniform over the workgroup
+	}
+	const long long begin = t * kSeqTile;
+	const int n = static_cast<int>(min(static_cast<long long>(kSeqTile), m.length - begin));
+	Acc acc = identity<Acc, OP>();
+	with_element_type(m.src_type, [&](auto aType) {
+		using T = decltype(aType);
+		constexpr int SV = 16 / static_cast<int>(sizeof(T));
+		constexpr int NS = kSeqPer / SV;
+		struct parts {
+			T e[SV];
+		};
+		const char* src = m.src + line_base(m.src_other_stride, m, line) + begin * static_cast<long long>(sizeof(T));
+		const int vectors = n / SV;
+		vec_t raw[NS];
+#pragma unroll
+		for (int i = 0; i < NS; ++i) {
+			raw[i] = *reinterpret_cast<const vec_t*>(src + 16 * min(static_cast<int>(threadIdx.x) + kSeqThreads * i,
+				vectors - 1));
+		}
+#pragma unroll
+		for (int i = 0; i < NS; ++i) {
+			const parts p = __builtin_bit_cast(parts, raw[i]);
+			const bool valid = static_cast<int>(threadIdx.x) + kSeqThreads * i < vectors;
+#pragma unroll
+			for (int j = 0; j < SV; ++j) {
+				acc = apply<Acc, OP>(acc, valid ? static_cast<Acc>(p.e[j]) : identity<Acc, OP>());
+			}
+		}
+	});
+	Acc total;
+	(void)block_exclusive_scan<Acc, OP>(acc, total, shared);
+	if (threadIdx.x == 0) {
+		aPartials[task] = total;
+	}
+}
+
 // Carries of one line: slot k <- init (+) totals of tiles 0..k-1 (scans), or the
 // line's result (reduction, written to the destination). Each thread takes
 // kSeqPer totals, loaded together: a consecutive segment (scans) or every
@@ -668,12 +719,30 @@ int vector_lines(const seq_map& m)
 	return vectors <= 64 ? 1 : (vectors <= 128 ? 2 : (vectors <= 256 ? 4 : 0));
 }
 
+// Lines of whole 16-byte source vectors (unit step, 16-byte aligned line
+// starts): full-size tiles take row_totals_vec_kernel.
+bool vector_source_rows(const seq_map& m)
+{
+	const long long ss = dtype_size(m.src_type);
+	if (m.src_step != ss || (m.length * ss) % 16 != 0 || reinterpret_cast<uintptr_t>(m.src) % 16 != 0) {
+		return false;
+	}
+	for (int k = 0; k < m.other_rank; ++k) {
+		if (m.src_other_stride[k] % 16 != 0) {
+			return false;
+		}
+	}
+	return true;
+}
+
 // Tile totals with as many loads per thread as the tile needs (4 or 16).
 template<class Acc, int OP>
 void launch_row_totals(const iane_sequence_plan& p, const seq_map& m, Acc* aPartials, long long aTasks, int aSkipLast,
 	hipStream_t aStream)
 {
-	if (p.per <= 4) {
+	if (p.per == kSeqPer && vector_source_rows(m)) {
+		row_totals_vec_kernel<Acc, OP><<<grid_of(aTasks), kSeqThreads, 0, aStream>>>(m, aPartials, p.tiles, aSkipLast);
+	} else if (p.per <= 4) {
 		row_totals_kernel<Acc, OP, 4><<<grid_of(aTasks), kSeqThreads, 0, aStream>>>(m, aPartials, p.tiles, p.per,
 			aSkipLast);
 	} else {
