@@ -596,6 +596,98 @@ __global__ __launch_bounds__(kSeqThreads) void row_output_kernel(seq_map m, cons
 	}
 }
 
+// row_output_kernel for full-size tiles (kSeqPer elements per thread) of
+// contiguous lines whose source and destination rows are whole 16-byte vectors:
+// the global loads and stores move 16-byte vectors (thread t takes vectors t,
+// t + 256, .. of the tile, each wave access one coalesced run) to and from the
+// same LDS stage, and the segments, scans and carries between them are those
+// of row_output_kernel, so the results are the same bits.
+template<class Acc, int OP, int KIND>
+__global__ __launch_bounds__(kSeqThreads) void row_output_vec_kernel(seq_map m, const Acc* aCarries, long long aTiles,
+	Acc aInit)
+{
+	using vec_t = unsigned __attribute__((ext_vector_type(4)));
+	__shared__ Acc stage[kSeqTile + kSeqTile / 16];
+	const auto slot = [](int aIndex) { return aIndex + (aIndex >> 4); };
+	__shared__ Acc shared[4];
+	const long long task = blockIdx.x;
+	const long long line = task / aTiles;
+	const long long t = task - line * aTiles;
+	const long long begin = t * kSeqTile;
+	const int n = static_cast<int>(min(static_cast<long long>(kSeqTile), m.length - begin));
+	const int tid = static_cast<int>(threadIdx.x);
+	with_element_type(m.src_type, [&](auto aType) {
+		using T = decltype(aType);
+		constexpr int SV = 16 / static_cast<int>(sizeof(T));
+		constexpr int NS = kSeqPer / SV;
+		struct parts {
+			T e[SV];
+		};
+		const char* src = m.src + line_base(m.src_other_stride, m, line) + begin * static_cast<long long>(sizeof(T));
+		const int vectors = n / SV;
+		vec_t raw[NS];
+#pragma unroll
+		for (int i = 0; i < NS; ++i) {
+			raw[i] = *reinterpret_cast<const vec_t*>(src + 16 * min(tid + kSeqThreads * i, vectors - 1));
+		}
+#pragma unroll
+		for (int i = 0; i < NS; ++i) {
+			const int v = tid + kSeqThreads * i;
+			if (v < vectors) {
+				const parts p = __builtin_bit_cast(parts, raw[i]);
+#pragma unroll
+				for (int j = 0; j < SV; ++j) {
+					stage[slot(v * SV + j)] = static_cast<Acc>(p.e[j]);
+				}
+			}
+		}
+	});
+	__syncthreads();
+	Acc local[kSeqPer];
+	Acc run = identity<Acc, OP>();
+#pragma unroll
+	for (int e = 0; e < kSeqPer; ++e) {
+		const int index = kSeqPer * tid + e;
+		const Acc x = index < n ? stage[slot(index)] : identity<Acc, OP>();
+		local[e] = run;
+		run = apply<Acc, OP>(run, x);
+		if constexpr (KIND == IANE_SEQ_INCLUSIVE) {
+			local[e] = run;
+		}
+	}
+	Acc total;
+	const Acc prefix = block_exclusive_scan<Acc, OP>(run, total, shared);
+	const Acc carry = aTiles > 1 ? aCarries[task] : aInit;
+	const Acc before = apply<Acc, OP>(carry, prefix);
+#pragma unroll
+	for (int e = 0; e < kSeqPer; ++e) {
+		const int index = kSeqPer * tid + e;
+		if (index < n) {
+			stage[slot(index)] = apply<Acc, OP>(before, local[e]);
+		}
+	}
+	__syncthreads();
+	constexpr int DV = 16 / static_cast<int>(sizeof(Acc));
+	constexpr int ND = kSeqPer / DV;
+	struct out_parts {
+		Acc e[DV];
+	};
+	char* dst = m.dst + line_base(m.dst_other_stride, m, line) + begin * static_cast<long long>(sizeof(Acc));
+	const int out_vectors = n / DV;
+#pragma unroll
+	for (int i = 0; i < ND; ++i) {
+		const int v = tid + kSeqThreads * i;
+		if (v < out_vectors) {
+			out_parts o;
+#pragma unroll
+			for (int j = 0; j < DV; ++j) {
+				o.e[j] = stage[slot(v * DV + j)];
+			}
+			*reinterpret_cast<vec_t*>(dst + 16LL * v) = __builtin_bit_cast(vec_t, o);
+		}
+	}
+}
+
 // ---- column mode ----------------------------------------------------------
 template<class Acc, int OP>
 __global__ __launch_bounds__(kSeqThreads) void col_totals_kernel(seq_map m, Acc* aPartials, long long aChunks,
@@ -735,6 +827,22 @@ bool vector_source_rows(const seq_map& m)
 	return true;
 }
 
+// Scan destinations of whole 16-byte vectors of Acc (unit step, 16-byte
+// aligned line starts): full-size tiles take row_output_vec_kernel.
+bool vector_dest_rows(const seq_map& m, size_t aAccBytes)
+{
+	const long long ds = static_cast<long long>(aAccBytes);
+	if (m.dst_step != ds || (m.length * ds) % 16 != 0 || reinterpret_cast<uintptr_t>(m.dst) % 16 != 0) {
+		return false;
+	}
+	for (int k = 0; k < m.other_rank; ++k) {
+		if (m.dst_other_stride[k] % 16 != 0) {
+			return false;
+		}
+	}
+	return true;
+}
+
 // Tile totals with as many loads per thread as the tile needs (4 or 16).
 template<class Acc, int OP>
 void launch_row_totals(const iane_sequence_plan& p, const seq_map& m, Acc* aPartials, long long aTasks, int aSkipLast,
@@ -800,7 +908,9 @@ void run_kind(const iane_sequence_plan& p, const seq_map& m, void* aPartials, hi
 				launch_row_totals<Acc, OP>(p, m, partials, tasks, 1, s);
 				row_carries_kernel<Acc, OP, KIND><<<grid_of(m.lines), kSeqThreads, 0, s>>>(m, partials, p.tiles, init);
 			}
-			if (p.per <= 4) {
+			if (p.per == kSeqPer && vector_source_rows(m) && vector_dest_rows(m, sizeof(Acc))) {
+				row_output_vec_kernel<Acc, OP, KIND><<<grid_of(tasks), kSeqThreads, 0, s>>>(m, partials, p.tiles, init);
+			} else if (p.per <= 4) {
 				row_output_kernel<Acc, OP, KIND, 4><<<grid_of(tasks), kSeqThreads, 0, s>>>(m, partials, p.tiles, p.per, init);
 			} else {
 				row_output_kernel<Acc, OP, KIND, kSeqPer><<<grid_of(tasks), kSeqThreads, 0, s>>>(m, partials, p.tiles, p.per,

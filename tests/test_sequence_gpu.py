@@ -194,3 +194,29 @@ def test_full_size_whole_scan_and_reduce_closed_form():
     assert total.item() == 2.0 ** 24
     flat = scan.flatten()
     assert torch.equal(flat, torch.arange(1, 2 ** 24 + 1, dtype=torch.float64, device="cuda"))
+
+
+@pytest.mark.parametrize("acc", [np.float32, np.float64, np.int32])
+def test_vector_and_scalar_tile_paths_agree(acc):
+    """Lines of several 4096-element tiles whose rows are whole 16-byte vectors take
+    the 16-byte tile-total and scan-output kernels; the same data through a view
+    4 bytes off alignment takes the per-element kernels. Integer-valued data makes
+    every grouping exact, so both must equal the oracle bit for bit (a misplaced
+    element or vector shows up as a wrong prefix)."""
+    import torch
+    rng = np.random.default_rng(11)
+    host = rng.integers(0, 8, (5, 3, 13292)).astype(np.float32 if acc != np.int32 else np.int32)
+    aligned = torch.from_numpy(host).cuda()
+    buf = torch.empty(host.size + 1, dtype=aligned.dtype, device="cuda")
+    buf[1:] = aligned.flatten()
+    shifted = buf[1:].view(host.shape)
+    assert aligned.data_ptr() % 16 == 0 and shifted.data_ptr() % 16 == 4
+    for kind, axis in itertools.product(KINDS, (-1, 0)):
+        ref = oracle.sequence(host, KINDS[kind], axis, "sum", 3, acc)
+        for src in (aligned, shifted):
+            dst = torch.full(out_shape(host.shape, kind, axis), 77, dtype=torch.from_numpy(np.zeros(1, acc)).dtype,
+                             device="cuda")
+            ctx = iane_amd.make_execution_context()
+            prepared = ctx.prepare(algorithm(kind, axis, "sum", 3), src, dst)
+            ctx.run(prepared, src, dst).wait()
+            np.testing.assert_array_equal(dst.cpu().numpy(), ref, err_msg=f"{kind} axis={axis}")
