@@ -472,49 +472,71 @@ __global__ __launch_bounds__(kSeqThreads) void row_totals_vec_kernel(seq_map m, 
 
 // Carries of one line: slot k <- init (+) totals of tiles 0..k-1 (scans), or the
 // line's result (reduction, written to the destination). Each thread takes
-// kSeqPer totals, loaded together: a consecutive segment (scans) or every
-// 256th (reductions).
+// kSeqPer totals: a reduction every 256th (each wave load one coalesced run); a
+// scan a consecutive segment, moved between coalesced loads / stores and the
+// segments through LDS (as the output pass does).
 template<class Acc, int OP, int KIND>
 __global__ __launch_bounds__(kSeqThreads) void row_carries_kernel(seq_map m, Acc* aPartials, long long aTiles, Acc aInit)
 {
+	constexpr bool scan = KIND != IANE_SEQ_REDUCE;
+	__shared__ Acc stage[scan ? kSeqTile + kSeqTile / 16 : 1];
+	const auto slot = [](int aIndex) { return aIndex + (aIndex >> 4); };
 	__shared__ Acc shared[4];
+	const int tid = static_cast<int>(threadIdx.x);
 	const long long line = blockIdx.x;
 	Acc* p = aPartials + line * aTiles;
-	const long long read = KIND == IANE_SEQ_REDUCE ? aTiles : aTiles - 1;
+	const long long read = scan ? aTiles - 1 : aTiles;
 	Acc carry = aInit;
 	for (long long start = 0; start < aTiles; start += kSeqTile) {
-		// a scan needs each thread's totals consecutive; a reduction takes them
-		// strided by the workgroup, so that each load of a wave is one coalesced run
-		const long long first = start + static_cast<long long>(threadIdx.x) * kSeqPer;
-		const auto index = [&](int aE) {
-			return KIND == IANE_SEQ_REDUCE ? start + threadIdx.x + static_cast<long long>(aE) * kSeqThreads : first + aE;
-		};
+		// clamped loads, masked afterwards: no branch between them
 		Acc v[kSeqPer];
 #pragma unroll
 		for (int e = 0; e < kSeqPer; ++e) {
-			v[e] = p[min(index(e), read - 1)];
+			v[e] = p[min(start + tid + static_cast<long long>(e) * kSeqThreads, read - 1)];
+		}
+#pragma unroll
+		for (int e = 0; e < kSeqPer; ++e) {
+			v[e] = start + tid + static_cast<long long>(e) * kSeqThreads < read ? v[e] : identity<Acc, OP>();
+		}
+		if constexpr (scan) {
+#pragma unroll
+			for (int e = 0; e < kSeqPer; ++e) {
+				stage[slot(tid + e * kSeqThreads)] = v[e];
+			}
+			__syncthreads();
+#pragma unroll
+			for (int e = 0; e < kSeqPer; ++e) {
+				v[e] = stage[slot(kSeqPer * tid + e)];
+			}
 		}
 		Acc run = identity<Acc, OP>();
 		Acc before_local[kSeqPer];
 #pragma unroll
 		for (int e = 0; e < kSeqPer; ++e) {
 			before_local[e] = run;
-			run = apply<Acc, OP>(run, index(e) < read ? v[e] : identity<Acc, OP>());
+			run = apply<Acc, OP>(run, v[e]);
 		}
 		Acc total;
 		const Acc before = block_exclusive_scan<Acc, OP>(run, total, shared);
-		if (KIND != IANE_SEQ_REDUCE) {
+		if constexpr (scan) {
 			const Acc head = apply<Acc, OP>(carry, before);
 #pragma unroll
 			for (int e = 0; e < kSeqPer; ++e) {
-				if (first + e < aTiles) {
-					p[first + e] = apply<Acc, OP>(head, before_local[e]);
+				stage[slot(kSeqPer * tid + e)] = apply<Acc, OP>(head, before_local[e]);
+			}
+			__syncthreads();
+#pragma unroll
+			for (int e = 0; e < kSeqPer; ++e) {
+				const long long k = start + tid + static_cast<long long>(e) * kSeqThreads;
+				if (k < aTiles) {
+					p[k] = stage[slot(tid + e * kSeqThreads)];
 				}
 			}
+			__syncthreads();
 		}
 		carry = apply<Acc, OP>(carry, total);
 	}
-	if (KIND == IANE_SEQ_REDUCE && threadIdx.x == 0) {
+	if (!scan && threadIdx.x == 0) {
 		store_value(m.dst + line_base(m.dst_other_stride, m, line), carry);
 	}
 }
